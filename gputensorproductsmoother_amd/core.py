@@ -1,0 +1,319 @@
+"""Host-side mirror of the reference's hot-path API over the C ABI.
+
+Class and method names follow the reference (PSMF namespace):
+
+* :class:`LaplaceOperator`  - include/laplace_operator.cuh:71-205
+* :class:`PatchSmoother`    - include/patch_smoother.cuh:86-124, 313-335
+* :class:`MGTransferCUDA`   - include/cuda_mg_transfer.cuh:57-358
+* :class:`MultigridSolvers` - include/solver.cuh:1541-2210
+
+Vectors are 1D ``torch`` tensors on the context's device (float32 for FP32
+objects, float64 for FP64), lexicographic with boundary DoFs included.  All
+work is enqueued on the context stream, which defaults to torch's current
+stream so that torch and gtps operations are ordered.  Errors raise
+:class:`GtpsError` (the reference raises deal.II ExcMessage / AssertCuda).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib as L
+
+FP32, FP64 = L.GTPS_FP32, L.GTPS_FP64
+
+_DTYPES = {FP32: torch.float32, FP64: torch.float64}
+
+
+def _prec(dtype) -> int:
+    if dtype in (FP32, torch.float32, "float32", "fp32", "float"):
+        return FP32
+    if dtype in (FP64, torch.float64, "float64", "fp64", "double"):
+        return FP64
+    raise ValueError(f"unsupported precision {dtype!r}")
+
+
+class Context:
+    """Device + HIP stream shared by all objects built on it."""
+
+    def __init__(self, device: int = 0, stream=None):
+        lib = L.load()
+        self.device = device
+        if stream is None and torch.cuda.is_available():
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = ctypes.c_void_p()
+        L.check(lib.gtps_ctx_create(device, ctypes.c_void_p(stream or 0), ctypes.byref(h)),
+                "ctx_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        L.check(L.load().gtps_ctx_synchronize(self._h), "synchronize")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.load().gtps_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # FP64 vector operations (deal.II Vector<double,CUDA> counterparts)
+    def dot(self, x: torch.Tensor, y: torch.Tensor) -> float:
+        _chk_vec(x, torch.float64)
+        _chk_vec(y, torch.float64, x.numel())
+        out = ctypes.c_double()
+        L.check(L.load().gtps_vec_dot(self._h, x.numel(), _ptr(x), _ptr(y), ctypes.byref(out)),
+                "dot")
+        return out.value
+
+    def l2_norm(self, x: torch.Tensor) -> float:
+        _chk_vec(x, torch.float64)
+        out = ctypes.c_double()
+        L.check(L.load().gtps_vec_nrm2(self._h, x.numel(), _ptr(x), ctypes.byref(out)), "nrm2")
+        return out.value
+
+    def sadd(self, s: float, a: float, x: torch.Tensor, y: torch.Tensor):
+        """y = s*y + a*x (deal.II Vector::sadd(s, a, x))."""
+        _chk_vec(x, torch.float64)
+        _chk_vec(y, torch.float64, x.numel())
+        L.check(L.load().gtps_vec_axpby(self._h, x.numel(), a, _ptr(x), s, _ptr(y)), "axpby")
+
+    def convert(self, src: torch.Tensor, dst: torch.Tensor):
+        """copy_vector: FP64 -> FP32 (round to nearest) or FP32 -> FP64."""
+        lib = L.load()
+        if src.dtype == torch.float64 and dst.dtype == torch.float32:
+            rc = lib.gtps_vec_convert_f64_f32(self._h, src.numel(), _ptr(src), _ptr(dst))
+        elif src.dtype == torch.float32 and dst.dtype == torch.float64:
+            rc = lib.gtps_vec_convert_f32_f64(self._h, src.numel(), _ptr(src), _ptr(dst))
+        else:
+            raise ValueError("convert needs one float32 and one float64 tensor")
+        _chk_vec(dst, dst.dtype, src.numel())
+        L.check(rc, "convert")
+
+
+def _ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _chk_vec(t: torch.Tensor, dtype, n: int | None = None):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch.Tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("vector must be contiguous")
+    if t.device.type != "cuda":
+        raise ValueError("vector must live on the GPU")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"vector has {t.numel()} entries, expected {n}")
+
+
+class LaplaceOperator:
+    """Matrix-free Laplace operator of one level (LaplaceOperator<dim,k,Number>)."""
+
+    def __init__(self, ctx: Context, dim: int, fe_degree: int, level: int, dtype=FP64):
+        self.ctx, self.dim, self.fe_degree, self.level = ctx, dim, fe_degree, level
+        self.prec = _prec(dtype)
+        self.dtype = _DTYPES[self.prec]
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_level_create(ctx.handle, dim, fe_degree, level, self.prec,
+                                           ctypes.byref(h)), "level_create")
+        self._h = h
+        self.n_dofs = int(L.load().gtps_level_n_dofs(h))
+        self.n_dofs_1d = fe_degree * (1 << level) + 1
+
+    @property
+    def handle(self):
+        return self._h
+
+    def initialize_dof_vector(self) -> torch.Tensor:
+        return torch.zeros(self.n_dofs, dtype=self.dtype, device=f"cuda:{self.ctx.device}")
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, self.dtype, self.n_dofs)
+        _chk_vec(src, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_laplace_vmult(self._h, _ptr(dst), _ptr(src)), "vmult")
+
+    Tvmult = vmult
+
+    def residual(self, dst: torch.Tensor, b: torch.Tensor, u: torch.Tensor):
+        """dst = b - A u."""
+        for t in (dst, b, u):
+            _chk_vec(t, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_laplace_residual(self._h, _ptr(dst), _ptr(b), _ptr(u)),
+                "residual")
+
+    def set_constrained_values(self, v: torch.Tensor, value: float = 0.0):
+        _chk_vec(v, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_vec_set_boundary(self._h, _ptr(v), value), "set_boundary")
+
+    def tensor_product_data(self) -> dict:
+        """FP64 1D data: cell/patch mass and Laplace, GPU eigenpairs."""
+        import numpy as np
+        k = self.fe_degree
+        n, np_, ni = k + 1, 2 * k + 1, 2 * k - 1
+        out = dict(cell_mass=np.zeros((n, n)), cell_laplace=np.zeros((n, n)),
+                   patch_mass=np.zeros((np_, np_)), patch_laplace=np.zeros((np_, np_)),
+                   eigenvalues=np.zeros(ni), eigenvectors=np.zeros((ni, ni)))
+        p = [out[key].ctypes.data_as(L._pd) for key in
+             ("cell_mass", "cell_laplace", "patch_mass", "patch_laplace",
+              "eigenvalues", "eigenvectors")]
+        L.check(L.load().gtps_level_tensor_product_data(self._h, *p), "tensor_product_data")
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_level_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class PatchSmoother:
+    """Vertex-patch Schwarz smoother with FD local solves (PatchSmoother)."""
+
+    MULTIPLICATIVE = L.GTPS_MULT_FWD
+    MULTIPLICATIVE_REVERSE = L.GTPS_MULT_REV
+    ADDITIVE = L.GTPS_ADDITIVE
+
+    def __init__(self, op: LaplaceOperator, variant: int = L.GTPS_MULT_FWD,
+                 relaxation: float = 1.0):
+        self.op, self.variant, self.relaxation = op, variant, relaxation
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_smoother_create(op.handle, variant, relaxation, ctypes.byref(h)),
+                "smoother_create")
+        self._h = h
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor):
+        """dst = 0, then one sweep."""
+        _chk_vec(dst, self.op.dtype, self.op.n_dofs)
+        _chk_vec(src, self.op.dtype, self.op.n_dofs)
+        L.check(L.load().gtps_smoother_vmult(self._h, _ptr(dst), _ptr(src)), "smoother_vmult")
+
+    def step(self, dst: torch.Tensor, src: torch.Tensor):
+        """One sweep in place on dst."""
+        _chk_vec(dst, self.op.dtype, self.op.n_dofs)
+        _chk_vec(src, self.op.dtype, self.op.n_dofs)
+        L.check(L.load().gtps_smoother_step(self._h, _ptr(dst), _ptr(src)), "smoother_step")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_smoother_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class MGTransferCUDA:
+    """Level transfer (MGTransferCUDA<dim,Number,Q>)."""
+
+    def __init__(self, ctx: Context, dim: int, fe_degree: int, min_level: int,
+                 max_level: int, dtype=FP64):
+        self.ctx, self.dim, self.fe_degree = ctx, dim, fe_degree
+        self.min_level, self.max_level = min_level, max_level
+        self.prec = _prec(dtype)
+        self.dtype = _DTYPES[self.prec]
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_transfer_create(ctx.handle, dim, fe_degree, min_level, max_level,
+                                              self.prec, ctypes.byref(h)), "transfer_create")
+        self._h = h
+
+    def _n(self, level):
+        n1 = self.fe_degree * (1 << level) + 1
+        return n1 ** self.dim
+
+    def prolongate(self, to_level: int, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, self.dtype, self._n(to_level))
+        _chk_vec(src, self.dtype, self._n(to_level - 1))
+        L.check(L.load().gtps_transfer_prolongate(self._h, to_level, _ptr(dst), _ptr(src)),
+                "prolongate")
+
+    def prolongate_and_add(self, to_level: int, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, self.dtype, self._n(to_level))
+        _chk_vec(src, self.dtype, self._n(to_level - 1))
+        L.check(L.load().gtps_transfer_prolongate_add(self._h, to_level, _ptr(dst), _ptr(src)),
+                "prolongate_add")
+
+    def restrict_and_add(self, from_level: int, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, self.dtype, self._n(from_level - 1))
+        _chk_vec(src, self.dtype, self._n(from_level))
+        L.check(L.load().gtps_transfer_restrict_add(self._h, from_level, _ptr(dst), _ptr(src)),
+                "restrict_add")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_transfer_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class MultigridSolvers:
+    """MG hierarchy (levels 1..max_level) with the FP64 MG-preconditioned CG.
+
+    ``vcycle_dtype`` float32 gives the mixed-precision form (FP32 smoother,
+    level operators and transfers under FP64 CG); float64 gives FP64
+    everywhere.  ``schedule`` selects the smoothing order (symmetric by
+    default, required for CG).
+    """
+
+    SYMMETRIC = L.GTPS_MG_SYMMETRIC
+    REFERENCE_ORDER = L.GTPS_MG_REFERENCE_ORDER
+    ADDITIVE = L.GTPS_MG_ADDITIVE
+
+    def __init__(self, ctx: Context, dim: int, fe_degree: int, max_level: int,
+                 vcycle_dtype=FP32, schedule: int = L.GTPS_MG_SYMMETRIC,
+                 relaxation: float = 1.0):
+        self.ctx, self.dim, self.fe_degree, self.max_level = ctx, dim, fe_degree, max_level
+        self.prec = _prec(vcycle_dtype)
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_mg_create(ctx.handle, dim, fe_degree, max_level, self.prec,
+                                        schedule, relaxation, ctypes.byref(h)), "mg_create")
+        self._h = h
+        self.n_dofs = int(L.load().gtps_mg_n_dofs(h))
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor):
+        """Preconditioner z = V(r) (MultigridSolvers::vmult)."""
+        _chk_vec(dst, torch.float64, self.n_dofs)
+        _chk_vec(src, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_mg_vcycle(self._h, _ptr(dst), _ptr(src)), "vcycle")
+
+    def do_matvec(self, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, torch.float64, self.n_dofs)
+        _chk_vec(src, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_mg_matvec(self._h, _ptr(dst), _ptr(src)), "matvec")
+
+    def solve_cg(self, b: torch.Tensor, x: torch.Tensor | None = None, rtol: float = 1e-8,
+                 maxit: int = 100, raise_on_failure: bool = True):
+        """Solve A x = b; returns (x, iterations, relative residual, history)."""
+        _chk_vec(b, torch.float64, self.n_dofs)
+        if x is None:
+            x = torch.empty_like(b)
+        _chk_vec(x, torch.float64, self.n_dofs)
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        hist = (ctypes.c_double * (maxit + 1))()
+        rc = L.load().gtps_mg_pcg_solve(self._h, _ptr(x), _ptr(b), rtol, maxit,
+                                        ctypes.byref(it), ctypes.byref(rel), hist)
+        if rc != L.GTPS_OK and (raise_on_failure or rc not in (L.GTPS_ENOTCONV,)):
+            L.check(rc, "pcg_solve")
+        return x, it.value, rel.value, list(hist[: it.value + 1])
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_mg_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

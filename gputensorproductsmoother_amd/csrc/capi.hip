@@ -1,0 +1,843 @@
+// capi.hip - the extern "C" boundary (include/gtps.h) and the host-side
+// multigrid / CG orchestration.
+//
+// Host code owns no numerics: it validates arguments, allocates level
+// vectors once, and enqueues kernels on the context stream.  The V-cycle
+// follows MultigridSolvers::v_cycle (include/solver.cuh:1952-2001) without
+// the per-stage device synchronisations; CG replaces SolverGMRES
+// (solver.cuh:1865-1910) and keeps its scalars on the device.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gtps_internal.h"
+
+using namespace gtps;
+
+namespace
+{
+  thread_local std::string g_last_error;
+
+  template <typename F>
+  int
+  guard(F &&f)
+  {
+    try
+      {
+        f();
+        g_last_error.clear();
+        return GTPS_OK;
+      }
+    catch (const Error &e)
+      {
+        g_last_error = e.what();
+        return e.code;
+      }
+    catch (const std::bad_alloc &)
+      {
+        g_last_error = "host allocation failed";
+        return GTPS_ENOMEM;
+      }
+    catch (const std::exception &e)
+      {
+        g_last_error = e.what();
+        return GTPS_EHIP;
+      }
+    catch (...)
+      {
+        g_last_error = "unknown error";
+        return GTPS_EHIP;
+      }
+  }
+
+  void
+  set_device(const gtps_ctx *ctx)
+  {
+    GTPS_HIP(hipSetDevice(ctx->device));
+  }
+
+  void *
+  dmalloc(size_t bytes)
+  {
+    void *p = nullptr;
+    if (bytes == 0)
+      bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess)
+      {
+        (void)hipGetLastError();
+        throw Error(GTPS_ENOMEM, "hipMalloc of " + std::to_string(bytes) +
+                                   " bytes failed");
+      }
+    return p;
+  }
+
+  void
+  dfree(void *p)
+  {
+    if (p)
+      (void)hipFree(p);
+  }
+
+  void
+  check_level_args(int dim, int degree, int level, int precision)
+  {
+    GTPS_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    GTPS_REQUIRE(degree >= 1 && degree <= MAX_DEGREE, "degree must be in 1..7");
+    GTPS_REQUIRE(level >= 1, "level must be >= 1");
+    GTPS_REQUIRE(precision == GTPS_FP32 || precision == GTPS_FP64,
+                 "precision must be GTPS_FP32 or GTPS_FP64");
+    // N1 must fit the kernels' 32-bit per-direction indices and the vector
+    // length a 64-bit index
+    GTPS_REQUIRE(level <= (dim == 3 ? 12 : 20), "level too large");
+  }
+
+  gtps_level *
+  make_level(gtps_ctx *ctx, int dim, int degree, int level, int precision)
+  {
+    check_level_args(dim, degree, level, precision);
+    auto lvl   = std::make_unique<gtps_level>();
+    lvl->ctx   = ctx;
+    lvl->dim   = dim;
+    lvl->k     = degree;
+    lvl->level = level;
+    lvl->n     = 1 << level;
+    lvl->N1    = degree * lvl->n + 1;
+    lvl->N     = dim == 3 ? (long long)lvl->N1 * lvl->N1 * lvl->N1
+                          : (long long)lvl->N1 * lvl->N1;
+    lvl->prec  = precision;
+    lvl->tab   = get_tables(ctx, degree);
+    lvl->scale = std::pow(std::ldexp(1.0, -level), dim - 2);
+    return lvl.release();
+  }
+
+  // one smoothing sweep; `resid` is level-sized scratch for the additive form
+  void
+  sweep(const gtps_level *lvl, void *u, const void *b, int variant,
+        double relax, void *resid)
+  {
+    const int ncol = 1 << lvl->dim;
+    if (variant == GTPS_ADDITIVE)
+      {
+        GTPS_REQUIRE(resid != nullptr, "additive sweep needs a residual buffer");
+        launch_laplace(lvl, resid, u, b, false);
+        for (int c = 0; c < ncol; ++c)
+          launch_patch_colour(lvl, u, resid, c, 1, relax);
+        return;
+      }
+    for (int cc = 0; cc < ncol; ++cc)
+      {
+        const int c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
+        launch_patch_colour(lvl, u, b, c, 0, relax);
+      }
+  }
+} // namespace
+
+struct gtps_mg
+{
+  gtps_ctx                  *ctx = nullptr;
+  int                        dim = 3, k = 1, maxl = 1, prec = GTPS_FP32;
+  int                        schedule = GTPS_MG_SYMMETRIC;
+  double                     relax    = 1.0;
+  std::vector<gtps_level *>  lv;       // [0..maxl], index 0 unused
+  gtps_level                *fine64 = nullptr;
+  bool                       own_fine64 = false;
+  gtps_transfer             *tr = nullptr;
+  std::vector<void *>        d, u, t;  // level vectors in prec
+  double                    *r = nullptr, *p = nullptr, *q = nullptr;
+  hipEvent_t                 ev = nullptr;
+
+  int
+  pre_variant() const
+  {
+    return schedule == GTPS_MG_ADDITIVE ? GTPS_ADDITIVE : GTPS_MULT_FWD;
+  }
+  int
+  post_variant() const
+  {
+    if (schedule == GTPS_MG_ADDITIVE)
+      return GTPS_ADDITIVE;
+    return schedule == GTPS_MG_SYMMETRIC ? GTPS_MULT_REV : GTPS_MULT_FWD;
+  }
+
+  // V-cycle on level l with defect dl and correction ul (level precision)
+  void
+  vcycle(int l, const void *dl, void *ul)
+  {
+    const gtps_level *L = lv[l];
+    launch_fill(ctx->stream, L->N, prec, ul, 0.0);
+    if (l == 1)
+      {
+        // one vertex patch covers the domain: exact solve
+        sweep(L, ul, dl, GTPS_MULT_FWD, 1.0, nullptr);
+        return;
+      }
+    sweep(L, ul, dl, pre_variant(), relax, t[l]);
+    launch_laplace(L, t[l], ul, dl, false);  // t = d - A u
+    launch_fill(ctx->stream, lv[l - 1]->N, prec, d[l - 1], 0.0);
+    launch_restrict_add(tr, l, d[l - 1], t[l]);
+    vcycle(l - 1, d[l - 1], u[l - 1]);
+    launch_prolongate_add(tr, l, ul, u[l - 1]);
+    sweep(L, ul, dl, post_variant(), relax, t[l]);
+  }
+
+  // z = V(r) in FP64, r and z finest-level vectors
+  void
+  precondition(const double *rin, double *zout, bool convert_in)
+  {
+    const long long N = lv[maxl]->N;
+    if (prec == GTPS_FP32)
+      {
+        if (convert_in)
+          launch_convert_f64_f32(ctx->stream, N, rin, (float *)d[maxl]);
+        vcycle(maxl, d[maxl], u[maxl]);
+        if (zout)
+          launch_convert_f32_f64(ctx->stream, N, (const float *)u[maxl], zout);
+      }
+    else
+      vcycle(maxl, rin, zout ? (void *)zout : u[maxl]);
+  }
+
+  ~gtps_mg()
+  {
+    for (size_t l = 0; l < lv.size(); ++l)
+      {
+        dfree(d.size() > l ? d[l] : nullptr);
+        dfree(u.size() > l ? u[l] : nullptr);
+        dfree(t.size() > l ? t[l] : nullptr);
+        delete lv[l];
+      }
+    if (own_fine64)
+      delete fine64;
+    if (tr)
+      {
+        dfree(tr->tmp1);
+        dfree(tr->tmp2);
+        delete tr;
+      }
+    dfree(r);
+    dfree(p);
+    dfree(q);
+    if (ev)
+      (void)hipEventDestroy(ev);
+  }
+};
+
+extern "C"
+{
+  const char *
+  gtps_last_error(void)
+  {
+    return g_last_error.c_str();
+  }
+
+  const char *
+  gtps_version(void)
+  {
+    return "gtps 0.1.0 gfx950";
+  }
+
+  int
+  gtps_ctx_create(int device, void *stream, gtps_ctx **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(out != nullptr, "out is NULL");
+      int ndev = 0;
+      GTPS_HIP(hipGetDeviceCount(&ndev));
+      GTPS_REQUIRE(device >= 0 && device < ndev, "no such device");
+      GTPS_HIP(hipSetDevice(device));
+      auto ctx    = std::make_unique<gtps_ctx>();
+      ctx->device = device;
+      if (stream)
+        ctx->stream = (hipStream_t)stream;
+      else
+        {
+          GTPS_HIP(hipStreamCreate(&ctx->stream));
+          ctx->own_stream = true;
+        }
+      ctx->ws.capacity = 65536;
+      ctx->ws.partials =
+        (double *)dmalloc(sizeof(double) * ctx->ws.capacity);
+      ctx->ws.scalars = (double *)dmalloc(sizeof(double) * cg::N_SLOTS);
+      GTPS_HIP(hipHostMalloc((void **)&ctx->ws.pinned,
+                             sizeof(double) * cg::N_SLOTS,
+                             hipHostMallocDefault));
+      *out = ctx.release();
+    });
+  }
+
+  int
+  gtps_ctx_destroy(gtps_ctx *ctx)
+  {
+    return guard([&] {
+      if (!ctx)
+        return;
+      (void)hipSetDevice(ctx->device);
+      (void)hipStreamSynchronize(ctx->stream);
+      for (auto &kv : ctx->tables)
+        {
+          dfree(kv.second->d64);
+          dfree(kv.second->d32);
+        }
+      dfree(ctx->ws.partials);
+      dfree(ctx->ws.scalars);
+      if (ctx->ws.pinned)
+        (void)hipHostFree(ctx->ws.pinned);
+      if (ctx->own_stream)
+        (void)hipStreamDestroy(ctx->stream);
+      delete ctx;
+    });
+  }
+
+  int
+  gtps_ctx_synchronize(gtps_ctx *ctx)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx, "ctx is NULL");
+      set_device(ctx);
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+    });
+  }
+
+  void *
+  gtps_ctx_stream(gtps_ctx *ctx)
+  {
+    return ctx ? (void *)ctx->stream : nullptr;
+  }
+
+  // ------------------------------------------------------------- levels
+  int
+  gtps_level_create(gtps_ctx *ctx, int dim, int degree, int level,
+                    int precision, gtps_level **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && out, "NULL argument");
+      set_device(ctx);
+      *out = make_level(ctx, dim, degree, level, precision);
+    });
+  }
+
+  int
+  gtps_level_destroy(gtps_level *lvl)
+  {
+    return guard([&] { delete lvl; });
+  }
+
+  long long
+  gtps_level_n_dofs(const gtps_level *lvl)
+  {
+    return lvl ? lvl->N : -1;
+  }
+
+  int
+  gtps_laplace_vmult(gtps_level *lvl, void *dst, const void *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && dst && src, "NULL argument");
+      GTPS_REQUIRE(dst != src, "vmult cannot run in place");
+      set_device(lvl->ctx);
+      launch_laplace(lvl, dst, src, nullptr, false);
+    });
+  }
+
+  int
+  gtps_laplace_residual(gtps_level *lvl, void *dst, const void *b,
+                        const void *u)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && dst && b && u, "NULL argument");
+      GTPS_REQUIRE(dst != u, "residual output cannot alias u");
+      set_device(lvl->ctx);
+      launch_laplace(lvl, dst, u, b, false);
+    });
+  }
+
+  int
+  gtps_level_tensor_product_data(gtps_level *lvl, double *cell_mass,
+                                 double *cell_laplace, double *patch_mass,
+                                 double *patch_laplace, double *eigenvalues,
+                                 double *eigenvectors)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl, "NULL level");
+      const Basis1D &b    = lvl->tab->basis;
+      auto           copy = [](const std::vector<double> &v, double *o) {
+        if (o)
+          std::memcpy(o, v.data(), sizeof(double) * v.size());
+      };
+      copy(b.cellM, cell_mass);
+      copy(b.cellK, cell_laplace);
+      copy(b.patchM, patch_mass);
+      copy(b.patchK, patch_laplace);
+      copy(b.lam, eigenvalues);
+      copy(b.S, eigenvectors);
+    });
+  }
+
+  // ------------------------------------------------------------ smoother
+  int
+  gtps_smoother_create(gtps_level *lvl, int variant, double relaxation,
+                       gtps_smoother **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && out, "NULL argument");
+      GTPS_REQUIRE(variant == GTPS_MULT_FWD || variant == GTPS_MULT_REV ||
+                     variant == GTPS_ADDITIVE,
+                   "unknown smoother variant");
+      GTPS_REQUIRE(std::isfinite(relaxation) && relaxation > 0.0,
+                   "relaxation must be positive");
+      set_device(lvl->ctx);
+      auto sm     = std::make_unique<gtps_smoother>();
+      sm->lvl     = lvl;
+      sm->variant = variant;
+      sm->relax   = relaxation;
+      if (variant == GTPS_ADDITIVE)
+        sm->resid = dmalloc(lvl->elem() * lvl->N);
+      *out = sm.release();
+    });
+  }
+
+  int
+  gtps_smoother_destroy(gtps_smoother *sm)
+  {
+    return guard([&] {
+      if (!sm)
+        return;
+      dfree(sm->resid);
+      delete sm;
+    });
+  }
+
+  int
+  gtps_smoother_step(gtps_smoother *sm, void *u, const void *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(sm && u && b, "NULL argument");
+      GTPS_REQUIRE(u != b, "u and b must differ");
+      set_device(sm->lvl->ctx);
+      sweep(sm->lvl, u, b, sm->variant, sm->relax, sm->resid);
+    });
+  }
+
+  int
+  gtps_smoother_vmult(gtps_smoother *sm, void *u, const void *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(sm && u && b, "NULL argument");
+      GTPS_REQUIRE(u != b, "u and b must differ");
+      set_device(sm->lvl->ctx);
+      launch_fill(sm->lvl->ctx->stream, sm->lvl->N, sm->lvl->prec, u, 0.0);
+      sweep(sm->lvl, u, b, sm->variant, sm->relax, sm->resid);
+    });
+  }
+
+  // ------------------------------------------------------------ transfer
+  int
+  gtps_transfer_create(gtps_ctx *ctx, int dim, int degree, int min_level,
+                       int max_level, int precision, gtps_transfer **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && out, "NULL argument");
+      check_level_args(dim, degree, max_level, precision);
+      GTPS_REQUIRE(min_level >= 1 && min_level < max_level,
+                   "need 1 <= min_level < max_level");
+      set_device(ctx);
+      auto tr       = std::make_unique<gtps_transfer>();
+      tr->ctx       = ctx;
+      tr->dim       = dim;
+      tr->k         = degree;
+      tr->min_level = min_level;
+      tr->max_level = max_level;
+      tr->prec      = precision;
+      tr->tab       = get_tables(ctx, degree);
+      const size_t e = transfer_tmp_elems(dim, degree, max_level);
+      const size_t s = precision == GTPS_FP32 ? sizeof(float) : sizeof(double);
+      tr->tmp1       = dmalloc(e * s);
+      tr->tmp2       = dmalloc(e * s);
+      *out           = tr.release();
+    });
+  }
+
+  int
+  gtps_transfer_destroy(gtps_transfer *tr)
+  {
+    return guard([&] {
+      if (!tr)
+        return;
+      dfree(tr->tmp1);
+      dfree(tr->tmp2);
+      delete tr;
+    });
+  }
+
+  int
+  gtps_transfer_prolongate_add(gtps_transfer *tr, int to_level, void *fine,
+                               const void *coarse)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(tr && fine && coarse, "NULL argument");
+      GTPS_REQUIRE(to_level > tr->min_level && to_level <= tr->max_level,
+                   "to_level out of range");
+      set_device(tr->ctx);
+      launch_prolongate_add(tr, to_level, fine, coarse);
+    });
+  }
+
+  int
+  gtps_transfer_prolongate(gtps_transfer *tr, int to_level, void *fine,
+                           const void *coarse)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(tr && fine && coarse, "NULL argument");
+      GTPS_REQUIRE(to_level > tr->min_level && to_level <= tr->max_level,
+                   "to_level out of range");
+      set_device(tr->ctx);
+      const long long nf = 1LL * tr->k * (1LL << to_level) + 1;
+      launch_fill(tr->ctx->stream, tr->dim == 3 ? nf * nf * nf : nf * nf,
+                  tr->prec, fine, 0.0);
+      launch_prolongate_add(tr, to_level, fine, coarse);
+    });
+  }
+
+  int
+  gtps_transfer_restrict_add(gtps_transfer *tr, int from_level, void *coarse,
+                             const void *fine)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(tr && fine && coarse, "NULL argument");
+      GTPS_REQUIRE(from_level > tr->min_level && from_level <= tr->max_level,
+                   "from_level out of range");
+      set_device(tr->ctx);
+      launch_restrict_add(tr, from_level, coarse, fine);
+    });
+  }
+
+  // ------------------------------------------------------------- vectors
+  int
+  gtps_vec_fill(gtps_ctx *ctx, long long n, int precision, void *x,
+                double value)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && (x || n == 0) && n >= 0, "bad argument");
+      set_device(ctx);
+      launch_fill(ctx->stream, n, precision, x, value);
+    });
+  }
+
+  int
+  gtps_vec_axpby(gtps_ctx *ctx, long long n, double a, const double *x,
+                 double b, double *y)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && n >= 0 && (n == 0 || (x && y)), "bad argument");
+      set_device(ctx);
+      launch_axpby(ctx->stream, n, a, x, b, y);
+    });
+  }
+
+  int
+  gtps_vec_dot(gtps_ctx *ctx, long long n, const double *x, const double *y,
+               double *result)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && result && n >= 0 && (n == 0 || (x && y)),
+                   "bad argument");
+      set_device(ctx);
+      if (n == 0)
+        {
+          *result = 0.0;
+          return;
+        }
+      const int nb = launch_dot_partials(ctx->stream, n, x, y, nullptr,
+                                         ctx->ws.partials, ctx->ws.capacity);
+      launch_finalize_sum(ctx->stream, ctx->ws.partials, nb, ctx->ws.scalars,
+                          cg::N_SLOTS - 1);
+      GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, ctx->ws.scalars + cg::N_SLOTS - 1,
+                              sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+      *result = ctx->ws.pinned[0];
+    });
+  }
+
+  int
+  gtps_vec_nrm2(gtps_ctx *ctx, long long n, const double *x, double *result)
+  {
+    double d  = 0.0;
+    int    rc = gtps_vec_dot(ctx, n, x, x, &d);
+    if (rc == GTPS_OK && result)
+      *result = std::sqrt(d);
+    return rc;
+  }
+
+  int
+  gtps_vec_convert_f64_f32(gtps_ctx *ctx, long long n, const double *x,
+                           float *y)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && n >= 0 && (n == 0 || (x && y)), "bad argument");
+      set_device(ctx);
+      launch_convert_f64_f32(ctx->stream, n, x, y);
+    });
+  }
+
+  int
+  gtps_vec_convert_f32_f64(gtps_ctx *ctx, long long n, const float *x,
+                           double *y)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && n >= 0 && (n == 0 || (x && y)), "bad argument");
+      set_device(ctx);
+      launch_convert_f32_f64(ctx->stream, n, x, y);
+    });
+  }
+
+  int
+  gtps_vec_set_boundary(gtps_level *lvl, void *v, double value)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && v, "NULL argument");
+      set_device(lvl->ctx);
+      launch_set_boundary(lvl, v, value);
+    });
+  }
+
+  // ------------------------------------------------------------ multigrid
+  int
+  gtps_mg_create(gtps_ctx *ctx, int dim, int degree, int max_level,
+                 int vcycle_precision, int schedule, double relaxation,
+                 gtps_mg **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && out, "NULL argument");
+      check_level_args(dim, degree, max_level, vcycle_precision);
+      GTPS_REQUIRE(max_level >= 1, "max_level must be >= 1");
+      GTPS_REQUIRE(schedule == GTPS_MG_SYMMETRIC ||
+                     schedule == GTPS_MG_REFERENCE_ORDER ||
+                     schedule == GTPS_MG_ADDITIVE,
+                   "unknown schedule");
+      GTPS_REQUIRE(std::isfinite(relaxation) && relaxation > 0.0,
+                   "relaxation must be positive");
+      set_device(ctx);
+      auto mg      = std::make_unique<gtps_mg>();
+      mg->ctx      = ctx;
+      mg->dim      = dim;
+      mg->k        = degree;
+      mg->maxl     = max_level;
+      mg->prec     = vcycle_precision;
+      mg->schedule = schedule;
+      mg->relax    = relaxation;
+      mg->lv.assign(max_level + 1, nullptr);
+      mg->d.assign(max_level + 1, nullptr);
+      mg->u.assign(max_level + 1, nullptr);
+      mg->t.assign(max_level + 1, nullptr);
+      const size_t es =
+        vcycle_precision == GTPS_FP32 ? sizeof(float) : sizeof(double);
+      for (int l = 1; l <= max_level; ++l)
+        {
+          mg->lv[l] = make_level(ctx, dim, degree, l, vcycle_precision);
+          const size_t bytes = es * mg->lv[l]->N;
+          mg->d[l]           = dmalloc(bytes);
+          mg->u[l]           = dmalloc(bytes);
+          mg->t[l]           = dmalloc(bytes);
+        }
+      if (vcycle_precision == GTPS_FP64)
+        mg->fine64 = mg->lv[max_level];
+      else
+        {
+          mg->fine64     = make_level(ctx, dim, degree, max_level, GTPS_FP64);
+          mg->own_fine64 = true;
+        }
+      if (max_level >= 2)
+        {
+          mg->tr            = new gtps_transfer();
+          mg->tr->ctx       = ctx;
+          mg->tr->dim       = dim;
+          mg->tr->k         = degree;
+          mg->tr->min_level = 1;
+          mg->tr->max_level = max_level;
+          mg->tr->prec      = vcycle_precision;
+          mg->tr->tab       = get_tables(ctx, degree);
+          const size_t e    = transfer_tmp_elems(dim, degree, max_level);
+          mg->tr->tmp1      = dmalloc(e * es);
+          mg->tr->tmp2      = dmalloc(e * es);
+        }
+      const long long N = mg->lv[max_level]->N;
+      mg->r             = (double *)dmalloc(sizeof(double) * N);
+      mg->p             = (double *)dmalloc(sizeof(double) * N);
+      mg->q             = (double *)dmalloc(sizeof(double) * N);
+      GTPS_HIP(hipEventCreateWithFlags(&mg->ev, hipEventDisableTiming));
+      // zero boundary entries of the level vectors once: kernels never write
+      // them except through vmult / residual
+      for (int l = 1; l <= max_level; ++l)
+        {
+          launch_fill(ctx->stream, mg->lv[l]->N, vcycle_precision, mg->d[l], 0.0);
+          launch_fill(ctx->stream, mg->lv[l]->N, vcycle_precision, mg->u[l], 0.0);
+          launch_fill(ctx->stream, mg->lv[l]->N, vcycle_precision, mg->t[l], 0.0);
+        }
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+      *out = mg.release();
+    });
+  }
+
+  int
+  gtps_mg_destroy(gtps_mg *mg)
+  {
+    return guard([&] {
+      if (!mg)
+        return;
+      (void)hipSetDevice(mg->ctx->device);
+      (void)hipStreamSynchronize(mg->ctx->stream);
+      delete mg;
+    });
+  }
+
+  long long
+  gtps_mg_n_dofs(const gtps_mg *mg)
+  {
+    return mg ? mg->lv[mg->maxl]->N : -1;
+  }
+
+  int
+  gtps_mg_vcycle(gtps_mg *mg, double *z, const double *r)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && z && r, "NULL argument");
+      GTPS_REQUIRE((const void *)z != (const void *)r, "z and r must differ");
+      set_device(mg->ctx);
+      mg->precondition(r, z, true);
+    });
+  }
+
+  int
+  gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && dst && src && (const void *)dst != (const void *)src,
+                   "bad argument");
+      set_device(mg->ctx);
+      launch_laplace(mg->fine64, dst, src, nullptr, false);
+    });
+  }
+
+  int
+  gtps_mg_pcg_solve(gtps_mg *mg, double *x, const double *b, double rtol,
+                    int maxit, int *iters, double *relres, double *history)
+  {
+    int  it_done = 0;
+    double rel   = 0.0;
+    bool converged = false, nonfinite = false;
+    int rc = guard([&] {
+      GTPS_REQUIRE(mg && x && b && (const void *)x != (const void *)b,
+                   "bad argument");
+      GTPS_REQUIRE(maxit >= 0 && rtol >= 0.0, "bad rtol / maxit");
+      set_device(mg->ctx);
+      gtps_ctx      *ctx = mg->ctx;
+      hipStream_t    s   = ctx->stream;
+      const long long N  = mg->fine64->N;
+      double        *sc  = ctx->ws.scalars;
+      double        *pt  = ctx->ws.partials;
+      const int      cap = ctx->ws.capacity;
+      const bool     f32 = mg->prec == GTPS_FP32;
+      float         *r32 = f32 ? (float *)mg->d[mg->maxl] : nullptr;
+      float         *z32 = f32 ? (float *)mg->u[mg->maxl] : nullptr;
+      double        *z64 = f32 ? nullptr : (double *)mg->u[mg->maxl];
+
+      auto read_rr = [&]() {
+        GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, sc + cg::RR, sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        GTPS_HIP(hipEventRecord(mg->ev, s));
+      };
+      auto wait_rr = [&]() {
+        GTPS_HIP(hipEventSynchronize(mg->ev));
+        return std::sqrt(ctx->ws.pinned[0]);
+      };
+      // z = V(r), rz = r.z, beta, p = z + beta p
+      auto precondition_and_direction = [&](bool first) {
+        if (f32)
+          mg->vcycle(mg->maxl, r32, z32);
+        else
+          mg->vcycle(mg->maxl, mg->r, z64);
+        launch_cg_scalar(s, sc, cg::OP_SHIFT);
+        const int nb = launch_dot_partials(s, N, mg->r, z64, z32, pt, cap);
+        launch_finalize_sum(s, pt, nb, sc, cg::RZ);
+        launch_cg_scalar(s, sc, first ? cg::OP_BETA0 : cg::OP_BETA);
+        launch_cg_update_p(s, N, mg->p, z64, z32, sc);
+      };
+
+      // x0 = lift(b), r0 = b - A x0
+      launch_lift_boundary(mg->fine64, b, x);
+      launch_laplace(mg->fine64, mg->r, x, b, false);
+      {
+        const int nb = launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap);
+        launch_finalize_sum(s, pt, nb, sc, cg::RR);
+      }
+      if (f32)
+        launch_convert_f64_f32(s, N, mg->r, r32);
+      launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
+      read_rr();
+      const double r0 = wait_rr();
+      if (history)
+        history[0] = r0;
+      if (!std::isfinite(r0))
+        {
+          nonfinite = true;
+          return;
+        }
+      if (r0 == 0.0)
+        {
+          converged = true;
+          return;
+        }
+      precondition_and_direction(true);
+      double rn = r0;
+      while (it_done < maxit)
+        {
+          // q = A p with fused p.q, alpha = rz / pq
+          const int nbq = launch_laplace(mg->fine64, mg->q, mg->p, nullptr, true);
+          launch_finalize_sum(s, pt, nbq, sc, cg::PQ);
+          launch_cg_scalar(s, sc, cg::OP_ALPHA);
+          const int nbr =
+            launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap);
+          launch_finalize_sum(s, pt, nbr, sc, cg::RR);
+          read_rr();
+          ++it_done;
+          // enqueue the next preconditioning step while the host checks
+          if (it_done < maxit)
+            precondition_and_direction(false);
+          rn = wait_rr();
+          if (history)
+            history[it_done] = rn;
+          if (!std::isfinite(rn))
+            {
+              nonfinite = true;
+              break;
+            }
+          if (rn <= rtol * r0)
+            {
+              converged = true;
+              break;
+            }
+        }
+      rel = rn / r0;
+      GTPS_HIP(hipStreamSynchronize(s));
+    });
+    if (iters)
+      *iters = it_done;
+    if (relres)
+      *relres = rel;
+    if (rc != GTPS_OK)
+      return rc;
+    if (nonfinite)
+      {
+        g_last_error = "non-finite residual in CG";
+        return GTPS_ENAN;
+      }
+    if (!converged)
+      {
+        g_last_error = "CG reached maxit without convergence";
+        return GTPS_ENOTCONV;
+      }
+    return GTPS_OK;
+  }
+}

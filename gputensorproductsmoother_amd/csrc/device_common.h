@@ -1,0 +1,107 @@
+// device_common.h - shared device-side definitions for the gfx950 kernels.
+//
+// Table layout: every degree K owns one packed array of 1D data per
+// precision (built once per context by setup.hip), passed to kernels as a
+// `const T* __restrict__` so that uniform, compile-time-indexed reads become
+// scalar loads.  Offsets are compile-time functions of K.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gtps
+{
+  constexpr int MAX_DEGREE = 7;
+
+  template <int K>
+  struct Tab
+  {
+    static constexpr int N  = K + 1;      // cell DoFs per direction
+    static constexpr int NP = 2 * K + 1;  // patch box per direction
+    static constexpr int NI = 2 * K - 1;  // patch interior per direction
+    static constexpr int W  = 2 * K + 1;  // width of a global 1D matrix row
+    // reference-cell mass and stiffness [N*N]
+    static constexpr int cellM = 0;
+    static constexpr int cellK = cellM + N * N;
+    // 2-cell patch mass and stiffness [NP*NP]
+    static constexpr int patchM = cellK + N * N;
+    static constexpr int patchK = patchM + NP * NP;
+    // M-orthonormal interior eigenvectors S[i*NI+j] (vector j) and values
+    static constexpr int S   = patchK + NP * NP;
+    static constexpr int lam = S + NI * NI;
+    // embedding (prolongation) matrix E[q*N+a], q in [0,2K], a in [0,K]
+    static constexpr int E = lam + NI;
+    // rows of the assembled global 1D matrices, by node position p in its
+    // cell (p = 0: vertex row), band[p*W + off + K] for off in [-K, K]
+    static constexpr int bandM = E + NP * N;
+    static constexpr int bandK = bandM + K * W;
+    static constexpr int size  = bandK + K * W;
+  };
+
+  // Column range of row r of a 2-cell patch matrix (block sparsity: rows
+  // inside cell 0 couple to nodes 0..K, rows inside cell 1 to K..2K, the
+  // shared vertex row to all).
+  template <int K>
+  __host__ __device__ constexpr int
+  prow_lo(int r)
+  {
+    return r <= K ? 0 : K;
+  }
+  template <int K>
+  __host__ __device__ constexpr int
+  prow_hi(int r)
+  {
+    return r < K ? K : 2 * K;
+  }
+  // Column offsets of a global 1D row for node position p within its cell.
+  template <int K>
+  __host__ __device__ constexpr int
+  band_lo(int p)
+  {
+    return p == 0 ? -K : -p;
+  }
+  template <int K>
+  __host__ __device__ constexpr int
+  band_hi(int p)
+  {
+    return p == 0 ? K : K - p;
+  }
+
+  template <typename T>
+  __device__ __forceinline__ T
+  wave_sum(T v)
+  {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      v += __shfl_xor(v, off, 64);
+    return v;
+  }
+
+  // Sum over the block; result valid in thread 0.  `red` holds >= NT/64
+  // doubles of LDS.
+  template <int NT>
+  __device__ __forceinline__ double
+  block_sum(double v, double *red)
+  {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+      red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+      {
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w)
+          s += red[w];
+      }
+    return s;
+  }
+
+  __device__ __forceinline__ int
+  linear_block_id()
+  {
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  }
+} // namespace gtps

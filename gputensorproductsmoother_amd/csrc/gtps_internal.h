@@ -1,0 +1,179 @@
+// gtps_internal.h - host-side objects and kernel launchers of libgtps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gtps.h"
+#include "device_common.h"
+
+namespace gtps
+{
+  struct Error : std::runtime_error
+  {
+    int code;
+    Error(int c, const std::string &m)
+      : std::runtime_error(m), code(c)
+    {}
+  };
+
+#define GTPS_HIP(call)                                                       \
+  do                                                                         \
+    {                                                                        \
+      hipError_t e_ = (call);                                                \
+      if (e_ != hipSuccess)                                                  \
+        throw ::gtps::Error(GTPS_EHIP, std::string(#call) + ": " +           \
+                                         hipGetErrorString(e_));             \
+    }                                                                        \
+  while (0)
+
+#define GTPS_REQUIRE(cond, msg)                                              \
+  do                                                                         \
+    {                                                                        \
+      if (!(cond))                                                           \
+        throw ::gtps::Error(GTPS_EINVAL, msg);                               \
+    }                                                                        \
+  while (0)
+
+#define GTPS_LAUNCH_CHECK() GTPS_HIP(hipGetLastError())
+
+  // FP64 1D data of one degree (host copy); eigenpairs come from the GPU.
+  struct Basis1D
+  {
+    int                 k = 0;
+    std::vector<double> nodes, qp, qw;
+    std::vector<double> cellM, cellK, patchM, patchK, E, lam, S;
+  };
+
+  // host-side 1D basis (GL nodes, Gauss rule, cell and patch matrices,
+  // embedding); eigenpairs are filled by setup_tables on the device.
+  Basis1D make_basis_host(int k);
+
+  struct DegreeTables
+  {
+    Basis1D basis;
+    double *d64 = nullptr;  // packed Tab<K> layout, FP64
+    float  *d32 = nullptr;  // same, FP32
+  };
+
+  struct Workspace
+  {
+    double *partials = nullptr;  // per-block partial sums
+    int     capacity = 0;
+    double *scalars  = nullptr;  // device scalars (CG state)
+    double *pinned   = nullptr;  // host-pinned mirror
+  };
+} // namespace gtps
+
+struct gtps_ctx
+{
+  int                                                     device = 0;
+  hipStream_t                                             stream = nullptr;
+  bool                                                    own_stream = false;
+  gtps::Workspace                                         ws;
+  std::map<int, std::unique_ptr<gtps::DegreeTables>>      tables;
+};
+
+struct gtps_level
+{
+  gtps_ctx                *ctx = nullptr;
+  int                      dim = 3, k = 1, level = 1, n = 2, N1 = 3;
+  long long                N   = 0;
+  int                      prec = GTPS_FP64;
+  gtps::DegreeTables      *tab  = nullptr;
+  double                   scale = 1.0;  // h^(dim-2)
+  const void *
+  dtab() const
+  {
+    return prec == GTPS_FP32 ? (const void *)tab->d32 : (const void *)tab->d64;
+  }
+  size_t
+  elem() const
+  {
+    return prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
+  }
+};
+
+struct gtps_smoother
+{
+  gtps_level *lvl      = nullptr;
+  int         variant  = GTPS_MULT_FWD;
+  double      relax    = 1.0;
+  void       *resid    = nullptr;  // additive variant: global residual
+};
+
+struct gtps_transfer
+{
+  gtps_ctx *ctx = nullptr;
+  int       dim = 3, k = 1, min_level = 1, max_level = 1, prec = GTPS_FP64;
+  void     *tmp1 = nullptr, *tmp2 = nullptr;
+  gtps::DegreeTables *tab = nullptr;
+};
+
+namespace gtps
+{
+  // setup.hip
+  DegreeTables *get_tables(gtps_ctx *ctx, int k);
+
+  // laplace.hip: y = A x (resid: y = b - A x); optional fused partial dot
+  // sum_i x_i y_i written per block to ctx->ws.partials; returns #blocks.
+  int launch_laplace(const gtps_level *lvl, void *y, const void *x,
+                     const void *b, bool fused_dot);
+
+  // smoother.hip: one colour pass. mode 0 multiplicative (src = b),
+  // mode 1 additive (src = global residual).
+  void launch_patch_colour(const gtps_level *lvl, void *u, const void *src,
+                           int colour, int mode, double relax);
+
+  // transfer.hip
+  void launch_prolongate_add(const gtps_transfer *tr, int to_level, void *fine,
+                             const void *coarse);
+  void launch_restrict_add(const gtps_transfer *tr, int from_level,
+                           void *coarse, const void *fine);
+  size_t transfer_tmp_elems(int dim, int k, int max_level);
+
+  // vector.hip
+  void launch_fill(hipStream_t s, long long n, int prec, void *x, double v);
+  void launch_axpby(hipStream_t s, long long n, double a, const double *x,
+                    double b, double *y);
+  void launch_convert_f64_f32(hipStream_t s, long long n, const double *x,
+                              float *y);
+  void launch_convert_f32_f64(hipStream_t s, long long n, const float *x,
+                              double *y);
+  int  launch_dot_partials(hipStream_t s, long long n, const double *x,
+                           const double *y, const float *y32, double *partials,
+                           int capacity);
+  // sum partials[0..nb) into scalars[slot] (deterministic, one block)
+  void launch_finalize_sum(hipStream_t s, const double *partials, int nb,
+                           double *scalars, int slot);
+  void launch_set_boundary(const gtps_level *lvl, void *v, double value);
+  void launch_lift_boundary(const gtps_level *lvl, const double *b, double *x);
+
+  // CG building blocks (vector.hip); scalar slots are defined there
+  void launch_cg_scalar(hipStream_t s, double *scalars, int op);
+  int  launch_cg_update_xr(hipStream_t s, long long n, double *x, double *r,
+                           const double *p, const double *q, float *r32,
+                           const double *scalars, double *partials,
+                           int capacity);
+  void launch_cg_update_p(hipStream_t s, long long n, double *p,
+                          const double *z, const float *z32,
+                          const double *scalars);
+
+  namespace cg
+  {
+    // device scalar slots
+    constexpr int RZ = 0, RZ_OLD = 1, PQ = 2, ALPHA = 3, BETA = 4, RR = 5,
+                  N_SLOTS = 8;
+    // scalar ops
+    constexpr int OP_ALPHA = 0;  // alpha = rz / pq
+    constexpr int OP_BETA  = 1;  // beta = rz / rz_old
+    constexpr int OP_BETA0 = 2;  // beta = 0 (first iteration)
+    constexpr int OP_SHIFT = 3;  // rz_old = rz
+  } // namespace cg
+} // namespace gtps

@@ -1,0 +1,483 @@
+// laplace.hip - matrix-free Laplace vmult on the uniform Q_k mesh.
+//
+// On a uniform Cartesian mesh with constant coefficient, the cell operator
+// the reference evaluates with Gauss(k+1) quadrature
+// (laplace_operator.cuh:34-65, dealii/cuda_fee.cuh:273-529) sums to the
+// Kronecker sum A = h^(d-2) sum_d K (x) M (x) ... of the assembled 1D
+// matrices (Gauss(k+1) is exact for the degree-2k 1D integrands).  The
+// kernels therefore work DoF-centred: every output row is owned by exactly
+// one work-group, so there are no atomics and results are deterministic.
+//
+// 3D kernel: a work-group owns an x-y tile of TCX x TCY cells and marches in
+// z over a chunk of cells.  Each thread keeps, for the columns of the tile
+// plus a halo of k nodes, a window of 2k+1 z-planes in registers; per output
+// plane it applies the z-rows of M and K (pass 1), then the y-rows (pass 2)
+// and x-rows (pass 3) through LDS.  The node position p inside a cell is a
+// compile-time loop index in every pass, so stencil coefficients are uniform
+// (scalar) reads.  Dirichlet rows copy the input (y_B = x_B), as
+// copy_constrained_values does (dealii/cuda_mf.template.cuh:494-508).
+//
+// HBM traffic per call: read x once, write y once (2 N sizeof(T)); halo
+// columns are re-read from L2 by neighbouring tiles.
+#include "gtps_internal.h"
+
+namespace gtps
+{
+  namespace
+  {
+    template <int K>
+    struct VT3;
+    // cells per tile in x and y, threads per work-group
+    template <>
+    struct VT3<1>
+    {
+      static constexpr int TCX = 48, TCY = 12, NT = 256;
+    };
+    template <>
+    struct VT3<2>
+    {
+      static constexpr int TCX = 24, TCY = 6, NT = 256;
+    };
+    template <>
+    struct VT3<3>
+    {
+      static constexpr int TCX = 16, TCY = 4, NT = 256;
+    };
+    template <>
+    struct VT3<4>
+    {
+      static constexpr int TCX = 12, TCY = 3, NT = 256;
+    };
+    template <>
+    struct VT3<5>
+    {
+      static constexpr int TCX = 10, TCY = 3, NT = 512;
+    };
+    template <>
+    struct VT3<6>
+    {
+      static constexpr int TCX = 8, TCY = 2, NT = 512;
+    };
+    template <>
+    struct VT3<7>
+    {
+      static constexpr int TCX = 8, TCY = 2, NT = 512;
+    };
+
+    template <int K, typename T, bool RESID, bool DOT>
+    __global__ void __launch_bounds__(VT3<K>::NT)
+    laplace3d_kernel(const T *__restrict__ x, T *__restrict__ y,
+                     const T *__restrict__ bv, const T *__restrict__ tab,
+                     const int n, const int zc, const T scale,
+                     double *__restrict__ partial)
+    {
+      using TB               = Tab<K>;
+      constexpr int TCX      = VT3<K>::TCX;
+      constexpr int TCY      = VT3<K>::TCY;
+      constexpr int NT       = VT3<K>::NT;
+      constexpr int W        = 2 * K + 1;
+      constexpr int WX       = K * (TCX + 2) + 1;
+      constexpr int WY       = K * (TCY + 2) + 1;
+      constexpr int TY       = K * TCY;
+      constexpr int NCOLS    = WX * WY;
+      constexpr int NCOL     = (NCOLS + NT - 1) / NT;
+
+      __shared__ T      sA[NCOLS], sB[NCOLS], sC1[TY * WX], sC2[TY * WX];
+      __shared__ double sred[NT / 64];
+
+      const int       N1  = K * n + 1;
+      const long long P   = (long long)N1 * N1;
+      const int       tid = threadIdx.x;
+      const int       cx0 = blockIdx.x * TCX, cy0 = blockIdx.y * TCY;
+      const int       cz0 = blockIdx.z * zc;
+      const int       cz1 = min(cz0 + zc, n);
+      const int       gx0 = K * cx0 - K, gy0 = K * cy0 - K;
+      const int       ox0 = K * cx0, oy0 = K * cy0;
+      const int       ox1 = (cx0 + TCX >= n) ? N1 : K * (cx0 + TCX);
+      const int       oy1 = (cy0 + TCY >= n) ? N1 : K * (cy0 + TCY);
+      const T *bM = tab + TB::bandM;
+      const T *bK = tab + TB::bandK;
+
+      T         w[NCOL][W];
+      long long cidx[NCOL];
+      bool      own[NCOL], bcol[NCOL];
+#pragma unroll
+      for (int c = 0; c < NCOL; ++c)
+        {
+          const int  col   = tid + c * NT;
+          const int  lx    = col % WX, ly = col / WX;
+          const int  gx    = gx0 + lx, gy = gy0 + ly;
+          const bool valid = col < NCOLS && gx >= 0 && gx < N1 && gy >= 0 &&
+                             gy < N1;
+          cidx[c] = valid ? gx + (long long)gy * N1 : 0;
+          own[c]  = valid && gx >= ox0 && gx < ox1 && gy >= oy0 && gy < oy1;
+          bcol[c] = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1;
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            {
+              const int gz = K * (cz0 - 1) + j;
+              w[c][j] = (valid && gz >= 0 && gz < N1) ? x[cidx[c] + gz * P] : T(0);
+            }
+          if (!valid)
+            cidx[c] = -1;
+        }
+
+      double dot = 0.0;
+      for (int cz = cz0; cz < cz1; ++cz)
+        {
+#pragma unroll
+          for (int p = 0; p < K; ++p)
+            {
+              const int       gz   = K * cz + p;
+              const long long zoff = gz * P;
+              // pass 1: z-rows of M and K on every column of tile + halo
+#pragma unroll
+              for (int c = 0; c < NCOL; ++c)
+                {
+                  const int col = tid + c * NT;
+                  if (col < NCOLS)
+                    {
+                      T a = 0, b = 0;
+#pragma unroll
+                      for (int o = band_lo<K>(p); o <= band_hi<K>(p); ++o)
+                        {
+                          a += bM[p * W + o + K] * w[c][K + p + o];
+                          b += bK[p * W + o + K] * w[c][K + p + o];
+                        }
+                      sA[col] = a;
+                      sB[col] = b;
+                      if (own[c] && (bcol[c] || gz == 0))
+                        {
+                          const long long g  = cidx[c] + zoff;
+                          const T         xv = w[c][K + p];
+                          const T         yv = RESID ? bv[g] - xv : xv;
+                          y[g]               = yv;
+                          if (DOT)
+                            dot += (double)xv * (double)yv;
+                        }
+                    }
+                }
+              __syncthreads();
+              if (gz != 0)
+                {
+                  // pass 2: y-rows; C1 = My (Mz x), C2 = Ky (Mz x) + My (Kz x)
+                  for (int task = tid; task < WX * TCY; task += NT)
+                    {
+                      const int lx = task % WX, cyl = task / WX;
+                      T         a[W], b[W];
+#pragma unroll
+                      for (int j = 0; j < W; ++j)
+                        {
+                          a[j] = sA[(K * cyl + j) * WX + lx];
+                          b[j] = sB[(K * cyl + j) * WX + lx];
+                        }
+#pragma unroll
+                      for (int q = 0; q < K; ++q)
+                        {
+                          T c1 = 0, c2 = 0;
+#pragma unroll
+                          for (int o = band_lo<K>(q); o <= band_hi<K>(q); ++o)
+                            {
+                              c1 += bM[q * W + o + K] * a[K + q + o];
+                              c2 += bK[q * W + o + K] * a[K + q + o];
+                              c2 += bM[q * W + o + K] * b[K + q + o];
+                            }
+                          sC1[(K * cyl + q) * WX + lx] = c1;
+                          sC2[(K * cyl + q) * WX + lx] = c2;
+                        }
+                    }
+                  __syncthreads();
+                  // pass 3: x-rows; y = h^(d-2) (Kx C1 + Mx C2) on interior rows
+                  for (int task = tid; task < TCX * TY; task += NT)
+                    {
+                      const int cxl = task % TCX, ty = task / TCX;
+                      const int gy  = K * cy0 + ty;
+                      T         c1[W], c2[W];
+#pragma unroll
+                      for (int j = 0; j < W; ++j)
+                        {
+                          c1[j] = sC1[ty * WX + K * cxl + j];
+                          c2[j] = sC2[ty * WX + K * cxl + j];
+                        }
+#pragma unroll
+                      for (int q = 0; q < K; ++q)
+                        {
+                          const int gx = K * (cx0 + cxl) + q;
+                          if (gx >= 1 && gx < N1 - 1 && gy >= 1 && gy < N1 - 1)
+                            {
+                              T v = 0;
+#pragma unroll
+                              for (int o = band_lo<K>(q); o <= band_hi<K>(q);
+                                   ++o)
+                                {
+                                  v += bK[q * W + o + K] * c1[K + q + o];
+                                  v += bM[q * W + o + K] * c2[K + q + o];
+                                }
+                              v *= scale;
+                              const long long g  = gx + (long long)gy * N1 + zoff;
+                              const T         yv = RESID ? bv[g] - v : v;
+                              y[g]               = yv;
+                              if (DOT)
+                                dot += (double)x[g] * (double)yv;
+                            }
+                        }
+                    }
+                }
+              __syncthreads();
+            }
+          // shift the window by one cell and load the next k planes
+#pragma unroll
+          for (int c = 0; c < NCOL; ++c)
+            {
+#pragma unroll
+              for (int j = 0; j <= K; ++j)
+                w[c][j] = w[c][j + K];
+              if (cz + 1 < cz1)
+                {
+#pragma unroll
+                  for (int j = K + 1; j < W; ++j)
+                    {
+                      const int gz = K * cz + j;
+                      w[c][j] = (cidx[c] >= 0 && gz < N1) ? x[cidx[c] + gz * P] : T(0);
+                    }
+                }
+            }
+        }
+      // last plane of the domain: Dirichlet rows
+      if (cz1 == n)
+        {
+          const long long zoff = (long long)(N1 - 1) * P;
+#pragma unroll
+          for (int c = 0; c < NCOL; ++c)
+            if (own[c])
+              {
+                const long long g  = cidx[c] + zoff;
+                const T         xv = w[c][K];
+                const T         yv = RESID ? bv[g] - xv : xv;
+                y[g]               = yv;
+                if (DOT)
+                  dot += (double)xv * (double)yv;
+              }
+        }
+      if (DOT)
+        {
+          const double s = block_sum<NT>(dot, sred);
+          if (tid == 0)
+            partial[linear_block_id()] = s;
+        }
+    }
+
+    // 2D: a work-group owns TCX x TCY cells; tile + halo staged in LDS.
+    template <int K>
+    struct VT2
+    {
+      static constexpr int TCX = (64 + K - 1) / K, TCY = (16 + K - 1) / K,
+                           NT = 256;
+    };
+
+    template <int K, typename T, bool RESID, bool DOT>
+    __global__ void __launch_bounds__(256)
+    laplace2d_kernel(const T *__restrict__ x, T *__restrict__ y,
+                     const T *__restrict__ bv, const T *__restrict__ tab,
+                     const int n, const T scale, double *__restrict__ partial)
+    {
+      using TB          = Tab<K>;
+      constexpr int TCX = VT2<K>::TCX, TCY = VT2<K>::TCY, NT = VT2<K>::NT;
+      constexpr int W   = 2 * K + 1;
+      constexpr int WX  = K * (TCX + 2) + 1, WY = K * (TCY + 2) + 1;
+      constexpr int TY  = K * TCY;
+      __shared__ T      sX[WX * WY], sC1[TY * WX], sC2[TY * WX];
+      __shared__ double sred[NT / 64];
+      const int         N1  = K * n + 1;
+      const int         tid = threadIdx.x;
+      const int         cx0 = blockIdx.x * TCX, cy0 = blockIdx.y * TCY;
+      const int         gx0 = K * cx0 - K, gy0 = K * cy0 - K;
+      const int         ox0 = K * cx0, oy0 = K * cy0;
+      const int         ox1 = (cx0 + TCX >= n) ? N1 : K * (cx0 + TCX);
+      const int         oy1 = (cy0 + TCY >= n) ? N1 : K * (cy0 + TCY);
+      const T *bM = tab + TB::bandM;
+      const T *bK = tab + TB::bandK;
+      double   dot = 0.0;
+      for (int i = tid; i < WX * WY; i += NT)
+        {
+          const int lx = i % WX, ly = i / WX;
+          const int gx = gx0 + lx, gy = gy0 + ly;
+          const bool valid = gx >= 0 && gx < N1 && gy >= 0 && gy < N1;
+          const T    xv    = valid ? x[gx + (long long)gy * N1] : T(0);
+          sX[i]            = xv;
+          if (valid && gx >= ox0 && gx < ox1 && gy >= oy0 && gy < oy1 &&
+              (gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1))
+            {
+              const long long g  = gx + (long long)gy * N1;
+              const T         yv = RESID ? bv[g] - xv : xv;
+              y[g]               = yv;
+              if (DOT)
+                dot += (double)xv * (double)yv;
+            }
+        }
+      __syncthreads();
+      for (int task = tid; task < WX * TCY; task += NT)
+        {
+          const int lx = task % WX, cyl = task / WX;
+          T         a[W];
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            a[j] = sX[(K * cyl + j) * WX + lx];
+#pragma unroll
+          for (int q = 0; q < K; ++q)
+            {
+              T c1 = 0, c2 = 0;
+#pragma unroll
+              for (int o = band_lo<K>(q); o <= band_hi<K>(q); ++o)
+                {
+                  c1 += bM[q * W + o + K] * a[K + q + o];
+                  c2 += bK[q * W + o + K] * a[K + q + o];
+                }
+              sC1[(K * cyl + q) * WX + lx] = c1;
+              sC2[(K * cyl + q) * WX + lx] = c2;
+            }
+        }
+      __syncthreads();
+      for (int task = tid; task < TCX * TY; task += NT)
+        {
+          const int cxl = task % TCX, ty = task / TCX;
+          const int gy  = K * cy0 + ty;
+          T         c1[W], c2[W];
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            {
+              c1[j] = sC1[ty * WX + K * cxl + j];
+              c2[j] = sC2[ty * WX + K * cxl + j];
+            }
+#pragma unroll
+          for (int q = 0; q < K; ++q)
+            {
+              const int gx = K * (cx0 + cxl) + q;
+              if (gx >= 1 && gx < N1 - 1 && gy >= 1 && gy < N1 - 1)
+                {
+                  T v = 0;
+#pragma unroll
+                  for (int o = band_lo<K>(q); o <= band_hi<K>(q); ++o)
+                    {
+                      v += bK[q * W + o + K] * c1[K + q + o];
+                      v += bM[q * W + o + K] * c2[K + q + o];
+                    }
+                  v *= scale;
+                  const long long g  = gx + (long long)gy * N1;
+                  const T         yv = RESID ? bv[g] - v : v;
+                  y[g]               = yv;
+                  if (DOT)
+                    dot += (double)x[g] * (double)yv;
+                }
+            }
+        }
+      if (DOT)
+        {
+          const double s = block_sum<NT>(dot, sred);
+          if (tid == 0)
+            partial[linear_block_id()] = s;
+        }
+    }
+
+    template <int K, typename T>
+    int
+    launch_k(const gtps_level *lvl, void *y, const void *x, const void *b,
+             bool fused_dot)
+    {
+      const T   *tab   = reinterpret_cast<const T *>(lvl->dtab());
+      const T    scale = (T)lvl->scale;
+      const int  n     = lvl->n;
+      double    *part  = lvl->ctx->ws.partials;
+      hipStream_t s    = lvl->ctx->stream;
+      const T   *xp    = reinterpret_cast<const T *>(x);
+      T         *yp    = reinterpret_cast<T *>(y);
+      const T   *bp    = reinterpret_cast<const T *>(b);
+      if (lvl->dim == 3)
+        {
+          using C       = VT3<K>;
+          const int tx  = (n + C::TCX - 1) / C::TCX;
+          const int ty  = (n + C::TCY - 1) / C::TCY;
+          // z-chunks: aim for >= 1024 work-groups on the 256 CUs
+          int zc = (int)(((long long)n * tx * ty + 1023) / 1024);
+          zc     = std::max(1, std::min(zc, n));
+          const int tz = (n + zc - 1) / zc;
+          const dim3 grid(tx, ty, tz);
+          const int  nb = tx * ty * tz;
+          if (fused_dot)
+            GTPS_REQUIRE(nb <= lvl->ctx->ws.capacity, "too many blocks for partials");
+#define GTPS_L3(R, D)                                                         \
+  hipLaunchKernelGGL((laplace3d_kernel<K, T, R, D>), grid, dim3(C::NT), 0, s, \
+                     xp, yp, bp, tab, n, zc, scale, part)
+          if (b)
+            {
+              if (fused_dot)
+                GTPS_L3(true, true);
+              else
+                GTPS_L3(true, false);
+            }
+          else
+            {
+              if (fused_dot)
+                GTPS_L3(false, true);
+              else
+                GTPS_L3(false, false);
+            }
+#undef GTPS_L3
+          GTPS_LAUNCH_CHECK();
+          return nb;
+        }
+      using C      = VT2<K>;
+      const dim3 grid((n + C::TCX - 1) / C::TCX, (n + C::TCY - 1) / C::TCY);
+      const int  nb = grid.x * grid.y;
+      if (fused_dot)
+        GTPS_REQUIRE(nb <= lvl->ctx->ws.capacity, "too many blocks for partials");
+#define GTPS_L2(R, D)                                                         \
+  hipLaunchKernelGGL((laplace2d_kernel<K, T, R, D>), grid, dim3(C::NT), 0, s, \
+                     xp, yp, bp, tab, n, scale, part)
+      if (b)
+        {
+          if (fused_dot)
+            GTPS_L2(true, true);
+          else
+            GTPS_L2(true, false);
+        }
+      else
+        {
+          if (fused_dot)
+            GTPS_L2(false, true);
+          else
+            GTPS_L2(false, false);
+        }
+#undef GTPS_L2
+      GTPS_LAUNCH_CHECK();
+      return nb;
+    }
+
+    template <typename T>
+    int
+    launch_t(const gtps_level *lvl, void *y, const void *x, const void *b,
+             bool fused_dot)
+    {
+      switch (lvl->k)
+        {
+          case 1: return launch_k<1, T>(lvl, y, x, b, fused_dot);
+          case 2: return launch_k<2, T>(lvl, y, x, b, fused_dot);
+          case 3: return launch_k<3, T>(lvl, y, x, b, fused_dot);
+          case 4: return launch_k<4, T>(lvl, y, x, b, fused_dot);
+          case 5: return launch_k<5, T>(lvl, y, x, b, fused_dot);
+          case 6: return launch_k<6, T>(lvl, y, x, b, fused_dot);
+          case 7: return launch_k<7, T>(lvl, y, x, b, fused_dot);
+          default: throw Error(GTPS_EINVAL, "degree must be in 1..7");
+        }
+    }
+  } // namespace
+
+  int
+  launch_laplace(const gtps_level *lvl, void *y, const void *x, const void *b,
+                 bool fused_dot)
+  {
+    if (lvl->prec == GTPS_FP32)
+      return launch_t<float>(lvl, y, x, b, fused_dot);
+    return launch_t<double>(lvl, y, x, b, fused_dot);
+  }
+} // namespace gtps

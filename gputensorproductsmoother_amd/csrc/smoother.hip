@@ -1,0 +1,655 @@
+// smoother.hip - vertex-patch Schwarz smoother, one colour per launch.
+//
+// A vertex patch is the 2^d cells around an interior mesh vertex; its DoF
+// box has (2k+1)^d nodes and (2k-1)^d interior nodes.  Patches are coloured
+// by the parity of their lower-left cell index in each direction, so the
+// interiors of one colour are disjoint and no patch of a colour reads
+// another's interior (patch_base.template.cuh:1208-1285).
+//
+// Multiplicative pass (reference FUSED_CF, loop_kernel.cuh:969-1060 with
+// evaluate_kernel.cuh:2058-2163): u_I <- w (u_I + A_II^{-1} (b - A_P u)_I).
+// This kernel evaluates the algebraically identical form
+// u_I <- w A_II^{-1} (b_I - A_IB u_B) of the reference's FUSED_BD kernel
+// (loop_kernel.cuh:674-783): it reads u only on the patch skeleton and b
+// only on the interior, and writes the interior.  A_P is the Kronecker sum
+// of the 2-cell patch 1D matrices (block-sparse rows), evaluated by
+// sum factorisation z -> y -> x with only interior output rows kept, and
+//   A_II^{-1} = (S (x) S (x) S) diag(1 / (h^(d-2) (l_i + l_j + l_k))) (S (x) S (x) S)^T
+// with M-orthonormal eigenvectors S of the interior pencil (fast
+// diagonalisation, evaluate_kernel.cuh:1327-1434).
+//
+// Additive pass: u_I += w A_II^{-1} r_I for a given global residual r.
+//
+// Layout: NPB same-colour patches per work-group, stages separated by
+// barriers, every stage a loop of independent 1D line tasks over LDS.
+#include "gtps_internal.h"
+
+namespace gtps
+{
+  namespace
+  {
+    constexpr int SM_NT = 256;
+
+    constexpr int
+    cmax(int a, int b)
+    {
+      return a > b ? a : b;
+    }
+    constexpr int
+    cmin(int a, int b)
+    {
+      return a < b ? a : b;
+    }
+
+    template <int K, typename T>
+    struct SM3
+    {
+      static constexpr int NP  = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr int R1  = cmax(NP * NP * NP, 2 * NI * NI * NP);
+      static constexpr int R2  = cmax(2 * NI * NP * NP, NI * NI * NI);
+      static constexpr int SB  = NI * NI * NI;
+      static constexpr int PER = R1 + R2 + SB;
+      static constexpr int NPB_T = cmax(1, cmin(64, SM_NT / (NI * NI)));
+      // keep the dynamic LDS at or below 64 KiB when more than one patch
+      static constexpr int NPB_L = cmax(1, (int)(65536 / (PER * sizeof(T))));
+      static constexpr int NPB   = cmin(NPB_T, NPB_L);
+      static constexpr size_t LDS = (size_t)NPB * PER * sizeof(T);
+    };
+
+    template <int K, typename T>
+    struct SM2
+    {
+      static constexpr int NP  = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr int R1  = NP * NP;
+      static constexpr int R2  = cmax(2 * NI * NP, NI * NI);
+      static constexpr int SB  = NI * NI;
+      static constexpr int PER = R1 + R2 + SB;
+      static constexpr int NPB_T = cmax(1, cmin(64, SM_NT / NP));
+      static constexpr int NPB_L = cmax(1, (int)(65536 / (PER * sizeof(T))));
+      static constexpr int NPB   = cmin(NPB_T, NPB_L);
+      static constexpr size_t LDS = (size_t)NPB * PER * sizeof(T);
+    };
+
+    // patches of colour c along one direction: lower-left cell indices
+    // 2i + parity in [0, n-2]
+    __device__ __forceinline__ int
+    n_patches_1d(int n, int parity)
+    {
+      return (n - parity) / 2;
+    }
+
+    template <int K, typename T, bool ADD>
+    __global__ void __launch_bounds__(SM_NT)
+    patch3d_kernel(T *__restrict__ u, const T *__restrict__ src,
+                   const T *__restrict__ tab, const int n, const int colour,
+                   const T scale, const T relax)
+    {
+      using TB             = Tab<K>;
+      using C              = SM3<K, T>;
+      constexpr int NP     = C::NP, NI = C::NI, NPB = C::NPB;
+      constexpr int R1     = C::R1, R2 = C::R2, SB = C::SB, NT = SM_NT;
+      extern __shared__ double smem_raw[];
+      T *sR1 = reinterpret_cast<T *>(smem_raw);
+      T *sR2 = sR1 + NPB * R1;
+      T *sSB = sR2 + NPB * R2;
+      __shared__ long long sfirst[NPB];
+
+      const int       N1  = K * n + 1;
+      const long long P   = (long long)N1 * N1;
+      const int       tid = threadIdx.x;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
+                npz    = n_patches_1d(n, pz);
+      const int npatch = npx * npy * npz;
+      if (tid < NPB)
+        {
+          const int q = blockIdx.x * NPB + tid;
+          if (q < npatch)
+            {
+              const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+              sfirst[tid]  = (long long)K * (2 * qx + px) +
+                            (long long)K * (2 * qy + py) * N1 +
+                            (long long)K * (2 * qz + pz) * P;
+            }
+          else
+            sfirst[tid] = -1;
+        }
+      __syncthreads();
+      const T *pM = tab + TB::patchM;
+      const T *pK = tab + TB::patchK;
+      const T *S  = tab + TB::S;
+      const T *lam = tab + TB::lam;
+
+      if (!ADD)
+        {
+          // stage 0: skeleton of u (interior zeroed) and interior of b
+          for (int t = tid; t < NPB * NP * NP * NP; t += NT)
+            {
+              const int       lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
+              const long long f  = sfirst[lp];
+              if (f < 0)
+                continue;
+              const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
+              const long long g = f + ix + (long long)iy * N1 + iz * P;
+              const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
+                                 iz > 0 && iz < NP - 1;
+              if (inner)
+                {
+                  sR1[lp * R1 + r] = T(0);
+                  sSB[lp * SB + ((iz - 1) * NI + iy - 1) * NI + ix - 1] = src[g];
+                }
+              else
+                sR1[lp * R1 + r] = u[g];
+            }
+          __syncthreads();
+          // stage 1: z-rows (interior rows only) of M and K
+          for (int t = tid; t < NPB * NP * NP; t += NT)
+            {
+              const int lp = t / (NP * NP), r = t - lp * (NP * NP);
+              T         v[NP];
+#pragma unroll
+              for (int iz = 0; iz < NP; ++iz)
+                v[iz] = sR1[lp * R1 + iz * NP * NP + r];
+#pragma unroll
+              for (int zi = 0; zi < NI; ++zi)
+                {
+                  const int row = zi + 1;
+                  T         m = 0, kk = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      m += pM[row * NP + c] * v[c];
+                      kk += pK[row * NP + c] * v[c];
+                    }
+                  sR2[lp * R2 + zi * NP * NP + r]           = m;
+                  sR2[lp * R2 + (NI + zi) * NP * NP + r]    = kk;
+                }
+            }
+          __syncthreads();
+          // stage 2: y-rows; C1 = My Mz u, C2 = Ky Mz u + My Kz u
+          for (int t = tid; t < NPB * NI * NP; t += NT)
+            {
+              const int lp = t / (NI * NP), r = t - lp * (NI * NP);
+              const int zi = r / NP, ix = r - zi * NP;
+              T         a[NP], b[NP];
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                {
+                  a[iy] = sR2[lp * R2 + (zi * NP + iy) * NP + ix];
+                  b[iy] = sR2[lp * R2 + ((NI + zi) * NP + iy) * NP + ix];
+                }
+#pragma unroll
+              for (int yi = 0; yi < NI; ++yi)
+                {
+                  const int row = yi + 1;
+                  T         c1 = 0, c2 = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      c1 += pM[row * NP + c] * a[c];
+                      c2 += pK[row * NP + c] * a[c];
+                      c2 += pM[row * NP + c] * b[c];
+                    }
+                  sR1[lp * R1 + (zi * NI + yi) * NP + ix]            = c1;
+                  sR1[lp * R1 + NI * NI * NP + (zi * NI + yi) * NP + ix] = c2;
+                }
+            }
+          __syncthreads();
+          // stage 3: x-rows, r_I = b_I - h (Kx C1 + Mx C2), then S^T along x
+          for (int t = tid; t < NPB * NI * NI; t += NT)
+            {
+              const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+              T         c1[NP], c2[NP], res[NI];
+#pragma unroll
+              for (int ix = 0; ix < NP; ++ix)
+                {
+                  c1[ix] = sR1[lp * R1 + r * NP + ix];
+                  c2[ix] = sR1[lp * R1 + NI * NI * NP + r * NP + ix];
+                }
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                {
+                  const int row = xi + 1;
+                  T         v   = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      v += pK[row * NP + c] * c1[c];
+                      v += pM[row * NP + c] * c2[c];
+                    }
+                  res[xi] = sSB[lp * SB + r * NI + xi] - scale * v;
+                }
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                {
+                  T s = 0;
+#pragma unroll
+                  for (int i = 0; i < NI; ++i)
+                    s += S[i * NI + j] * res[i];
+                  sR2[lp * R2 + r * NI + j] = s;
+                }
+            }
+        }
+      else
+        {
+          // additive: interior residual, S^T along x
+          for (int t = tid; t < NPB * NI * NI; t += NT)
+            {
+              const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
+              const long long f  = sfirst[lp];
+              if (f < 0)
+                continue;
+              const int       zi = r / NI, yi = r - zi * NI;
+              const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+              T               res[NI];
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                res[xi] = src[g0 + xi];
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                {
+                  T s = 0;
+#pragma unroll
+                  for (int i = 0; i < NI; ++i)
+                    s += S[i * NI + j] * res[i];
+                  sR2[lp * R2 + r * NI + j] = s;
+                }
+            }
+        }
+      __syncthreads();
+      // stage 4: S^T along y
+      for (int t = tid; t < NPB * NI * NI; t += NT)
+        {
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sR2 + lp * R2 + zi * NI * NI + xi;
+          T         v[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * v[i];
+              base[j * NI] = s;
+            }
+        }
+      __syncthreads();
+      // stage 5: S^T along z, divide by the eigenvalue sums, S along z
+      for (int t = tid; t < NPB * NI * NI; t += NT)
+        {
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          const int yi = r / NI, xi = r - yi * NI;
+          T        *base = sR2 + lp * R2 + yi * NI + xi;
+          const T   lxy  = lam[xi] + lam[yi];
+          T         v[NI], tt[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * v[i];
+              tt[j] = s / (scale * (lxy + lam[j]));
+            }
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * tt[j];
+              base[i * NI * NI] = s;
+            }
+        }
+      __syncthreads();
+      // stage 6: S along y
+      for (int t = tid; t < NPB * NI * NI; t += NT)
+        {
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sR2 + lp * R2 + zi * NI * NI + xi;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = base[j * NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              base[i * NI] = s;
+            }
+        }
+      __syncthreads();
+      // stage 7: S along x and write the interior
+      for (int t = tid; t < NPB * NI * NI; t += NT)
+        {
+          const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
+          const long long f  = sfirst[lp];
+          if (f < 0)
+            continue;
+          const int zi = r / NI, yi = r - zi * NI;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = sR2[lp * R2 + r * NI + j];
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              if (ADD)
+                u[g0 + i] += relax * s;
+              else
+                u[g0 + i] = relax * s;
+            }
+        }
+    }
+
+    template <int K, typename T, bool ADD>
+    __global__ void __launch_bounds__(SM_NT)
+    patch2d_kernel(T *__restrict__ u, const T *__restrict__ src,
+                   const T *__restrict__ tab, const int n, const int colour,
+                   const T scale, const T relax)
+    {
+      using TB         = Tab<K>;
+      using C          = SM2<K, T>;
+      constexpr int NP = C::NP, NI = C::NI, NPB = C::NPB;
+      constexpr int R1 = C::R1, R2 = C::R2, SB = C::SB, NT = SM_NT;
+      extern __shared__ double smem_raw[];
+      T *sR1 = reinterpret_cast<T *>(smem_raw);
+      T *sR2 = sR1 + NPB * R1;
+      T *sSB = sR2 + NPB * R2;
+      __shared__ long long sfirst[NPB];
+      const int N1  = K * n + 1;
+      const int tid = threadIdx.x;
+      const int px = colour & 1, py = (colour >> 1) & 1;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py);
+      const int npatch = npx * npy;
+      if (tid < NPB)
+        {
+          const int q = blockIdx.x * NPB + tid;
+          if (q < npatch)
+            {
+              const int qx = q % npx, qy = q / npx;
+              sfirst[tid]  = (long long)K * (2 * qx + px) +
+                            (long long)K * (2 * qy + py) * N1;
+            }
+          else
+            sfirst[tid] = -1;
+        }
+      __syncthreads();
+      const T *pM  = tab + TB::patchM;
+      const T *pK  = tab + TB::patchK;
+      const T *S   = tab + TB::S;
+      const T *lam = tab + TB::lam;
+      if (!ADD)
+        {
+          for (int t = tid; t < NPB * NP * NP; t += NT)
+            {
+              const int       lp = t / (NP * NP), r = t - lp * (NP * NP);
+              const long long f  = sfirst[lp];
+              if (f < 0)
+                continue;
+              const int ix = r % NP, iy = r / NP;
+              const long long g = f + ix + (long long)iy * N1;
+              const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1;
+              if (inner)
+                {
+                  sR1[lp * R1 + r] = T(0);
+                  sSB[lp * SB + (iy - 1) * NI + ix - 1] = src[g];
+                }
+              else
+                sR1[lp * R1 + r] = u[g];
+            }
+          __syncthreads();
+          // y-rows of M (for Kx My) and K (for Mx Ky)
+          for (int t = tid; t < NPB * NP; t += NT)
+            {
+              const int lp = t / NP, ix = t - lp * NP;
+              T         v[NP];
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                v[iy] = sR1[lp * R1 + iy * NP + ix];
+#pragma unroll
+              for (int yi = 0; yi < NI; ++yi)
+                {
+                  const int row = yi + 1;
+                  T         m = 0, kk = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      m += pM[row * NP + c] * v[c];
+                      kk += pK[row * NP + c] * v[c];
+                    }
+                  sR2[lp * R2 + yi * NP + ix]        = m;
+                  sR2[lp * R2 + (NI + yi) * NP + ix] = kk;
+                }
+            }
+          __syncthreads();
+          for (int t = tid; t < NPB * NI; t += NT)
+            {
+              const int lp = t / NI, yi = t - lp * NI;
+              T         c1[NP], c2[NP], res[NI];
+#pragma unroll
+              for (int ix = 0; ix < NP; ++ix)
+                {
+                  c1[ix] = sR2[lp * R2 + yi * NP + ix];
+                  c2[ix] = sR2[lp * R2 + (NI + yi) * NP + ix];
+                }
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                {
+                  const int row = xi + 1;
+                  T         v   = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      v += pK[row * NP + c] * c1[c];
+                      v += pM[row * NP + c] * c2[c];
+                    }
+                  res[xi] = sSB[lp * SB + yi * NI + xi] - scale * v;
+                }
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                {
+                  T s = 0;
+#pragma unroll
+                  for (int i = 0; i < NI; ++i)
+                    s += S[i * NI + j] * res[i];
+                  sR1[lp * R1 + yi * NI + j] = s;
+                }
+            }
+        }
+      else
+        {
+          for (int t = tid; t < NPB * NI; t += NT)
+            {
+              const int       lp = t / NI, yi = t - lp * NI;
+              const long long f  = sfirst[lp];
+              if (f < 0)
+                continue;
+              const long long g0 = f + 1 + (long long)(yi + 1) * N1;
+              T               res[NI];
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                res[xi] = src[g0 + xi];
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                {
+                  T s = 0;
+#pragma unroll
+                  for (int i = 0; i < NI; ++i)
+                    s += S[i * NI + j] * res[i];
+                  sR1[lp * R1 + yi * NI + j] = s;
+                }
+            }
+        }
+      __syncthreads();
+      // y: S^T, divide, S
+      for (int t = tid; t < NPB * NI; t += NT)
+        {
+          const int lp = t / NI, xi = t - lp * NI;
+          T        *base = sR1 + lp * R1 + xi;
+          T         v[NI], tt[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * v[i];
+              tt[j] = s / (scale * (lam[xi] + lam[j]));
+            }
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * tt[j];
+              base[i * NI] = s;
+            }
+        }
+      __syncthreads();
+      for (int t = tid; t < NPB * NI; t += NT)
+        {
+          const int       lp = t / NI, yi = t - lp * NI;
+          const long long f  = sfirst[lp];
+          if (f < 0)
+            continue;
+          T v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = sR1[lp * R1 + yi * NI + j];
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1;
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              if (ADD)
+                u[g0 + i] += relax * s;
+              else
+                u[g0 + i] = relax * s;
+            }
+        }
+    }
+
+    template <typename KernelPtr>
+    void
+    ensure_lds(KernelPtr kern, size_t bytes)
+    {
+      if (bytes > 65536)
+        GTPS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)bytes));
+    }
+
+    template <int K, typename T>
+    void
+    launch_k(const gtps_level *lvl, void *u, const void *src, int colour,
+             int mode, double relax)
+    {
+      const T    *tab = reinterpret_cast<const T *>(lvl->dtab());
+      const int   n   = lvl->n;
+      hipStream_t s   = lvl->ctx->stream;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      long long np = (long long)((n - px) / 2) * ((n - py) / 2);
+      if (lvl->dim == 3)
+        np *= (n - pz) / 2;
+      if (np <= 0)
+        return;
+      T       *up = reinterpret_cast<T *>(u);
+      const T *sp = reinterpret_cast<const T *>(src);
+      if (lvl->dim == 3)
+        {
+          using C          = SM3<K, T>;
+          const long long nb = (np + C::NPB - 1) / C::NPB;
+          if (mode == 0)
+            {
+              auto kern = patch3d_kernel<K, T, false>;
+              ensure_lds(kern, C::LDS);
+              hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(SM_NT), C::LDS,
+                                 s, up, sp, tab, n, colour, (T)lvl->scale,
+                                 (T)relax);
+            }
+          else
+            {
+              auto kern = patch3d_kernel<K, T, true>;
+              ensure_lds(kern, C::LDS);
+              hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(SM_NT), C::LDS,
+                                 s, up, sp, tab, n, colour, (T)lvl->scale,
+                                 (T)relax);
+            }
+        }
+      else
+        {
+          using C          = SM2<K, T>;
+          const long long nb = (np + C::NPB - 1) / C::NPB;
+          if (mode == 0)
+            {
+              auto kern = patch2d_kernel<K, T, false>;
+              ensure_lds(kern, C::LDS);
+              hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(SM_NT), C::LDS,
+                                 s, up, sp, tab, n, colour, (T)lvl->scale,
+                                 (T)relax);
+            }
+          else
+            {
+              auto kern = patch2d_kernel<K, T, true>;
+              ensure_lds(kern, C::LDS);
+              hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(SM_NT), C::LDS,
+                                 s, up, sp, tab, n, colour, (T)lvl->scale,
+                                 (T)relax);
+            }
+        }
+      GTPS_LAUNCH_CHECK();
+    }
+
+    template <typename T>
+    void
+    launch_t(const gtps_level *lvl, void *u, const void *src, int colour,
+             int mode, double relax)
+    {
+      switch (lvl->k)
+        {
+          case 1: return launch_k<1, T>(lvl, u, src, colour, mode, relax);
+          case 2: return launch_k<2, T>(lvl, u, src, colour, mode, relax);
+          case 3: return launch_k<3, T>(lvl, u, src, colour, mode, relax);
+          case 4: return launch_k<4, T>(lvl, u, src, colour, mode, relax);
+          case 5: return launch_k<5, T>(lvl, u, src, colour, mode, relax);
+          case 6: return launch_k<6, T>(lvl, u, src, colour, mode, relax);
+          case 7: return launch_k<7, T>(lvl, u, src, colour, mode, relax);
+          default: throw Error(GTPS_EINVAL, "degree must be in 1..7");
+        }
+    }
+  } // namespace
+
+  void
+  launch_patch_colour(const gtps_level *lvl, void *u, const void *src,
+                      int colour, int mode, double relax)
+  {
+    if (lvl->prec == GTPS_FP32)
+      launch_t<float>(lvl, u, src, colour, mode, relax);
+    else
+      launch_t<double>(lvl, u, src, colour, mode, relax);
+  }
+} // namespace gtps

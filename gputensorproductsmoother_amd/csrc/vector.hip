@@ -1,0 +1,304 @@
+// vector.hip - FP64/FP32 vector kernels and the fused CG building blocks.
+//
+// Reductions are two-stage and deterministic: a kernel writes one partial
+// sum per work-group (wavefront shuffles, then LDS across the waves), and a
+// single work-group adds the partials in a fixed order into a device scalar
+// slot.  CG scalars (alpha, beta) are formed on the device, so an iteration
+// needs no host round trip except the convergence check.
+//
+// Reference counterparts: deal.II Vector<double,CUDA> sadd/add/l2_norm/dot
+// used by the solver (include/solver.cuh:1712,1830,1853,1977, SolverGMRES
+// internals), vec_equ/plain_copy (include/cuda_vector.cuh:79-118) and
+// copy_vector<is_d2f> (include/solver.cuh:64-78).
+#include "gtps_internal.h"
+
+namespace gtps
+{
+  namespace
+  {
+    constexpr int VNT = 256;
+
+    int
+    grid_for(long long n, int cap = 8192)
+    {
+      long long nb = (n + VNT - 1) / VNT;
+      return (int)std::max<long long>(1, std::min<long long>(nb, cap));
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(VNT)
+    fill_kernel(long long n, T *__restrict__ x, T v)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        x[i] = v;
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    axpby_kernel(long long n, double a, const double *__restrict__ x, double b,
+                 double *__restrict__ y)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        y[i] = a * x[i] + b * y[i];
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    d2f_kernel(long long n, const double *__restrict__ x, float *__restrict__ y)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        y[i] = (float)x[i];
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    f2d_kernel(long long n, const float *__restrict__ x, double *__restrict__ y)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        y[i] = (double)x[i];
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    dot_kernel(long long n, const double *__restrict__ x,
+               const double *__restrict__ y, const float *__restrict__ y32,
+               double *__restrict__ partials)
+    {
+      __shared__ double red[VNT / 64];
+      double            s = 0.0;
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        s += x[i] * (y32 ? (double)y32[i] : y[i]);
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    finalize_kernel(const double *__restrict__ partials, int nb,
+                    double *__restrict__ scalars, int slot)
+    {
+      __shared__ double red[VNT / 64];
+      double            s = 0.0;
+      for (int i = threadIdx.x; i < nb; i += VNT)
+        s += partials[i];
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x == 0)
+        scalars[slot] = s;
+    }
+
+    __global__ void
+    cg_scalar_kernel(double *sc, int op)
+    {
+      if (threadIdx.x != 0)
+        return;
+      switch (op)
+        {
+          case cg::OP_ALPHA:
+            sc[cg::ALPHA] = sc[cg::RZ] / sc[cg::PQ];
+            break;
+          case cg::OP_BETA:
+            sc[cg::BETA] = sc[cg::RZ] / sc[cg::RZ_OLD];
+            break;
+          case cg::OP_BETA0:
+            sc[cg::BETA] = 0.0;
+            break;
+          case cg::OP_SHIFT:
+            sc[cg::RZ_OLD] = sc[cg::RZ];
+            break;
+          default:
+            break;
+        }
+    }
+
+    // x += alpha p; r -= alpha q; partial ||r||^2; r32 = (float) r
+    __global__ void __launch_bounds__(VNT)
+    cg_update_xr_kernel(long long n, double *__restrict__ x,
+                        double *__restrict__ r, const double *__restrict__ p,
+                        const double *__restrict__ q, float *__restrict__ r32,
+                        const double *__restrict__ sc,
+                        double *__restrict__ partials)
+    {
+      __shared__ double red[VNT / 64];
+      const double      alpha = sc[cg::ALPHA];
+      double            s     = 0.0;
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          x[i] += alpha * p[i];
+          const double ri = r[i] - alpha * q[i];
+          r[i]            = ri;
+          if (r32)
+            r32[i] = (float)ri;
+          s += ri * ri;
+        }
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+    }
+
+    // p = z + beta p, z given in FP64 or FP32
+    __global__ void __launch_bounds__(VNT)
+    cg_update_p_kernel(long long n, double *__restrict__ p,
+                       const double *__restrict__ z,
+                       const float *__restrict__ z32,
+                       const double *__restrict__ sc)
+    {
+      const double beta = sc[cg::BETA];
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          const double zi = z32 ? (double)z32[i] : z[i];
+          p[i]            = zi + beta * p[i];
+        }
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(VNT)
+    set_boundary_kernel(int dim, int N1, long long n, T *__restrict__ v, T val)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          const int  x  = (int)(i % N1);
+          const int  y  = (int)((i / N1) % N1);
+          const int  z  = (int)(i / ((long long)N1 * N1));
+          const bool bd = x == 0 || x == N1 - 1 || y == 0 || y == N1 - 1 ||
+                          (dim == 3 && (z == 0 || z == N1 - 1));
+          if (bd)
+            v[i] = val;
+        }
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    lift_boundary_kernel(int dim, int N1, long long n,
+                         const double *__restrict__ b, double *__restrict__ x)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          const int  xx = (int)(i % N1);
+          const int  y  = (int)((i / N1) % N1);
+          const int  z  = (int)(i / ((long long)N1 * N1));
+          const bool bd = xx == 0 || xx == N1 - 1 || y == 0 || y == N1 - 1 ||
+                          (dim == 3 && (z == 0 || z == N1 - 1));
+          x[i] = bd ? b[i] : 0.0;
+        }
+    }
+  } // namespace
+
+  void
+  launch_fill(hipStream_t s, long long n, int prec, void *x, double v)
+  {
+    if (n <= 0)
+      return;
+    if (prec == GTPS_FP32)
+      hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(VNT), 0,
+                         s, n, (float *)x, (float)v);
+    else
+      hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n)), dim3(VNT), 0,
+                         s, n, (double *)x, v);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_axpby(hipStream_t s, long long n, double a, const double *x, double b,
+               double *y)
+  {
+    if (n <= 0)
+      return;
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(VNT), 0, s, n, a,
+                       x, b, y);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_convert_f64_f32(hipStream_t s, long long n, const double *x, float *y)
+  {
+    if (n <= 0)
+      return;
+    hipLaunchKernelGGL(d2f_kernel, dim3(grid_for(n)), dim3(VNT), 0, s, n, x, y);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_convert_f32_f64(hipStream_t s, long long n, const float *x, double *y)
+  {
+    if (n <= 0)
+      return;
+    hipLaunchKernelGGL(f2d_kernel, dim3(grid_for(n)), dim3(VNT), 0, s, n, x, y);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  int
+  launch_dot_partials(hipStream_t s, long long n, const double *x,
+                      const double *y, const float *y32, double *partials,
+                      int capacity)
+  {
+    const int nb = grid_for(n, std::min(capacity, 2048));
+    hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(VNT), 0, s, n, x, y, y32,
+                       partials);
+    GTPS_LAUNCH_CHECK();
+    return nb;
+  }
+
+  void
+  launch_finalize_sum(hipStream_t s, const double *partials, int nb,
+                      double *scalars, int slot)
+  {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(VNT), 0, s, partials, nb,
+                       scalars, slot);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_cg_scalar(hipStream_t s, double *scalars, int op)
+  {
+    hipLaunchKernelGGL(cg_scalar_kernel, dim3(1), dim3(64), 0, s, scalars, op);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  int
+  launch_cg_update_xr(hipStream_t s, long long n, double *x, double *r,
+                      const double *p, const double *q, float *r32,
+                      const double *scalars, double *partials, int capacity)
+  {
+    const int nb = grid_for(n, std::min(capacity, 2048));
+    hipLaunchKernelGGL(cg_update_xr_kernel, dim3(nb), dim3(VNT), 0, s, n, x, r,
+                       p, q, r32, scalars, partials);
+    GTPS_LAUNCH_CHECK();
+    return nb;
+  }
+
+  void
+  launch_cg_update_p(hipStream_t s, long long n, double *p, const double *z,
+                     const float *z32, const double *scalars)
+  {
+    hipLaunchKernelGGL(cg_update_p_kernel, dim3(grid_for(n)), dim3(VNT), 0, s,
+                       n, p, z, z32, scalars);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_set_boundary(const gtps_level *lvl, void *v, double value)
+  {
+    hipStream_t s = lvl->ctx->stream;
+    if (lvl->prec == GTPS_FP32)
+      hipLaunchKernelGGL(set_boundary_kernel<float>, dim3(grid_for(lvl->N)),
+                         dim3(VNT), 0, s, lvl->dim, lvl->N1, lvl->N, (float *)v,
+                         (float)value);
+    else
+      hipLaunchKernelGGL(set_boundary_kernel<double>, dim3(grid_for(lvl->N)),
+                         dim3(VNT), 0, s, lvl->dim, lvl->N1, lvl->N,
+                         (double *)v, value);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_lift_boundary(const gtps_level *lvl, const double *b, double *x)
+  {
+    hipLaunchKernelGGL(lift_boundary_kernel, dim3(grid_for(lvl->N)), dim3(VNT),
+                       0, lvl->ctx->stream, lvl->dim, lvl->N1, lvl->N, b, x);
+    GTPS_LAUNCH_CHECK();
+  }
+} // namespace gtps

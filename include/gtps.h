@@ -1,0 +1,186 @@
+/*
+ * gtps.h - C ABI of the MI355X-native geometric-multigrid hot path
+ * (vertex-patch Schwarz smoother with fast-diagonalisation local solves,
+ * matrix-free sum-factorised Laplace, level transfer, MG-preconditioned CG).
+ *
+ * Mesh and data layout (the contract every entry point shares):
+ *   - unit square (dim 2) or unit cube (dim 3), uniform Cartesian mesh with
+ *     2^level cells per direction, continuous Q_k Lagrange elements on
+ *     Gauss-Lobatto nodes;
+ *   - a level vector holds N = (k 2^level + 1)^dim values in lexicographic
+ *     order (x fastest), boundary DoFs included; it is a plain device array
+ *     owned by the caller (float for GTPS_FP32 objects, double for GTPS_FP64);
+ *   - Dirichlet DoFs have identity rows: vmult copies them (y_B = x_B).
+ *
+ * Conventions: every call returns 0 on success or a GTPS_E* code, never
+ * throws, and records a message readable with gtps_last_error() (per thread).
+ * Work is enqueued asynchronously on the context's HIP stream; the caller
+ * synchronises (gtps_ctx_synchronize) before reading results on the host.
+ * Handles are not thread-safe.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository).
+ */
+#ifndef GTPS_H
+#define GTPS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTPS_OK 0
+#define GTPS_EINVAL 1   /* bad argument (dimension, degree, level, size) */
+#define GTPS_EHIP 2     /* HIP runtime error */
+#define GTPS_ENOMEM 3   /* device allocation failed */
+#define GTPS_ENOTCONV 4 /* solver reached maxit without convergence */
+#define GTPS_ENAN 5     /* non-finite residual detected */
+
+#define GTPS_FP32 0
+#define GTPS_FP64 1
+
+/* smoother variants */
+#define GTPS_MULT_FWD 0  /* multiplicative, colours 0..2^dim-1 */
+#define GTPS_MULT_REV 1  /* multiplicative, colours reversed */
+#define GTPS_ADDITIVE 2  /* additive (one residual, all patches, damped) */
+
+/* multigrid smoothing schedules */
+#define GTPS_MG_SYMMETRIC 0       /* pre MULT_FWD, post MULT_REV (CG-safe) */
+#define GTPS_MG_REFERENCE_ORDER 1 /* pre and post MULT_FWD (solver.cuh:1969,1997) */
+#define GTPS_MG_ADDITIVE 2        /* pre and post ADDITIVE */
+
+typedef struct gtps_ctx gtps_ctx;
+typedef struct gtps_level gtps_level;
+typedef struct gtps_smoother gtps_smoother;
+typedef struct gtps_transfer gtps_transfer;
+typedef struct gtps_mg gtps_mg;
+
+/* ---------------------------------------------------------------- context */
+/* Device selection and stream; replaces the reference's implicit CUDA
+ * default stream and findCudaDevice (apps/poisson.cu:488-489).
+ * stream == NULL creates a stream owned by the context. */
+int gtps_ctx_create(int device, void *stream, gtps_ctx **out);
+int gtps_ctx_destroy(gtps_ctx *ctx);
+int gtps_ctx_synchronize(gtps_ctx *ctx);
+void *gtps_ctx_stream(gtps_ctx *ctx);
+const char *gtps_last_error(void);
+const char *gtps_version(void);
+
+/* ------------------------------------------------------ level / operator */
+/* One MG level with its matrix-free Laplace operator.
+ * Replaces PSMF::LaplaceOperator<dim,k,Number,DoFLayout::Q>::initialize
+ * (include/laplace_operator.cuh:83-84) together with the level
+ * PSMF::MatrixFree::reinit it wraps (include/dealii/cuda_mf.template.cuh:856). */
+int gtps_level_create(gtps_ctx *ctx, int dim, int degree, int level,
+                      int precision, gtps_level **out);
+int gtps_level_destroy(gtps_level *lvl);
+long long gtps_level_n_dofs(const gtps_level *lvl);
+
+/* dst = A src (Dirichlet rows copied).
+ * Replaces LaplaceOperator::vmult (include/laplace_operator.cuh:86-101). */
+int gtps_laplace_vmult(gtps_level *lvl, void *dst, const void *src);
+
+/* dst = b - A u (the V-cycle's defect update t = d - A u).
+ * Replaces matrix[l].vmult + t.sadd(-1,1,d) (include/solver.cuh:1976-1977). */
+int gtps_laplace_residual(gtps_level *lvl, void *dst, const void *b,
+                          const void *u);
+
+/* Host copy (FP64) of the 1D data the level was set up with: reference-cell
+ * mass/stiffness [(k+1)^2 each], patch mass/stiffness [(2k+1)^2 each],
+ * interior eigenvalues [2k-1] and M-orthonormal eigenvectors [(2k-1)^2,
+ * column j = vector j] computed on the GPU in FP64.  Any pointer may be NULL.
+ * Replaces TensorProductData::get_{mass_matrix,derivative_matrix,
+ * eigenvalues,eigenvectors} (include/tensor_product.h:205-260). */
+int gtps_level_tensor_product_data(gtps_level *lvl, double *cell_mass,
+                                   double *cell_laplace, double *patch_mass,
+                                   double *patch_laplace, double *eigenvalues,
+                                   double *eigenvectors);
+
+/* --------------------------------------------------------------- smoother */
+/* Vertex-patch Schwarz smoother with exact fast-diagonalisation local solves.
+ * Replaces PSMF::PatchSmoother::initialize (include/patch_smoother.cuh:313-335)
+ * with AdditionalData{relaxation} (patch_smoother.cuh:86-124). */
+int gtps_smoother_create(gtps_level *lvl, int variant, double relaxation,
+                         gtps_smoother **out);
+int gtps_smoother_destroy(gtps_smoother *sm);
+/* One sweep in place: u <- S(u, b).  Multiplicative: per colour, per patch,
+ * u_I <- relaxation * (u_I + A_II^{-1} (b - A u)_I).  Additive:
+ * u <- u + relaxation * sum_P R_P^T A_P^{-1} R_P (b - A u).
+ * Replaces PatchSmoother::step (patch_smoother.cuh:92-124 -> step_impl
+ * 187-292). */
+int gtps_smoother_step(gtps_smoother *sm, void *u, const void *b);
+/* u <- 0, then one sweep.  Replaces PatchSmoother::vmult
+ * (patch_smoother.cuh:157-163). */
+int gtps_smoother_vmult(gtps_smoother *sm, void *u, const void *b);
+
+/* --------------------------------------------------------------- transfer */
+/* Level transfer for levels min_level..max_level of one (dim, degree).
+ * Replaces PSMF::MGTransferCUDA<dim,Number,Q> (include/cuda_mg_transfer.cuh:57)
+ * with build() (cuda_mg_transfer.template.cuh:453-559). */
+int gtps_transfer_create(gtps_ctx *ctx, int dim, int degree, int min_level,
+                         int max_level, int precision, gtps_transfer **out);
+int gtps_transfer_destroy(gtps_transfer *tr);
+/* fine(to_level) += P coarse(to_level-1); coarse Dirichlet values ignored.
+ * Replaces MGTransferCUDA::prolongate_and_add (.template.cuh:573-621). */
+int gtps_transfer_prolongate_add(gtps_transfer *tr, int to_level, void *fine,
+                                 const void *coarse);
+/* fine = P coarse.  Replaces MGTransferCUDA::prolongate (.template.cuh:561). */
+int gtps_transfer_prolongate(gtps_transfer *tr, int to_level, void *fine,
+                             const void *coarse);
+/* coarse(from_level-1) += P^T fine(from_level) on non-Dirichlet coarse DoFs.
+ * Replaces MGTransferCUDA::restrict_and_add (.template.cuh:623-671). */
+int gtps_transfer_restrict_add(gtps_transfer *tr, int from_level, void *coarse,
+                               const void *fine);
+
+/* ---------------------------------------------------------------- vectors */
+/* FP64 vector kernels on the context stream (the deal.II
+ * LinearAlgebra::distributed::Vector<double,CUDA> operations the solver uses,
+ * include/solver.cuh:1712,1830,1853,1977, and the precision conversion
+ * copy_vector, include/solver.cuh:64-78, 2070-2095).
+ * Reductions are deterministic (fixed two-stage order) and return their value
+ * to the host (they synchronise the stream). */
+int gtps_vec_fill(gtps_ctx *ctx, long long n, int precision, void *x,
+                  double value);
+int gtps_vec_axpby(gtps_ctx *ctx, long long n, double a, const double *x,
+                   double b, double *y);
+int gtps_vec_dot(gtps_ctx *ctx, long long n, const double *x, const double *y,
+                 double *result);
+int gtps_vec_nrm2(gtps_ctx *ctx, long long n, const double *x, double *result);
+int gtps_vec_convert_f64_f32(gtps_ctx *ctx, long long n, const double *x,
+                             float *y);
+int gtps_vec_convert_f32_f64(gtps_ctx *ctx, long long n, const float *x,
+                             double *y);
+/* v_B = value on the level's Dirichlet DoFs.  Replaces
+ * set_inhomogeneous_dofs / set_constrained_dofs (include/solver.cuh:45-62,
+ * dealii/cuda_mf.template.cuh:514-528). */
+int gtps_vec_set_boundary(gtps_level *lvl, void *v, double value);
+
+/* ---------------------------------------------------------- multigrid+CG */
+/* Multigrid hierarchy on levels 1..max_level (level 1: one vertex patch, the
+ * exact coarse solve).  vcycle_precision: GTPS_FP32 (FP32 smoother, level
+ * operators and transfers under FP64 CG) or GTPS_FP64 (FP64 everywhere).
+ * Replaces the PSMF::MultigridSolvers constructor (include/solver.cuh:1545-1744). */
+int gtps_mg_create(gtps_ctx *ctx, int dim, int degree, int max_level,
+                   int vcycle_precision, int schedule, double relaxation,
+                   gtps_mg **out);
+int gtps_mg_destroy(gtps_mg *mg);
+long long gtps_mg_n_dofs(const gtps_mg *mg);
+/* z = V(r) on the finest level, FP64 in and out (conversion fused).
+ * Replaces MultigridSolvers::vmult (include/solver.cuh:1915-1930). */
+int gtps_mg_vcycle(gtps_mg *mg, double *z, const double *r);
+/* MG-preconditioned FP64 CG for A x = b from x = lift(b) (x_B = b_B,
+ * x_I = 0) until ||r_k|| <= rtol ||r_0|| or maxit iterations.  iters and
+ * relres (optional) receive the iteration count and ||r||/||r_0||;
+ * history (optional, maxit+1 entries) receives ||r_k||.
+ * Replaces MultigridSolvers::solve_gmres (include/solver.cuh:1865-1910): the
+ * outer Krylov method is CG with a symmetric V-cycle instead of GMRES. */
+int gtps_mg_pcg_solve(gtps_mg *mg, double *x, const double *b, double rtol,
+                      int maxit, int *iters, double *relres, double *history);
+/* FP64 finest-level operator of the hierarchy: dst = A src.  Replaces
+ * MultigridSolvers::do_matvec (include/solver.cuh:1934-1938). */
+int gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GTPS_H */

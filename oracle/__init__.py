@@ -1,0 +1,165 @@
+"""ctypes front-end of the FP64 CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this package, always as the checker.  The product
+library never routes through it.  The C++ restatement lives in
+``oracle/gtps_oracle.cpp``; ``build()`` compiles it with the committed
+Makefile.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgtps_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    """Compile the oracle shared library in-tree (make -C oracle)."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "libgtps_oracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        i, d, ll, ull = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_ulonglong
+        L.oracle_basis_1d.argtypes = [i] + [_dp] * 8
+        L.oracle_gen_eig.argtypes = [i, _dp, _dp, _dp, _dp]
+        L.oracle_n_dofs.argtypes = [i, i, i]
+        L.oracle_n_dofs.restype = ll
+        L.oracle_laplace_vmult.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_smoother_step.argtypes = [i, i, i, i, d, _dp, _dp]
+        L.oracle_prolongate_add.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_restrict_add.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_vcycle.argtypes = [i, i, i, i, d, _dp, _dp]
+        L.oracle_pcg.argtypes = [i, i, i, i, d, _dp, _dp, d, i,
+                                 ctypes.POINTER(i), _dp, _dp]
+        L.oracle_rhs_random.argtypes = [i, i, i, ull, _dp]
+        L.oracle_rhs_assembled.argtypes = [i, i, i, i, _dp]
+        L.oracle_l2_error_sine.argtypes = [i, i, i, _dp]
+        L.oracle_l2_error_sine.restype = d
+        L.oracle_set_threads.argtypes = [i]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed (rc={rc})")
+
+
+def n_dofs(dim: int, k: int, level: int) -> int:
+    return int(lib().oracle_n_dofs(dim, k, level))
+
+
+def basis_1d(k: int) -> dict:
+    n, np_, ni = k + 1, 2 * k + 1, 2 * k - 1
+    out = dict(nodes=np.zeros(n), M=np.zeros((n, n)), K=np.zeros((n, n)),
+               E=np.zeros((np_, n)), MP=np.zeros((np_, np_)), KP=np.zeros((np_, np_)),
+               lam=np.zeros(ni), S=np.zeros((ni, ni)))
+    _chk(lib().oracle_basis_1d(k, *[_p(out[key]) for key in
+                                    ("nodes", "M", "K", "E", "MP", "KP", "lam", "S")]),
+         "basis_1d")
+    return out
+
+
+def gen_eig(K: np.ndarray, M: np.ndarray):
+    n = K.shape[0]
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    lam, S = np.zeros(n), np.zeros((n, n))
+    _chk(lib().oracle_gen_eig(n, _p(K), _p(M), _p(lam), _p(S)), "gen_eig")
+    return lam, S
+
+
+def laplace_vmult(dim, k, level, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    _chk(lib().oracle_laplace_vmult(dim, k, level, _p(x), _p(y)), "vmult")
+    return y
+
+
+MULT_FWD, MULT_REV, ADDITIVE = 0, 1, 2
+
+
+def smoother_step(dim, k, level, b, u, variant=MULT_FWD, relax=1.0):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    u = np.array(u, dtype=np.float64, copy=True)
+    _chk(lib().oracle_smoother_step(dim, k, level, variant, relax, _p(b), _p(u)), "smoother")
+    return u
+
+
+def prolongate_add(dim, k, fine_level, coarse, fine):
+    coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+    fine = np.array(fine, dtype=np.float64, copy=True)
+    _chk(lib().oracle_prolongate_add(dim, k, fine_level, _p(coarse), _p(fine)), "prolongate")
+    return fine
+
+
+def restrict_add(dim, k, fine_level, fine, coarse):
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    coarse = np.array(coarse, dtype=np.float64, copy=True)
+    _chk(lib().oracle_restrict_add(dim, k, fine_level, _p(fine), _p(coarse)), "restrict")
+    return coarse
+
+
+MG_SYMMETRIC, MG_REFERENCE_ORDER, MG_ADDITIVE = 0, 1, 2
+
+
+def vcycle(dim, k, level, r, mg_variant=MG_SYMMETRIC, relax=1.0):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    _chk(lib().oracle_vcycle(dim, k, level, mg_variant, relax, _p(r), _p(z)), "vcycle")
+    return z
+
+
+def pcg(dim, k, level, b, rtol=1e-8, maxit=100, mg_variant=MG_SYMMETRIC, relax=1.0):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0.0)
+    hist = np.zeros(maxit + 1)
+    rc = lib().oracle_pcg(dim, k, level, mg_variant, relax, _p(b), _p(x), rtol, maxit,
+                          ctypes.byref(it), ctypes.byref(rel), _p(hist))
+    _chk(rc, "pcg")
+    return x, it.value, rel.value, hist[: it.value + 1]
+
+
+def rhs_random(dim, k, level, seed=20261202):
+    b = np.zeros(n_dofs(dim, k, level))
+    _chk(lib().oracle_rhs_random(dim, k, level, seed, _p(b)), "rhs_random")
+    return b
+
+
+RHS_CONST, RHS_SINE = 0, 1
+
+
+def rhs_assembled(dim, k, level, kind=RHS_CONST):
+    b = np.zeros(n_dofs(dim, k, level))
+    _chk(lib().oracle_rhs_assembled(dim, k, level, kind, _p(b)), "rhs_assembled")
+    return b
+
+
+def l2_error_sine(dim, k, level, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return float(lib().oracle_l2_error_sine(dim, k, level, _p(u)))
+
+
+def set_threads(n: int) -> int:
+    return int(lib().oracle_set_threads(n))

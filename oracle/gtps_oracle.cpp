@@ -1,0 +1,1133 @@
+// gtps_oracle.cpp - FP64 CPU oracle for the geometric-multigrid hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: tests/, the smoke()
+// entry and bench.py's CPU-baseline leg load it through ctypes. The product
+// library (gputensorproductsmoother_amd/csrc) never links or calls it.
+//
+// It restates, in plain FP64 C++ (OpenMP over independent cells/patches),
+// the algorithm of the reference's hot path on a uniform Cartesian mesh of
+// the unit square/cube with Q_k Lagrange elements (Gauss-Lobatto nodes,
+// lexicographic numbering, boundary DoFs stored with identity rows):
+//
+//   * 1D basis: GL nodes, Gauss(k+1) quadrature, cell mass/stiffness
+//     (patch_base.template.cuh:709-760, tensor_product.h:107-201)
+//   * generalized eigenpairs of the 2-cell patch interior pencil
+//     K v = lambda M v (tensor_product.h:205-211 -> deal.II
+//     TensorProductMatrixSymmetricSum), solved here by Cholesky + cyclic
+//     Jacobi in FP64
+//   * Laplace vmult as a cell loop that evaluates gradients at Gauss points,
+//     integrates back and scatters (laplace_operator.cuh:34-101,
+//     dealii/cuda_fee.cuh:273-529), then copies the Dirichlet rows
+//     (dealii/cuda_mf.template.cuh:494-508)
+//   * vertex-patch Schwarz smoother, coloured by patch-index parity
+//     (patch_base.template.cuh:1208-1285), with the full-box local residual
+//     r = b - A_P u and the fast-diagonalisation inverse
+//     S (x) S (x) S diag(1/sum lambda) (S (x) S (x) S)^T written back as
+//     relaxation * (u_I + A_II^{-1} r_I)
+//     (loop_kernel.cuh:969-1060, evaluate_kernel.cuh:2058-2163, 501-590,
+//     1327-1434); additive variant accumulates patch solves of one global
+//     residual colour by colour
+//   * MG transfer as a coarse-cell loop with the 1D embedding matrix and
+//     valence weights (cuda_mg_transfer.template.cuh:48-369, 561-671)
+//   * V(1,1)-cycle (solver.cuh:1952-2001) with optional reversed colour order
+//     for post-smoothing, and MG-preconditioned CG.
+//
+// Every function is pinned by tests/test_oracle_*.py against closed forms,
+// high-precision golden fixtures (tests/golden) and brute-force dense
+// restatements on tiny grids.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+namespace
+{
+  constexpr int MAXK = 8;
+
+  // --------------------------------------------------------------- 1D basis
+  void
+  legendre(int n, long double x, long double &p, long double &dp)
+  {
+    long double p0 = 1.0L, p1 = x;
+    if (n == 0)
+      {
+        p  = 1.0L;
+        dp = 0.0L;
+        return;
+      }
+    for (int m = 1; m < n; ++m)
+      {
+        long double p2 = ((2 * m + 1) * x * p1 - m * p0) / (m + 1);
+        p0             = p1;
+        p1             = p2;
+      }
+    p = p1;
+    // P_n' = n (x P_n - P_{n-1}) / (x^2 - 1), valid away from +-1
+    dp = n * (x * p1 - p0) / (x * x - 1.0L);
+  }
+
+  // Gauss-Lobatto nodes on [0,1]: endpoints and the roots of P_k'.
+  std::vector<double>
+  gl_nodes(int k)
+  {
+    std::vector<double> x(k + 1);
+    x[0] = 0.0;
+    x[k] = 1.0;
+    for (int i = 1; i < k; ++i)
+      {
+        long double t = -std::cos(M_PI * i / k);
+        for (int it = 0; it < 100; ++it)
+          {
+            long double p, dp;
+            legendre(k, t, p, dp);
+            // P'' from Legendre's equation
+            long double d2p = (2 * t * dp - k * (k + 1) * p) / (1 - t * t);
+            long double dt  = dp / d2p;
+            t -= dt;
+            if (std::fabs((double)dt) < 1e-19)
+              break;
+          }
+        x[i] = (double)((t + 1.0L) / 2.0L);
+      }
+    return x;
+  }
+
+  // Gauss-Legendre rule with n points on [0,1].
+  void
+  gauss_rule(int n, std::vector<double> &pts, std::vector<double> &wts)
+  {
+    pts.resize(n);
+    wts.resize(n);
+    for (int i = 0; i < n; ++i)
+      {
+        long double t = -std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        long double p = 0, dp = 0;
+        for (int it = 0; it < 100; ++it)
+          {
+            legendre(n, t, p, dp);
+            long double dt = p / dp;
+            t -= dt;
+            if (std::fabs((double)dt) < 1e-19)
+              break;
+          }
+        legendre(n, t, p, dp);
+        pts[i] = (double)((t + 1.0L) / 2.0L);
+        wts[i] = (double)(1.0L / ((1 - t * t) * dp * dp));
+      }
+  }
+
+  double
+  lagrange(const std::vector<double> &nodes, int i, double x)
+  {
+    double v = 1.0;
+    for (size_t j = 0; j < nodes.size(); ++j)
+      if ((int)j != i)
+        v *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+    return v;
+  }
+
+  double
+  lagrange_deriv(const std::vector<double> &nodes, int i, double x)
+  {
+    double s = 0.0;
+    for (size_t m = 0; m < nodes.size(); ++m)
+      {
+        if ((int)m == i)
+          continue;
+        double v = 1.0 / (nodes[i] - nodes[m]);
+        for (size_t j = 0; j < nodes.size(); ++j)
+          if ((int)j != i && j != m)
+            v *= (x - nodes[j]) / (nodes[i] - nodes[j]);
+        s += v;
+      }
+    return s;
+  }
+
+  // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (row-major).
+  void
+  jacobi_eig(int n, std::vector<double> A, std::vector<double> &w,
+             std::vector<double> &V)
+  {
+    V.assign(n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      V[i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep)
+      {
+        double off = 0.0, diag = 0.0;
+        for (int p = 0; p < n; ++p)
+          for (int q = 0; q < n; ++q)
+            (p == q ? diag : off) += A[p * n + q] * A[p * n + q];
+        if (off <= 1e-32 * diag)
+          break;
+        for (int p = 0; p < n; ++p)
+          for (int q = p + 1; q < n; ++q)
+            {
+              const double apq = A[p * n + q];
+              if (std::fabs(apq) < 1e-300)
+                continue;
+              const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+              const double t =
+                (theta >= 0 ? 1.0 : -1.0) /
+                (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+              const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+              for (int r = 0; r < n; ++r)
+                {
+                  const double ap = A[r * n + p], aq = A[r * n + q];
+                  A[r * n + p]    = c * ap - s * aq;
+                  A[r * n + q]    = s * ap + c * aq;
+                }
+              for (int r = 0; r < n; ++r)
+                {
+                  const double ap = A[p * n + r], aq = A[q * n + r];
+                  A[p * n + r]    = c * ap - s * aq;
+                  A[q * n + r]    = s * ap + c * aq;
+                }
+              for (int r = 0; r < n; ++r)
+                {
+                  const double vp = V[r * n + p], vq = V[r * n + q];
+                  V[r * n + p]    = c * vp - s * vq;
+                  V[r * n + q]    = s * vp + c * vq;
+                }
+            }
+      }
+    w.resize(n);
+    for (int i = 0; i < n; ++i)
+      w[i] = A[i * n + i];
+  }
+
+  // Generalized symmetric-definite eigenproblem K s = lambda M s, with
+  // S^T M S = I, eigenvalues ascending. S[i*n+j] = component i of vector j.
+  void
+  gen_eig(int n, const double *K, const double *M, double *lam, double *S)
+  {
+    std::vector<double> L(n * n, 0.0);
+    for (int j = 0; j < n; ++j)
+      {
+        double s = M[j * n + j];
+        for (int m = 0; m < j; ++m)
+          s -= L[j * n + m] * L[j * n + m];
+        L[j * n + j] = std::sqrt(s);
+        for (int i = j + 1; i < n; ++i)
+          {
+            double t = M[i * n + j];
+            for (int m = 0; m < j; ++m)
+              t -= L[i * n + m] * L[j * n + m];
+            L[i * n + j] = t / L[j * n + j];
+          }
+      }
+    // Linv (lower triangular)
+    std::vector<double> Li(n * n, 0.0);
+    for (int j = 0; j < n; ++j)
+      {
+        Li[j * n + j] = 1.0 / L[j * n + j];
+        for (int i = j + 1; i < n; ++i)
+          {
+            double t = 0.0;
+            for (int m = j; m < i; ++m)
+              t -= L[i * n + m] * Li[m * n + j];
+            Li[i * n + j] = t / L[i * n + i];
+          }
+      }
+    // C = Li K Li^T
+    std::vector<double> T(n * n, 0.0), C(n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        {
+          double t = 0.0;
+          for (int m = 0; m < n; ++m)
+            t += Li[i * n + m] * K[m * n + j];
+          T[i * n + j] = t;
+        }
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        {
+          double t = 0.0;
+          for (int m = 0; m < n; ++m)
+            t += T[i * n + m] * Li[j * n + m];
+          C[i * n + j] = t;
+        }
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < i; ++j)
+        C[i * n + j] = C[j * n + i] = 0.5 * (C[i * n + j] + C[j * n + i]);
+    std::vector<double> w, Q;
+    jacobi_eig(n, C, w, Q);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i)
+      order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](int a, int b) { return w[a] < w[b]; });
+    for (int jj = 0; jj < n; ++jj)
+      {
+        const int j = order[jj];
+        lam[jj]     = w[j];
+        for (int i = 0; i < n; ++i)
+          {
+            // S = Li^T Q
+            double t = 0.0;
+            for (int m = 0; m < n; ++m)
+              t += Li[m * n + i] * Q[m * n + j];
+            S[i * n + jj] = t;
+          }
+      }
+  }
+
+  struct Basis
+  {
+    int                 k = 0;
+    std::vector<double> nodes, qp, qw;
+    std::vector<double> N, D;    // [q*(k+1)+i] values/derivatives at Gauss points
+    std::vector<double> M, K;    // cell (k+1)^2, reference cell [0,1]
+    std::vector<double> E;       // embedding (2k+1) x (k+1)
+    std::vector<double> MP, KP;  // 2-cell patch (2k+1)^2
+    std::vector<double> lam, S;  // interior (2k-1) eigenpairs
+  };
+
+  Basis
+  make_basis(int k)
+  {
+    Basis b;
+    b.k     = k;
+    b.nodes = gl_nodes(k);
+    gauss_rule(k + 1, b.qp, b.qw);
+    const int n = k + 1;
+    b.N.resize(n * n);
+    b.D.resize(n * n);
+    for (int q = 0; q < n; ++q)
+      for (int i = 0; i < n; ++i)
+        {
+          b.N[q * n + i] = lagrange(b.nodes, i, b.qp[q]);
+          b.D[q * n + i] = lagrange_deriv(b.nodes, i, b.qp[q]);
+        }
+    b.M.assign(n * n, 0.0);
+    b.K.assign(n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        for (int q = 0; q < n; ++q)
+          {
+            b.M[i * n + j] += b.qw[q] * b.N[q * n + i] * b.N[q * n + j];
+            b.K[i * n + j] += b.qw[q] * b.D[q * n + i] * b.D[q * n + j];
+          }
+    const int np = 2 * k + 1;
+    b.E.resize(np * n);
+    for (int q = 0; q < np; ++q)
+      {
+        const double x =
+          q <= k ? 0.5 * b.nodes[q] : 0.5 + 0.5 * b.nodes[q - k];
+        for (int a = 0; a < n; ++a)
+          b.E[q * n + a] = lagrange(b.nodes, a, x);
+      }
+    b.MP.assign(np * np, 0.0);
+    b.KP.assign(np * np, 0.0);
+    for (int c = 0; c < 2; ++c)
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          {
+            b.MP[(c * k + i) * np + c * k + j] += b.M[i * n + j];
+            b.KP[(c * k + i) * np + c * k + j] += b.K[i * n + j];
+          }
+    const int ni = 2 * k - 1;
+    std::vector<double> MI(ni * ni), KI(ni * ni);
+    for (int i = 0; i < ni; ++i)
+      for (int j = 0; j < ni; ++j)
+        {
+          MI[i * ni + j] = b.MP[(i + 1) * np + j + 1];
+          KI[i * ni + j] = b.KP[(i + 1) * np + j + 1];
+        }
+    b.lam.resize(ni);
+    b.S.resize(ni * ni);
+    gen_eig(ni, KI.data(), MI.data(), b.lam.data(), b.S.data());
+    return b;
+  }
+
+  const Basis &
+  basis(int k)
+  {
+    static Basis cache[MAXK + 1];
+    static bool  have[MAXK + 1] = {};
+#pragma omp critical(gtps_oracle_basis)
+    {
+      if (!have[k])
+        {
+          cache[k] = make_basis(k);
+          have[k]  = true;
+        }
+    }
+    return cache[k];
+  }
+
+  // ------------------------------------------------- small tensor utilities
+  // Arrays are (n2, n1, n0) with axis 0 fastest. Apply A (rows x cols) along
+  // `axis`: out[.., r, ..] (+)= sum_c A[r*cols+c] in[.., c, ..].
+  void
+  apply_axis(const double *A, int rows, int cols, const double *in,
+             double *out, const int *shape, int axis, bool add,
+             bool transpose = false)
+  {
+    int s_out[3] = {shape[0], shape[1], shape[2]};
+    s_out[axis]  = rows;
+    const int stride_in[3]  = {1, shape[0], shape[0] * shape[1]};
+    const int stride_out[3] = {1, s_out[0], s_out[0] * s_out[1]};
+    for (int i2 = 0; i2 < s_out[2]; ++i2)
+      for (int i1 = 0; i1 < s_out[1]; ++i1)
+        for (int i0 = 0; i0 < s_out[0]; ++i0)
+          {
+            int idx[3] = {i0, i1, i2};
+            const int r = idx[axis];
+            idx[axis]   = 0;
+            const int base_in =
+              idx[0] * stride_in[0] + idx[1] * stride_in[1] + idx[2] * stride_in[2];
+            double s = 0.0;
+            for (int c = 0; c < cols; ++c)
+              {
+                const double a = transpose ? A[c * rows + r] : A[r * cols + c];
+                s += a * in[base_in + c * stride_in[axis]];
+              }
+            const int o = i0 * stride_out[0] + i1 * stride_out[1] +
+                          i2 * stride_out[2];
+            out[o] = add ? out[o] + s : s;
+          }
+  }
+
+  struct Grid
+  {
+    int       dim, k, n;  // n cells per direction
+    long long N1, N;      // DoFs per direction, total
+    double    h;
+    Grid(int dim_, int k_, int level)
+      : dim(dim_), k(k_), n(1 << level)
+    {
+      N1 = (long long)k * n + 1;
+      N  = dim == 2 ? N1 * N1 : N1 * N1 * N1;
+      h  = 1.0 / n;
+    }
+    bool
+    on_boundary(long long i) const
+    {
+      const long long x = i % N1, y = (i / N1) % N1, z = i / (N1 * N1);
+      if (x == 0 || x == N1 - 1 || y == 0 || y == N1 - 1)
+        return true;
+      return dim == 3 && (z == 0 || z == N1 - 1);
+    }
+  };
+
+  // ------------------------------------------------------------- operator
+  // Cell-wise quadrature form: y_K = sum_d (D_d^T W N^T..) (D_d N..) u_K with
+  // JxW = h^dim w_q and J^{-1} = I/h; scattered into y.
+  void
+  laplace_vmult(const Grid &g, const double *x, double *y)
+  {
+    const Basis &b   = basis(g.k);
+    const int    n1d = g.k + 1;
+    const int    dim = g.dim;
+    const int    nl  = dim == 2 ? n1d * n1d : n1d * n1d * n1d;
+    const double scale = std::pow(g.h, dim - 2);
+    std::vector<double> wq(nl);
+    for (int l = 0; l < (dim == 3 ? n1d : 1); ++l)
+      for (int j = 0; j < n1d; ++j)
+        for (int i = 0; i < n1d; ++i)
+          wq[(l * n1d + j) * n1d + i] =
+            b.qw[i] * b.qw[j] * (dim == 3 ? b.qw[l] : 1.0) * scale;
+    std::fill(y, y + g.N, 0.0);
+    const int ncol = 1 << dim;
+    for (int col = 0; col < ncol; ++col)
+      {
+        const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+        const int nz = dim == 3 ? g.n : 1;
+#pragma omp parallel
+        {
+          std::vector<double> u(nl), t1(nl), t2(nl), gq(nl), out(nl), acc(nl);
+          const int shape[3] = {n1d, n1d, dim == 3 ? n1d : 1};
+#pragma omp for collapse(2) schedule(static)
+          for (int cz = pz * (dim == 3); cz < nz; cz += (dim == 3 ? 2 : 1))
+            for (int cy = py; cy < g.n; cy += 2)
+              {
+                if (dim == 2 && pz == 1)
+                  continue;
+                for (int cx = px; cx < g.n; cx += 2)
+                  {
+                    const long long base = (long long)g.k * cx +
+                                           g.N1 * ((long long)g.k * cy) +
+                                           g.N1 * g.N1 *
+                                             ((long long)g.k * cz * (dim == 3));
+                    for (int l = 0; l < shape[2]; ++l)
+                      for (int j = 0; j < n1d; ++j)
+                        for (int i = 0; i < n1d; ++i)
+                          u[(l * n1d + j) * n1d + i] =
+                            x[base + i + g.N1 * j + g.N1 * g.N1 * l];
+                    std::fill(acc.begin(), acc.end(), 0.0);
+                    for (int d = 0; d < dim; ++d)
+                      {
+                        // gradient component d at quadrature points
+                        const double *src = u.data();
+                        double       *bufs[2] = {t1.data(), t2.data()};
+                        int           nb      = 0;
+                        for (int a = 0; a < dim; ++a)
+                          {
+                            double *dst = (a == dim - 1) ? gq.data() : bufs[nb++ & 1];
+                            apply_axis(a == d ? b.D.data() : b.N.data(), n1d,
+                                       n1d, src, dst, shape, a, false);
+                            src = dst;
+                          }
+                        for (int q = 0; q < nl; ++q)
+                          gq[q] *= wq[q];
+                        // integrate: transpose of the same matrices
+                        src = gq.data();
+                        nb  = 0;
+                        for (int a = 0; a < dim; ++a)
+                          {
+                            double *dst = (a == dim - 1) ? out.data() : bufs[nb++ & 1];
+                            apply_axis(a == d ? b.D.data() : b.N.data(), n1d,
+                                       n1d, src, dst, shape, a, false, true);
+                            src = dst;
+                          }
+                        for (int q = 0; q < nl; ++q)
+                          acc[q] += out[q];
+                      }
+                    for (int l = 0; l < shape[2]; ++l)
+                      for (int j = 0; j < n1d; ++j)
+                        for (int i = 0; i < n1d; ++i)
+                          y[base + i + g.N1 * j + g.N1 * g.N1 * l] +=
+                            acc[(l * n1d + j) * n1d + i];
+                  }
+              }
+        }
+      }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < g.N; ++i)
+      if (g.on_boundary(i))
+        y[i] = x[i];
+  }
+
+  // ------------------------------------------------------------- smoother
+  enum Variant
+  {
+    MULT_FWD = 0,
+    MULT_REV = 1,
+    ADDITIVE = 2
+  };
+
+  // One patch: box values ub, bb (full box), returns interior correction in
+  // `delta` (interior, lexicographic) for r = rb_I (if residual_given) or
+  // r = b - A_P u otherwise.
+  struct PatchWork
+  {
+    std::vector<double> box, t1, t2, r, t;
+    explicit PatchWork(int np)
+      : box(np * np * np), t1(np * np * np), t2(np * np * np),
+        r(np * np * np), t(np * np * np)
+    {}
+  };
+
+  void
+  fd_inverse(const Grid &g, const Basis &b, double *r, double *tmp)
+  {
+    // r: interior (ni^dim), in place -> A_II^{-1} r
+    const int    ni     = 2 * g.k - 1;
+    const int    shape[3] = {ni, ni, g.dim == 3 ? ni : 1};
+    const double scale  = std::pow(g.h, g.dim - 2);
+    double      *bufs[2] = {r, tmp};
+    int          cur     = 0;
+    for (int a = 0; a < g.dim; ++a)
+      {
+        apply_axis(b.S.data(), ni, ni, bufs[cur], bufs[cur ^ 1], shape, a,
+                   false, true);  // S^T
+        cur ^= 1;
+      }
+    for (int l = 0; l < shape[2]; ++l)
+      for (int j = 0; j < ni; ++j)
+        for (int i = 0; i < ni; ++i)
+          {
+            const double lam =
+              b.lam[i] + b.lam[j] + (g.dim == 3 ? b.lam[l] : 0.0);
+            bufs[cur][(l * ni + j) * ni + i] /= scale * lam;
+          }
+    for (int a = 0; a < g.dim; ++a)
+      {
+        apply_axis(b.S.data(), ni, ni, bufs[cur], bufs[cur ^ 1], shape, a,
+                   false, false);  // S
+        cur ^= 1;
+      }
+    if (cur != 0)
+      std::memcpy(r, bufs[cur], sizeof(double) * shape[0] * shape[1] * shape[2]);
+  }
+
+  void
+  smoother_step(const Grid &g, int variant, double relax, const double *b_,
+                double *u)
+  {
+    const Basis &b   = basis(g.k);
+    const int    k   = g.k;
+    const int    np  = 2 * k + 1;
+    const int    ni  = 2 * k - 1;
+    const int    dim = g.dim;
+    const int    npatch = g.n - 1;  // patches per direction
+    const double scale  = std::pow(g.h, dim - 2);
+    std::vector<double> res;
+    if (variant == ADDITIVE)
+      {
+        res.resize(g.N);
+        laplace_vmult(g, u, res.data());
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < g.N; ++i)
+          res[i] = b_[i] - res[i];
+      }
+    const int ncol = 1 << dim;
+    for (int cc = 0; cc < ncol; ++cc)
+      {
+        const int col = variant == MULT_REV ? ncol - 1 - cc : cc;
+        const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+        const int nz = dim == 3 ? npatch : 1;
+#pragma omp parallel
+        {
+          PatchWork   w(np);
+          const int   shape[3]  = {np, np, dim == 3 ? np : 1};
+          const int   shi[3]    = {ni, ni, dim == 3 ? ni : 1};
+          const int   nbox      = shape[0] * shape[1] * shape[2];
+          const int   nint      = shi[0] * shi[1] * shi[2];
+          std::vector<double> bb(nbox), delta(nint);
+#pragma omp for collapse(2) schedule(static)
+          for (int iz = (dim == 3 ? pz : 0); iz < nz; iz += (dim == 3 ? 2 : 1))
+            for (int iy = py; iy < npatch; iy += 2)
+              {
+                if (dim == 2 && pz == 1)
+                  continue;
+                for (int ix = px; ix < npatch; ix += 2)
+                  {
+                    const long long first =
+                      (long long)k * ix + g.N1 * ((long long)k * iy) +
+                      g.N1 * g.N1 * ((long long)k * iz * (dim == 3));
+                    auto gidx = [&](int i, int j, int l) {
+                      return first + i + g.N1 * j + g.N1 * g.N1 * l;
+                    };
+                    if (variant == ADDITIVE)
+                      {
+                        for (int l = 0; l < shi[2]; ++l)
+                          for (int j = 0; j < ni; ++j)
+                            for (int i = 0; i < ni; ++i)
+                              delta[(l * ni + j) * ni + i] =
+                                res[gidx(i + 1, j + 1, l + (dim == 3))];
+                      }
+                    else
+                      {
+                        for (int l = 0; l < shape[2]; ++l)
+                          for (int j = 0; j < np; ++j)
+                            for (int i = 0; i < np; ++i)
+                              {
+                                w.box[(l * np + j) * np + i] = u[gidx(i, j, l)];
+                                bb[(l * np + j) * np + i]    = b_[gidx(i, j, l)];
+                              }
+                        // A_P u = scale * sum_d KP_d (x) MP_others
+                        std::fill(w.r.begin(), w.r.begin() + nbox, 0.0);
+                        for (int d = 0; d < dim; ++d)
+                          {
+                            const double *src = w.box.data();
+                            double       *bufs[2] = {w.t1.data(), w.t2.data()};
+                            int           nb      = 0;
+                            for (int a = 0; a < dim; ++a)
+                              {
+                                double *dst = bufs[nb++ & 1];
+                                apply_axis(a == d ? b.KP.data() : b.MP.data(),
+                                           np, np, src, dst, shape, a, false);
+                                src = dst;
+                              }
+                            for (int q = 0; q < nbox; ++q)
+                              w.r[q] += src[q];
+                          }
+                        for (int l = 0; l < shi[2]; ++l)
+                          for (int j = 0; j < ni; ++j)
+                            for (int i = 0; i < ni; ++i)
+                              {
+                                const int q = ((l + (dim == 3)) * np + j + 1) * np + i + 1;
+                                delta[(l * ni + j) * ni + i] =
+                                  bb[q] - scale * w.r[q];
+                              }
+                      }
+                    fd_inverse(g, b, delta.data(), w.t.data());
+                    for (int l = 0; l < shi[2]; ++l)
+                      for (int j = 0; j < ni; ++j)
+                        for (int i = 0; i < ni; ++i)
+                          {
+                            const long long gi = gidx(i + 1, j + 1, l + (dim == 3));
+                            const double    dv = delta[(l * ni + j) * ni + i];
+                            if (variant == ADDITIVE)
+                              u[gi] += relax * dv;
+                            else
+                              u[gi] = relax * (u[gi] + dv);
+                          }
+                  }
+              }
+        }
+      }
+  }
+
+  // ------------------------------------------------------------- transfer
+  // weight of fine local node q (0..2k) of coarse cell c (0..nc-1): 1/2 on
+  // coarse-cell faces shared with a neighbour cell, else 1 (valence weights).
+  inline double
+  vweight(int q, int c, int nc, int k)
+  {
+    if (q == 0 && c > 0)
+      return 0.5;
+    if (q == 2 * k && c < nc - 1)
+      return 0.5;
+    return 1.0;
+  }
+
+  void
+  transfer(const Grid &gf, bool prolongate, double *coarse_, double *fine_,
+           const double *coarse_in, const double *fine_in)
+  {
+    // gf: fine grid; coarse grid has gf.n/2 cells
+    const Basis &b   = basis(gf.k);
+    const int    k   = gf.k;
+    const int    dim = gf.dim;
+    const int    nc  = gf.n / 2;
+    const long long Nc1 = (long long)k * nc + 1;
+    const int    n1  = k + 1, np = 2 * k + 1;
+    const int    shc[3] = {n1, n1, dim == 3 ? n1 : 1};
+    const int    ncoarse_loc = shc[0] * shc[1] * shc[2];
+    const int    shf[3] = {np, np, dim == 3 ? np : 1};
+    const int    nfine_loc = shf[0] * shf[1] * shf[2];
+    const int    ncol = 1 << dim;
+    for (int col = 0; col < ncol; ++col)
+      {
+        const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+        if (dim == 2 && pz)
+          continue;
+        const int nz = dim == 3 ? nc : 1;
+#pragma omp parallel
+        {
+          std::vector<double> cv(ncoarse_loc), fv(nfine_loc), t1(nfine_loc),
+            t2(nfine_loc);
+#pragma omp for collapse(2) schedule(static)
+          for (int cz = (dim == 3 ? pz : 0); cz < nz; cz += (dim == 3 ? 2 : 1))
+            for (int cy = py; cy < nc; cy += 2)
+              for (int cx = px; cx < nc; cx += 2)
+                {
+                  const int cc[3] = {cx, cy, cz};
+                  auto cidx = [&](int i, int j, int l) {
+                    return (long long)k * cx + i +
+                           Nc1 * ((long long)k * cy + j) +
+                           Nc1 * Nc1 * ((long long)k * cz * (dim == 3) + l);
+                  };
+                  auto fidx = [&](int i, int j, int l) {
+                    return (long long)2 * k * cx + i +
+                           gf.N1 * ((long long)2 * k * cy + j) +
+                           gf.N1 * gf.N1 *
+                             ((long long)2 * k * cz * (dim == 3) + l);
+                  };
+                  auto c_on_bdry = [&](int i, int j, int l) {
+                    const long long gx = (long long)k * cx + i,
+                                    gy = (long long)k * cy + j,
+                                    gz = (long long)k * cz + l;
+                    if (gx == 0 || gx == Nc1 - 1 || gy == 0 || gy == Nc1 - 1)
+                      return true;
+                    return dim == 3 && (gz == 0 || gz == Nc1 - 1);
+                  };
+                  auto fw = [&](int i, int j, int l) {
+                    double w = vweight(i, cc[0], nc, k) * vweight(j, cc[1], nc, k);
+                    if (dim == 3)
+                      w *= vweight(l, cc[2], nc, k);
+                    return w;
+                  };
+                  if (prolongate)
+                    {
+                      for (int l = 0; l < shc[2]; ++l)
+                        for (int j = 0; j < n1; ++j)
+                          for (int i = 0; i < n1; ++i)
+                            cv[(l * n1 + j) * n1 + i] =
+                              c_on_bdry(i, j, l) ? 0.0 : coarse_in[cidx(i, j, l)];
+                      int sh[3] = {shc[0], shc[1], shc[2]};
+                      const double *src = cv.data();
+                      double       *bufs[2] = {t1.data(), t2.data()};
+                      for (int a = 0; a < dim; ++a)
+                        {
+                          double *dst = (a == dim - 1) ? fv.data() : bufs[a & 1];
+                          apply_axis(b.E.data(), np, n1, src, dst, sh, a, false);
+                          sh[a] = np;
+                          src   = dst;
+                        }
+                      for (int l = 0; l < shf[2]; ++l)
+                        for (int j = 0; j < np; ++j)
+                          for (int i = 0; i < np; ++i)
+                            fine_[fidx(i, j, l)] +=
+                              fw(i, j, l) * fv[(l * np + j) * np + i];
+                    }
+                  else
+                    {
+                      for (int l = 0; l < shf[2]; ++l)
+                        for (int j = 0; j < np; ++j)
+                          for (int i = 0; i < np; ++i)
+                            fv[(l * np + j) * np + i] =
+                              fw(i, j, l) * fine_in[fidx(i, j, l)];
+                      int sh[3] = {shf[0], shf[1], shf[2]};
+                      const double *src = fv.data();
+                      double       *bufs[2] = {t1.data(), t2.data()};
+                      for (int a = 0; a < dim; ++a)
+                        {
+                          double *dst = (a == dim - 1) ? cv.data() : bufs[a & 1];
+                          apply_axis(b.E.data(), n1, np, src, dst, sh, a,
+                                     false, true);
+                          sh[a] = n1;
+                          src   = dst;
+                        }
+                      for (int l = 0; l < shc[2]; ++l)
+                        for (int j = 0; j < n1; ++j)
+                          for (int i = 0; i < n1; ++i)
+                            if (!c_on_bdry(i, j, l))
+                              coarse_[cidx(i, j, l)] += cv[(l * n1 + j) * n1 + i];
+                    }
+                }
+        }
+      }
+  }
+
+  // ------------------------------------------------------------ multigrid
+  struct MG
+  {
+    int dim, k, minlevel, maxlevel, variant;  // variant: 0 sym, 1 ref-order, 2 additive
+    double relax;
+    std::vector<Grid> grids;
+    mutable std::vector<std::vector<double>> d, u, t;
+    MG(int dim_, int k_, int maxl, int var, double rel)
+      : dim(dim_), k(k_), minlevel(1), maxlevel(maxl), variant(var), relax(rel)
+    {
+      for (int l = 0; l <= maxlevel; ++l)
+        grids.emplace_back(dim, k, std::max(l, 1));
+      d.resize(maxlevel + 1);
+      u.resize(maxlevel + 1);
+      t.resize(maxlevel + 1);
+      for (int l = minlevel; l <= maxlevel; ++l)
+        {
+          d[l].assign(grids[l].N, 0.0);
+          u[l].assign(grids[l].N, 0.0);
+          t[l].assign(grids[l].N, 0.0);
+        }
+    }
+    int pre_variant() const { return variant == 2 ? ADDITIVE : MULT_FWD; }
+    int post_variant() const
+    {
+      return variant == 2 ? ADDITIVE : (variant == 0 ? MULT_REV : MULT_FWD);
+    }
+    void
+    vcycle(int l) const
+    {
+      const Grid &g = grids[l];
+      std::fill(u[l].begin(), u[l].end(), 0.0);
+      if (l == minlevel)
+        {
+          // single vertex patch: exact solve
+          smoother_step(g, MULT_FWD, 1.0, d[l].data(), u[l].data());
+          return;
+        }
+      smoother_step(g, pre_variant(), relax, d[l].data(), u[l].data());
+      laplace_vmult(g, u[l].data(), t[l].data());
+      for (long long i = 0; i < g.N; ++i)
+        t[l][i] = d[l][i] - t[l][i];
+      std::fill(d[l - 1].begin(), d[l - 1].end(), 0.0);
+      transfer(g, false, d[l - 1].data(), nullptr, nullptr, t[l].data());
+      vcycle(l - 1);
+      transfer(g, true, nullptr, u[l].data(), u[l - 1].data(), nullptr);
+      smoother_step(g, post_variant(), relax, d[l].data(), u[l].data());
+    }
+    void
+    apply(const double *r, double *z) const
+    {
+      std::copy(r, r + grids[maxlevel].N, d[maxlevel].begin());
+      vcycle(maxlevel);
+      std::copy(u[maxlevel].begin(), u[maxlevel].end(), z);
+    }
+  };
+
+  double
+  dot(long long n, const double *a, const double *b)
+  {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long long i = 0; i < n; ++i)
+      s += a[i] * b[i];
+    return s;
+  }
+
+  uint64_t
+  splitmix64(uint64_t x)
+  {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+  }
+} // namespace
+
+extern "C"
+{
+  int
+  oracle_max_degree()
+  {
+    return MAXK;
+  }
+
+  // 1D data for degree k: nodes[k+1], cell M,K [(k+1)^2], embedding
+  // E[(2k+1)(k+1)], patch MP,KP [(2k+1)^2], interior eigenpairs lam[2k-1],
+  // S[(2k-1)^2] (column j = eigenvector j).
+  int
+  oracle_basis_1d(int k, double *nodes, double *M, double *K, double *E,
+                  double *MP, double *KP, double *lam, double *S)
+  {
+    if (k < 1 || k > MAXK)
+      return 1;
+    const Basis &b = basis(k);
+    if (nodes) std::copy(b.nodes.begin(), b.nodes.end(), nodes);
+    if (M) std::copy(b.M.begin(), b.M.end(), M);
+    if (K) std::copy(b.K.begin(), b.K.end(), K);
+    if (E) std::copy(b.E.begin(), b.E.end(), E);
+    if (MP) std::copy(b.MP.begin(), b.MP.end(), MP);
+    if (KP) std::copy(b.KP.begin(), b.KP.end(), KP);
+    if (lam) std::copy(b.lam.begin(), b.lam.end(), lam);
+    if (S) std::copy(b.S.begin(), b.S.end(), S);
+    return 0;
+  }
+
+  int
+  oracle_gen_eig(int n, const double *K, const double *M, double *lam,
+                 double *S)
+  {
+    if (n < 1 || n > 64)
+      return 1;
+    gen_eig(n, K, M, lam, S);
+    return 0;
+  }
+
+  long long
+  oracle_n_dofs(int dim, int k, int level)
+  {
+    return Grid(dim, k, level).N;
+  }
+
+  int
+  oracle_laplace_vmult(int dim, int k, int level, const double *x, double *y)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1)
+      return 1;
+    laplace_vmult(Grid(dim, k, level), x, y);
+    return 0;
+  }
+
+  // variant: 0 forward colours, 1 reverse colours, 2 additive
+  int
+  oracle_smoother_step(int dim, int k, int level, int variant, double relax,
+                       const double *b, double *u)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1 ||
+        variant < 0 || variant > 2)
+      return 1;
+    smoother_step(Grid(dim, k, level), variant, relax, b, u);
+    return 0;
+  }
+
+  int
+  oracle_prolongate_add(int dim, int k, int fine_level, const double *coarse,
+                        double *fine)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || fine_level < 2)
+      return 1;
+    transfer(Grid(dim, k, fine_level), true, nullptr, fine, coarse, nullptr);
+    return 0;
+  }
+
+  int
+  oracle_restrict_add(int dim, int k, int fine_level, const double *fine,
+                      double *coarse)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || fine_level < 2)
+      return 1;
+    transfer(Grid(dim, k, fine_level), false, coarse, nullptr, nullptr, fine);
+    return 0;
+  }
+
+  // mg_variant: 0 multiplicative with reversed post-smoothing (symmetric),
+  // 1 multiplicative with the reference's colour order, 2 additive.
+  int
+  oracle_vcycle(int dim, int k, int level, int mg_variant, double relax,
+                const double *r, double *z)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1)
+      return 1;
+    MG mg(dim, k, level, mg_variant, relax);
+    mg.apply(r, z);
+    return 0;
+  }
+
+  // Preconditioned CG on the zero-Dirichlet system. Boundary entries of b are
+  // taken as the boundary values of x (identity rows); stops when
+  // ||r_k|| <= rtol ||r_0||. history (optional, maxit+1) gets ||r_k||.
+  int
+  oracle_pcg(int dim, int k, int level, int mg_variant, double relax,
+             const double *b, double *x, double rtol, int maxit, int *iters,
+             double *relres, double *history)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1)
+      return 1;
+    const Grid g(dim, k, level);
+    MG         mg(dim, k, level, mg_variant, relax);
+    const long long N = g.N;
+    std::vector<double> r(N), z(N), p(N), q(N);
+    for (long long i = 0; i < N; ++i)
+      x[i] = g.on_boundary(i) ? b[i] : 0.0;
+    laplace_vmult(g, x, q.data());
+    for (long long i = 0; i < N; ++i)
+      r[i] = b[i] - q[i];
+    const double r0 = std::sqrt(dot(N, r.data(), r.data()));
+    if (history)
+      history[0] = r0;
+    int    it  = 0;
+    double rn  = r0;
+    if (r0 == 0.0)
+      {
+        *iters  = 0;
+        *relres = 0.0;
+        return 0;
+      }
+    mg.apply(r.data(), z.data());
+    std::copy(z.begin(), z.end(), p.begin());
+    double rz = dot(N, r.data(), z.data());
+    while (it < maxit)
+      {
+        laplace_vmult(g, p.data(), q.data());
+        const double alpha = rz / dot(N, p.data(), q.data());
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < N; ++i)
+          {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+          }
+        ++it;
+        rn = std::sqrt(dot(N, r.data(), r.data()));
+        if (history)
+          history[it] = rn;
+        if (!(rn > rtol * r0))
+          break;
+        mg.apply(r.data(), z.data());
+        const double rz_new = dot(N, r.data(), z.data());
+        const double beta   = rz_new / rz;
+        rz                  = rz_new;
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < N; ++i)
+          p[i] = z[i] + beta * p[i];
+      }
+    *iters  = it;
+    *relres = rn / r0;
+    return std::isfinite(rn) ? 0 : 2;
+  }
+
+  // b_i = 2*u01(i) - 1 with u01(i) = (splitmix64(seed ^ i) >> 11) 2^-53;
+  // Dirichlet entries set to zero.
+  int
+  oracle_rhs_random(int dim, int k, int level, unsigned long long seed,
+                    double *b)
+  {
+    const Grid g(dim, k, level);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < g.N; ++i)
+      {
+        const double u01 =
+          (double)(splitmix64(seed ^ (uint64_t)i) >> 11) * 0x1.0p-53;
+        b[i] = g.on_boundary(i) ? 0.0 : 2.0 * u01 - 1.0;
+      }
+    return 0;
+  }
+
+  // Assembled load vector b_i = int f phi_i for a separable f(x) =
+  // c * prod_d f1(x_d) with f1 = 1 (kind 0) or f1 = sin(3 pi x) and
+  // c = dim (3 pi)^2 (kind 1, the reference's Poisson problem), Gauss(k+1)
+  // quadrature; Dirichlet entries zero.
+  int
+  oracle_rhs_assembled(int dim, int k, int level, int kind, double *b)
+  {
+    const Grid   g(dim, k, level);
+    const Basis &bs = basis(k);
+    std::vector<double> m(g.N1, 0.0);
+    for (int c = 0; c < g.n; ++c)
+      for (int i = 0; i <= k; ++i)
+        for (int q = 0; q <= k; ++q)
+          {
+            const double xq = (c + bs.qp[q]) * g.h;
+            const double f1 = kind == 0 ? 1.0 : std::sin(3.0 * M_PI * xq);
+            m[(long long)k * c + i] += bs.qw[q] * g.h * f1 * bs.N[q * (k + 1) + i];
+          }
+    const double cst = kind == 0 ? 1.0 : dim * 9.0 * M_PI * M_PI;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < g.N; ++i)
+      {
+        const long long x = i % g.N1, y = (i / g.N1) % g.N1, z = i / (g.N1 * g.N1);
+        double v = cst * m[x] * m[y];
+        if (dim == 3)
+          v *= m[z];
+        b[i] = g.on_boundary(i) ? 0.0 : v;
+      }
+    return 0;
+  }
+
+  // L2 error of the FE function u against prod_d sin(3 pi x_d), with a
+  // Gauss(k+3) rule per cell.
+  double
+  oracle_l2_error_sine(int dim, int k, int level, const double *u)
+  {
+    const Grid   g(dim, k, level);
+    const Basis &bs = basis(k);
+    std::vector<double> qp, qw;
+    gauss_rule(k + 3, qp, qw);
+    const int nq = k + 3, n1 = k + 1;
+    std::vector<double> phi(nq * n1);
+    for (int q = 0; q < nq; ++q)
+      for (int i = 0; i < n1; ++i)
+        phi[q * n1 + i] = lagrange(bs.nodes, i, qp[q]);
+    double err = 0.0;
+    const int nz = dim == 3 ? g.n : 1;
+#pragma omp parallel for collapse(2) reduction(+ : err) schedule(static)
+    for (int cz = 0; cz < nz; ++cz)
+      for (int cy = 0; cy < g.n; ++cy)
+        for (int cx = 0; cx < g.n; ++cx)
+          for (int qz = 0; qz < (dim == 3 ? nq : 1); ++qz)
+            for (int qy = 0; qy < nq; ++qy)
+              for (int qx = 0; qx < nq; ++qx)
+                {
+                  double uh = 0.0;
+                  for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+                    for (int j = 0; j < n1; ++j)
+                      for (int i = 0; i < n1; ++i)
+                        {
+                          double w = phi[qx * n1 + i] * phi[qy * n1 + j];
+                          if (dim == 3)
+                            w *= phi[qz * n1 + l];
+                          const long long gi =
+                            (long long)k * cx + i + g.N1 * ((long long)k * cy + j) +
+                            g.N1 * g.N1 * ((long long)k * cz * (dim == 3) + l);
+                          uh += w * u[gi];
+                        }
+                  const double x = (cx + qp[qx]) * g.h, y = (cy + qp[qy]) * g.h,
+                               z = (cz + qp[qz]) * g.h;
+                  double ue = std::sin(3 * M_PI * x) * std::sin(3 * M_PI * y);
+                  double w  = qw[qx] * qw[qy] * g.h * g.h;
+                  if (dim == 3)
+                    {
+                      ue *= std::sin(3 * M_PI * z);
+                      w *= qw[qz] * g.h;
+                    }
+                  err += w * (uh - ue) * (uh - ue);
+                }
+    return std::sqrt(err);
+  }
+
+  int
+  oracle_set_threads(int n)
+  {
+    if (n > 0)
+      omp_set_num_threads(n);
+    return omp_get_max_threads();
+  }
+}

@@ -1,0 +1,150 @@
+"""GPU V-cycle and MG-preconditioned CG against the FP64 CPU oracle, plus
+size-independent properties at the metric configuration (3D Q3, 64^3 cells).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import boundary_mask, n_dofs, rel_err, rng
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def core():
+    from gputensorproductsmoother_amd import core as C
+    return C
+
+
+@pytest.fixture(scope="module")
+def ctx(core):
+    return core.Context(0)
+
+
+def _announce(*args):
+    print("CASE", *args, flush=True)
+
+
+SCHED = {"sym": (O.MG_SYMMETRIC, 0, 1.0), "ref": (O.MG_REFERENCE_ORDER, 1, 1.0),
+         "add": (O.MG_ADDITIVE, 2, 0.5)}
+
+
+@pytest.mark.parametrize("sched", ["sym", "ref", "add"])
+@pytest.mark.parametrize("dim,k,level", [(2, 2, 4), (3, 1, 4), (3, 3, 3), (3, 4, 2)])
+def test_vcycle_fp64(core, ctx, dim, k, level, sched):
+    _announce("vcycle64", dim, k, level, sched)
+    s = SCHED[sched]
+    r = O.rhs_random(dim, k, level)
+    ref = O.vcycle(dim, k, level, r, mg_variant=s[0], relax=s[2])
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float64, s[1], s[2])
+    rd = torch.tensor(r, device="cuda:0")
+    zd = torch.full_like(rd, float("nan"))
+    mg.vmult(zd, rd)
+    torch.cuda.synchronize()
+    assert rel_err(zd.cpu().numpy(), ref) < 1e-11
+
+
+@pytest.mark.parametrize("dim,k,level", [(2, 2, 4), (3, 3, 3), (3, 2, 4)])
+def test_vcycle_fp32(core, ctx, dim, k, level):
+    _announce("vcycle32", dim, k, level)
+    r = O.rhs_random(dim, k, level)
+    ref = O.vcycle(dim, k, level, r)
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float32)
+    rd = torch.tensor(r, device="cuda:0")
+    zd = torch.full_like(rd, float("nan"))
+    mg.vmult(zd, rd)
+    torch.cuda.synchronize()
+    assert rel_err(zd.cpu().numpy(), ref) < 1e-4
+
+
+def test_vcycle_symmetric(core, ctx):
+    """With reversed post-smoothing the V-cycle is a symmetric operator:
+    <V r, s> = <r, V s> (needed by CG; the reference's order is not)."""
+    _announce("vcycle-symmetry")
+    dim, k, level = 3, 3, 3
+    g = rng(11)
+    bm = boundary_mask(dim, k, level)
+    r = g.uniform(-1, 1, n_dofs(dim, k, level))
+    s = g.uniform(-1, 1, n_dofs(dim, k, level))
+    r[bm] = 0
+    s[bm] = 0
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+    out = []
+    for v in (r, s):
+        vd = torch.tensor(v, device="cuda:0")
+        zd = torch.empty_like(vd)
+        mg.vmult(zd, vd)
+        out.append(zd.cpu().numpy())
+    a, b = out[0] @ s, r @ out[1]
+    assert abs(a - b) < 1e-12 * max(abs(a), 1.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dim,k,level,rtol", [(2, 2, 5, 1e-10), (3, 3, 3, 1e-8),
+                                               (3, 3, 4, 1e-8), (3, 2, 4, 1e-8),
+                                               (3, 4, 3, 1e-8)])
+def test_pcg_against_oracle(core, ctx, dim, k, level, rtol, prec):
+    _announce("pcg", dim, k, level, prec)
+    b = O.rhs_random(dim, k, level)
+    x_ref, it_ref, rel_ref, _ = O.pcg(dim, k, level, b, rtol=1e-13, maxit=200)
+    _, it64, _, _ = O.pcg(dim, k, level, b, rtol=rtol)
+    mg = core.MultigridSolvers(ctx, dim, k, level,
+                               torch.float64 if prec == "fp64" else torch.float32)
+    bd = torch.tensor(b, device="cuda:0")
+    x, it, rel, hist = mg.solve_cg(bd, rtol=rtol, maxit=100)
+    x = x.cpu().numpy()
+    assert rel <= rtol
+    # true residual from the oracle operator
+    r = b - O.laplace_vmult(dim, k, level, x)
+    assert np.linalg.norm(r) <= 2 * rtol * np.linalg.norm(b)
+    # FP64 form takes the oracle's iteration count; the FP32 V-cycle at most one more
+    if prec == "fp64":
+        assert it == it64
+    else:
+        assert it <= it64 + 1
+    # error against the tightly converged FP64 oracle solution
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 100 * rtol
+
+
+def test_pcg_additive_2d(core, ctx):
+    """Config-1 shape at reduced size: 2D Q2, additive Schwarz, CG to 1e-10."""
+    _announce("pcg-additive-2d")
+    dim, k, level = 2, 2, 5
+    b = O.rhs_assembled(dim, k, level, O.RHS_CONST)
+    _, it_ref, _, _ = O.pcg(dim, k, level, b, rtol=1e-10, mg_variant=O.MG_ADDITIVE, relax=0.5)
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float64, 2, 0.5)
+    x, it, rel, _ = mg.solve_cg(torch.tensor(b, device="cuda:0"), rtol=1e-10)
+    assert rel <= 1e-10
+    assert it == it_ref
+
+
+def test_pcg_metric_config(core, ctx):
+    """3D Q3 on 64^3 cells (N = 193^3): FP32-V-cycle and FP64 solves both
+    reach 1e-8, their solutions agree, and the true FP64 residual of the
+    mixed-precision solution (GPU FP64 operator) is below 1e-8."""
+    dim, k, level = 3, 3, 6
+    _announce("pcg-metric", dim, k, level)
+    b = O.rhs_random(dim, k, level)
+    bd = torch.tensor(b, device="cuda:0")
+    sols = {}
+    for dt in (torch.float32, torch.float64):
+        mg = core.MultigridSolvers(ctx, dim, k, level, dt)
+        x, it, rel, hist = mg.solve_cg(bd, rtol=1e-8, maxit=50)
+        assert rel <= 1e-8
+        assert it <= 12
+        sols[dt] = (x.clone(), mg)
+        print("precision", dt, "iterations", it, "relres", rel, flush=True)
+    x32, mg32 = sols[torch.float32]
+    x64, _ = sols[torch.float64]
+    r = torch.empty_like(bd)
+    mg32.do_matvec(r, x32)
+    r = bd - r
+    assert (r.norm() / bd.norm()).item() < 1e-8
+    d = ((x32 - x64).norm() / x64.norm()).item()
+    assert d < 1e-6
+    # GPU operator against the oracle at full size (one matvec, FP64)
+    y_ref = O.laplace_vmult(dim, k, level, b)
+    y = torch.empty_like(bd)
+    mg32.do_matvec(y, bd)
+    assert rel_err(y.cpu().numpy(), y_ref) < 1e-12
