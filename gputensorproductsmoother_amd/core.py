@@ -34,6 +34,18 @@ def _prec(dtype) -> int:
     raise ValueError(f"unsupported precision {dtype!r}")
 
 
+def default_relaxation(dim: int, schedule: int) -> float:
+    """Damping used when none is given: 1 for the multiplicative schedules
+    (relaxation = 1 at every call site of the reference,
+    include/solver.cuh:1719-1737); for the additive schedule 0.4 in 2D and
+    0.2 in 3D, since every cell-interior DoF lies in 2^dim overlapping
+    vertex-patch interiors (the CG iteration counts behind this choice are
+    recorded with the additive tests)."""
+    if schedule == L.GTPS_MG_ADDITIVE:
+        return 0.4 if dim == 2 else 0.2
+    return 1.0
+
+
 class Context:
     """Device + HIP stream shared by all objects built on it."""
 
@@ -274,9 +286,12 @@ class MultigridSolvers:
 
     def __init__(self, ctx: Context, dim: int, fe_degree: int, max_level: int,
                  vcycle_dtype=FP32, schedule: int = L.GTPS_MG_SYMMETRIC,
-                 relaxation: float = 1.0):
+                 relaxation: float | None = None):
         self.ctx, self.dim, self.fe_degree, self.max_level = ctx, dim, fe_degree, max_level
         self.prec = _prec(vcycle_dtype)
+        if relaxation is None:
+            relaxation = default_relaxation(dim, schedule)
+        self.relaxation = relaxation
         h = ctypes.c_void_p()
         L.check(L.load().gtps_mg_create(ctx.handle, dim, fe_degree, max_level, self.prec,
                                         schedule, relaxation, ctypes.byref(h)), "mg_create")
@@ -309,6 +324,28 @@ class MultigridSolvers:
         if rc != L.GTPS_OK and (raise_on_failure or rc not in (L.GTPS_ENOTCONV,)):
             L.check(rc, "pcg_solve")
         return x, it.value, rel.value, list(hist[: it.value + 1])
+
+    def set_profiling(self, groups=("smooth_fine",)):
+        """Enable HIP-event stage timers for the named groups (see
+        include/gtps.h GTPS_PROF_*); an empty sequence disables them."""
+        mask = 0
+        for g in groups:
+            mask |= 1 << L.PROF_GROUPS[g]
+        L.check(L.load().gtps_mg_set_profiling(self._h, mask), "set_profiling")
+
+    def profile(self) -> dict:
+        """{group: (intervals, total_ms)} since the last reset."""
+        out = {}
+        for name, g in L.PROF_GROUPS.items():
+            c = ctypes.c_longlong()
+            ms = ctypes.c_double()
+            L.check(L.load().gtps_mg_profile(self._h, g, ctypes.byref(c), ctypes.byref(ms)),
+                    "profile")
+            out[name] = (c.value, ms.value)
+        return out
+
+    def profile_reset(self):
+        L.check(L.load().gtps_mg_profile_reset(self._h), "profile_reset")
 
     def __del__(self):
         try:

@@ -111,10 +111,18 @@ namespace
     return lvl.release();
   }
 
+  // brackets every colour launch of a sweep when a timer is attached
+  struct LaunchTimer
+  {
+    virtual long tic_launch()        = 0;
+    virtual void toc_launch(long e)  = 0;
+    virtual ~LaunchTimer()           = default;
+  };
+
   // one smoothing sweep; `resid` is level-sized scratch for the additive form
   void
   sweep(const gtps_level *lvl, void *u, const void *b, int variant,
-        double relax, void *resid)
+        double relax, void *resid, LaunchTimer *timer = nullptr)
   {
     const int ncol = 1 << lvl->dim;
     if (variant == GTPS_ADDITIVE)
@@ -122,18 +130,26 @@ namespace
         GTPS_REQUIRE(resid != nullptr, "additive sweep needs a residual buffer");
         launch_laplace(lvl, resid, u, b, false);
         for (int c = 0; c < ncol; ++c)
-          launch_patch_colour(lvl, u, resid, c, 1, relax);
+          {
+            const long e = timer ? timer->tic_launch() : -1;
+            launch_patch_colour(lvl, u, resid, c, 1, relax);
+            if (timer)
+              timer->toc_launch(e);
+          }
         return;
       }
     for (int cc = 0; cc < ncol; ++cc)
       {
-        const int c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
+        const int  c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
+        const long e = timer ? timer->tic_launch() : -1;
         launch_patch_colour(lvl, u, b, c, 0, relax);
+        if (timer)
+          timer->toc_launch(e);
       }
   }
 } // namespace
 
-struct gtps_mg
+struct gtps_mg : LaunchTimer
 {
   gtps_ctx                  *ctx = nullptr;
   int                        dim = 3, k = 1, maxl = 1, prec = GTPS_FP32;
@@ -146,6 +162,70 @@ struct gtps_mg
   std::vector<void *>        d, u, t;  // level vectors in prec
   double                    *r = nullptr, *p = nullptr, *q = nullptr;
   hipEvent_t                 ev = nullptr;
+
+  // stage timers (HIP events on the context stream)
+  int                        prof_mask = 0;
+  std::vector<hipEvent_t>    ev_pool;
+  size_t                     ev_used = 0;
+  std::vector<std::pair<int, size_t>> pending;  // (group, index of start event)
+  long long                  prof_count[GTPS_PROF_N_GROUPS] = {};
+  double                     prof_ms[GTPS_PROF_N_GROUPS]    = {};
+
+  hipEvent_t
+  next_event()
+  {
+    if (ev_used == ev_pool.size())
+      {
+        hipEvent_t e;
+        GTPS_HIP(hipEventCreate(&e));
+        ev_pool.push_back(e);
+      }
+    return ev_pool[ev_used++];
+  }
+  // returns the start-event index, or -1 when the group is disabled
+  long
+  tic(int group)
+  {
+    if (!(prof_mask & (1 << group)))
+      return -1;
+    const size_t i = ev_used;
+    GTPS_HIP(hipEventRecord(next_event(), ctx->stream));
+    return (long)i;
+  }
+  void
+  toc(int group, long start)
+  {
+    if (start < 0)
+      return;
+    GTPS_HIP(hipEventRecord(next_event(), ctx->stream));
+    pending.emplace_back(group, (size_t)start);
+  }
+  // after the stream is synchronised: fold pending intervals into totals
+  void
+  collect()
+  {
+    for (const auto &pr : pending)
+      {
+        float ms = 0.f;
+        GTPS_HIP(hipEventElapsedTime(&ms, ev_pool[pr.second],
+                                     ev_pool[pr.second + 1]));
+        prof_count[pr.first] += 1;
+        prof_ms[pr.first] += ms;
+      }
+    pending.clear();
+    ev_used = 0;
+  }
+
+  long
+  tic_launch() override
+  {
+    return tic(GTPS_PROF_SMOOTH_FINE);
+  }
+  void
+  toc_launch(long e) override
+  {
+    toc(GTPS_PROF_SMOOTH_FINE, e);
+  }
 
   int
   pre_variant() const
@@ -172,13 +252,27 @@ struct gtps_mg
         sweep(L, ul, dl, GTPS_MULT_FWD, 1.0, nullptr);
         return;
       }
-    sweep(L, ul, dl, pre_variant(), relax, t[l]);
+    // finest level: every colour launch is timed on its own; coarser
+    // levels: whole sweeps
+    const bool   fine = l == maxl;
+    LaunchTimer *lt   = fine ? this : nullptr;
+    long         e    = fine ? -1 : tic(GTPS_PROF_SMOOTH_COARSE);
+    sweep(L, ul, dl, pre_variant(), relax, t[l], lt);
+    toc(GTPS_PROF_SMOOTH_COARSE, e);
+    e = fine ? tic(GTPS_PROF_RESIDUAL_FINE) : -1;
     launch_laplace(L, t[l], ul, dl, false);  // t = d - A u
+    toc(GTPS_PROF_RESIDUAL_FINE, e);
     launch_fill(ctx->stream, lv[l - 1]->N, prec, d[l - 1], 0.0);
+    e = tic(GTPS_PROF_TRANSFER);
     launch_restrict_add(tr, l, d[l - 1], t[l]);
+    toc(GTPS_PROF_TRANSFER, e);
     vcycle(l - 1, d[l - 1], u[l - 1]);
+    e = tic(GTPS_PROF_TRANSFER);
     launch_prolongate_add(tr, l, ul, u[l - 1]);
-    sweep(L, ul, dl, post_variant(), relax, t[l]);
+    toc(GTPS_PROF_TRANSFER, e);
+    e = fine ? -1 : tic(GTPS_PROF_SMOOTH_COARSE);
+    sweep(L, ul, dl, post_variant(), relax, t[l], lt);
+    toc(GTPS_PROF_SMOOTH_COARSE, e);
   }
 
   // z = V(r) in FP64, r and z finest-level vectors
@@ -220,6 +314,8 @@ struct gtps_mg
     dfree(q);
     if (ev)
       (void)hipEventDestroy(ev);
+    for (hipEvent_t e : ev_pool)
+      (void)hipEventDestroy(e);
   }
 };
 
@@ -705,7 +801,52 @@ extern "C"
       GTPS_REQUIRE(mg && z && r, "NULL argument");
       GTPS_REQUIRE((const void *)z != (const void *)r, "z and r must differ");
       set_device(mg->ctx);
+      const long e = mg->tic(GTPS_PROF_VCYCLE);
       mg->precondition(r, z, true);
+      mg->toc(GTPS_PROF_VCYCLE, e);
+    });
+  }
+
+  int
+  gtps_mg_set_profiling(gtps_mg *mg, int mask)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      GTPS_REQUIRE(mask >= 0 && mask < (1 << GTPS_PROF_N_GROUPS), "bad mask");
+      mg->prof_mask = mask;
+    });
+  }
+
+  int
+  gtps_mg_profile(gtps_mg *mg, int group, long long *count, double *total_ms)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      GTPS_REQUIRE(group >= 0 && group < GTPS_PROF_N_GROUPS, "bad group");
+      set_device(mg->ctx);
+      GTPS_HIP(hipStreamSynchronize(mg->ctx->stream));
+      mg->collect();
+      if (count)
+        *count = mg->prof_count[group];
+      if (total_ms)
+        *total_ms = mg->prof_ms[group];
+    });
+  }
+
+  int
+  gtps_mg_profile_reset(gtps_mg *mg)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      set_device(mg->ctx);
+      GTPS_HIP(hipStreamSynchronize(mg->ctx->stream));
+      mg->pending.clear();
+      mg->ev_used = 0;
+      for (int g = 0; g < GTPS_PROF_N_GROUPS; ++g)
+        {
+          mg->prof_count[g] = 0;
+          mg->prof_ms[g]    = 0.0;
+        }
     });
   }
 
@@ -754,10 +895,12 @@ extern "C"
       };
       // z = V(r), rz = r.z, beta, p = z + beta p
       auto precondition_and_direction = [&](bool first) {
+        const long e = mg->tic(GTPS_PROF_VCYCLE);
         if (f32)
           mg->vcycle(mg->maxl, r32, z32);
         else
           mg->vcycle(mg->maxl, mg->r, z64);
+        mg->toc(GTPS_PROF_VCYCLE, e);
         launch_cg_scalar(s, sc, cg::OP_SHIFT);
         const int nb = launch_dot_partials(s, N, mg->r, z64, z32, pt, cap);
         launch_finalize_sum(s, pt, nb, sc, cg::RZ);
@@ -794,7 +937,9 @@ extern "C"
       while (it_done < maxit)
         {
           // q = A p with fused p.q, alpha = rz / pq
+          const long e  = mg->tic(GTPS_PROF_CG_MATVEC);
           const int nbq = launch_laplace(mg->fine64, mg->q, mg->p, nullptr, true);
+          mg->toc(GTPS_PROF_CG_MATVEC, e);
           launch_finalize_sum(s, pt, nbq, sc, cg::PQ);
           launch_cg_scalar(s, sc, cg::OP_ALPHA);
           const int nbr =
@@ -821,6 +966,7 @@ extern "C"
         }
       rel = rn / r0;
       GTPS_HIP(hipStreamSynchronize(s));
+      mg->collect();
     });
     if (iters)
       *iters = it_done;

@@ -179,6 +179,32 @@ int gtps_mg_pcg_solve(gtps_mg *mg, double *x, const double *b, double rtol,
  * MultigridSolvers::do_matvec (include/solver.cuh:1934-1938). */
 int gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src);
 
+/* ---------------------------------------------------------- profiling */
+/* Stage timers with HIP events on the context stream, the counterpart of the
+ * reference's per-level timing buckets (MultigridSolvers::print_wall_times,
+ * include/solver.cuh:1747-1771, filled at 1957-1999) without its per-stage
+ * device synchronisations.  `mask` enables groups (bit g = group g):
+ *   GTPS_PROF_SMOOTH_FINE   one patch-kernel (colour) launch, finest level
+ *   GTPS_PROF_SMOOTH_COARSE smoothing sweeps on the coarser levels
+ *   GTPS_PROF_RESIDUAL_FINE t = d - A u on the finest level
+ *   GTPS_PROF_TRANSFER      restriction and prolongation, all levels
+ *   GTPS_PROF_CG_MATVEC     the FP64 q = A p of each CG iteration
+ *   GTPS_PROF_VCYCLE        one whole V-cycle
+ * gtps_mg_profile synchronises the stream and returns, for one group, the
+ * number of timed intervals and their total in milliseconds since the last
+ * reset. */
+#define GTPS_PROF_SMOOTH_FINE 0
+#define GTPS_PROF_SMOOTH_COARSE 1
+#define GTPS_PROF_RESIDUAL_FINE 2
+#define GTPS_PROF_TRANSFER 3
+#define GTPS_PROF_CG_MATVEC 4
+#define GTPS_PROF_VCYCLE 5
+#define GTPS_PROF_N_GROUPS 6
+int gtps_mg_set_profiling(gtps_mg *mg, int mask);
+int gtps_mg_profile(gtps_mg *mg, int group, long long *count,
+                    double *total_ms);
+int gtps_mg_profile_reset(gtps_mg *mg);
+
 #ifdef __cplusplus
 }
 #endif
