@@ -1,0 +1,47 @@
+"""The header-only C++ facade compiles and links against libgtps.so; on a
+CPU-only host the program reports the missing device through gtps::Error."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def exe(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    out = str(tmp_path_factory.mktemp("facade") / "facade_example")
+    libdir = os.path.join(ROOT, "gputensorproductsmoother_amd")
+    cmd = [gxx, "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+           "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "cpp", "facade_example.cpp"),
+           "-L", libdir, "-lgtps", "-L", "/opt/rocm/lib", "-lamdhip64",
+           f"-Wl,-rpath,{libdir}", "-Wl,-rpath,/opt/rocm/lib", "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return out
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+def test_facade_runs_or_fails_cleanly(exe):
+    if _has_gpu():
+        pytest.skip("covered by the gpu-marked run")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 3, (p.returncode, p.stdout, p.stderr)
+    assert "gtps::Error" in p.stdout
+
+
+@pytest.mark.gpu
+def test_facade_solves_on_gpu(exe):
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, (p.stdout, p.stderr)
+    assert "iterations" in p.stdout
