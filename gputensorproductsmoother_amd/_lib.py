@@ -73,6 +73,7 @@ SIGNATURES = {
     "gtps_mg_vcycle": (_i, [_vp, _vp, _vp]),
     "gtps_mg_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _pd]),
     "gtps_mg_matvec": (_i, [_vp, _vp, _vp]),
+    "gtps_mg_set_graph": (_i, [_vp, _i]),
     "gtps_mg_set_profiling": (_i, [_vp, _i]),
     "gtps_mg_profile": (_i, [_vp, _i, ctypes.POINTER(_ll), _pd]),
     "gtps_mg_profile_reset": (_i, [_vp]),

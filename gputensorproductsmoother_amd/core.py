@@ -325,6 +325,11 @@ class MultigridSolvers:
             L.check(rc, "pcg_solve")
         return x, it.value, rel.value, list(hist[: it.value + 1])
 
+    def set_graph(self, enable: bool = True):
+        """Replay the per-iteration V-cycle sequence as one HIP graph
+        (default) or launch its kernels one by one."""
+        L.check(L.load().gtps_mg_set_graph(self._h, 1 if enable else 0), "set_graph")
+
     def set_profiling(self, groups=("smooth_fine",)):
         """Enable HIP-event stage timers for the named groups (see
         include/gtps.h GTPS_PROF_*); an empty sequence disables them."""

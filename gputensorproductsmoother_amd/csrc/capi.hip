@@ -119,12 +119,19 @@ namespace
     virtual ~LaunchTimer()           = default;
   };
 
-  // one smoothing sweep; `resid` is level-sized scratch for the additive form
+  // one smoothing sweep; `resid` is level-sized scratch for the additive
+  // form.  from_zero: u is taken as zero on entry (vmult semantics); the
+  // forward multiplicative sweep then starts with the colour-0 pass that
+  // writes the whole mesh (interiors solved, skeleton zeroed) instead of a
+  // separate fill.
   void
   sweep(const gtps_level *lvl, void *u, const void *b, int variant,
-        double relax, void *resid, LaunchTimer *timer = nullptr)
+        double relax, void *resid, LaunchTimer *timer = nullptr,
+        bool from_zero = false)
   {
     const int ncol = 1 << lvl->dim;
+    if (from_zero && variant != GTPS_MULT_FWD)
+      launch_fill(lvl->ctx->stream, lvl->N, lvl->prec, u, 0.0);
     if (variant == GTPS_ADDITIVE)
       {
         GTPS_REQUIRE(resid != nullptr, "additive sweep needs a residual buffer");
@@ -142,7 +149,9 @@ namespace
       {
         const int  c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
         const long e = timer ? timer->tic_launch() : -1;
-        launch_patch_colour(lvl, u, b, c, 0, relax);
+        launch_patch_colour(lvl, u, b, c,
+                            (from_zero && variant == GTPS_MULT_FWD && cc == 0) ? 2 : 0,
+                            relax);
         if (timer)
           timer->toc_launch(e);
       }
@@ -162,6 +171,12 @@ struct gtps_mg : LaunchTimer
   std::vector<void *>        d, u, t;  // level vectors in prec
   double                    *r = nullptr, *p = nullptr, *q = nullptr;
   hipEvent_t                 ev = nullptr;
+
+  // the per-iteration preconditioning sequence (V-cycle, r.z, beta,
+  // p update) replayed as one HIP graph; captured lazily on a private stream
+  bool                       use_graph  = true;
+  hipStream_t                cap_stream = nullptr;
+  hipGraphExec_t             dir_exec   = nullptr;
 
   // stage timers (HIP events on the context stream)
   int                        prof_mask = 0;
@@ -245,11 +260,10 @@ struct gtps_mg : LaunchTimer
   vcycle(int l, const void *dl, void *ul)
   {
     const gtps_level *L = lv[l];
-    launch_fill(ctx->stream, L->N, prec, ul, 0.0);
     if (l == 1)
       {
         // one vertex patch covers the domain: exact solve
-        sweep(L, ul, dl, GTPS_MULT_FWD, 1.0, nullptr);
+        sweep(L, ul, dl, GTPS_MULT_FWD, 1.0, nullptr, nullptr, true);
         return;
       }
     // finest level: every colour launch is timed on its own; coarser
@@ -257,7 +271,7 @@ struct gtps_mg : LaunchTimer
     const bool   fine = l == maxl;
     LaunchTimer *lt   = fine ? this : nullptr;
     long         e    = fine ? -1 : tic(GTPS_PROF_SMOOTH_COARSE);
-    sweep(L, ul, dl, pre_variant(), relax, t[l], lt);
+    sweep(L, ul, dl, pre_variant(), relax, t[l], lt, true);
     toc(GTPS_PROF_SMOOTH_COARSE, e);
     e = fine ? tic(GTPS_PROF_RESIDUAL_FINE) : -1;
     launch_laplace(L, t[l], ul, dl, false);  // t = d - A u
@@ -273,6 +287,80 @@ struct gtps_mg : LaunchTimer
     e = fine ? -1 : tic(GTPS_PROF_SMOOTH_COARSE);
     sweep(L, ul, dl, post_variant(), relax, t[l], lt);
     toc(GTPS_PROF_SMOOTH_COARSE, e);
+  }
+
+  // z = V(r), rz_old = rz, rz = r.z, beta = rz / rz_old, p = z + beta p;
+  // every buffer is owned by the hierarchy, so the sequence can be captured
+  void
+  enqueue_direction()
+  {
+    hipStream_t     s   = ctx->stream;
+    double         *sc  = ctx->ws.scalars;
+    double         *pt  = ctx->ws.partials;
+    const long long N   = fine64->N;
+    const bool      f32 = prec == GTPS_FP32;
+    float          *z32 = f32 ? (float *)u[maxl] : nullptr;
+    double         *z64 = f32 ? nullptr : (double *)u[maxl];
+    const long      e   = tic(GTPS_PROF_VCYCLE);
+    if (f32)
+      vcycle(maxl, d[maxl], z32);
+    else
+      vcycle(maxl, r, z64);
+    toc(GTPS_PROF_VCYCLE, e);
+    launch_cg_scalar(s, sc, cg::OP_SHIFT);
+    const int nb = launch_dot_partials(s, N, r, z64, z32, pt, ctx->ws.capacity);
+    launch_finalize_sum(s, pt, nb, sc, cg::RZ);
+    launch_cg_scalar(s, sc, cg::OP_BETA);
+    launch_cg_update_p(s, N, p, z64, z32, sc);
+  }
+
+  void
+  capture_direction()
+  {
+    if (!cap_stream)
+      GTPS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+    hipStream_t saved = ctx->stream;
+    ctx->stream       = cap_stream;
+    hipGraph_t graph  = nullptr;
+    try
+      {
+        GTPS_HIP(hipStreamBeginCapture(cap_stream,
+                                       hipStreamCaptureModeThreadLocal));
+        enqueue_direction();
+        GTPS_HIP(hipStreamEndCapture(cap_stream, &graph));
+      }
+    catch (...)
+      {
+        ctx->stream = saved;
+        if (!graph)
+          {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cap_stream, &g);
+            if (g)
+              (void)hipGraphDestroy(g);
+          }
+        else
+          (void)hipGraphDestroy(graph);
+        throw;
+      }
+    ctx->stream = saved;
+    hipError_t e = hipGraphInstantiate(&dir_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GTPS_HIP(e);
+  }
+
+  void
+  direction()
+  {
+    // stage timers need eager launches
+    if (!use_graph || prof_mask != 0)
+      {
+        enqueue_direction();
+        return;
+      }
+    if (!dir_exec)
+      capture_direction();
+    GTPS_HIP(hipGraphLaunch(dir_exec, ctx->stream));
   }
 
   // z = V(r) in FP64, r and z finest-level vectors
@@ -316,6 +404,10 @@ struct gtps_mg : LaunchTimer
       (void)hipEventDestroy(ev);
     for (hipEvent_t e : ev_pool)
       (void)hipEventDestroy(e);
+    if (dir_exec)
+      (void)hipGraphExecDestroy(dir_exec);
+    if (cap_stream)
+      (void)hipStreamDestroy(cap_stream);
   }
 };
 
@@ -522,8 +614,7 @@ extern "C"
       GTPS_REQUIRE(sm && u && b, "NULL argument");
       GTPS_REQUIRE(u != b, "u and b must differ");
       set_device(sm->lvl->ctx);
-      launch_fill(sm->lvl->ctx->stream, sm->lvl->N, sm->lvl->prec, u, 0.0);
-      sweep(sm->lvl, u, b, sm->variant, sm->relax, sm->resid);
+      sweep(sm->lvl, u, b, sm->variant, sm->relax, sm->resid, nullptr, true);
     });
   }
 
@@ -808,6 +899,15 @@ extern "C"
   }
 
   int
+  gtps_mg_set_graph(gtps_mg *mg, int enable)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      mg->use_graph = enable != 0;
+    });
+  }
+
+  int
   gtps_mg_set_profiling(gtps_mg *mg, int mask)
   {
     return guard([&] {
@@ -881,8 +981,6 @@ extern "C"
       const int      cap = ctx->ws.capacity;
       const bool     f32 = mg->prec == GTPS_FP32;
       float         *r32 = f32 ? (float *)mg->d[mg->maxl] : nullptr;
-      float         *z32 = f32 ? (float *)mg->u[mg->maxl] : nullptr;
-      double        *z64 = f32 ? nullptr : (double *)mg->u[mg->maxl];
 
       auto read_rr = [&]() {
         GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, sc + cg::RR, sizeof(double),
@@ -892,20 +990,6 @@ extern "C"
       auto wait_rr = [&]() {
         GTPS_HIP(hipEventSynchronize(mg->ev));
         return std::sqrt(ctx->ws.pinned[0]);
-      };
-      // z = V(r), rz = r.z, beta, p = z + beta p
-      auto precondition_and_direction = [&](bool first) {
-        const long e = mg->tic(GTPS_PROF_VCYCLE);
-        if (f32)
-          mg->vcycle(mg->maxl, r32, z32);
-        else
-          mg->vcycle(mg->maxl, mg->r, z64);
-        mg->toc(GTPS_PROF_VCYCLE, e);
-        launch_cg_scalar(s, sc, cg::OP_SHIFT);
-        const int nb = launch_dot_partials(s, N, mg->r, z64, z32, pt, cap);
-        launch_finalize_sum(s, pt, nb, sc, cg::RZ);
-        launch_cg_scalar(s, sc, first ? cg::OP_BETA0 : cg::OP_BETA);
-        launch_cg_update_p(s, N, mg->p, z64, z32, sc);
       };
 
       // x0 = lift(b), r0 = b - A x0
@@ -917,7 +1001,9 @@ extern "C"
       }
       if (f32)
         launch_convert_f64_f32(s, N, mg->r, r32);
+      // p = 0 and rz = 1 make the first direction update p = z + beta 0
       launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
+      launch_cg_scalar(s, sc, cg::OP_INIT);
       read_rr();
       const double r0 = wait_rr();
       if (history)
@@ -932,7 +1018,7 @@ extern "C"
           converged = true;
           return;
         }
-      precondition_and_direction(true);
+      mg->direction();
       double rn = r0;
       while (it_done < maxit)
         {
@@ -949,7 +1035,7 @@ extern "C"
           ++it_done;
           // enqueue the next preconditioning step while the host checks
           if (it_done < maxit)
-            precondition_and_direction(false);
+            mg->direction();
           rn = wait_rr();
           if (history)
             history[it_done] = rn;

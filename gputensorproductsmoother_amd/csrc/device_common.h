@@ -36,7 +36,10 @@ namespace gtps
     // cell (p = 0: vertex row), band[p*W + off + K] for off in [-K, K]
     static constexpr int bandM = E + NP * N;
     static constexpr int bandK = bandM + K * W;
-    static constexpr int size  = bandK + K * W;
+    // 1 / (l_i + l_j + l_k), [k][j][i], and 1 / (l_i + l_j), [j][i]
+    static constexpr int lamInv3 = bandK + K * W;
+    static constexpr int lamInv2 = lamInv3 + NI * NI * NI;
+    static constexpr int size    = lamInv2 + NI * NI;
   };
 
   // Column range of row r of a 2-cell patch matrix (block sparsity: rows

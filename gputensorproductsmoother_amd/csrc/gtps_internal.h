@@ -127,7 +127,9 @@ namespace gtps
                      const void *b, bool fused_dot);
 
   // smoother.hip: one colour pass. mode 0 multiplicative (src = b),
-  // mode 1 additive (src = global residual).
+  // mode 1 additive (src = global residual), mode 2 multiplicative on
+  // u = 0 that also zeroes the patch skeleton (replaces "u = 0" + colour 0
+  // because colour-0 patch boxes cover the mesh).
   void launch_patch_colour(const gtps_level *lvl, void *u, const void *src,
                            int colour, int mode, double relax);
 
@@ -173,7 +175,7 @@ namespace gtps
     // scalar ops
     constexpr int OP_ALPHA = 0;  // alpha = rz / pq
     constexpr int OP_BETA  = 1;  // beta = rz / rz_old
-    constexpr int OP_BETA0 = 2;  // beta = 0 (first iteration)
     constexpr int OP_SHIFT = 3;  // rz_old = rz
+    constexpr int OP_INIT  = 4;  // rz = 1 (first direction: p = 0 + z)
   } // namespace cg
 } // namespace gtps

@@ -31,22 +31,22 @@ namespace gtps
     template <>
     struct VT3<1>
     {
-      static constexpr int TCX = 48, TCY = 12, NT = 256;
+      static constexpr int TCX = 48, TCY = 16, NT = 512;
     };
     template <>
     struct VT3<2>
     {
-      static constexpr int TCX = 24, TCY = 6, NT = 256;
+      static constexpr int TCX = 24, TCY = 10, NT = 512;
     };
     template <>
     struct VT3<3>
     {
-      static constexpr int TCX = 16, TCY = 4, NT = 256;
+      static constexpr int TCX = 16, TCY = 8, NT = 512;
     };
     template <>
     struct VT3<4>
     {
-      static constexpr int TCX = 12, TCY = 3, NT = 256;
+      static constexpr int TCX = 12, TCY = 4, NT = 512;
     };
     template <>
     struct VT3<5>
@@ -83,6 +83,7 @@ namespace gtps
       constexpr int NCOL     = (NCOLS + NT - 1) / NT;
 
       __shared__ T      sA[NCOLS], sB[NCOLS], sC1[TY * WX], sC2[TY * WX];
+      __shared__ T      sX0[DOT ? NCOLS : 1];
       __shared__ double sred[NT / 64];
 
       const int       N1  = K * n + 1;
@@ -98,7 +99,11 @@ namespace gtps
       const T *bM = tab + TB::bandM;
       const T *bK = tab + TB::bandK;
 
-      T         w[NCOL][W];
+      // register window of 2K+1 z-planes per column (planes K(c-1)..K(c+1)
+      // for cell c) and, for K <= 4, the K planes of the next cell loaded one
+      // cell ahead (larger K would spill)
+      constexpr bool PF = K <= 4;
+      T         w[NCOL][W], nx[NCOL][PF ? K : 1];
       long long cidx[NCOL];
       bool      own[NCOL], bcol[NCOL];
 #pragma unroll
@@ -109,7 +114,7 @@ namespace gtps
           const int  gx    = gx0 + lx, gy = gy0 + ly;
           const bool valid = col < NCOLS && gx >= 0 && gx < N1 && gy >= 0 &&
                              gy < N1;
-          cidx[c] = valid ? gx + (long long)gy * N1 : 0;
+          cidx[c] = valid ? gx + (long long)gy * N1 : -1;
           own[c]  = valid && gx >= ox0 && gx < ox1 && gy >= oy0 && gy < oy1;
           bcol[c] = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1;
 #pragma unroll
@@ -118,8 +123,15 @@ namespace gtps
               const int gz = K * (cz0 - 1) + j;
               w[c][j] = (valid && gz >= 0 && gz < N1) ? x[cidx[c] + gz * P] : T(0);
             }
-          if (!valid)
-            cidx[c] = -1;
+          if constexpr (PF)
+            {
+#pragma unroll
+              for (int j = 0; j < K; ++j)
+                {
+                  const int gz = K * (cz0 + 1) + 1 + j;
+                  nx[c][j] = (valid && cz0 + 1 < cz1 && gz < N1) ? x[cidx[c] + gz * P] : T(0);
+                }
+            }
         }
 
       double dot = 0.0;
@@ -146,6 +158,8 @@ namespace gtps
                         }
                       sA[col] = a;
                       sB[col] = b;
+                      if (DOT)
+                        sX0[col] = w[c][K + p];
                       if (own[c] && (bcol[c] || gz == 0))
                         {
                           const long long g  = cidx[c] + zoff;
@@ -218,21 +232,38 @@ namespace gtps
                               const T         yv = RESID ? bv[g] - v : v;
                               y[g]               = yv;
                               if (DOT)
-                                dot += (double)x[g] * (double)yv;
+                                dot += (double)sX0[(ty + K) * WX + K * cxl + q + K] *
+                                       (double)yv;
                             }
                         }
                     }
                 }
               __syncthreads();
             }
-          // shift the window by one cell and load the next k planes
+          // shift the window by one cell, move the prefetched planes in and
+          // start loading the cell after next
 #pragma unroll
           for (int c = 0; c < NCOL; ++c)
             {
 #pragma unroll
               for (int j = 0; j <= K; ++j)
                 w[c][j] = w[c][j + K];
-              if (cz + 1 < cz1)
+              if constexpr (PF)
+                {
+#pragma unroll
+                  for (int j = 0; j < K; ++j)
+                    w[c][K + 1 + j] = nx[c][j];
+                  if (cz + 2 < cz1)
+                    {
+#pragma unroll
+                      for (int j = 0; j < K; ++j)
+                        {
+                          const int gz = K * (cz + 2) + 1 + j;
+                          nx[c][j] = (cidx[c] >= 0 && gz < N1) ? x[cidx[c] + gz * P] : T(0);
+                        }
+                    }
+                }
+              else if (cz + 1 < cz1)
                 {
 #pragma unroll
                   for (int j = K + 1; j < W; ++j)
@@ -397,8 +428,8 @@ namespace gtps
           using C       = VT3<K>;
           const int tx  = (n + C::TCX - 1) / C::TCX;
           const int ty  = (n + C::TCY - 1) / C::TCY;
-          // z-chunks: aim for >= 1024 work-groups on the 256 CUs
-          int zc = (int)(((long long)n * tx * ty + 1023) / 1024);
+          // z-chunks: about two 512-thread work-groups per CU
+          int zc = (int)(((long long)n * tx * ty + 511) / 512);
           zc     = std::max(1, std::min(zc, n));
           const int tz = (n + zc - 1) / zc;
           const dim3 grid(tx, ty, tz);

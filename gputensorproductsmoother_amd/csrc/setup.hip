@@ -323,6 +323,15 @@ namespace gtps
                 tab[off + p * W + o + K] = v;
               }
         }
+      const int NI = 2 * K - 1;
+      for (int c = 0; c < NI; ++c)
+        for (int b2 = 0; b2 < NI; ++b2)
+          {
+            tab[TB::lamInv2 + c * NI + b2] = 1.0 / (b.lam[c] + b.lam[b2]);
+            for (int a = 0; a < NI; ++a)
+              tab[TB::lamInv3 + (c * NI + b2) * NI + a] =
+                1.0 / (b.lam[c] + b.lam[b2] + b.lam[a]);
+          }
       return tab;
     }
 

@@ -553,6 +553,312 @@ namespace gtps
         }
     }
 
+    // ---------------------------------------------------------------------
+    // Wave kernel (3D, small k): one wavefront per work-group owns NPW
+    // x-adjacent patches of one colour.  Phase A: lane (patch, x) streams its
+    // (y,z) plane of the skeleton from HBM (lanes of a wave read contiguous
+    // x-runs) and applies the z- and y-rows of the patch matrices in
+    // registers, leaving C1 = My Mz u and C2 = Ky Mz u + My Kz u on the
+    // interior (y,z) rows.  Phase B: lane (patch, interior (z,y) line)
+    // applies the x-rows, forms r = b - h^(d-2)(Kx C1 + Mx C2) and S^T along
+    // x.  Phase C: lane (patch, x eigen-index) holds its (z,y) plane: S^T y,
+    // S^T z, 1/(h^(d-2) sum lambda), S z, S y.  Phase D: S along x and the
+    // interior store.  Phases exchange through LDS (C1/C2 rows, later the
+    // transformed interior in place).
+    //   MODE 0: multiplicative (src = b)
+    //   MODE 1: additive, u_I += w A_II^{-1} r_I (src = global residual r)
+    //   MODE 2: multiplicative on u = 0 (src = b): u is not read; the patch
+    //           skeleton is written with zeros, so a colour whose boxes cover
+    //           the mesh (colour 0) replaces "u = 0; first colour".
+    template <int K>
+    struct SW3
+    {
+      static constexpr int NP  = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr int NPW = 64 / NP;
+      // per-patch LDS stride: C1 and C2 rows (2 NI^2 rows of NP), padded so
+      // that phase-A lanes (patch, x) hit distinct banks
+      static constexpr int RAW = 2 * NI * NI * NP;
+      static constexpr int PER = RAW + ((NP - RAW % 32) + 32) % 32;
+    };
+
+    template <int K, typename T>
+    constexpr bool
+    use_wave_kernel()
+    {
+      // register budget: the phase-A accumulators (2 NI^2) and the plane
+      // column must stay well below 128 VGPRs
+      return K <= 2 || (K == 3 && sizeof(T) == 4);
+    }
+
+    template <int K, typename T, int MODE>
+    __global__ void __launch_bounds__(64)
+    patch3d_wave_kernel(T *__restrict__ u, const T *__restrict__ src,
+                        const T *__restrict__ tab, const int n,
+                        const int colour, const T scale, const T inv_scale,
+                        const T relax)
+    {
+      using TB          = Tab<K>;
+      using C           = SW3<K>;
+      constexpr int NP  = C::NP, NI = C::NI, NPW = C::NPW, PER = C::PER;
+      extern __shared__ double smem_raw[];
+      T *sC = reinterpret_cast<T *>(smem_raw);
+      __shared__ long long sfirst[NPW];
+
+      const int       N1   = K * n + 1;
+      const long long P    = (long long)N1 * N1;
+      const int       lane = threadIdx.x;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
+                npz = n_patches_1d(n, pz);
+      if (lane < NPW)
+        {
+          const int q = blockIdx.x * NPW + lane;
+          if (q < npx * npy * npz)
+            {
+              const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+              sfirst[lane] = (long long)K * (2 * qx + px) +
+                             (long long)K * (2 * qy + py) * N1 +
+                             (long long)K * (2 * qz + pz) * P;
+            }
+          else
+            sfirst[lane] = -1;
+        }
+      __syncthreads();
+      const T *pM   = tab + TB::patchM;
+      const T *pK   = tab + TB::patchK;
+      const T *S    = tab + TB::S;
+      const T *linv = tab + TB::lamInv3;
+
+      if (MODE == 2)
+        {
+          // zero the skeleton of every patch box
+          for (int t = lane; t < NPW * NP * NP * NP; t += 64)
+            {
+              const int       lp = t % NPW, r = t / NPW;
+              const long long f  = sfirst[lp];
+              const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
+              const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
+                                 iz > 0 && iz < NP - 1;
+              if (f >= 0 && !inner)
+                u[f + ix + (long long)iy * N1 + iz * P] = T(0);
+            }
+        }
+      if (MODE == 0)
+        {
+          // phase A
+          const int       lp = lane / NP, ix = lane - lp * NP;
+          const long long f  = lp < NPW ? sfirst[lp] : -1;
+          if (f >= 0)
+            {
+              const bool xface = ix == 0 || ix == NP - 1;
+              T          c1[NI][NI], c2[NI][NI];
+#pragma unroll
+              for (int a = 0; a < NI; ++a)
+#pragma unroll
+                for (int b = 0; b < NI; ++b)
+                  {
+                    c1[a][b] = 0;
+                    c2[a][b] = 0;
+                  }
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                {
+                  const bool yface = iy == 0 || iy == NP - 1;
+                  T          v[NP];
+#pragma unroll
+                  for (int iz = 0; iz < NP; ++iz)
+                    {
+                      const bool      zface = iz == 0 || iz == NP - 1;
+                      const long long g = f + ix + (long long)iy * N1 + iz * P;
+                      if (yface || zface)
+                        v[iz] = u[g];
+                      else
+                        v[iz] = xface ? u[g] : T(0);
+                    }
+                  T mz[NI], kz[NI];
+#pragma unroll
+                  for (int zi = 0; zi < NI; ++zi)
+                    {
+                      const int row = zi + 1;
+                      T         m = 0, kk = 0;
+#pragma unroll
+                      for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                        {
+                          m += pM[row * NP + c] * v[c];
+                          kk += pK[row * NP + c] * v[c];
+                        }
+                      mz[zi] = m;
+                      kz[zi] = kk;
+                    }
+#pragma unroll
+                  for (int yi = 0; yi < NI; ++yi)
+                    {
+                      const int row = yi + 1;
+                      if (iy < prow_lo<K>(row) || iy > prow_hi<K>(row))
+                        continue;
+                      const T my = pM[row * NP + iy], ky = pK[row * NP + iy];
+#pragma unroll
+                      for (int zi = 0; zi < NI; ++zi)
+                        {
+                          c1[zi][yi] += my * mz[zi];
+                          c2[zi][yi] += ky * mz[zi] + my * kz[zi];
+                        }
+                    }
+                }
+              T *base = sC + lp * PER + ix;
+#pragma unroll
+              for (int zi = 0; zi < NI; ++zi)
+#pragma unroll
+                for (int yi = 0; yi < NI; ++yi)
+                  {
+                    base[(zi * NI + yi) * NP]             = c1[zi][yi];
+                    base[((NI + zi) * NI + yi) * NP]      = c2[zi][yi];
+                  }
+            }
+          __syncthreads();
+        }
+      // phase B: residual on interior x-lines, S^T along x (in place over
+      // the C1 row of the line)
+      for (int t = lane; t < NPW * NI * NI; t += 64)
+        {
+          const int       lp = t % NPW, r = t / NPW;
+          const long long f  = sfirst[lp];
+          if (f < 0)
+            continue;
+          const int       zi = r / NI, yi = r - zi * NI;
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+          T               res[NI];
+          if (MODE == 0)
+            {
+              T c1[NP], c2[NP];
+#pragma unroll
+              for (int ix = 0; ix < NP; ++ix)
+                {
+                  c1[ix] = sC[lp * PER + r * NP + ix];
+                  c2[ix] = sC[lp * PER + (NI * NI + r) * NP + ix];
+                }
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                {
+                  const int row = xi + 1;
+                  T         v   = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      v += pK[row * NP + c] * c1[c];
+                      v += pM[row * NP + c] * c2[c];
+                    }
+                  res[xi] = src[g0 + xi] - scale * v;
+                }
+            }
+          else
+            {
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                res[xi] = src[g0 + xi];
+            }
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * res[i];
+              sC[lp * PER + r * NP + j] = s;
+            }
+        }
+      __syncthreads();
+      // phase C: per x eigen-index, the (z,y) plane through y and z
+      for (int t = lane; t < NPW * NI; t += 64)
+        {
+          const int lp = t % NPW, xi = t / NPW;
+          if (sfirst[lp] < 0)
+            continue;
+          T *base = sC + lp * PER + xi;
+          T  V[NI][NI], W[NI][NI];
+#pragma unroll
+          for (int zi = 0; zi < NI; ++zi)
+#pragma unroll
+            for (int yi = 0; yi < NI; ++yi)
+              V[zi][yi] = base[(zi * NI + yi) * NP];
+          // S^T along y
+#pragma unroll
+          for (int zi = 0; zi < NI; ++zi)
+#pragma unroll
+            for (int j = 0; j < NI; ++j)
+              {
+                T s = 0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+                  s += S[i * NI + j] * V[zi][i];
+                W[zi][j] = s;
+              }
+          // S^T along z, eigenvalue scaling
+#pragma unroll
+          for (int jy = 0; jy < NI; ++jy)
+#pragma unroll
+            for (int jz = 0; jz < NI; ++jz)
+              {
+                T s = 0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+                  s += S[i * NI + jz] * W[i][jy];
+                V[jz][jy] = s * (linv[(jz * NI + jy) * NI + xi] * inv_scale);
+              }
+          // S along z
+#pragma unroll
+          for (int jy = 0; jy < NI; ++jy)
+#pragma unroll
+            for (int iz = 0; iz < NI; ++iz)
+              {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < NI; ++j)
+                  s += S[iz * NI + j] * V[j][jy];
+                W[iz][jy] = s;
+              }
+          // S along y
+#pragma unroll
+          for (int iz = 0; iz < NI; ++iz)
+#pragma unroll
+            for (int iy = 0; iy < NI; ++iy)
+              {
+                T s = 0;
+#pragma unroll
+                for (int j = 0; j < NI; ++j)
+                  s += S[iy * NI + j] * W[iz][j];
+                base[(iz * NI + iy) * NP] = s;
+              }
+        }
+      __syncthreads();
+      // phase D: S along x, store the interior
+      for (int t = lane; t < NPW * NI * NI; t += 64)
+        {
+          const int       lp = t % NPW, r = t / NPW;
+          const long long f  = sfirst[lp];
+          if (f < 0)
+            continue;
+          const int zi = r / NI, yi = r - zi * NI;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = sC[lp * PER + r * NP + j];
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              if (MODE == 1)
+                u[g0 + i] += relax * s;
+              else
+                u[g0 + i] = relax * s;
+            }
+        }
+    }
+
     template <typename KernelPtr>
     void
     ensure_lds(KernelPtr kern, size_t bytes)
@@ -579,6 +885,33 @@ namespace gtps
         return;
       T       *up = reinterpret_cast<T *>(u);
       const T *sp = reinterpret_cast<const T *>(src);
+      if constexpr (use_wave_kernel<K, T>())
+        if (lvl->dim == 3)
+        {
+          using C            = SW3<K>;
+          const long long nb = (np + C::NPW - 1) / C::NPW;
+          const size_t    lds = sizeof(T) * C::NPW * C::PER;
+          const T scale = (T)lvl->scale, inv = (T)(1.0 / lvl->scale);
+#define GTPS_SW(M)                                                            \
+  hipLaunchKernelGGL((patch3d_wave_kernel<K, T, M>), dim3((unsigned)nb),     \
+                     dim3(64), lds, s, up, sp, tab, n, colour, scale, inv,    \
+                     (T)relax)
+          if (mode == 0)
+            GTPS_SW(0);
+          else if (mode == 1)
+            GTPS_SW(1);
+          else
+            GTPS_SW(2);
+#undef GTPS_SW
+          GTPS_LAUNCH_CHECK();
+          return;
+        }
+      if (mode == 2)
+        {
+          // no zero-skeleton form: clear u, then a multiplicative pass
+          launch_fill(s, lvl->N, lvl->prec, u, 0.0);
+          mode = 0;
+        }
       if (lvl->dim == 3)
         {
           using C          = SM3<K, T>;

@@ -9,9 +9,11 @@
 // transpose R = P^T (261-344, 623-671).  Dirichlet DoFs of the coarse level
 // are excluded on both sides (set_mg_constrained_dofs, .template.cuh:586,668).
 //
-// Each direction is one pass (x, then y, then z) through two level-sized
-// temporaries owned by the transfer object; the last pass accumulates into
-// the destination, so no atomics are needed and the result is deterministic.
+// 3D prolongation (all k) and 3D restriction (k <= 3) run as one fused tile
+// kernel each (below); 2D and the remaining restrictions run one pass per
+// direction (x, then y, then z) through two level-sized temporaries owned by
+// the transfer object.  Either way the last step accumulates into the
+// destination, so no atomics are needed and results are deterministic.
 #include "gtps_internal.h"
 
 namespace gtps
@@ -116,6 +118,188 @@ namespace gtps
         }
     }
 
+    // ------------------------------------------------------------------
+    // Fused 3D kernels: one work-group owns a tile of CX x CY x CZ coarse
+    // cells.  Prolongation stages the coarse tile (k C + 1 nodes per
+    // direction, Dirichlet nodes as zero) in LDS, applies E along x, y, z in
+    // LDS and adds the 2k C fine nodes per direction it owns into the fine
+    // vector.  Restriction stages the fine window [2k c0 - 2k + 1,
+    // 2k (c0 + C) - 1] per direction, applies the transposed 1D rows along
+    // x, y, z and adds the k C coarse nodes per direction it owns into the
+    // coarse vector (Dirichlet nodes skipped).
+    template <int K>
+    struct TT
+    {
+      static constexpr int CX = K <= 3 ? 4 : 2, CY = K <= 3 ? 2 : 1,
+                           CZ = K <= 3 ? 2 : 1, NT = 256;
+    };
+
+    template <int K, typename T>
+    __global__ void __launch_bounds__(256)
+    prolongate3d_kernel(const T *__restrict__ coarse, T *__restrict__ fine,
+                        const T *__restrict__ tab, const int nc)
+    {
+      using TB          = Tab<K>;
+      constexpr int CX  = TT<K>::CX, CY = TT<K>::CY, CZ = TT<K>::CZ, NT = 256;
+      constexpr int N   = K + 1;
+      constexpr int EX  = K * CX + 1, EY = K * CY + 1, EZ = K * CZ + 1;
+      constexpr int FX  = 2 * K * CX, FY = 2 * K * CY, FZ = 2 * K * CZ;
+      __shared__ T sE[(2 * K + 1) * N];
+      __shared__ T sC[EZ * EY * EX], sX[EZ * EY * FX], sY[EZ * FY * FX];
+      const int       tid = threadIdx.x;
+      const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
+      const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
+      const int c0x = blockIdx.x * CX, c0y = blockIdx.y * CY, c0z = blockIdx.z * CZ;
+      for (int i = tid; i < (2 * K + 1) * N; i += NT)
+        sE[i] = tab[TB::E + i];
+      for (int i = tid; i < EZ * EY * EX; i += NT)
+        {
+          const int x = i % EX, y = (i / EX) % EY, z = i / (EX * EY);
+          const int gx = K * c0x + x, gy = K * c0y + y, gz = K * c0z + z;
+          const bool inner = gx > 0 && gx < Nc1 - 1 && gy > 0 && gy < Nc1 - 1 &&
+                             gz > 0 && gz < Nc1 - 1;
+          sC[i] = inner ? coarse[gx + (long long)gy * Nc1 + gz * Pc] : T(0);
+        }
+      __syncthreads();
+      for (int i = tid; i < EZ * EY * FX; i += NT)
+        {
+          const int xf = i % FX, r = i / FX;
+          const int cl = xf / (2 * K), q = xf - 2 * K * cl;
+          T         v  = 0;
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            v += sE[q * N + a] * sC[r * EX + K * cl + a];
+          sX[i] = v;
+        }
+      __syncthreads();
+      for (int i = tid; i < EZ * FY * FX; i += NT)
+        {
+          const int xf = i % FX, yf = (i / FX) % FY, z = i / (FX * FY);
+          const int cl = yf / (2 * K), q = yf - 2 * K * cl;
+          T         v  = 0;
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            v += sE[q * N + a] * sX[(z * EY + K * cl + a) * FX + xf];
+          sY[i] = v;
+        }
+      __syncthreads();
+      for (int i = tid; i < FZ * FY * FX; i += NT)
+        {
+          const int xf = i % FX, yf = (i / FX) % FY, zf = i / (FX * FY);
+          const int gx = 2 * K * c0x + xf, gy = 2 * K * c0y + yf,
+                    gz = 2 * K * c0z + zf;
+          if (gx >= Nf1 - 1 || gy >= Nf1 - 1 || gz >= Nf1 - 1)
+            continue;  // beyond the mesh, or the last (Dirichlet) plane: +0
+          const int cl = zf / (2 * K), q = zf - 2 * K * cl;
+          T         v  = 0;
+#pragma unroll
+          for (int a = 0; a < N; ++a)
+            v += sE[q * N + a] * sY[((K * cl + a) * FY + yf) * FX + xf];
+          fine[gx + (long long)gy * Nf1 + gz * Pf] += v;
+        }
+    }
+
+    // transposed 1D row of the prolongation for coarse node k*cl + a of the
+    // tile; `f` points at fine window entry of node 2k*(c0+cl) (window
+    // stride `st`); cc = c0 + cl is the coarse cell index
+    template <int K, typename T>
+    __device__ __forceinline__ T
+    restrict_row(const T *sE, const T *f, int st, int a, int cc)
+    {
+      constexpr int N = K + 1;
+      T             v = 0;
+      if (a != 0)
+        {
+#pragma unroll
+          for (int q = 1; q < 2 * K; ++q)
+            v += sE[q * N + a] * f[q * st];
+        }
+      else
+        {
+          v = f[0];
+#pragma unroll
+          for (int q = 1; q < 2 * K; ++q)
+            v += sE[q * N] * f[q * st];
+          if (cc > 0)
+            {
+#pragma unroll
+              for (int q = 1; q < 2 * K; ++q)
+                v += sE[q * N + K] * f[(q - 2 * K) * st];
+            }
+        }
+      return v;
+    }
+
+    template <int K>
+    struct RT
+    {
+      static constexpr int CX = K <= 2 ? 4 : 2, CY = K <= 2 ? 2 : (K <= 4 ? 2 : 1),
+                           CZ = K <= 2 ? 2 : (K <= 4 ? 2 : 1);
+    };
+
+    template <int K, typename T>
+    __global__ void __launch_bounds__(256)
+    restrict3d_kernel(const T *__restrict__ fine, T *__restrict__ coarse,
+                      const T *__restrict__ tab, const int nc)
+    {
+      using TB         = Tab<K>;
+      constexpr int CX = RT<K>::CX, CY = RT<K>::CY, CZ = RT<K>::CZ, NT = 256;
+      constexpr int N  = K + 1;
+      // fine window per direction and coarse nodes owned per direction
+      constexpr int WX = 2 * K * CX + 2 * K - 1, WY = 2 * K * CY + 2 * K - 1,
+                    WZ = 2 * K * CZ + 2 * K - 1;
+      constexpr int OX = K * CX, OY = K * CY, OZ = K * CZ;
+      __shared__ T sE[(2 * K + 1) * N];
+      __shared__ T sF[WZ * WY * WX], sX[WZ * WY * OX], sY[WZ * OY * OX];
+      const int       tid = threadIdx.x;
+      const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
+      const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
+      const int c0x = blockIdx.x * CX, c0y = blockIdx.y * CY, c0z = blockIdx.z * CZ;
+      const int w0x = 2 * K * c0x - 2 * K + 1, w0y = 2 * K * c0y - 2 * K + 1,
+                w0z = 2 * K * c0z - 2 * K + 1;
+      for (int i = tid; i < (2 * K + 1) * N; i += NT)
+        sE[i] = tab[TB::E + i];
+      for (int i = tid; i < WZ * WY * WX; i += NT)
+        {
+          const int x = i % WX, y = (i / WX) % WY, z = i / (WX * WY);
+          const int gx = w0x + x, gy = w0y + y, gz = w0z + z;
+          const bool in = gx >= 0 && gx < Nf1 && gy >= 0 && gy < Nf1 && gz >= 0 &&
+                          gz < Nf1;
+          sF[i] = in ? fine[gx + (long long)gy * Nf1 + gz * Pf] : T(0);
+        }
+      __syncthreads();
+      // window index of fine node 2k (c0 + cl) is 2k cl + 2k - 1
+      for (int i = tid; i < WZ * WY * OX; i += NT)
+        {
+          const int xo = i % OX, r = i / OX;
+          const int cl = xo / K, a = xo - K * cl;
+          sX[i] = restrict_row<K>(sE, sF + r * WX + 2 * K * cl + 2 * K - 1, 1, a,
+                                  c0x + cl);
+        }
+      __syncthreads();
+      for (int i = tid; i < WZ * OY * OX; i += NT)
+        {
+          const int xo = i % OX, yo = (i / OX) % OY, z = i / (OX * OY);
+          const int cl = yo / K, a = yo - K * cl;
+          sY[i] = restrict_row<K>(sE, sX + (z * WY + 2 * K * cl + 2 * K - 1) * OX + xo,
+                                  OX, a, c0y + cl);
+        }
+      __syncthreads();
+      for (int i = tid; i < OZ * OY * OX; i += NT)
+        {
+          const int xo = i % OX, yo = (i / OX) % OY, zo = i / (OX * OY);
+          const int gx = K * c0x + xo, gy = K * c0y + yo, gz = K * c0z + zo;
+          if (gx <= 0 || gx >= Nc1 - 1 || gy <= 0 || gy >= Nc1 - 1 || gz <= 0 ||
+              gz >= Nc1 - 1)
+            continue;
+          const int cl = zo / K, a = zo - K * cl;
+          const T   v  = restrict_row<K>(
+            sE, sY + ((2 * K * cl + 2 * K - 1) * OY + yo) * OX + xo, OY * OX, a,
+            c0z + cl);
+          coarse[gx + (long long)gy * Nc1 + gz * Pc] += v;
+        }
+    }
+
     template <int K, typename T, bool PROLONG, bool MASK_IN, int OUT>
     void
     pass(hipStream_t s, const T *in, T *out, const T *tab, int dim, int axis,
@@ -144,6 +328,29 @@ namespace gtps
       const int   Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
       T          *t1  = reinterpret_cast<T *>(tr->tmp1);
       T          *t2  = reinterpret_cast<T *>(tr->tmp2);
+      if (dim == 3 && prolong)
+        {
+          using C = TT<K>;
+          const dim3 grid((nc + C::CX - 1) / C::CX, (nc + C::CY - 1) / C::CY,
+                          (nc + C::CZ - 1) / C::CZ);
+          hipLaunchKernelGGL((prolongate3d_kernel<K, T>), grid, dim3(256), 0, s,
+                             coarse_in, fine, tab, nc);
+          GTPS_LAUNCH_CHECK();
+          return;
+        }
+      // the restriction window outgrows 64 KiB of LDS beyond k = 3: those
+      // degrees keep the three separable passes below
+      if constexpr (K <= 3)
+        if (dim == 3)
+          {
+            using C = RT<K>;
+            const dim3 grid((nc + C::CX - 1) / C::CX, (nc + C::CY - 1) / C::CY,
+                            (nc + C::CZ - 1) / C::CZ);
+            hipLaunchKernelGGL((restrict3d_kernel<K, T>), grid, dim3(256), 0, s,
+                               fine_in, coarse, tab, nc);
+            GTPS_LAUNCH_CHECK();
+            return;
+          }
       if (prolong)
         {
           int e0[3] = {Nc1, Nc1, dim == 3 ? Nc1 : 1};

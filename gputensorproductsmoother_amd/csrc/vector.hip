@@ -100,11 +100,11 @@ namespace gtps
           case cg::OP_BETA:
             sc[cg::BETA] = sc[cg::RZ] / sc[cg::RZ_OLD];
             break;
-          case cg::OP_BETA0:
-            sc[cg::BETA] = 0.0;
-            break;
           case cg::OP_SHIFT:
             sc[cg::RZ_OLD] = sc[cg::RZ];
+            break;
+          case cg::OP_INIT:
+            sc[cg::RZ] = 1.0;
             break;
           default:
             break;

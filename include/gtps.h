@@ -179,6 +179,15 @@ int gtps_mg_pcg_solve(gtps_mg *mg, double *x, const double *b, double rtol,
  * MultigridSolvers::do_matvec (include/solver.cuh:1934-1938). */
 int gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src);
 
+/* Replay the per-iteration preconditioning sequence (V-cycle, r.z, beta,
+ * p update; about 130 kernels for 6 levels) as one HIP graph captured on the
+ * first solve (enable != 0, the default), or launch it kernel by kernel.
+ * The graph uses no memset nodes.  Stage timers force kernel-by-kernel
+ * launches.  No reference counterpart (the reference launches every kernel
+ * on the default stream and synchronises after each stage,
+ * include/solver.cuh:1959-1998). */
+int gtps_mg_set_graph(gtps_mg *mg, int enable);
+
 /* ---------------------------------------------------------- profiling */
 /* Stage timers with HIP events on the context stream, the counterpart of the
  * reference's per-level timing buckets (MultigridSolvers::print_wall_times,
