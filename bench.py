@@ -106,6 +106,58 @@ def cpu_baseline(args):
             "kind": "port", "sample": sample}
 
 
+def kernel_bench(args):
+    """Per-kernel timing of the level-`level` building blocks (not the
+    contract line): finest smoother sweep FP32/FP64, vmult FP64 and FP32
+    residual, FP32 transfers, and a plain device copy of known size used to
+    calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    from gputensorproductsmoother_amd import core
+
+    dim, k, level, R = args.dim, args.degree, args.level, args.repeat
+    ctx = core.Context(0)
+    out = {}
+
+    def timed(name, fn, nbytes, launches):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(R):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / (R * launches)
+        out[name] = {"us_per_launch": round(us, 2), "launches": launches,
+                     "GB_s_algorithmic": round(nbytes / launches / (us * 1e-6) / 1e9, 1)}
+
+    g = torch.Generator(device="cuda:0").manual_seed(SEED)
+    vals = smoother_colour_values(dim, k, level)
+    for dt, s in ((torch.float32, 4), (torch.float64, 8)):
+        op = core.LaplaceOperator(ctx, dim, k, level, dt)
+        sm = core.PatchSmoother(op)
+        u = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+        b = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+        tag = "fp32" if s == 4 else "fp64"
+        timed(f"smoother_colour_{tag}", lambda: sm.step(u, b), s * sum(vals), len(vals))
+        y = torch.empty_like(u)
+        timed(f"vmult_{tag}", lambda: op.vmult(y, u), 2 * s * op.n_dofs, 1)
+        timed(f"residual_{tag}", lambda: op.residual(y, b, u), 3 * s * op.n_dofs, 1)
+        if level >= 2:
+            tr = core.MGTransferCUDA(ctx, dim, k, 1, level, dt)
+            nc = (k * (1 << (level - 1)) + 1) ** dim
+            c = torch.rand(nc, generator=g, device="cuda:0", dtype=dt)
+            timed(f"prolongate_add_{tag}", lambda: tr.prolongate_and_add(level, u, c),
+                  s * (nc + 2 * op.n_dofs), 1)
+            timed(f"restrict_add_{tag}", lambda: tr.restrict_and_add(level, c, u),
+                  s * (op.n_dofs + 2 * nc), 1)
+    src = torch.rand(1 << 26, generator=g, device="cuda:0")  # 256 MiB
+    dst = torch.empty_like(src)
+    timed("copy_256MiB", lambda: dst.copy_(src), 2 * src.numel() * 4, 1)
+    print(json.dumps({"kernels": out, "workload": f"{dim}D Q{k} level {level}",
+                      "repeat": R}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +171,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-everywhere run")
     ap.add_argument("--breakdown", action="store_true",
                     help="also time every stage group (extra solves, not in value)")
+    ap.add_argument("--kernels", action="store_true",
+                    help="per-kernel timings of the building blocks instead of the solve")
+    ap.add_argument("--repeat", type=int, default=10)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,6 +181,10 @@ def main():
               "(the multi-GPU decomposition is a design in DESIGN.md, not a run)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.kernels:
+        kernel_bench(args)
+        return
 
     import torch
     from gputensorproductsmoother_amd import core

@@ -590,8 +590,10 @@ namespace gtps
       return K <= 2 || (K == 3 && sizeof(T) == 4);
     }
 
+    // at most 93 VGPRs (5 waves per SIMD) for the Q3 multiplicative form,
+    // without spilling
     template <int K, typename T, int MODE>
-    __global__ void __launch_bounds__(64)
+    __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
     patch3d_wave_kernel(T *__restrict__ u, const T *__restrict__ src,
                         const T *__restrict__ tab, const int n,
                         const int colour, const T scale, const T inv_scale,
@@ -718,7 +720,9 @@ namespace gtps
           __syncthreads();
         }
       // phase B: residual on interior x-lines, S^T along x (in place over
-      // the C1 row of the line)
+      // the C1 row of the line); the fixed trip count is unrolled so that a
+      // lane's lines overlap their loads and arithmetic
+#pragma unroll
       for (int t = lane; t < NPW * NI * NI; t += 64)
         {
           const int       lp = t % NPW, r = t / NPW;
@@ -832,6 +836,7 @@ namespace gtps
         }
       __syncthreads();
       // phase D: S along x, store the interior
+#pragma unroll
       for (int t = lane; t < NPW * NI * NI; t += 64)
         {
           const int       lp = t % NPW, r = t / NPW;
