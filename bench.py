@@ -154,6 +154,20 @@ def kernel_bench(args):
     src = torch.rand(1 << 26, generator=g, device="cuda:0")  # 256 MiB
     dst = torch.empty_like(src)
     timed("copy_256MiB", lambda: dst.copy_(src), 2 * src.numel() * 4, 1)
+    if args.smoother_levels:
+        # both smoother kernels on every level (colour-launch latency vs size)
+        for lv in range(2, level + 1):
+            vals_l = smoother_colour_values(dim, k, lv)
+            for dt, s in ((torch.float32, 4), (torch.float64, 8)):
+                op = core.LaplaceOperator(ctx, dim, k, lv, dt)
+                sm = core.PatchSmoother(op)
+                u = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+                b = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+                for path, lim in (("single", 1 << 40), ("multi", 0)):
+                    prev = core.set_tuning("small_colour_patches", lim)
+                    timed(f"smoother_l{lv}_{'fp32' if s == 4 else 'fp64'}_{path}",
+                          lambda: sm.step(u, b), s * sum(vals_l), len(vals_l))
+                    core.set_tuning("small_colour_patches", prev)
     print(json.dumps({"kernels": out, "workload": f"{dim}D Q{k} level {level}",
                       "repeat": R}), flush=True)
 
@@ -174,6 +188,8 @@ def main():
     ap.add_argument("--kernels", action="store_true",
                     help="per-kernel timings of the building blocks instead of the solve")
     ap.add_argument("--repeat", type=int, default=10)
+    ap.add_argument("--smoother-levels", action="store_true",
+                    help="with --kernels: both smoother kernels on every level")
     args = ap.parse_args()
 
     if args.gpus != 1:

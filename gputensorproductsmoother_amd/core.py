@@ -46,6 +46,15 @@ def default_relaxation(dim: int, schedule: int) -> float:
     return 1.0
 
 
+def set_tuning(key: str, value: int) -> int:
+    """Set a process-wide tuning knob (see gtps_set_tuning in
+    include/gtps.h); returns the previous value."""
+    prev = L.load().gtps_set_tuning(key.encode(), int(value))
+    if prev < 0:
+        L.check(L.GTPS_EINVAL, "set_tuning")
+    return int(prev)
+
+
 class Context:
     """Device + HIP stream shared by all objects built on it."""
 

@@ -15,6 +15,13 @@
 
 using namespace gtps;
 
+gtps::Tuning &
+gtps::tuning()
+{
+  static Tuning t;
+  return t;
+}
+
 namespace
 {
   thread_local std::string g_last_error;
@@ -896,6 +903,25 @@ extern "C"
       mg->precondition(r, z, true);
       mg->toc(GTPS_PROF_VCYCLE, e);
     });
+  }
+
+  long long
+  gtps_set_tuning(const char *key, long long value)
+  {
+    long long prev = -1;
+    int rc = guard([&] {
+      GTPS_REQUIRE(key != nullptr, "NULL key");
+      const std::string k(key);
+      if (k == "small_colour_patches")
+        {
+          GTPS_REQUIRE(value >= 0, "value must be >= 0");
+          prev                              = tuning().small_colour_patches;
+          tuning().small_colour_patches     = value;
+        }
+      else
+        throw Error(GTPS_EINVAL, "unknown tuning key " + k);
+    });
+    return rc == GTPS_OK ? prev : -1;
   }
 
   int

@@ -107,4 +107,18 @@ namespace gtps
   {
     return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   }
+
+  // MI355X dispatches consecutive work-group ids round-robin over its 8
+  // XCDs, each with a private L2.  This bijection maps the hardware id to a
+  // logical id such that every XCD works through one contiguous range of
+  // logical ids, so neighbouring tiles (which share halo / face data) run
+  // on the same XCD and hit the same L2.
+  constexpr int N_XCD = 8;
+  __host__ __device__ __forceinline__ int
+  xcd_logical_id(int hw, int n)
+  {
+    const int q = n / N_XCD, r = n % N_XCD;
+    const int xcd = hw % N_XCD, slot = hw / N_XCD;
+    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+  }
 } // namespace gtps

@@ -118,6 +118,15 @@ struct gtps_transfer
 
 namespace gtps
 {
+  // process-wide tuning knobs (gtps_set_tuning)
+  struct Tuning
+  {
+    // smoother colours with at most this many patches use the single-patch
+    // (latency) kernel, larger ones the multi-patch (throughput) kernel
+    long long small_colour_patches = 4096;
+  };
+  Tuning &tuning();
+
   // setup.hip
   DegreeTables *get_tables(gtps_ctx *ctx, int k);
 

@@ -89,8 +89,13 @@ namespace gtps
       const int       N1  = K * n + 1;
       const long long P   = (long long)N1 * N1;
       const int       tid = threadIdx.x;
-      const int       cx0 = blockIdx.x * TCX, cy0 = blockIdx.y * TCY;
-      const int       cz0 = blockIdx.z * zc;
+      // logical tile: z-chunk fastest, then x, then y, so that the tiles an
+      // XCD works through share halo columns and window planes
+      const int       L   = xcd_logical_id(linear_block_id(),
+                                           gridDim.x * gridDim.y * gridDim.z);
+      const int       bz  = L % gridDim.z, bxy = L / gridDim.z;
+      const int       cx0 = (bxy % gridDim.x) * TCX, cy0 = (bxy / gridDim.x) * TCY;
+      const int       cz0 = bz * zc;
       const int       cz1 = min(cz0 + zc, n);
       const int       gx0 = K * cx0 - K, gy0 = K * cy0 - K;
       const int       ox0 = K * cx0, oy0 = K * cy0;
@@ -321,7 +326,8 @@ namespace gtps
       __shared__ double sred[NT / 64];
       const int         N1  = K * n + 1;
       const int         tid = threadIdx.x;
-      const int         cx0 = blockIdx.x * TCX, cy0 = blockIdx.y * TCY;
+      const int         L   = xcd_logical_id(linear_block_id(), gridDim.x * gridDim.y);
+      const int         cx0 = (L % gridDim.x) * TCX, cy0 = (L / gridDim.x) * TCY;
       const int         gx0 = K * cx0 - K, gy0 = K * cy0 - K;
       const int         ox0 = K * cx0, oy0 = K * cy0;
       const int         ox1 = (cx0 + TCX >= n) ? N1 : K * (cx0 + TCX);

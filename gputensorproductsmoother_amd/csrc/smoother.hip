@@ -101,9 +101,10 @@ namespace gtps
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
                 npz    = n_patches_1d(n, pz);
       const int npatch = npx * npy * npz;
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
       if (tid < NPB)
         {
-          const int q = blockIdx.x * NPB + tid;
+          const int q = blk * NPB + tid;
           if (q < npatch)
             {
               const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
@@ -378,9 +379,10 @@ namespace gtps
       const int px = colour & 1, py = (colour >> 1) & 1;
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py);
       const int npatch = npx * npy;
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
       if (tid < NPB)
         {
-          const int q = blockIdx.x * NPB + tid;
+          const int q = blk * NPB + tid;
           if (q < npatch)
             {
               const int qx = q % npx, qy = q / npx;
@@ -612,9 +614,10 @@ namespace gtps
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
                 npz = n_patches_1d(n, pz);
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
       if (lane < NPW)
         {
-          const int q = blockIdx.x * NPW + lane;
+          const int q = blk * NPW + lane;
           if (q < npx * npy * npz)
             {
               const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
@@ -864,6 +867,260 @@ namespace gtps
         }
     }
 
+    // ---------------------------------------------------------------------
+    // Single-patch kernel for small colours (coarse levels): one wavefront
+    // per patch, so a launch with few patches is not serialised through a
+    // few waves.  Lane (y,x) loads its z-column of the skeleton and lane
+    // (z,y) its interior b-row at the start (all loads in flight at once);
+    // the seven contraction / FD stages then run through LDS inside one
+    // wave.  Same MODE semantics as the wave kernel.  Needs (2k+1)^2 <= 64.
+    template <int K, typename T, int MODE>
+    __global__ void __launch_bounds__(64)
+    patch3d_single_kernel(T *__restrict__ u, const T *__restrict__ src,
+                          const T *__restrict__ tab, const int n,
+                          const int colour, const T scale, const T inv_scale,
+                          const T relax)
+    {
+      using TB         = Tab<K>;
+      constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
+      static_assert(NP * NP <= 64, "one lane per z-pencil");
+      __shared__ T sZ[2 * NI * NP * NP], sY[2 * NI * NI * NP], sR[NI * NI * NI];
+      const int       N1   = K * n + 1;
+      const long long P    = (long long)N1 * N1;
+      const int       lane = threadIdx.x;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py);
+      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      const long long f = (long long)K * (2 * qx + px) +
+                          (long long)K * (2 * qy + py) * N1 +
+                          (long long)K * (2 * qz + pz) * P;
+      const T *pM   = tab + TB::patchM;
+      const T *pK   = tab + TB::patchK;
+      const T *S    = tab + TB::S;
+      const T *linv = tab + TB::lamInv3;
+
+      // interior right-hand side (or residual) rows, loaded up front
+      T rb[NI];
+      if (lane < NI * NI)
+        {
+          const int       zi = lane / NI, yi = lane - zi * NI;
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+#pragma unroll
+          for (int xi = 0; xi < NI; ++xi)
+            rb[xi] = src[g0 + xi];
+        }
+      if (MODE == 0)
+        {
+          if (lane < NP * NP)
+            {
+              const int  iy = lane / NP, ix = lane - iy * NP;
+              const bool face = ix == 0 || ix == NP - 1 || iy == 0 || iy == NP - 1;
+              T          v[NP];
+#pragma unroll
+              for (int iz = 0; iz < NP; ++iz)
+                {
+                  const long long g = f + ix + (long long)iy * N1 + iz * P;
+                  v[iz] = (face || iz == 0 || iz == NP - 1) ? u[g] : T(0);
+                }
+#pragma unroll
+              for (int zi = 0; zi < NI; ++zi)
+                {
+                  const int row = zi + 1;
+                  T         m = 0, kk = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      m += pM[row * NP + c] * v[c];
+                      kk += pK[row * NP + c] * v[c];
+                    }
+                  sZ[zi * NP * NP + lane]        = m;
+                  sZ[(NI + zi) * NP * NP + lane] = kk;
+                }
+            }
+          __syncthreads();
+          if (lane < NI * NP)
+            {
+              const int zi = lane / NP, ix = lane - zi * NP;
+              T         a[NP], b[NP];
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                {
+                  a[iy] = sZ[(zi * NP + iy) * NP + ix];
+                  b[iy] = sZ[((NI + zi) * NP + iy) * NP + ix];
+                }
+#pragma unroll
+              for (int yi = 0; yi < NI; ++yi)
+                {
+                  const int row = yi + 1;
+                  T         c1 = 0, c2 = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      c1 += pM[row * NP + c] * a[c];
+                      c2 += pK[row * NP + c] * a[c];
+                      c2 += pM[row * NP + c] * b[c];
+                    }
+                  sY[(zi * NI + yi) * NP + ix]            = c1;
+                  sY[((NI + zi) * NI + yi) * NP + ix]     = c2;
+                }
+            }
+          __syncthreads();
+        }
+      if (MODE == 2)
+        {
+          for (int t = lane; t < NP * NP * NP; t += 64)
+            {
+              const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
+              const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
+                                 iz > 0 && iz < NP - 1;
+              if (!inner)
+                u[f + ix + (long long)iy * N1 + iz * P] = T(0);
+            }
+        }
+      if (lane < NI * NI)
+        {
+          const int r = lane;
+          T         res[NI];
+          if (MODE == 0)
+            {
+              T c1[NP], c2[NP];
+#pragma unroll
+              for (int ix = 0; ix < NP; ++ix)
+                {
+                  c1[ix] = sY[r * NP + ix];
+                  c2[ix] = sY[(NI * NI + r) * NP + ix];
+                }
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                {
+                  const int row = xi + 1;
+                  T         v   = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      v += pK[row * NP + c] * c1[c];
+                      v += pM[row * NP + c] * c2[c];
+                    }
+                  res[xi] = rb[xi] - scale * v;
+                }
+            }
+          else
+            {
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                res[xi] = rb[xi];
+            }
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * res[i];
+              sR[r * NI + j] = s;
+            }
+        }
+      __syncthreads();
+      // S^T along y
+      if (lane < NI * NI)
+        {
+          const int zi = lane / NI, xi = lane - zi * NI;
+          T        *base = sR + zi * NI * NI + xi;
+          T         v[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * v[i];
+              base[j * NI] = s;
+            }
+        }
+      __syncthreads();
+      // S^T along z, eigenvalue scaling, S along z
+      if (lane < NI * NI)
+        {
+          const int yi = lane / NI, xi = lane - yi * NI;
+          T        *base = sR + yi * NI + xi;
+          T         v[NI], tt[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + j] * v[i];
+              tt[j] = s * (linv[(j * NI + yi) * NI + xi] * inv_scale);
+            }
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * tt[j];
+              base[i * NI * NI] = s;
+            }
+        }
+      __syncthreads();
+      // S along y
+      if (lane < NI * NI)
+        {
+          const int zi = lane / NI, xi = lane - zi * NI;
+          T        *base = sR + zi * NI * NI + xi;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = base[j * NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              base[i * NI] = s;
+            }
+        }
+      __syncthreads();
+      // S along x, store
+      if (lane < NI * NI)
+        {
+          const int zi = lane / NI, yi = lane - zi * NI;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = sR[lane * NI + j];
+          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[i * NI + j] * v[j];
+              if (MODE == 1)
+                u[g0 + i] += relax * s;
+              else
+                u[g0 + i] = relax * s;
+            }
+        }
+    }
+
+    long long
+    small_colour_limit()
+    {
+      return tuning().small_colour_patches;
+    }
+
     template <typename KernelPtr>
     void
     ensure_lds(KernelPtr kern, size_t bytes)
@@ -890,6 +1147,24 @@ namespace gtps
         return;
       T       *up = reinterpret_cast<T *>(u);
       const T *sp = reinterpret_cast<const T *>(src);
+      if constexpr (2 * K + 1 <= 8)
+        if (lvl->dim == 3 && np <= small_colour_limit())
+          {
+            const T scale = (T)lvl->scale, inv = (T)(1.0 / lvl->scale);
+#define GTPS_SS(M)                                                            \
+  hipLaunchKernelGGL((patch3d_single_kernel<K, T, M>), dim3((unsigned)np),    \
+                     dim3(64), 0, s, up, sp, tab, n, colour, scale, inv,      \
+                     (T)relax)
+            if (mode == 0)
+              GTPS_SS(0);
+            else if (mode == 1)
+              GTPS_SS(1);
+            else
+              GTPS_SS(2);
+#undef GTPS_SS
+            GTPS_LAUNCH_CHECK();
+            return;
+          }
       if constexpr (use_wave_kernel<K, T>())
         if (lvl->dim == 3)
         {

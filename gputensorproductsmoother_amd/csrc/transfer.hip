@@ -149,7 +149,10 @@ namespace gtps
       const int       tid = threadIdx.x;
       const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
       const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
-      const int c0x = blockIdx.x * CX, c0y = blockIdx.y * CY, c0z = blockIdx.z * CZ;
+      const int L   = xcd_logical_id(linear_block_id(),
+                                     gridDim.x * gridDim.y * gridDim.z);
+      const int c0x = (L % gridDim.x) * CX, c0y = ((L / gridDim.x) % gridDim.y) * CY,
+                c0z = (L / (gridDim.x * gridDim.y)) * CZ;
       for (int i = tid; i < (2 * K + 1) * N; i += NT)
         sE[i] = tab[TB::E + i];
       for (int i = tid; i < EZ * EY * EX; i += NT)
@@ -254,7 +257,10 @@ namespace gtps
       const int       tid = threadIdx.x;
       const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
       const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
-      const int c0x = blockIdx.x * CX, c0y = blockIdx.y * CY, c0z = blockIdx.z * CZ;
+      const int L   = xcd_logical_id(linear_block_id(),
+                                     gridDim.x * gridDim.y * gridDim.z);
+      const int c0x = (L % gridDim.x) * CX, c0y = ((L / gridDim.x) % gridDim.y) * CY,
+                c0z = (L / (gridDim.x * gridDim.y)) * CZ;
       const int w0x = 2 * K * c0x - 2 * K + 1, w0y = 2 * K * c0y - 2 * K + 1,
                 w0z = 2 * K * c0z - 2 * K + 1;
       for (int i = tid; i < (2 * K + 1) * N; i += NT)

@@ -64,6 +64,16 @@ int gtps_ctx_synchronize(gtps_ctx *ctx);
 void *gtps_ctx_stream(gtps_ctx *ctx);
 const char *gtps_last_error(void);
 const char *gtps_version(void);
+/* Process-wide tuning knobs; returns the previous value, or -1 (with
+ * gtps_last_error set) for an unknown key or invalid value.  Keys:
+ *   "small_colour_patches"  smoother colours with at most this many patches
+ *                           run the single-patch (latency-bound, coarse
+ *                           level) kernel, larger ones the multi-patch kernel
+ *                           (default 4096).
+ * Results do not depend on the knobs beyond round-off.  No reference
+ * counterpart (the reference picks kernels at compile time,
+ * scripts/ct_parameter.py.in:146-148). */
+long long gtps_set_tuning(const char *key, long long value);
 
 /* ------------------------------------------------------ level / operator */
 /* One MG level with its matrix-free Laplace operator.

@@ -178,3 +178,33 @@ def test_tensor_product_data(core, ctx):
         assert np.abs(lam - ref["lam"]).max() < 1e-11 * lam.max()
         assert np.abs(d["cell_mass"] - ref["M"]).max() < 1e-14
         assert np.abs(d["cell_laplace"] - ref["K"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("variant", ["fwd", "add"])
+@pytest.mark.parametrize("path", ["single", "multi"])
+@pytest.mark.parametrize("dim,k,level", [(3, 1, 4), (3, 2, 3), (3, 3, 3), (3, 3, 4)])
+def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, path):
+    """The single-patch (coarse-level) and multi-patch (fine-level) smoother
+    kernels agree with the oracle on the same level."""
+    _announce("smoother-path", dim, k, level, variant, prec, path)
+    g = rng(level * 7 + k)
+    N = n_dofs(dim, k, level)
+    b, u = g.uniform(-1, 1, (2, N))
+    if prec == "fp32":
+        u = u.astype(np.float32).astype(np.float64)
+        b = b.astype(np.float32).astype(np.float64)
+    ov, cv, relax = {"fwd": (O.MULT_FWD, core.PatchSmoother.MULTIPLICATIVE, 1.0),
+                     "add": (O.ADDITIVE, core.PatchSmoother.ADDITIVE, 0.5)}[variant]
+    ref = O.smoother_step(dim, k, level, b, u, variant=ov, relax=relax)
+    prev = core.set_tuning("small_colour_patches", 1 << 40 if path == "single" else 0)
+    try:
+        op = core.LaplaceOperator(ctx, dim, k, level, _dt(prec))
+        sm = core.PatchSmoother(op, cv, relax)
+        ud = _to_dev(u, prec)
+        sm.step(ud, _to_dev(b, prec))
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("small_colour_patches", prev)
+    tol = TOL[prec] * (20 if prec == "fp32" else 1)
+    assert rel_err(ud.cpu().numpy(), ref) < tol
