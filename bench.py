@@ -76,6 +76,23 @@ def smoother_colour_values(dim: int, k: int, level: int):
     return out
 
 
+def measured_traffic(kernel: str, dim: int, k: int, level: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE runs, each
+    scaled by the ratio a plain copy of known size reads for the same
+    counter), or None when no summary matches this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in data.get("kernels", []):
+        if e.get("kernel") == kernel and e.get("workload") == [dim, k, level]:
+            return int(e["bytes_per_launch"])
+    return None
+
+
 def cpu_baseline(args):
     """FP64 C++ oracle MG-PCG on the host cores (bounded sample)."""
     import oracle as O
@@ -246,9 +263,13 @@ def main():
     bytes_per_launch = 4.0 * sum(vals) / len(vals)
     avg_launch_ms = tot_ms / max(cnt, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    kernel_name = (f"patch3d_single_kernel<{k}, float, 0>" if dim == 3 and k <= 3
+                   else f"patch3d_kernel<{k}, float, false>")
+    traffic = measured_traffic(kernel_name, dim, k, level)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"patch3d_kernel<{k},float> finest-level colour launch",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "kernel": f"{kernel_name} (finest-level colour launch)",
                 "avg_launch_us": round(avg_launch_ms * 1e3, 2), "launches": cnt,
                 "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 

@@ -283,9 +283,10 @@ struct gtps_mg : LaunchTimer
     e = fine ? tic(GTPS_PROF_RESIDUAL_FINE) : -1;
     launch_laplace(L, t[l], ul, dl, false);  // t = d - A u
     toc(GTPS_PROF_RESIDUAL_FINE, e);
-    launch_fill(ctx->stream, lv[l - 1]->N, prec, d[l - 1], 0.0);
+    // d_{l-1} = R t: the store form writes every interior coarse DoF; its
+    // Dirichlet entries were zeroed at setup and are never written
     e = tic(GTPS_PROF_TRANSFER);
-    launch_restrict_add(tr, l, d[l - 1], t[l]);
+    launch_restrict_store(tr, l, d[l - 1], t[l]);
     toc(GTPS_PROF_TRANSFER, e);
     vcycle(l - 1, d[l - 1], u[l - 1]);
     e = tic(GTPS_PROF_TRANSFER);

@@ -122,8 +122,11 @@ namespace gtps
   struct Tuning
   {
     // smoother colours with at most this many patches use the single-patch
-    // (latency) kernel, larger ones the multi-patch (throughput) kernel
-    long long small_colour_patches = 4096;
+    // kernel, larger ones the multi-patch kernel (k <= 3).  Measured on
+    // MI355X at 64^3 Q3 the single-patch kernel is at least as fast on every
+    // level (4.3-7.7 us per coarse-level launch instead of 11-13 us; level 6
+    // FP32 30.7 vs 31.2 us, FP64 41 vs 104 us), so it is the default.
+    long long small_colour_patches = 1LL << 62;
   };
   Tuning &tuning();
 
@@ -147,6 +150,10 @@ namespace gtps
                              const void *coarse);
   void launch_restrict_add(const gtps_transfer *tr, int from_level,
                            void *coarse, const void *fine);
+  // coarse = P^T fine on non-Dirichlet coarse DoFs (Dirichlet entries are
+  // left untouched)
+  void launch_restrict_store(const gtps_transfer *tr, int from_level,
+                             void *coarse, const void *fine);
   size_t transfer_tmp_elems(int dim, int k, int max_level);
 
   // vector.hip

@@ -22,7 +22,7 @@ namespace gtps
   {
     constexpr int TR_NT = 256;
 
-    // OUT: 0 store, 1 add, 2 add except on coarse Dirichlet nodes
+    // OUT: 0 store, 1 add, 2 add / 3 store except on coarse Dirichlet nodes
     template <int K, typename T, bool PROLONG, bool MASK_IN, int OUT>
     __global__ void __launch_bounds__(TR_NT)
     transfer_pass_kernel(const T *__restrict__ in, T *__restrict__ out,
@@ -113,7 +113,12 @@ namespace gtps
                 if (oc[d] == 0 || oc[d] == Nc1 - 1)
                   bdry = true;
               if (!bdry)
-                out[o] += acc;
+                {
+                  if (OUT == 2)
+                    out[o] += acc;
+                  else
+                    out[o] = acc;
+                }
             }
         }
     }
@@ -144,17 +149,15 @@ namespace gtps
       constexpr int N   = K + 1;
       constexpr int EX  = K * CX + 1, EY = K * CY + 1, EZ = K * CZ + 1;
       constexpr int FX  = 2 * K * CX, FY = 2 * K * CY, FZ = 2 * K * CZ;
-      __shared__ T sE[(2 * K + 1) * N];
       __shared__ T sC[EZ * EY * EX], sX[EZ * EY * FX], sY[EZ * FY * FX];
-      const int       tid = threadIdx.x;
-      const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
+      const T  *E   = tab + TB::E;
+      const int tid = threadIdx.x;
+      const int Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
       const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
       const int L   = xcd_logical_id(linear_block_id(),
                                      gridDim.x * gridDim.y * gridDim.z);
       const int c0x = (L % gridDim.x) * CX, c0y = ((L / gridDim.x) % gridDim.y) * CY,
                 c0z = (L / (gridDim.x * gridDim.y)) * CZ;
-      for (int i = tid; i < (2 * K + 1) * N; i += NT)
-        sE[i] = tab[TB::E + i];
       for (int i = tid; i < EZ * EY * EX; i += NT)
         {
           const int x = i % EX, y = (i / EX) % EY, z = i / (EX * EY);
@@ -164,73 +167,112 @@ namespace gtps
           sC[i] = inner ? coarse[gx + (long long)gy * Nc1 + gz * Pc] : T(0);
         }
       __syncthreads();
-      for (int i = tid; i < EZ * EY * FX; i += NT)
+      // x: one coarse row (z, y) per thread, all 2k CX fine outputs unrolled
+      for (int t = tid; t < EZ * EY; t += NT)
         {
-          const int xf = i % FX, r = i / FX;
-          const int cl = xf / (2 * K), q = xf - 2 * K * cl;
-          T         v  = 0;
+          T r[EX];
 #pragma unroll
-          for (int a = 0; a < N; ++a)
-            v += sE[q * N + a] * sC[r * EX + K * cl + a];
-          sX[i] = v;
+          for (int i = 0; i < EX; ++i)
+            r[i] = sC[t * EX + i];
+#pragma unroll
+          for (int cl = 0; cl < CX; ++cl)
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q)
+              {
+                T v = 0;
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  v += E[q * N + a] * r[K * cl + a];
+                sX[t * FX + 2 * K * cl + q] = v;
+              }
         }
       __syncthreads();
-      for (int i = tid; i < EZ * FY * FX; i += NT)
+      // y: one column (z, xf) per thread
+      for (int t = tid; t < EZ * FX; t += NT)
         {
-          const int xf = i % FX, yf = (i / FX) % FY, z = i / (FX * FY);
-          const int cl = yf / (2 * K), q = yf - 2 * K * cl;
-          T         v  = 0;
+          const int z = t / FX, xf = t - z * FX;
+          T         r[EY];
 #pragma unroll
-          for (int a = 0; a < N; ++a)
-            v += sE[q * N + a] * sX[(z * EY + K * cl + a) * FX + xf];
-          sY[i] = v;
+          for (int i = 0; i < EY; ++i)
+            r[i] = sX[(z * EY + i) * FX + xf];
+#pragma unroll
+          for (int cl = 0; cl < CY; ++cl)
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q)
+              {
+                T v = 0;
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  v += E[q * N + a] * r[K * cl + a];
+                sY[(z * FY + 2 * K * cl + q) * FX + xf] = v;
+              }
         }
       __syncthreads();
-      for (int i = tid; i < FZ * FY * FX; i += NT)
+      // z: one column (yf, xf) per thread, accumulated into the fine vector
+      for (int t = tid; t < FY * FX; t += NT)
         {
-          const int xf = i % FX, yf = (i / FX) % FY, zf = i / (FX * FY);
-          const int gx = 2 * K * c0x + xf, gy = 2 * K * c0y + yf,
-                    gz = 2 * K * c0z + zf;
-          if (gx >= Nf1 - 1 || gy >= Nf1 - 1 || gz >= Nf1 - 1)
-            continue;  // beyond the mesh, or the last (Dirichlet) plane: +0
-          const int cl = zf / (2 * K), q = zf - 2 * K * cl;
-          T         v  = 0;
+          const int yf = t / FX, xf = t - yf * FX;
+          const int gx = 2 * K * c0x + xf, gy = 2 * K * c0y + yf;
+          if (gx >= Nf1 - 1 || gy >= Nf1 - 1)
+            continue;  // beyond the mesh or the last (Dirichlet) plane: +0
+          T r[EZ];
 #pragma unroll
-          for (int a = 0; a < N; ++a)
-            v += sE[q * N + a] * sY[((K * cl + a) * FY + yf) * FX + xf];
-          fine[gx + (long long)gy * Nf1 + gz * Pf] += v;
+          for (int i = 0; i < EZ; ++i)
+            r[i] = sY[(i * FY + yf) * FX + xf];
+          T *out = fine + gx + (long long)gy * Nf1;
+#pragma unroll
+          for (int cl = 0; cl < CZ; ++cl)
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q)
+              {
+                const int gz = 2 * K * (c0z + cl) + q;
+                if (gz >= Nf1 - 1)
+                  continue;
+                T v = 0;
+#pragma unroll
+                for (int a = 0; a < N; ++a)
+                  v += E[q * N + a] * r[K * cl + a];
+                out[gz * Pf] += v;
+              }
         }
     }
 
-    // transposed 1D row of the prolongation for coarse node k*cl + a of the
-    // tile; `f` points at fine window entry of node 2k*(c0+cl) (window
-    // stride `st`); cc = c0 + cl is the coarse cell index
-    template <int K, typename T>
-    __device__ __forceinline__ T
-    restrict_row(const T *sE, const T *f, int st, int a, int cc)
+    // Transposed 1D rows of the prolongation for the k C coarse nodes of a
+    // tile, read from a fine window line r[] whose entry 2k cl + 2k - 1 is
+    // the fine node on coarse vertex c0 + cl; c0 = first coarse cell.
+    template <int K, int C, typename T>
+    __device__ __forceinline__ void
+    restrict_line(const T *E, const T *r, int c0, T *o)
     {
       constexpr int N = K + 1;
-      T             v = 0;
-      if (a != 0)
-        {
 #pragma unroll
-          for (int q = 1; q < 2 * K; ++q)
-            v += sE[q * N + a] * f[q * st];
-        }
-      else
-        {
-          v = f[0];
+      for (int cl = 0; cl < C; ++cl)
 #pragma unroll
-          for (int q = 1; q < 2 * K; ++q)
-            v += sE[q * N] * f[q * st];
-          if (cc > 0)
-            {
+        for (int a = 0; a < K; ++a)
+          {
+            const int base = 2 * K * cl + 2 * K - 1;  // fine node 2k(c0+cl)
+            T         v    = 0;
+            if (a != 0)
+              {
 #pragma unroll
-              for (int q = 1; q < 2 * K; ++q)
-                v += sE[q * N + K] * f[(q - 2 * K) * st];
-            }
-        }
-      return v;
+                for (int q = 1; q < 2 * K; ++q)
+                  v += E[q * N + a] * r[base + q];
+              }
+            else
+              {
+                v = r[base];
+#pragma unroll
+                for (int q = 1; q < 2 * K; ++q)
+                  v += E[q * N] * r[base + q];
+                if (c0 + cl > 0)
+                  {
+#pragma unroll
+                    for (int q = 1; q < 2 * K; ++q)
+                      v += E[q * N + K] * r[base + q - 2 * K];
+                  }
+              }
+            o[K * cl + a] = v;
+          }
     }
 
     template <int K>
@@ -240,22 +282,23 @@ namespace gtps
                            CZ = K <= 2 ? 2 : (K <= 4 ? 2 : 1);
     };
 
-    template <int K, typename T>
+    // STORE: coarse = P^T fine on interior coarse nodes (Dirichlet entries
+    // untouched); otherwise coarse += P^T fine
+    template <int K, typename T, bool STORE>
     __global__ void __launch_bounds__(256)
     restrict3d_kernel(const T *__restrict__ fine, T *__restrict__ coarse,
                       const T *__restrict__ tab, const int nc)
     {
       using TB         = Tab<K>;
       constexpr int CX = RT<K>::CX, CY = RT<K>::CY, CZ = RT<K>::CZ, NT = 256;
-      constexpr int N  = K + 1;
       // fine window per direction and coarse nodes owned per direction
       constexpr int WX = 2 * K * CX + 2 * K - 1, WY = 2 * K * CY + 2 * K - 1,
                     WZ = 2 * K * CZ + 2 * K - 1;
       constexpr int OX = K * CX, OY = K * CY, OZ = K * CZ;
-      __shared__ T sE[(2 * K + 1) * N];
       __shared__ T sF[WZ * WY * WX], sX[WZ * WY * OX], sY[WZ * OY * OX];
-      const int       tid = threadIdx.x;
-      const int       Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
+      const T  *E   = tab + TB::E;
+      const int tid = threadIdx.x;
+      const int Nc1 = K * nc + 1, Nf1 = 2 * K * nc + 1;
       const long long Pc = (long long)Nc1 * Nc1, Pf = (long long)Nf1 * Nf1;
       const int L   = xcd_logical_id(linear_block_id(),
                                      gridDim.x * gridDim.y * gridDim.z);
@@ -263,8 +306,6 @@ namespace gtps
                 c0z = (L / (gridDim.x * gridDim.y)) * CZ;
       const int w0x = 2 * K * c0x - 2 * K + 1, w0y = 2 * K * c0y - 2 * K + 1,
                 w0z = 2 * K * c0z - 2 * K + 1;
-      for (int i = tid; i < (2 * K + 1) * N; i += NT)
-        sE[i] = tab[TB::E + i];
       for (int i = tid; i < WZ * WY * WX; i += NT)
         {
           const int x = i % WX, y = (i / WX) % WY, z = i / (WX * WY);
@@ -274,35 +315,54 @@ namespace gtps
           sF[i] = in ? fine[gx + (long long)gy * Nf1 + gz * Pf] : T(0);
         }
       __syncthreads();
-      // window index of fine node 2k (c0 + cl) is 2k cl + 2k - 1
-      for (int i = tid; i < WZ * WY * OX; i += NT)
+      for (int t = tid; t < WZ * WY; t += NT)
         {
-          const int xo = i % OX, r = i / OX;
-          const int cl = xo / K, a = xo - K * cl;
-          sX[i] = restrict_row<K>(sE, sF + r * WX + 2 * K * cl + 2 * K - 1, 1, a,
-                                  c0x + cl);
+          T r[WX], o[OX];
+#pragma unroll
+          for (int i = 0; i < WX; ++i)
+            r[i] = sF[t * WX + i];
+          restrict_line<K, CX>(E, r, c0x, o);
+#pragma unroll
+          for (int i = 0; i < OX; ++i)
+            sX[t * OX + i] = o[i];
         }
       __syncthreads();
-      for (int i = tid; i < WZ * OY * OX; i += NT)
+      for (int t = tid; t < WZ * OX; t += NT)
         {
-          const int xo = i % OX, yo = (i / OX) % OY, z = i / (OX * OY);
-          const int cl = yo / K, a = yo - K * cl;
-          sY[i] = restrict_row<K>(sE, sX + (z * WY + 2 * K * cl + 2 * K - 1) * OX + xo,
-                                  OX, a, c0y + cl);
+          const int z = t / OX, xo = t - z * OX;
+          T         r[WY], o[OY];
+#pragma unroll
+          for (int i = 0; i < WY; ++i)
+            r[i] = sX[(z * WY + i) * OX + xo];
+          restrict_line<K, CY>(E, r, c0y, o);
+#pragma unroll
+          for (int i = 0; i < OY; ++i)
+            sY[(z * OY + i) * OX + xo] = o[i];
         }
       __syncthreads();
-      for (int i = tid; i < OZ * OY * OX; i += NT)
+      for (int t = tid; t < OY * OX; t += NT)
         {
-          const int xo = i % OX, yo = (i / OX) % OY, zo = i / (OX * OY);
-          const int gx = K * c0x + xo, gy = K * c0y + yo, gz = K * c0z + zo;
-          if (gx <= 0 || gx >= Nc1 - 1 || gy <= 0 || gy >= Nc1 - 1 || gz <= 0 ||
-              gz >= Nc1 - 1)
+          const int yo = t / OX, xo = t - yo * OX;
+          const int gx = K * c0x + xo, gy = K * c0y + yo;
+          if (gx <= 0 || gx >= Nc1 - 1 || gy <= 0 || gy >= Nc1 - 1)
             continue;
-          const int cl = zo / K, a = zo - K * cl;
-          const T   v  = restrict_row<K>(
-            sE, sY + ((2 * K * cl + 2 * K - 1) * OY + yo) * OX + xo, OY * OX, a,
-            c0z + cl);
-          coarse[gx + (long long)gy * Nc1 + gz * Pc] += v;
+          T r[WZ], o[OZ];
+#pragma unroll
+          for (int i = 0; i < WZ; ++i)
+            r[i] = sY[(i * OY + yo) * OX + xo];
+          restrict_line<K, CZ>(E, r, c0z, o);
+          T *out = coarse + gx + (long long)gy * Nc1;
+#pragma unroll
+          for (int i = 0; i < OZ; ++i)
+            {
+              const int gz = K * c0z + i;
+              if (gz <= 0 || gz >= Nc1 - 1)
+                continue;
+              if (STORE)
+                out[gz * Pc] = o[i];
+              else
+                out[gz * Pc] += o[i];
+            }
         }
     }
 
@@ -323,7 +383,7 @@ namespace gtps
     template <int K, typename T>
     void
     run(const gtps_transfer *tr, int fine_level, T *fine, const T *fine_in,
-        T *coarse, const T *coarse_in, bool prolong)
+        T *coarse, const T *coarse_in, bool prolong, bool store)
     {
       const T    *tab = reinterpret_cast<const T *>(
         tr->prec == GTPS_FP32 ? (const void *)tr->tab->d32
@@ -352,8 +412,12 @@ namespace gtps
             using C = RT<K>;
             const dim3 grid((nc + C::CX - 1) / C::CX, (nc + C::CY - 1) / C::CY,
                             (nc + C::CZ - 1) / C::CZ);
-            hipLaunchKernelGGL((restrict3d_kernel<K, T>), grid, dim3(256), 0, s,
-                               fine_in, coarse, tab, nc);
+            if (store)
+              hipLaunchKernelGGL((restrict3d_kernel<K, T, true>), grid, dim3(256),
+                                 0, s, fine_in, coarse, tab, nc);
+            else
+              hipLaunchKernelGGL((restrict3d_kernel<K, T, false>), grid,
+                                 dim3(256), 0, s, fine_in, coarse, tab, nc);
             GTPS_LAUNCH_CHECK();
             return;
           }
@@ -381,13 +445,19 @@ namespace gtps
           if (dim == 2)
             {
               pass<K, T, false, false, 0>(s, fine_in, t1, tab, dim, 0, e0, e1, nc, Nc1);
-              pass<K, T, false, false, 2>(s, t1, coarse, tab, dim, 1, e1, e2, nc, Nc1);
+              if (store)
+                pass<K, T, false, false, 3>(s, t1, coarse, tab, dim, 1, e1, e2, nc, Nc1);
+              else
+                pass<K, T, false, false, 2>(s, t1, coarse, tab, dim, 1, e1, e2, nc, Nc1);
             }
           else
             {
               pass<K, T, false, false, 0>(s, fine_in, t1, tab, dim, 0, e0, e1, nc, Nc1);
               pass<K, T, false, false, 0>(s, t1, t2, tab, dim, 1, e1, e2, nc, Nc1);
-              pass<K, T, false, false, 2>(s, t2, coarse, tab, dim, 2, e2, e3, nc, Nc1);
+              if (store)
+                pass<K, T, false, false, 3>(s, t2, coarse, tab, dim, 2, e2, e3, nc, Nc1);
+              else
+                pass<K, T, false, false, 2>(s, t2, coarse, tab, dim, 2, e2, e3, nc, Nc1);
             }
         }
     }
@@ -396,7 +466,7 @@ namespace gtps
     void
     run_t(const gtps_transfer *tr, int fine_level, void *fine,
           const void *fine_in, void *coarse, const void *coarse_in,
-          bool prolong)
+          bool prolong, bool store = false)
     {
       T       *f  = reinterpret_cast<T *>(fine);
       const T *fi = reinterpret_cast<const T *>(fine_in);
@@ -404,13 +474,13 @@ namespace gtps
       const T *ci = reinterpret_cast<const T *>(coarse_in);
       switch (tr->k)
         {
-          case 1: return run<1, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 2: return run<2, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 3: return run<3, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 4: return run<4, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 5: return run<5, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 6: return run<6, T>(tr, fine_level, f, fi, c, ci, prolong);
-          case 7: return run<7, T>(tr, fine_level, f, fi, c, ci, prolong);
+          case 1: return run<1, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 2: return run<2, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 3: return run<3, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 4: return run<4, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 5: return run<5, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 6: return run<6, T>(tr, fine_level, f, fi, c, ci, prolong, store);
+          case 7: return run<7, T>(tr, fine_level, f, fi, c, ci, prolong, store);
           default: throw Error(GTPS_EINVAL, "degree must be in 1..7");
         }
     }
@@ -442,5 +512,15 @@ namespace gtps
       run_t<float>(tr, from_level, nullptr, fine, coarse, nullptr, false);
     else
       run_t<double>(tr, from_level, nullptr, fine, coarse, nullptr, false);
+  }
+
+  void
+  launch_restrict_store(const gtps_transfer *tr, int from_level, void *coarse,
+                        const void *fine)
+  {
+    if (tr->prec == GTPS_FP32)
+      run_t<float>(tr, from_level, nullptr, fine, coarse, nullptr, false, true);
+    else
+      run_t<double>(tr, from_level, nullptr, fine, coarse, nullptr, false, true);
   }
 } // namespace gtps

@@ -69,7 +69,9 @@ const char *gtps_version(void);
  *   "small_colour_patches"  smoother colours with at most this many patches
  *                           run the single-patch (latency-bound, coarse
  *                           level) kernel, larger ones the multi-patch kernel
- *                           (default 4096).
+ *                           (default 2^62, i.e. always the
+ *                           single-patch kernel where it applies: 3D,
+ *                           k <= 3).
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */
