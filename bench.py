@@ -157,6 +157,15 @@ def kernel_bench(args):
         b = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
         tag = "fp32" if s == 4 else "fp64"
         timed(f"smoother_colour_{tag}", lambda: sm.step(u, b), s * sum(vals), len(vals))
+        if dim == 3 and k <= 3:
+            # patch-group kernel at 1, 2 and 4 patches per wavefront
+            prev_min = core.set_tuning("group_min_patches", 0)
+            for npw in (1, 2, 4):
+                prev_npw = core.set_tuning("patches_per_wave", npw)
+                timed(f"smoother_colour_{tag}_npw{npw}", lambda: sm.step(u, b),
+                      s * sum(vals), len(vals))
+                core.set_tuning("patches_per_wave", prev_npw)
+            core.set_tuning("group_min_patches", prev_min)
         y = torch.empty_like(u)
         timed(f"vmult_{tag}", lambda: op.vmult(y, u), 2 * s * op.n_dofs, 1)
         timed(f"residual_{tag}", lambda: op.residual(y, b, u), 3 * s * op.n_dofs, 1)
@@ -207,6 +216,8 @@ def main():
     ap.add_argument("--repeat", type=int, default=10)
     ap.add_argument("--smoother-levels", action="store_true",
                     help="with --kernels: both smoother kernels on every level")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="library tuning knob (gtps_set_tuning) applied before the run")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -214,6 +225,12 @@ def main():
               "(the multi-GPU decomposition is a design in DESIGN.md, not a run)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.tune:
+        from gputensorproductsmoother_amd import core as _core
+        for kv in args.tune:
+            key, val = kv.split("=", 1)
+            _core.set_tuning(key, int(val))
 
     if args.kernels:
         kernel_bench(args)

@@ -919,6 +919,19 @@ extern "C"
           prev                              = tuning().small_colour_patches;
           tuning().small_colour_patches     = value;
         }
+      else if (k == "group_min_patches")
+        {
+          GTPS_REQUIRE(value >= 0, "value must be >= 0");
+          prev                          = tuning().group_min_patches;
+          tuning().group_min_patches    = value;
+        }
+      else if (k == "patches_per_wave")
+        {
+          GTPS_REQUIRE(value == 1 || value == 2 || value == 4,
+                       "patches_per_wave must be 1, 2 or 4");
+          prev                         = tuning().patches_per_wave;
+          tuning().patches_per_wave    = (int)value;
+        }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });

@@ -127,6 +127,10 @@ namespace gtps
     // level (4.3-7.7 us per coarse-level launch instead of 11-13 us; level 6
     // FP32 30.7 vs 31.2 us, FP64 41 vs 104 us), so it is the default.
     long long small_colour_patches = 1LL << 62;
+    // patch-group kernel: colours with more than group_min_patches patches
+    // pack patches_per_wave (1, 2 or 4) patches into each wavefront
+    long long group_min_patches = 4096;
+    int       patches_per_wave  = 2;
   };
   Tuning &tuning();
 

@@ -868,53 +868,104 @@ namespace gtps
     }
 
     // ---------------------------------------------------------------------
-    // Single-patch kernel for small colours (coarse levels): one wavefront
-    // per patch, so a launch with few patches is not serialised through a
-    // few waves.  Lane (y,x) loads its z-column of the skeleton and lane
-    // (z,y) its interior b-row at the start (all loads in flight at once);
-    // the seven contraction / FD stages then run through LDS inside one
-    // wave.  Same MODE semantics as the wave kernel.  Needs (2k+1)^2 <= 64.
-    template <int K, typename T, int MODE>
+    // Patch-group kernel: one wavefront per work-group handles NPW patches
+    // (NPW = 1 for small colours on coarse levels, where latency dominates;
+    // 2-4 on the finest level, where packing the 25-line FD stages of
+    // several patches into one wave cuts the issued VALU instructions).
+    // Lane tasks per stage: (patch, z-pencil) loads the skeleton column and
+    // applies the z-rows; (patch, z-row, x) applies the y-rows; (patch,
+    // interior (z,y) line) forms the residual and S^T along x; then S^T y,
+    // S^T z + 1/(h^(d-2) sum lambda) + S z, S y and S x with the store.
+    // The interior b-rows are loaded at the start so that every global load
+    // is in flight at once.  Same MODE semantics as the wave kernel.
+    // Needs (2k+1)^2 <= 64.
+    template <int K>
+    struct SG
+    {
+      static constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr int ZS = 2 * NI * NP * NP;  // z-rows of M and K
+      static constexpr int YS = 2 * NI * NI * NP;  // y-rows (C1, C2)
+      // FD work buffer (NI^3) reuses the z-row region after stage 2
+      static constexpr int PER = (ZS + YS) | 1;
+    };
+
+    template <int K, typename T, int MODE, int NPW>
     __global__ void __launch_bounds__(64)
-    patch3d_single_kernel(T *__restrict__ u, const T *__restrict__ src,
-                          const T *__restrict__ tab, const int n,
-                          const int colour, const T scale, const T inv_scale,
-                          const T relax)
+    patch3d_group_kernel(T *__restrict__ u, const T *__restrict__ src,
+                         const T *__restrict__ tab, const int n,
+                         const int colour, const T scale, const T inv_scale,
+                         const T relax)
     {
       using TB         = Tab<K>;
-      constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
+      using C          = SG<K>;
+      constexpr int NP = C::NP, NI = C::NI, PER = C::PER, ZS = C::ZS;
       static_assert(NP * NP <= 64, "one lane per z-pencil");
-      __shared__ T sZ[2 * NI * NP * NP], sY[2 * NI * NI * NP], sR[NI * NI * NI];
+      constexpr int P1 = (NPW * NP * NP + 63) / 64;  // passes per stage
+      constexpr int P2 = (NPW * NI * NP + 63) / 64;
+      constexpr int P3 = (NPW * NI * NI + 63) / 64;
+      __shared__ T      sm[NPW * PER];
+      __shared__ long long sfirst[NPW];
       const int       N1   = K * n + 1;
       const long long P    = (long long)N1 * N1;
       const int       lane = threadIdx.x;
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
-      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py);
-      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
-      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
-      const long long f = (long long)K * (2 * qx + px) +
-                          (long long)K * (2 * qy + py) * N1 +
-                          (long long)K * (2 * qz + pz) * P;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
+                npz = n_patches_1d(n, pz);
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
+      if (lane < NPW)
+        {
+          const int q = blk * NPW + lane;
+          if (q < npx * npy * npz)
+            {
+              const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+              sfirst[lane] = (long long)K * (2 * qx + px) +
+                             (long long)K * (2 * qy + py) * N1 +
+                             (long long)K * (2 * qz + pz) * P;
+            }
+          else
+            sfirst[lane] = -1;
+        }
+      __syncthreads();
       const T *pM   = tab + TB::patchM;
       const T *pK   = tab + TB::patchK;
       const T *S    = tab + TB::S;
       const T *linv = tab + TB::lamInv3;
 
-      // interior right-hand side (or residual) rows, loaded up front
-      T rb[NI];
-      if (lane < NI * NI)
-        {
-          const int       zi = lane / NI, yi = lane - zi * NI;
-          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+      // interior right-hand side (or residual) rows of this lane's stage-3
+      // tasks, loaded up front
+      T rb[P3][NI];
 #pragma unroll
-          for (int xi = 0; xi < NI; ++xi)
-            rb[xi] = src[g0 + xi];
+      for (int pp = 0; pp < P3; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t < NPW * NI * NI)
+            {
+              const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
+              const long long f  = sfirst[lp];
+              const int       zi = r / NI, yi = r - zi * NI;
+              if (f >= 0)
+                {
+                  const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+#pragma unroll
+                  for (int xi = 0; xi < NI; ++xi)
+                    rb[pp][xi] = src[g0 + xi];
+                }
+            }
         }
       if (MODE == 0)
         {
-          if (lane < NP * NP)
+          // stage 1: z-pencils of the skeleton, z-rows of M and K
+#pragma unroll
+          for (int pp = 0; pp < P1; ++pp)
             {
-              const int  iy = lane / NP, ix = lane - iy * NP;
+              const int t = lane + 64 * pp;
+              if (t >= NPW * NP * NP)
+                continue;
+              const int       lp = t / (NP * NP), pc = t - lp * (NP * NP);
+              const long long f  = sfirst[lp];
+              if (f < 0)
+                continue;
+              const int  iy = pc / NP, ix = pc - iy * NP;
               const bool face = ix == 0 || ix == NP - 1 || iy == 0 || iy == NP - 1;
               T          v[NP];
 #pragma unroll
@@ -923,6 +974,7 @@ namespace gtps
                   const long long g = f + ix + (long long)iy * N1 + iz * P;
                   v[iz] = (face || iz == 0 || iz == NP - 1) ? u[g] : T(0);
                 }
+              T *sz = sm + lp * PER;
 #pragma unroll
               for (int zi = 0; zi < NI; ++zi)
                 {
@@ -934,20 +986,30 @@ namespace gtps
                       m += pM[row * NP + c] * v[c];
                       kk += pK[row * NP + c] * v[c];
                     }
-                  sZ[zi * NP * NP + lane]        = m;
-                  sZ[(NI + zi) * NP * NP + lane] = kk;
+                  sz[zi * NP * NP + pc]        = m;
+                  sz[(NI + zi) * NP * NP + pc] = kk;
                 }
             }
           __syncthreads();
-          if (lane < NI * NP)
+          // stage 2: y-rows, C1 = My Mz u, C2 = Ky Mz u + My Kz u
+#pragma unroll
+          for (int pp = 0; pp < P2; ++pp)
             {
-              const int zi = lane / NP, ix = lane - zi * NP;
+              const int t = lane + 64 * pp;
+              if (t >= NPW * NI * NP)
+                continue;
+              const int lp = t / (NI * NP), r = t - lp * (NI * NP);
+              if (sfirst[lp] < 0)
+                continue;
+              const int zi = r / NP, ix = r - zi * NP;
+              const T  *sz = sm + lp * PER;
+              T        *sy = sm + lp * PER + ZS;
               T         a[NP], b[NP];
 #pragma unroll
               for (int iy = 0; iy < NP; ++iy)
                 {
-                  a[iy] = sZ[(zi * NP + iy) * NP + ix];
-                  b[iy] = sZ[((NI + zi) * NP + iy) * NP + ix];
+                  a[iy] = sz[(zi * NP + iy) * NP + ix];
+                  b[iy] = sz[((NI + zi) * NP + iy) * NP + ix];
                 }
 #pragma unroll
               for (int yi = 0; yi < NI; ++yi)
@@ -961,35 +1023,45 @@ namespace gtps
                       c2 += pK[row * NP + c] * a[c];
                       c2 += pM[row * NP + c] * b[c];
                     }
-                  sY[(zi * NI + yi) * NP + ix]            = c1;
-                  sY[((NI + zi) * NI + yi) * NP + ix]     = c2;
+                  sy[(zi * NI + yi) * NP + ix]        = c1;
+                  sy[((NI + zi) * NI + yi) * NP + ix] = c2;
                 }
             }
           __syncthreads();
         }
       if (MODE == 2)
         {
-          for (int t = lane; t < NP * NP * NP; t += 64)
+          for (int t = lane; t < NPW * NP * NP * NP; t += 64)
             {
-              const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
+              const int       lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
+              const long long f  = sfirst[lp];
+              const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
               const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
                                  iz > 0 && iz < NP - 1;
-              if (!inner)
+              if (f >= 0 && !inner)
                 u[f + ix + (long long)iy * N1 + iz * P] = T(0);
             }
         }
-      if (lane < NI * NI)
+      // stage 3: x-rows, residual, S^T along x -> FD buffer (z-row region)
+#pragma unroll
+      for (int pp = 0; pp < P3; ++pp)
         {
-          const int r = lane;
-          T         res[NI];
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (sfirst[lp] < 0)
+            continue;
+          T res[NI];
           if (MODE == 0)
             {
-              T c1[NP], c2[NP];
+              const T *sy = sm + lp * PER + ZS;
+              T        c1[NP], c2[NP];
 #pragma unroll
               for (int ix = 0; ix < NP; ++ix)
                 {
-                  c1[ix] = sY[r * NP + ix];
-                  c2[ix] = sY[(NI * NI + r) * NP + ix];
+                  c1[ix] = sy[r * NP + ix];
+                  c2[ix] = sy[(NI * NI + r) * NP + ix];
                 }
 #pragma unroll
               for (int xi = 0; xi < NI; ++xi)
@@ -1002,15 +1074,16 @@ namespace gtps
                       v += pK[row * NP + c] * c1[c];
                       v += pM[row * NP + c] * c2[c];
                     }
-                  res[xi] = rb[xi] - scale * v;
+                  res[xi] = rb[pp][xi] - scale * v;
                 }
             }
           else
             {
 #pragma unroll
               for (int xi = 0; xi < NI; ++xi)
-                res[xi] = rb[xi];
+                res[xi] = rb[pp][xi];
             }
+          T *sr = sm + lp * PER;
 #pragma unroll
           for (int j = 0; j < NI; ++j)
             {
@@ -1018,15 +1091,22 @@ namespace gtps
 #pragma unroll
               for (int i = 0; i < NI; ++i)
                 s += S[i * NI + j] * res[i];
-              sR[r * NI + j] = s;
+              sr[r * NI + j] = s;
             }
         }
       __syncthreads();
-      // S^T along y
-      if (lane < NI * NI)
+      // stage 4: S^T along y
+#pragma unroll
+      for (int pp = 0; pp < P3; ++pp)
         {
-          const int zi = lane / NI, xi = lane - zi * NI;
-          T        *base = sR + zi * NI * NI + xi;
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (sfirst[lp] < 0)
+            continue;
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sm + lp * PER + zi * NI * NI + xi;
           T         v[NI];
 #pragma unroll
           for (int i = 0; i < NI; ++i)
@@ -1042,11 +1122,18 @@ namespace gtps
             }
         }
       __syncthreads();
-      // S^T along z, eigenvalue scaling, S along z
-      if (lane < NI * NI)
+      // stage 5: S^T along z, eigenvalue scaling, S along z
+#pragma unroll
+      for (int pp = 0; pp < P3; ++pp)
         {
-          const int yi = lane / NI, xi = lane - yi * NI;
-          T        *base = sR + yi * NI + xi;
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (sfirst[lp] < 0)
+            continue;
+          const int yi = r / NI, xi = r - yi * NI;
+          T        *base = sm + lp * PER + yi * NI + xi;
           T         v[NI], tt[NI];
 #pragma unroll
           for (int i = 0; i < NI; ++i)
@@ -1071,11 +1158,18 @@ namespace gtps
             }
         }
       __syncthreads();
-      // S along y
-      if (lane < NI * NI)
+      // stage 6: S along y
+#pragma unroll
+      for (int pp = 0; pp < P3; ++pp)
         {
-          const int zi = lane / NI, xi = lane - zi * NI;
-          T        *base = sR + zi * NI * NI + xi;
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (sfirst[lp] < 0)
+            continue;
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sm + lp * PER + zi * NI * NI + xi;
           T         v[NI];
 #pragma unroll
           for (int j = 0; j < NI; ++j)
@@ -1091,14 +1185,22 @@ namespace gtps
             }
         }
       __syncthreads();
-      // S along x, store
-      if (lane < NI * NI)
+      // stage 7: S along x, store
+#pragma unroll
+      for (int pp = 0; pp < P3; ++pp)
         {
-          const int zi = lane / NI, yi = lane - zi * NI;
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
+          const long long f  = sfirst[lp];
+          if (f < 0)
+            continue;
+          const int zi = r / NI, yi = r - zi * NI;
           T         v[NI];
 #pragma unroll
           for (int j = 0; j < NI; ++j)
-            v[j] = sR[lane * NI + j];
+            v[j] = sm[lp * PER + r * NI + j];
           const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
 #pragma unroll
           for (int i = 0; i < NI; ++i)
@@ -1150,18 +1252,33 @@ namespace gtps
       if constexpr (2 * K + 1 <= 8)
         if (lvl->dim == 3 && np <= small_colour_limit())
           {
-            const T scale = (T)lvl->scale, inv = (T)(1.0 / lvl->scale);
-#define GTPS_SS(M)                                                            \
-  hipLaunchKernelGGL((patch3d_single_kernel<K, T, M>), dim3((unsigned)np),    \
-                     dim3(64), 0, s, up, sp, tab, n, colour, scale, inv,      \
-                     (T)relax)
+            const T   scale = (T)lvl->scale, inv = (T)(1.0 / lvl->scale);
+            // one patch per wave while a colour has too few patches to fill
+            // the GPU several times over; packed groups beyond that
+            const int npw = np <= tuning().group_min_patches ? 1 : tuning().patches_per_wave;
+#define GTPS_SG(M, W)                                                         \
+  hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W>),                      \
+                     dim3((unsigned)((np + W - 1) / W)), dim3(64), 0, s, up,  \
+                     sp, tab, n, colour, scale, inv, (T)relax)
+#define GTPS_SG_W(M)                                                          \
+  do                                                                          \
+    {                                                                         \
+      if (npw == 2)                                                           \
+        GTPS_SG(M, 2);                                                        \
+      else if (npw == 4)                                                      \
+        GTPS_SG(M, 4);                                                        \
+      else                                                                    \
+        GTPS_SG(M, 1);                                                        \
+    }                                                                         \
+  while (0)
             if (mode == 0)
-              GTPS_SS(0);
+              GTPS_SG_W(0);
             else if (mode == 1)
-              GTPS_SS(1);
+              GTPS_SG_W(1);
             else
-              GTPS_SS(2);
-#undef GTPS_SS
+              GTPS_SG_W(2);
+#undef GTPS_SG_W
+#undef GTPS_SG
             GTPS_LAUNCH_CHECK();
             return;
           }

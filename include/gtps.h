@@ -72,6 +72,10 @@ const char *gtps_version(void);
  *                           (default 2^62, i.e. always the
  *                           single-patch kernel where it applies: 3D,
  *                           k <= 3).
+ *   "group_min_patches"     colours with more patches than this pack
+ *                           several patches into each wavefront (default
+ *                           4096);
+ *   "patches_per_wave"      how many (1, 2 or 4; default 2).
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */

@@ -180,13 +180,23 @@ def test_tensor_product_data(core, ctx):
         assert np.abs(d["cell_laplace"] - ref["K"]).max() < 1e-12
 
 
+SMOOTHER_PATHS = {
+    # small_colour_patches, group_min_patches, patches_per_wave
+    "single": (1 << 40, 1 << 40, 2),
+    "group2": (1 << 40, 0, 2),
+    "group4": (1 << 40, 0, 4),
+    "multi": (0, 4096, 2),
+}
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("variant", ["fwd", "add"])
-@pytest.mark.parametrize("path", ["single", "multi"])
+@pytest.mark.parametrize("path", list(SMOOTHER_PATHS))
 @pytest.mark.parametrize("dim,k,level", [(3, 1, 4), (3, 2, 3), (3, 3, 3), (3, 3, 4)])
 def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, path):
-    """The single-patch (coarse-level) and multi-patch (fine-level) smoother
-    kernels agree with the oracle on the same level."""
+    """The patch-group kernel at 1, 2 and 4 patches per wavefront (coarse and
+    fine levels) and the multi-wave kernel agree with the oracle on the same
+    level; odd patch counts leave a partly empty last group."""
     _announce("smoother-path", dim, k, level, variant, prec, path)
     g = rng(level * 7 + k)
     N = n_dofs(dim, k, level)
@@ -197,7 +207,8 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
     ov, cv, relax = {"fwd": (O.MULT_FWD, core.PatchSmoother.MULTIPLICATIVE, 1.0),
                      "add": (O.ADDITIVE, core.PatchSmoother.ADDITIVE, 0.5)}[variant]
     ref = O.smoother_step(dim, k, level, b, u, variant=ov, relax=relax)
-    prev = core.set_tuning("small_colour_patches", 1 << 40 if path == "single" else 0)
+    keys = ("small_colour_patches", "group_min_patches", "patches_per_wave")
+    prev = [core.set_tuning(kk, v) for kk, v in zip(keys, SMOOTHER_PATHS[path])]
     try:
         op = core.LaplaceOperator(ctx, dim, k, level, _dt(prec))
         sm = core.PatchSmoother(op, cv, relax)
@@ -205,6 +216,7 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
         sm.step(ud, _to_dev(b, prec))
         torch.cuda.synchronize()
     finally:
-        core.set_tuning("small_colour_patches", prev)
+        for kk, v in zip(keys, prev):
+            core.set_tuning(kk, v)
     tol = TOL[prec] * (20 if prec == "fp32" else 1)
     assert rel_err(ud.cpu().numpy(), ref) < tol
