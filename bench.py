@@ -37,25 +37,9 @@ HBM_PEAK_GBS = 8000.0
 
 def rhs_splitmix(dim: int, k: int, level: int, seed: int = SEED):
     """b_i = 2 u01(i) - 1, u01(i) = (splitmix64(seed ^ i) >> 11) 2^-53,
-    zero on Dirichlet DoFs (numpy, uint64 arithmetic wraps mod 2^64)."""
-    import numpy as np
-    n1 = k * (1 << level) + 1
-    n = n1 ** dim
-    with np.errstate(over="ignore"):
-        x = np.arange(n, dtype=np.uint64) ^ np.uint64(seed)
-        x = x + np.uint64(0x9E3779B97F4A7C15)
-        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        x = x ^ (x >> np.uint64(31))
-    b = 2.0 * ((x >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
-    idx = np.arange(n1)
-    e = (idx == 0) | (idx == n1 - 1)
-    if dim == 3:
-        bd = e[:, None, None] | e[None, :, None] | e[None, None, :]
-    else:
-        bd = e[:, None] | e[None, :]
-    b[bd.ravel()] = 0.0
-    return b
+    zero on Dirichlet DoFs."""
+    from gputensorproductsmoother_amd import problems
+    return problems.rhs_random(dim, k, level, seed)
 
 
 def smoother_colour_values(dim: int, k: int, level: int):

@@ -52,6 +52,9 @@ SIGNATURES = {
     "gtps_laplace_vmult": (_i, [_vp, _vp, _vp]),
     "gtps_laplace_residual": (_i, [_vp, _vp, _vp, _vp]),
     "gtps_level_tensor_product_data": (_i, [_vp, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "gtps_level_set_coefficient": (_i, [_vp, _vp]),
+    "gtps_laplace_vmult_transpose": (_i, [_vp, _vp, _vp]),
+    "gtps_level_coefficient_gradient": (_i, [_vp, _vp, _vp, _vp]),
     "gtps_smoother_create": (_i, [_vp, _i, _d, ctypes.POINTER(_vp)]),
     "gtps_smoother_destroy": (_i, [_vp]),
     "gtps_smoother_step": (_i, [_vp, _vp, _vp]),
@@ -74,6 +77,9 @@ SIGNATURES = {
     "gtps_mg_vcycle": (_i, [_vp, _vp, _vp]),
     "gtps_mg_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _pd]),
     "gtps_mg_matvec": (_i, [_vp, _vp, _vp]),
+    "gtps_mg_set_coefficient": (_i, [_vp, _vp]),
+    "gtps_mg_matvec_transpose": (_i, [_vp, _vp, _vp]),
+    "gtps_mg_coefficient_gradient": (_i, [_vp, _vp, _vp, _vp]),
     "gtps_mg_set_graph": (_i, [_vp, _i]),
     "gtps_mg_set_profiling": (_i, [_vp, _i]),
     "gtps_mg_profile": (_i, [_vp, _i, ctypes.POINTER(_ll), _pd]),

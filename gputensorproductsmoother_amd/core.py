@@ -172,6 +172,35 @@ class LaplaceOperator:
         L.check(L.load().gtps_laplace_residual(self._h, _ptr(dst), _ptr(b), _ptr(u)),
                 "residual")
 
+    @property
+    def n_cells(self) -> int:
+        return (1 << self.level) ** self.dim
+
+    def set_coefficient(self, coef: torch.Tensor | None):
+        """Per-cell coefficient c (FP64 on the GPU, cells cx + n cy + n^2 cz)
+        for -div(c grad u); None restores c = 1 (3D only)."""
+        if coef is None:
+            L.check(L.load().gtps_level_set_coefficient(self._h, None), "set_coefficient")
+            return
+        _chk_vec(coef, torch.float64, self.n_cells)
+        L.check(L.load().gtps_level_set_coefficient(self._h, _ptr(coef)), "set_coefficient")
+
+    def vmult_transpose(self, dst: torch.Tensor, src: torch.Tensor):
+        """dst = A^T src (identity Dirichlet rows of A become A_IB^T columns)."""
+        _chk_vec(dst, self.dtype, self.n_dofs)
+        _chk_vec(src, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_laplace_vmult_transpose(self._h, _ptr(dst), _ptr(src)),
+                "vmult_transpose")
+
+    def coefficient_gradient(self, grad: torch.Tensor, lam: torch.Tensor, u: torch.Tensor):
+        """grad_K = -int_K grad(lambda~) . grad(u) (lambda's Dirichlet entries
+        masked)."""
+        _chk_vec(grad, self.dtype, self.n_cells)
+        _chk_vec(lam, self.dtype, self.n_dofs)
+        _chk_vec(u, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_level_coefficient_gradient(self._h, _ptr(grad), _ptr(lam),
+                                                         _ptr(u)), "coefficient_gradient")
+
     def set_constrained_values(self, v: torch.Tensor, value: float = 0.0):
         _chk_vec(v, self.dtype, self.n_dofs)
         L.check(L.load().gtps_vec_set_boundary(self._h, _ptr(v), value), "set_boundary")
@@ -306,6 +335,8 @@ class MultigridSolvers:
                                         schedule, relaxation, ctypes.byref(h)), "mg_create")
         self._h = h
         self.n_dofs = int(L.load().gtps_mg_n_dofs(h))
+        self._coef = None
+        self._bmask = None
 
     def vmult(self, dst: torch.Tensor, src: torch.Tensor):
         """Preconditioner z = V(r) (MultigridSolvers::vmult)."""
@@ -333,6 +364,53 @@ class MultigridSolvers:
         if rc != L.GTPS_OK and (raise_on_failure or rc not in (L.GTPS_ENOTCONV,)):
             L.check(rc, "pcg_solve")
         return x, it.value, rel.value, list(hist[: it.value + 1])
+
+    @property
+    def n_cells(self) -> int:
+        return (1 << self.max_level) ** self.dim
+
+    def set_coefficient(self, coef: torch.Tensor | None):
+        """Per-cell coefficient of the finest level (FP64 on the GPU); coarse
+        levels get the mean of their children.  None restores c = 1."""
+        if coef is None:
+            L.check(L.load().gtps_mg_set_coefficient(self._h, None), "set_coefficient")
+            self._coef = None
+            return
+        _chk_vec(coef, torch.float64, self.n_cells)
+        L.check(L.load().gtps_mg_set_coefficient(self._h, _ptr(coef)), "set_coefficient")
+        self._coef = coef
+
+    def matvec_transpose(self, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, torch.float64, self.n_dofs)
+        _chk_vec(src, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_mg_matvec_transpose(self._h, _ptr(dst), _ptr(src)),
+                "matvec_transpose")
+
+    def coefficient_gradient(self, grad: torch.Tensor, lam: torch.Tensor, u: torch.Tensor):
+        _chk_vec(grad, torch.float64, self.n_cells)
+        _chk_vec(lam, torch.float64, self.n_dofs)
+        _chk_vec(u, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_mg_coefficient_gradient(self._h, _ptr(grad), _ptr(lam), _ptr(u)),
+                "coefficient_gradient")
+
+    def boundary_mask(self) -> torch.Tensor:
+        """Boolean mask of the Dirichlet DoFs of the finest level."""
+        if getattr(self, "_bmask", None) is None:
+            n1 = self.fe_degree * (1 << self.max_level) + 1
+            i = torch.arange(self.n_dofs, device=f"cuda:{self.ctx.device}")
+            x, y = i % n1, (i // n1) % n1
+            m = (x == 0) | (x == n1 - 1) | (y == 0) | (y == n1 - 1)
+            if self.dim == 3:
+                z = i // (n1 * n1)
+                m |= (z == 0) | (z == n1 - 1)
+            self._bmask = m
+        return self._bmask
+
+    def solve(self, b: torch.Tensor, coef: torch.Tensor | None = None, rtol: float = 1e-10,
+              maxit: int = 200) -> torch.Tensor:
+        """Differentiable solve u = A(c)^{-1} b (torch.autograd): gradients
+        flow to b and, when given, to the per-cell coefficient c."""
+        return MGSolve.apply(self, b, coef, rtol, maxit)
 
     def set_graph(self, enable: bool = True):
         """Replay the per-iteration V-cycle sequence as one HIP graph
@@ -368,3 +446,54 @@ class MultigridSolvers:
                 self._h = None
         except Exception:
             pass
+
+
+class MGSolve(torch.autograd.Function):
+    """u = A(c)^{-1} b through the MG-preconditioned CG, differentiable in b
+    and in the per-cell coefficient c (c = None: the solver's current
+    operator, which must not change before the backward pass).
+
+    A has identity Dirichlet rows and keeps its boundary columns, so with
+    I/B the interior/Dirichlet DoFs, A^T lambda = g splits into
+    A_II lambda_I = g_I (solved by the same symmetric MG-CG on a right-hand
+    side with zero Dirichlet entries) and lambda_B = g_B - A_IB^T lambda_I
+    (one transpose product).  Then dL/db = lambda and
+    dL/dc_K = -lambda~_K^T A^1_K u_K with lambda~ = lambda_I (zero on B),
+    computed by the cell-wise gradient kernel.
+    """
+
+    @staticmethod
+    def forward(ctx, solver: "MultigridSolvers", b: torch.Tensor, coef, rtol: float,
+                maxit: int):
+        c = None
+        if coef is not None:
+            c = coef.detach().contiguous()
+            solver.set_coefficient(c)
+        u, it, rel, _ = solver.solve_cg(b.detach().contiguous(), rtol=rtol, maxit=maxit)
+        ctx.solver, ctx.rtol, ctx.maxit = solver, rtol, maxit
+        ctx.has_coef = c is not None
+        ctx.forward_info = (it, rel)
+        ctx.save_for_backward(u, c)
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_u: torch.Tensor):
+        solver = ctx.solver
+        u, coef = ctx.saved_tensors
+        if ctx.has_coef and (solver._coef is None
+                             or solver._coef.data_ptr() != coef.data_ptr()):
+            solver.set_coefficient(coef.contiguous())
+        g = grad_u.detach().to(torch.float64).contiguous()
+        mask = solver.boundary_mask()
+        rhs = torch.where(mask, torch.zeros_like(g), g)
+        lam, _, _, _ = solver.solve_cg(rhs, rtol=ctx.rtol, maxit=ctx.maxit)
+        grad_b = None
+        if ctx.needs_input_grad[1]:
+            t = torch.empty_like(lam)
+            solver.matvec_transpose(t, lam)
+            grad_b = torch.where(mask, g - t, lam)
+        grad_c = None
+        if ctx.has_coef and ctx.needs_input_grad[2]:
+            grad_c = torch.empty(solver.n_cells, dtype=torch.float64, device=g.device)
+            solver.coefficient_gradient(grad_c, lam, u)
+        return None, grad_b, grad_c, None, None

@@ -118,6 +118,44 @@ namespace
     return lvl.release();
   }
 
+  void
+  ensure_celem(gtps_level *lvl)
+  {
+    if (!lvl->celem)
+      {
+        const long long ncells = (long long)lvl->n * lvl->n * lvl->n;
+        const long long n3     = (long long)(lvl->k + 1) * (lvl->k + 1) * (lvl->k + 1);
+        lvl->celem             = dmalloc(lvl->elem() * ncells * n3);
+      }
+  }
+
+  // copy (and round) a device FP64 per-cell coefficient into the level
+  void
+  set_level_coefficient(gtps_level *lvl, const double *c)
+  {
+    GTPS_REQUIRE(lvl->dim == 3, "the variable-coefficient operator is 3D");
+    const long long ncells = (long long)lvl->n * lvl->n * lvl->n;
+    if (!lvl->coef)
+      lvl->coef = dmalloc(lvl->elem() * ncells);
+    ensure_celem(lvl);
+    if (lvl->prec == GTPS_FP32)
+      launch_convert_f64_f32(lvl->ctx->stream, ncells, c, (float *)lvl->coef);
+    else
+      GTPS_HIP(hipMemcpyAsync(lvl->coef, c, sizeof(double) * ncells,
+                              hipMemcpyDeviceToDevice, lvl->ctx->stream));
+  }
+
+  void
+  clear_level_coefficient(gtps_level *lvl)
+  {
+    if (lvl->coef)
+      {
+        GTPS_HIP(hipStreamSynchronize(lvl->ctx->stream));
+        dfree(lvl->coef);
+        lvl->coef = nullptr;
+      }
+  }
+
   // brackets every colour launch of a sweep when a timer is attached
   struct LaunchTimer
   {
@@ -137,6 +175,27 @@ namespace
         bool from_zero = false)
   {
     const int ncol = 1 << lvl->dim;
+    if (lvl->coef)
+      {
+        // variable coefficient: correction-form sweep with the inexact local
+        // solve (no "from zero" colour pass, no additive form)
+        GTPS_REQUIRE(variant != GTPS_ADDITIVE,
+                     "the additive smoother has no variable-coefficient form");
+        GTPS_REQUIRE(coeff_smoother_supported(lvl->k, lvl->prec),
+                     "the variable-coefficient smoother supports degree <= 5 "
+                     "in FP64 and <= 7 in FP32");
+        if (from_zero)
+          launch_fill(lvl->ctx->stream, lvl->N, lvl->prec, u, 0.0);
+        for (int cc = 0; cc < ncol; ++cc)
+          {
+            const int  c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
+            const long e = timer ? timer->tic_launch() : -1;
+            launch_patch_coeff_colour(lvl, u, b, c, relax);
+            if (timer)
+              timer->toc_launch(e);
+          }
+        return;
+      }
     if (from_zero && variant != GTPS_MULT_FWD)
       launch_fill(lvl->ctx->stream, lvl->N, lvl->prec, u, 0.0);
     if (variant == GTPS_ADDITIVE)
@@ -545,6 +604,44 @@ extern "C"
       GTPS_REQUIRE(dst != u, "residual output cannot alias u");
       set_device(lvl->ctx);
       launch_laplace(lvl, dst, u, b, false);
+    });
+  }
+
+  int
+  gtps_laplace_vmult_transpose(gtps_level *lvl, void *dst, const void *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && dst && src, "NULL argument");
+      GTPS_REQUIRE(dst != src, "vmult cannot run in place");
+      GTPS_REQUIRE(lvl->dim == 3, "the transpose operator is 3D");
+      set_device(lvl->ctx);
+      ensure_celem(lvl);
+      launch_coeff_apply(lvl, dst, src, nullptr, false, true);
+    });
+  }
+
+  int
+  gtps_level_set_coefficient(gtps_level *lvl, const double *coef)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl, "NULL level");
+      set_device(lvl->ctx);
+      if (coef)
+        set_level_coefficient(lvl, coef);
+      else
+        clear_level_coefficient(lvl);
+    });
+  }
+
+  int
+  gtps_level_coefficient_gradient(gtps_level *lvl, void *grad,
+                                  const void *lambda, const void *u)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && grad && lambda && u, "NULL argument");
+      GTPS_REQUIRE(lvl->dim == 3, "the coefficient gradient is 3D");
+      set_device(lvl->ctx);
+      launch_coeff_gradient(lvl, grad, lambda, u);
     });
   }
 
@@ -998,6 +1095,90 @@ extern "C"
                    "bad argument");
       set_device(mg->ctx);
       launch_laplace(mg->fine64, dst, src, nullptr, false);
+    });
+  }
+
+  int
+  gtps_mg_matvec_transpose(gtps_mg *mg, double *dst, const double *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && dst && src && (const void *)dst != (const void *)src,
+                   "bad argument");
+      GTPS_REQUIRE(mg->dim == 3, "the transpose operator is 3D");
+      set_device(mg->ctx);
+      ensure_celem(mg->fine64);
+      launch_coeff_apply(mg->fine64, dst, src, nullptr, false, true);
+    });
+  }
+
+  int
+  gtps_mg_set_coefficient(gtps_mg *mg, const double *coef)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      set_device(mg->ctx);
+      hipStream_t s = mg->ctx->stream;
+      // kernels change: the captured preconditioning graph is stale
+      if (mg->dir_exec)
+        {
+          GTPS_HIP(hipStreamSynchronize(s));
+          (void)hipGraphExecDestroy(mg->dir_exec);
+          mg->dir_exec = nullptr;
+        }
+      if (!coef)
+        {
+          for (int l = 1; l <= mg->maxl; ++l)
+            clear_level_coefficient(mg->lv[l]);
+          if (mg->own_fine64)
+            clear_level_coefficient(mg->fine64);
+          return;
+        }
+      GTPS_REQUIRE(mg->dim == 3, "the variable-coefficient operator is 3D");
+      GTPS_REQUIRE(mg->schedule != GTPS_MG_ADDITIVE,
+                   "the additive smoother has no variable-coefficient form");
+      GTPS_REQUIRE(coeff_smoother_supported(mg->k, mg->prec),
+                   "the variable-coefficient smoother supports degree <= 5 in "
+                   "FP64 and <= 7 in FP32");
+      // FP64 chain of averaged coefficients, finest first
+      std::vector<double *> chain(mg->maxl + 1, nullptr);
+      try
+        {
+          for (int l = mg->maxl - 1; l >= 1; --l)
+            {
+              const long long nc = 1LL << (3 * l);
+              chain[l]           = (double *)dmalloc(sizeof(double) * nc);
+              launch_coarsen_coefficient(s, 1 << l,
+                                         l + 1 == mg->maxl ? coef : chain[l + 1],
+                                         chain[l]);
+            }
+          set_level_coefficient(mg->lv[mg->maxl], coef);
+          for (int l = 1; l < mg->maxl; ++l)
+            set_level_coefficient(mg->lv[l], chain[l]);
+          if (mg->own_fine64)
+            set_level_coefficient(mg->fine64, coef);
+          GTPS_HIP(hipStreamSynchronize(s));
+        }
+      catch (...)
+        {
+          (void)hipStreamSynchronize(s);
+          for (double *c : chain)
+            dfree(c);
+          throw;
+        }
+      for (double *c : chain)
+        dfree(c);
+    });
+  }
+
+  int
+  gtps_mg_coefficient_gradient(gtps_mg *mg, double *grad, const double *lambda,
+                               const double *u)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && grad && lambda && u, "NULL argument");
+      GTPS_REQUIRE(mg->dim == 3, "the coefficient gradient is 3D");
+      set_device(mg->ctx);
+      launch_coeff_gradient(mg->fine64, grad, lambda, u);
     });
   }
 

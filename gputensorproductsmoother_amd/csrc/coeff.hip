@@ -1,0 +1,738 @@
+// coeff.hip - variable-coefficient diffusion on the uniform mesh (3D):
+// A(c) = sum_K c_K A^1_K with c constant per cell, its transpose, the
+// gradient dJ/dc_K = -lambda~_K^T A^1_K u_K, coefficient coarsening, and a
+// vertex-patch smoother with an inexact fast-diagonalisation local solve.
+//
+// The reference carries a coefficient through its solver interface but
+// never uses it (include/solver.cuh:1548; the matrix-free hook
+// evaluate_coeff, include/dealii/cuda_mf.template.cuh:569-581, 815-824, is
+// not called on the hot path), so these kernels have no reference
+// counterpart; they reuse the layout and conventions of the
+// constant-coefficient path (lexicographic DoFs, identity Dirichlet rows,
+// boundary columns kept).  Cells are numbered cx + n cy + n^2 cz.
+//
+// Operator: cell-wise, the reference-cell Laplacian is the Kronecker sum
+// h (K^ (x) M^ (x) M^ + M^ (x) K^ (x) M^ + M^ (x) M^ (x) K^) of the 1D cell
+// matrices (exact with Gauss(k+1)), applied by sum factorisation: z-rows,
+// then y-rows, then x-rows, one thread per cell pencil.  Cell results go to
+// an element-vector buffer that a DoF-centred gather adds up in a fixed
+// order (deterministic, no atomics).
+//
+// Smoother: per colour and patch, u_I <- u_I + relax A~_P^{-1} (b - A u)_I
+// with A~_P = cmax_P A^1_P (cmax_P = largest coefficient of the patch's 8
+// cells), so A~_P >= A_P,II and the local step is a contraction; the local
+// residual is formed from the 8 cells of the patch in LDS.
+#include "gtps_internal.h"
+
+namespace gtps
+{
+  namespace
+  {
+    constexpr int CNT = 256;  // threads per work-group
+
+    __host__ __device__ constexpr int
+    cells_per_block(int K)
+    {
+      return CNT / ((K + 1) * (K + 1));
+    }
+
+    __device__ __forceinline__ bool
+    on_bdry(int gx, int gy, int gz, int N1)
+    {
+      return gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1 || gz == 0 ||
+             gz == N1 - 1;
+    }
+
+    // MODE 0: element vectors E[cell][n^3] = c_K scale A^1 x_K (Dirichlet
+    //         inputs masked when mask != 0);
+    // MODE 1: grad[cell] = -scale lambda~_K . A^1 u_K (lambda's Dirichlet
+    //         entries masked).
+    template <int K, typename T, int MODE>
+    __global__ void __launch_bounds__(CNT)
+    cell_kernel(const T *__restrict__ x, const T *__restrict__ lam,
+                const T *__restrict__ coef, T *__restrict__ out,
+                const T *__restrict__ tab, const int n, const int N1,
+                const T scale, const int mask)
+    {
+      using TB          = Tab<K>;
+      constexpr int NN  = K + 1, N2 = NN * NN, N3 = N2 * NN;
+      constexpr int CPB = cells_per_block(K);
+      __shared__ T  sa[CPB * N3], sb[CPB * N3];
+      __shared__ double red[MODE == 1 ? CNT : 1];
+      const T      *cM   = tab + TB::cellM;
+      const T      *cK   = tab + TB::cellK;
+      const int     tid  = threadIdx.x;
+      const int     lc   = tid / N2, t = tid - lc * N2;
+      const long long ncells = (long long)n * n * n;
+      const long long cell =
+        (long long)xcd_logical_id(blockIdx.x, gridDim.x) * CPB + lc;
+      const bool active = lc < CPB && cell < ncells;
+      const long long P = (long long)N1 * N1;
+      int cx = 0, cy = 0, cz = 0;
+      long long base = 0;
+      if (active)
+        {
+          cx   = (int)(cell % n);
+          cy   = (int)((cell / n) % n);
+          cz   = (int)(cell / ((long long)n * n));
+          base = (long long)K * cx + (long long)K * cy * N1 + (long long)K * cz * P;
+        }
+      T *ca = sa + (active ? lc : 0) * N3;
+      T *cb = sb + (active ? lc : 0) * N3;
+      // z-rows: thread (ix, iy) owns the z-pencil
+      if (active)
+        {
+          const int ix = t % NN, iy = t / NN;
+          T         v[NN];
+#pragma unroll
+          for (int iz = 0; iz < NN; ++iz)
+            {
+              const bool m = mask && on_bdry(K * cx + ix, K * cy + iy, K * cz + iz, N1);
+              v[iz] = m ? T(0) : x[base + ix + (long long)iy * N1 + iz * P];
+            }
+#pragma unroll
+          for (int lz = 0; lz < NN; ++lz)
+            {
+              T a = 0, b = 0;
+#pragma unroll
+              for (int iz = 0; iz < NN; ++iz)
+                {
+                  a += cM[lz * NN + iz] * v[iz];
+                  b += cK[lz * NN + iz] * v[iz];
+                }
+              ca[(lz * NN + iy) * NN + ix] = a;
+              cb[(lz * NN + iy) * NN + ix] = b;
+            }
+        }
+      __syncthreads();
+      // y-rows, in place per (lz, ix) column: c1 = My a, c2 = Ky a + My b
+      if (active)
+        {
+          const int ix = t % NN, lz = t / NN;
+          T         a[NN], b[NN];
+#pragma unroll
+          for (int iy = 0; iy < NN; ++iy)
+            {
+              a[iy] = ca[(lz * NN + iy) * NN + ix];
+              b[iy] = cb[(lz * NN + iy) * NN + ix];
+            }
+#pragma unroll
+          for (int ly = 0; ly < NN; ++ly)
+            {
+              T c1 = 0, c2 = 0;
+#pragma unroll
+              for (int iy = 0; iy < NN; ++iy)
+                {
+                  c1 += cM[ly * NN + iy] * a[iy];
+                  c2 += cK[ly * NN + iy] * a[iy] + cM[ly * NN + iy] * b[iy];
+                }
+              ca[(lz * NN + ly) * NN + ix] = c1;
+              cb[(lz * NN + ly) * NN + ix] = c2;
+            }
+        }
+      __syncthreads();
+      // x-rows per (lz, ly) row: y = Kx c1 + Mx c2
+      double part = 0.0;
+      if (active)
+        {
+          const int ly = t % NN, lz = t / NN;
+          T         c1[NN], c2[NN];
+#pragma unroll
+          for (int ix = 0; ix < NN; ++ix)
+            {
+              c1[ix] = ca[(lz * NN + ly) * NN + ix];
+              c2[ix] = cb[(lz * NN + ly) * NN + ix];
+            }
+          const T f = MODE == 0 ? (coef ? coef[cell] : T(1)) * scale : scale;
+#pragma unroll
+          for (int lx = 0; lx < NN; ++lx)
+            {
+              T y = 0;
+#pragma unroll
+              for (int ix = 0; ix < NN; ++ix)
+                y += cK[lx * NN + ix] * c1[ix] + cM[lx * NN + ix] * c2[ix];
+              if (MODE == 0)
+                out[cell * N3 + (lz * NN + ly) * NN + lx] = f * y;
+              else
+                {
+                  const int gx = K * cx + lx, gy = K * cy + ly, gz = K * cz + lz;
+                  if (!on_bdry(gx, gy, gz, N1))
+                    part += (double)lam[base + lx + (long long)ly * N1 + lz * P] *
+                            (double)(f * y);
+                }
+            }
+        }
+      if (MODE == 1)
+        {
+          red[tid] = part;
+          __syncthreads();
+          if (active && t == 0)
+            {
+              double s = 0.0;
+              for (int q = 0; q < N2; ++q)
+                s += red[lc * N2 + q];
+              out[cell] = (T)(-s);
+            }
+        }
+    }
+
+    // DoF-centred sum of the element vectors, fixed order (lower cell
+    // first per direction).  Rows: RESID y = b - A x; TRANSPOSE boundary
+    // rows y_B = sum + x_B; otherwise identity rows.  DOT: per-block
+    // partial sums of x.y.
+    template <int K, typename T, bool RESID, bool TRANSPOSE, bool DOT>
+    __global__ void __launch_bounds__(CNT)
+    gather_kernel(const long long N, const int N1, const int n,
+                  const T *__restrict__ E, const T *__restrict__ x,
+                  const T *__restrict__ b, T *__restrict__ y,
+                  double *__restrict__ partials)
+    {
+      constexpr int NN = K + 1, N3 = NN * NN * NN;
+      __shared__ double red[CNT / 64];
+      double            dsum = 0.0;
+      const long long   P    = (long long)N1 * N1;
+      for (long long i = (long long)blockIdx.x * CNT + threadIdx.x; i < N;
+           i += (long long)gridDim.x * CNT)
+        {
+          const int gx = (int)(i % N1), gy = (int)((i / N1) % N1),
+                    gz = (int)(i / P);
+          int cxs[2], lxs[2], cys[2], lys[2], czs[2], lzs[2];
+          int nx = 0, ny = 0, nz = 0;
+          auto split = [&](int g, int *cs, int *ls, int &cnt) {
+            const int c = g / K, l = g - c * K;
+            if (l == 0)
+              {
+                if (c > 0)
+                  {
+                    cs[cnt] = c - 1;
+                    ls[cnt] = K;
+                    ++cnt;
+                  }
+                if (c < n)
+                  {
+                    cs[cnt] = c;
+                    ls[cnt] = 0;
+                    ++cnt;
+                  }
+              }
+            else
+              {
+                cs[cnt] = c;
+                ls[cnt] = l;
+                ++cnt;
+              }
+          };
+          split(gx, cxs, lxs, nx);
+          split(gy, cys, lys, ny);
+          split(gz, czs, lzs, nz);
+          T s = 0;
+          for (int a = 0; a < nz; ++a)
+            for (int bb = 0; bb < ny; ++bb)
+              for (int c = 0; c < nx; ++c)
+                {
+                  const long long cell = cxs[c] + (long long)n * cys[bb] +
+                                         (long long)n * n * czs[a];
+                  s += E[cell * N3 + (lzs[a] * NN + lys[bb]) * NN + lxs[c]];
+                }
+          const bool bd = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1 ||
+                          gz == 0 || gz == N1 - 1;
+          T v;
+          if (bd)
+            v = TRANSPOSE ? s + x[i] : x[i];
+          else
+            v = s;
+          if (RESID)
+            v = b[i] - v;
+          y[i] = v;
+          if (DOT)
+            dsum += (double)x[i] * (double)v;
+        }
+      if (DOT)
+        {
+          dsum = block_sum<CNT>(dsum, red);
+          if (threadIdx.x == 0)
+            partials[blockIdx.x] = dsum;
+        }
+    }
+
+    template <int K, typename T>
+    struct PC
+    {
+      static constexpr int NN = K + 1, N2 = NN * NN, N3 = N2 * NN;
+      static constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr size_t lds =
+        sizeof(T) * (NP * NP * NP + 2 * 8 * N3 + NI * NI * NI + 8);
+      static constexpr bool ok = lds <= 65536;
+    };
+
+    __device__ __forceinline__ int
+    n_patches(int n, int parity)
+    {
+      return (n - parity) / 2;
+    }
+
+    // one vertex patch per work-group
+    template <int K, typename T>
+    __global__ void __launch_bounds__(CNT)
+    patch_coeff_kernel(T *__restrict__ u, const T *__restrict__ bvec,
+                       const T *__restrict__ coef, const T *__restrict__ tab,
+                       const int n, const int colour, const T scale,
+                       const T relax)
+    {
+      using TB         = Tab<K>;
+      using C          = PC<K, T>;
+      constexpr int NN = C::NN, N2 = C::N2, N3 = C::N3, NP = C::NP, NI = C::NI;
+      __shared__ T sbox[NP * NP * NP];
+      __shared__ T sa[8 * N3], sb[8 * N3];
+      __shared__ T sr[NI * NI * NI];
+      __shared__ T sc[8];
+      const T     *cM   = tab + TB::cellM;
+      const T     *cK   = tab + TB::cellK;
+      const T     *S    = tab + TB::S;
+      const T     *linv = tab + TB::lamInv3;
+      const int    tid  = threadIdx.x;
+      const int    N1   = K * n + 1;
+      const long long P = (long long)N1 * N1;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      const int npx = n_patches(n, px), npy = n_patches(n, py);
+      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      const int c0x = 2 * qx + px, c0y = 2 * qy + py, c0z = 2 * qz + pz;
+      const long long first = (long long)K * c0x + (long long)K * c0y * N1 +
+                              (long long)K * c0z * P;
+      if (tid < 8)
+        {
+          const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
+          sc[tid] = coef[(c0x + ox) + (long long)n * (c0y + oy) +
+                         (long long)n * n * (c0z + oz)];
+        }
+      for (int t = tid; t < NP * NP * NP; t += CNT)
+        {
+          const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
+          sbox[t] = u[first + ix + (long long)iy * N1 + iz * P];
+        }
+      __syncthreads();
+      // cell z-rows: task (cell, ix, iy)
+      for (int t = tid; t < 8 * N2; t += CNT)
+        {
+          const int c = t / N2, r = t - c * N2;
+          const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+          const int ix = r % NN, iy = r / NN;
+          T         v[NN];
+#pragma unroll
+          for (int iz = 0; iz < NN; ++iz)
+            v[iz] = sbox[((oz * K + iz) * NP + oy * K + iy) * NP + ox * K + ix];
+#pragma unroll
+          for (int lz = 0; lz < NN; ++lz)
+            {
+              T a = 0, b = 0;
+#pragma unroll
+              for (int iz = 0; iz < NN; ++iz)
+                {
+                  a += cM[lz * NN + iz] * v[iz];
+                  b += cK[lz * NN + iz] * v[iz];
+                }
+              sa[c * N3 + (lz * NN + iy) * NN + ix] = a;
+              sb[c * N3 + (lz * NN + iy) * NN + ix] = b;
+            }
+        }
+      __syncthreads();
+      // cell y-rows in place: task (cell, ix, lz)
+      for (int t = tid; t < 8 * N2; t += CNT)
+        {
+          const int c = t / N2, r = t - c * N2;
+          const int ix = r % NN, lz = r / NN;
+          T        *ca = sa + c * N3, *cb = sb + c * N3;
+          T         a[NN], b[NN];
+#pragma unroll
+          for (int iy = 0; iy < NN; ++iy)
+            {
+              a[iy] = ca[(lz * NN + iy) * NN + ix];
+              b[iy] = cb[(lz * NN + iy) * NN + ix];
+            }
+#pragma unroll
+          for (int ly = 0; ly < NN; ++ly)
+            {
+              T c1 = 0, c2 = 0;
+#pragma unroll
+              for (int iy = 0; iy < NN; ++iy)
+                {
+                  c1 += cM[ly * NN + iy] * a[iy];
+                  c2 += cK[ly * NN + iy] * a[iy] + cM[ly * NN + iy] * b[iy];
+                }
+              ca[(lz * NN + ly) * NN + ix] = c1;
+              cb[(lz * NN + ly) * NN + ix] = c2;
+            }
+        }
+      __syncthreads();
+      // cell x-rows in place into sa: task (cell, ly, lz)
+      for (int t = tid; t < 8 * N2; t += CNT)
+        {
+          const int c = t / N2, r = t - c * N2;
+          const int ly = r % NN, lz = r / NN;
+          T        *ca = sa + c * N3, *cb = sb + c * N3;
+          T         c1[NN], c2[NN];
+#pragma unroll
+          for (int ix = 0; ix < NN; ++ix)
+            {
+              c1[ix] = ca[(lz * NN + ly) * NN + ix];
+              c2[ix] = cb[(lz * NN + ly) * NN + ix];
+            }
+          const T f = sc[c] * scale;
+#pragma unroll
+          for (int lx = 0; lx < NN; ++lx)
+            {
+              T y = 0;
+#pragma unroll
+              for (int ix = 0; ix < NN; ++ix)
+                y += cK[lx * NN + ix] * c1[ix] + cM[lx * NN + ix] * c2[ix];
+              ca[(lz * NN + ly) * NN + lx] = f * y;
+            }
+        }
+      __syncthreads();
+      // interior residual r = b - sum of the cells containing the DoF
+      for (int t = tid; t < NI * NI * NI; t += CNT)
+        {
+          const int i = t % NI, j = (t / NI) % NI, l = t / (NI * NI);
+          const int p[3] = {i + 1, j + 1, l + 1};
+          int       co[3][2], lo[3][2], cnt[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            {
+              if (p[d] < K)
+                {
+                  co[d][0] = 0;
+                  lo[d][0] = p[d];
+                  cnt[d]   = 1;
+                }
+              else if (p[d] == K)
+                {
+                  co[d][0] = 0;
+                  lo[d][0] = K;
+                  co[d][1] = 1;
+                  lo[d][1] = 0;
+                  cnt[d]   = 2;
+                }
+              else
+                {
+                  co[d][0] = 1;
+                  lo[d][0] = p[d] - K;
+                  cnt[d]   = 1;
+                }
+            }
+          T s = 0;
+          for (int a = 0; a < cnt[2]; ++a)
+            for (int bb = 0; bb < cnt[1]; ++bb)
+              for (int c = 0; c < cnt[0]; ++c)
+                {
+                  const int cell = co[0][c] + 2 * co[1][bb] + 4 * co[2][a];
+                  s += sa[cell * N3 + (lo[2][a] * NN + lo[1][bb]) * NN + lo[0][c]];
+                }
+          const long long g = first + p[0] + (long long)p[1] * N1 + p[2] * P;
+          sr[t] = bvec[g] - s;
+        }
+      T cmax = sc[0];
+#pragma unroll
+      for (int c = 1; c < 8; ++c)
+        cmax = sc[c] > cmax ? sc[c] : cmax;
+      __syncthreads();
+      // fast diagonalisation: S^T along x, y, z; scale; S along z, y, x
+      constexpr int NL = NI * NI;
+      for (int t = tid; t < NL; t += CNT)  // x-lines (l, j)
+        {
+          T v[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = sr[t * NI + i];
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj)
+            {
+              T s = 0;
+#pragma unroll
+              for (int i = 0; i < NI; ++i)
+                s += S[i * NI + jj] * v[i];
+              sr[t * NI + jj] = s;
+            }
+        }
+      __syncthreads();
+      for (int t = tid; t < NL; t += CNT)  // y-lines (l, i)
+        {
+          const int i = t % NI, l = t / NI;
+          T        *base = sr + l * NL + i;
+          T         v[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = base[j * NI];
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj)
+            {
+              T s = 0;
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                s += S[j * NI + jj] * v[j];
+              base[jj * NI] = s;
+            }
+        }
+      __syncthreads();
+      const T inv = T(1) / (scale * cmax);
+      for (int t = tid; t < NL; t += CNT)  // z-lines (j, i)
+        {
+          T *base = sr + t;
+          T  v[NI], w[NI];
+#pragma unroll
+          for (int l = 0; l < NI; ++l)
+            v[l] = base[l * NL];
+#pragma unroll
+          for (int ll = 0; ll < NI; ++ll)
+            {
+              T s = 0;
+#pragma unroll
+              for (int l = 0; l < NI; ++l)
+                s += S[l * NI + ll] * v[l];
+              w[ll] = s * linv[ll * NL + t] * inv;
+            }
+#pragma unroll
+          for (int l = 0; l < NI; ++l)
+            {
+              T s = 0;
+#pragma unroll
+              for (int ll = 0; ll < NI; ++ll)
+                s += S[l * NI + ll] * w[ll];
+              base[l * NL] = s;
+            }
+        }
+      __syncthreads();
+      for (int t = tid; t < NL; t += CNT)  // y-lines
+        {
+          const int i = t % NI, l = t / NI;
+          T        *base = sr + l * NL + i;
+          T         v[NI];
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj)
+            v[jj] = base[jj * NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            {
+              T s = 0;
+#pragma unroll
+              for (int jj = 0; jj < NI; ++jj)
+                s += S[j * NI + jj] * v[jj];
+              base[j * NI] = s;
+            }
+        }
+      __syncthreads();
+      for (int t = tid; t < NL; t += CNT)  // x-lines and store
+        {
+          const int j = t % NI, l = t / NI;
+          T         v[NI];
+#pragma unroll
+          for (int jj = 0; jj < NI; ++jj)
+            v[jj] = sr[t * NI + jj];
+          const long long g0 = first + 1 + (long long)(j + 1) * N1 + (l + 1) * P;
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            {
+              T s = 0;
+#pragma unroll
+              for (int jj = 0; jj < NI; ++jj)
+                s += S[i * NI + jj] * v[jj];
+              const T uold = sbox[((l + 1) * NP + j + 1) * NP + i + 1];
+              u[g0 + i]    = uold + relax * s;
+            }
+        }
+    }
+
+    __global__ void __launch_bounds__(CNT)
+    coarsen_kernel(const int nc, const double *__restrict__ fine,
+                   double *__restrict__ coarse)
+    {
+      const long long ncc = (long long)nc * nc * nc;
+      const int       nf  = 2 * nc;
+      for (long long c = (long long)blockIdx.x * CNT + threadIdx.x; c < ncc;
+           c += (long long)gridDim.x * CNT)
+        {
+          const int cx = (int)(c % nc), cy = (int)((c / nc) % nc),
+                    cz = (int)(c / ((long long)nc * nc));
+          double s = 0.0;
+          for (int q = 0; q < 8; ++q)
+            {
+              const int fx = 2 * cx + (q & 1), fy = 2 * cy + ((q >> 1) & 1),
+                        fz = 2 * cz + (q >> 2);
+              s += fine[fx + (long long)nf * fy + (long long)nf * nf * fz];
+            }
+          coarse[c] = 0.125 * s;
+        }
+    }
+
+    int
+    grid_cap(long long work, int cap)
+    {
+      const long long nb = (work + CNT - 1) / CNT;
+      return (int)std::max<long long>(1, std::min<long long>(nb, cap));
+    }
+
+    template <int K, typename T>
+    void
+    cells_k(const gtps_level *lvl, T *out, const T *x, const T *lam, int mode,
+            bool mask)
+    {
+      const long long ncells = (long long)lvl->n * lvl->n * lvl->n;
+      constexpr int   CPB    = cells_per_block(K);
+      const long long nb     = (ncells + CPB - 1) / CPB;
+      GTPS_REQUIRE(nb < (1LL << 31), "mesh too large for the cell kernel");
+      const T *tab = (const T *)lvl->dtab();
+      if (mode == 0)
+        hipLaunchKernelGGL((cell_kernel<K, T, 0>), dim3((unsigned)nb), dim3(CNT),
+                           0, lvl->ctx->stream, x, nullptr,
+                           (const T *)lvl->coef, out, tab, lvl->n, lvl->N1,
+                           (T)lvl->scale, mask ? 1 : 0);
+      else
+        hipLaunchKernelGGL((cell_kernel<K, T, 1>), dim3((unsigned)nb), dim3(CNT),
+                           0, lvl->ctx->stream, x, lam, nullptr, out, tab,
+                           lvl->n, lvl->N1, (T)lvl->scale, 0);
+      GTPS_LAUNCH_CHECK();
+    }
+
+    template <int K, typename T>
+    int
+    apply_k(const gtps_level *lvl, T *y, const T *x, const T *b, bool dot,
+            bool transpose)
+    {
+      T *E = (T *)lvl->celem;
+      cells_k<K, T>(lvl, E, x, nullptr, 0, transpose);
+      hipStream_t s   = lvl->ctx->stream;
+      double     *pt  = lvl->ctx->ws.partials;
+      const int   nb  = grid_cap(lvl->N, dot ? std::min(lvl->ctx->ws.capacity, 2048) : 8192);
+#define GTPS_G(R, TR, D)                                                      \
+  hipLaunchKernelGGL((gather_kernel<K, T, R, TR, D>), dim3(nb), dim3(CNT), 0, \
+                     s, lvl->N, lvl->N1, lvl->n, (const T *)E, x, b, y, pt)
+      if (transpose)
+        {
+          GTPS_REQUIRE(b == nullptr && !dot, "transpose form has no residual/dot");
+          GTPS_G(false, true, false);
+        }
+      else if (b && dot)
+        GTPS_G(true, false, true);
+      else if (b)
+        GTPS_G(true, false, false);
+      else if (dot)
+        GTPS_G(false, false, true);
+      else
+        GTPS_G(false, false, false);
+#undef GTPS_G
+      GTPS_LAUNCH_CHECK();
+      return dot ? nb : 0;
+    }
+
+    template <int K, typename T>
+    void
+    patch_k(const gtps_level *lvl, T *u, const T *b, int colour, double relax)
+    {
+      if constexpr (PC<K, T>::ok)
+        {
+          const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
+          const int n  = lvl->n;
+          const long long np =
+            (long long)((n - px) / 2) * ((n - py) / 2) * ((n - pz) / 2);
+          if (np == 0)
+            return;
+          GTPS_REQUIRE(np < (1LL << 31), "mesh too large for the patch kernel");
+          hipLaunchKernelGGL((patch_coeff_kernel<K, T>), dim3((unsigned)np),
+                             dim3(CNT), 0, lvl->ctx->stream, u, b,
+                             (const T *)lvl->coef, (const T *)lvl->dtab(), n,
+                             colour, (T)lvl->scale, (T)relax);
+          GTPS_LAUNCH_CHECK();
+        }
+      else
+        throw Error(GTPS_EINVAL, "the variable-coefficient smoother supports "
+                                 "degree <= 5 in FP64 and <= 7 in FP32");
+    }
+
+    template <typename T, typename F>
+    auto
+    dispatch(int k, F &&f)
+    {
+      switch (k)
+        {
+          case 1: return f(std::integral_constant<int, 1>{});
+          case 2: return f(std::integral_constant<int, 2>{});
+          case 3: return f(std::integral_constant<int, 3>{});
+          case 4: return f(std::integral_constant<int, 4>{});
+          case 5: return f(std::integral_constant<int, 5>{});
+          case 6: return f(std::integral_constant<int, 6>{});
+          case 7: return f(std::integral_constant<int, 7>{});
+          default: throw Error(GTPS_EINVAL, "degree must be in 1..7");
+        }
+    }
+  } // namespace
+
+  bool
+  coeff_smoother_supported(int k, int prec)
+  {
+    return prec == GTPS_FP32 ? k <= 7 : k <= 5;
+  }
+
+  int
+  launch_coeff_apply(const gtps_level *lvl, void *y, const void *x,
+                     const void *b, bool dot, bool transpose)
+  {
+    GTPS_REQUIRE(lvl->dim == 3, "the variable-coefficient operator is 3D");
+    if (lvl->prec == GTPS_FP32)
+      return dispatch<float>(lvl->k, [&](auto kk) {
+        return apply_k<decltype(kk)::value, float>(lvl, (float *)y, (const float *)x,
+                                                   (const float *)b, dot, transpose);
+      });
+    return dispatch<double>(lvl->k, [&](auto kk) {
+      return apply_k<decltype(kk)::value, double>(lvl, (double *)y, (const double *)x,
+                                                  (const double *)b, dot, transpose);
+    });
+  }
+
+  void
+  launch_coeff_gradient(const gtps_level *lvl, void *grad, const void *lambda,
+                        const void *u)
+  {
+    GTPS_REQUIRE(lvl->dim == 3, "the variable-coefficient operator is 3D");
+    if (lvl->prec == GTPS_FP32)
+      dispatch<float>(lvl->k, [&](auto kk) {
+        cells_k<decltype(kk)::value, float>(lvl, (float *)grad, (const float *)u,
+                                            (const float *)lambda, 1, false);
+        return 0;
+      });
+    else
+      dispatch<double>(lvl->k, [&](auto kk) {
+        cells_k<decltype(kk)::value, double>(lvl, (double *)grad, (const double *)u,
+                                             (const double *)lambda, 1, false);
+        return 0;
+      });
+  }
+
+  void
+  launch_patch_coeff_colour(const gtps_level *lvl, void *u, const void *b,
+                            int colour, double relax)
+  {
+    GTPS_REQUIRE(lvl->dim == 3 && lvl->coef, "level has no 3D coefficient");
+    if (lvl->prec == GTPS_FP32)
+      dispatch<float>(lvl->k, [&](auto kk) {
+        patch_k<decltype(kk)::value, float>(lvl, (float *)u, (const float *)b,
+                                            colour, relax);
+        return 0;
+      });
+    else
+      dispatch<double>(lvl->k, [&](auto kk) {
+        patch_k<decltype(kk)::value, double>(lvl, (double *)u, (const double *)b,
+                                             colour, relax);
+        return 0;
+      });
+  }
+
+  void
+  launch_coarsen_coefficient(hipStream_t s, int nc, const double *fine,
+                             double *coarse)
+  {
+    const long long ncc = (long long)nc * nc * nc;
+    hipLaunchKernelGGL(coarsen_kernel, dim3(grid_cap(ncc, 8192)), dim3(CNT), 0,
+                       s, nc, fine, coarse);
+    GTPS_LAUNCH_CHECK();
+  }
+} // namespace gtps

@@ -88,6 +88,18 @@ struct gtps_level
   int                      prec = GTPS_FP64;
   gtps::DegreeTables      *tab  = nullptr;
   double                   scale = 1.0;  // h^(dim-2)
+  // variable coefficient (level precision, one value per cell, cells
+  // cx + n cy + n^2 cz) and the element-vector buffer of the cell-wise
+  // operator; both null for the constant-coefficient Laplacian
+  void                    *coef  = nullptr;
+  void                    *celem = nullptr;
+  ~gtps_level()
+  {
+    if (coef)
+      (void)hipFree(coef);
+    if (celem)
+      (void)hipFree(celem);
+  }
   const void *
   dtab() const
   {
@@ -141,6 +153,22 @@ namespace gtps
   // sum_i x_i y_i written per block to ctx->ws.partials; returns #blocks.
   int launch_laplace(const gtps_level *lvl, void *y, const void *x,
                      const void *b, bool fused_dot);
+
+  // coeff.hip: variable-coefficient operator (3D). y = A(c) x (resid:
+  // y = b - A(c) x; transpose: y = A(c)^T x); a null level coefficient
+  // means c = 1. Needs lvl->celem. Returns #dot partial blocks.
+  int  launch_coeff_apply(const gtps_level *lvl, void *y, const void *x,
+                          const void *b, bool fused_dot, bool transpose);
+  // grad[cell] = -lambda~_K . A^1_K u_K (lambda's Dirichlet entries masked)
+  void launch_coeff_gradient(const gtps_level *lvl, void *grad,
+                             const void *lambda, const void *u);
+  // one colour of the inexact-FD multiplicative smoother (lvl->coef set)
+  void launch_patch_coeff_colour(const gtps_level *lvl, void *u,
+                                 const void *b, int colour, double relax);
+  // coarse[c] = mean of the 8 children (FP64), nc coarse cells per direction
+  void launch_coarsen_coefficient(hipStream_t s, int nc, const double *fine,
+                                  double *coarse);
+  bool coeff_smoother_supported(int k, int prec);
 
   // smoother.hip: one colour pass. mode 0 multiplicative (src = b),
   // mode 1 additive (src = global residual), mode 2 multiplicative on

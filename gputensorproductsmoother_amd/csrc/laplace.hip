@@ -513,6 +513,8 @@ namespace gtps
   launch_laplace(const gtps_level *lvl, void *y, const void *x, const void *b,
                  bool fused_dot)
   {
+    if (lvl->coef)
+      return launch_coeff_apply(lvl, y, x, b, fused_dot, false);
     if (lvl->prec == GTPS_FP32)
       return launch_t<float>(lvl, y, x, b, fused_dot);
     return launch_t<double>(lvl, y, x, b, fused_dot);

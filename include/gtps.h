@@ -100,6 +100,36 @@ int gtps_laplace_vmult(gtps_level *lvl, void *dst, const void *src);
 int gtps_laplace_residual(gtps_level *lvl, void *dst, const void *b,
                           const void *u);
 
+/* ------------------------------------------------- variable coefficient */
+/* Variable-coefficient diffusion -div(c grad u) with c constant per cell
+ * (3D only; cells numbered cx + n cy + n^2 cz, n = 2^level).  The reference
+ * takes a coefficient in its solver constructor and ignores it
+ * (include/solver.cuh:1548); its matrix-free hook for per-cell data is
+ * MatrixFree::evaluate_coefficients (include/dealii/cuda_mf.template.cuh:
+ * 815-824), never called on the hot path.  These entry points have no
+ * further reference counterpart.
+ *
+ * Set (coef = device FP64 array of n^3 values, copied and rounded to the
+ * level precision) or clear (coef = NULL) the level's coefficient.  While
+ * set, gtps_laplace_vmult / gtps_laplace_residual apply A(c) =
+ * sum_K c_K A^1_K (same Dirichlet identity rows), and smoothers on the
+ * level use the inexact local solve cmax_P A^1_P (correction form
+ * u_I <- u_I + relaxation A~_P^{-1} (b - A u)_I; multiplicative only). */
+int gtps_level_set_coefficient(gtps_level *lvl, const double *coef);
+
+/* dst = A^T src for the matrix of gtps_laplace_vmult (identity Dirichlet
+ * rows, boundary columns kept): interior rows A_II src_I, boundary rows
+ * A_IB^T src_I + src_B.  Needed for the adjoint of the constrained solve.
+ * Uses the level's coefficient if set, else c = 1. */
+int gtps_laplace_vmult_transpose(gtps_level *lvl, void *dst, const void *src);
+
+/* grad[K] = -sum_{i,j in K} lambda~_i (A^1_K)_ij u_j = -int_K grad(lambda).
+ * grad(u) per cell (lambda~ = lambda with its Dirichlet entries zeroed):
+ * dJ/dc_K for J(u(c)) when lambda solves the adjoint system.  grad has n^3
+ * entries in the level precision. */
+int gtps_level_coefficient_gradient(gtps_level *lvl, void *grad,
+                                    const void *lambda, const void *u);
+
 /* Host copy (FP64) of the 1D data the level was set up with: reference-cell
  * mass/stiffness [(k+1)^2 each], patch mass/stiffness [(2k+1)^2 each],
  * interior eigenvalues [2k-1] and M-orthonormal eigenvectors [(2k-1)^2,
@@ -194,6 +224,20 @@ int gtps_mg_pcg_solve(gtps_mg *mg, double *x, const double *b, double rtol,
 /* FP64 finest-level operator of the hierarchy: dst = A src.  Replaces
  * MultigridSolvers::do_matvec (include/solver.cuh:1934-1938). */
 int gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src);
+
+/* Variable coefficient for the whole hierarchy: the finest levels take
+ * coef (device FP64, n^3 cells of the finest level), each coarser level the
+ * mean of its 8 children; NULL restores c = 1.  Multiplicative schedules
+ * only.  No reference counterpart (see gtps_level_set_coefficient). */
+int gtps_mg_set_coefficient(gtps_mg *mg, const double *coef);
+
+/* FP64 finest-level dst = A^T src (see gtps_laplace_vmult_transpose). */
+int gtps_mg_matvec_transpose(gtps_mg *mg, double *dst, const double *src);
+
+/* FP64 finest-level coefficient gradient (see
+ * gtps_level_coefficient_gradient); grad has n^3 doubles. */
+int gtps_mg_coefficient_gradient(gtps_mg *mg, double *grad,
+                                 const double *lambda, const double *u);
 
 /* Replay the per-iteration preconditioning sequence (V-cycle, r.z, beta,
  * p update; about 130 kernels for 6 levels) as one HIP graph captured on the

@@ -50,6 +50,15 @@ def lib():
         L.oracle_l2_error_sine.argtypes = [i, i, i, _dp]
         L.oracle_l2_error_sine.restype = d
         L.oracle_set_threads.argtypes = [i]
+        L.oracle_coeff_pcg.argtypes = [i, i, i, _dp, i, d, _dp, _dp, d, i,
+                                       ctypes.POINTER(i), _dp, _dp]
+        L.oracle_coeff_vmult.argtypes = [i, i, i, _dp, _dp, _dp, i]
+        L.oracle_coeff_smoother_step.argtypes = [i, i, i, _dp, i, d, _dp, _dp]
+        L.oracle_coeff_coarsen.argtypes = [i, i, _dp, _dp]
+        L.oracle_coeff_vcycle.argtypes = [i, i, i, _dp, i, d, _dp, _dp]
+        L.oracle_coeff_gradient.argtypes = [i, i, i, _dp, _dp, _dp]
+        L.oracle_coeff_problem.argtypes = [i, i, i, ull, d, _dp, _dp, i,
+                                           ctypes.POINTER(ll)]
         _lib = L
     return _lib
 
@@ -163,3 +172,84 @@ def l2_error_sine(dim, k, level, u):
 
 def set_threads(n: int) -> int:
     return int(lib().oracle_set_threads(n))
+
+
+# ---------------------------------------------------------------- variable
+# coefficient (cells numbered cx + n cy + n^2 cz, n = 2^level)
+
+def n_cells(dim: int, level: int) -> int:
+    return (1 << level) ** dim
+
+
+def _coef(c):
+    return None if c is None else _p(np.ascontiguousarray(c, dtype=np.float64))
+
+
+def coeff_vmult(dim, k, level, coef, x, transpose=False):
+    """y = A(c) x with identity Dirichlet rows (or its transpose)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    c = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    y = np.zeros_like(x)
+    _chk(lib().oracle_coeff_vmult(dim, k, level, None if c is None else _p(c), _p(x), _p(y),
+                                  1 if transpose else 0), "coeff_vmult")
+    return y
+
+
+def coeff_smoother_step(dim, k, level, coef, b, u, variant=MULT_FWD, relax=1.0):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    u = np.array(u, dtype=np.float64, copy=True)
+    _chk(lib().oracle_coeff_smoother_step(dim, k, level, _p(c), variant, relax, _p(b), _p(u)),
+         "coeff_smoother")
+    return u
+
+
+def coeff_coarsen(dim, coarse_level, fine):
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    out = np.zeros(n_cells(dim, coarse_level))
+    _chk(lib().oracle_coeff_coarsen(dim, coarse_level, _p(fine), _p(out)), "coeff_coarsen")
+    return out
+
+
+def coeff_vcycle(dim, k, level, coef, r, mg_variant=MG_SYMMETRIC, relax=1.0):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    z = np.zeros_like(r)
+    _chk(lib().oracle_coeff_vcycle(dim, k, level, _p(c), mg_variant, relax, _p(r), _p(z)),
+         "coeff_vcycle")
+    return z
+
+
+def coeff_pcg(dim, k, level, coef, b, rtol=1e-8, maxit=100, mg_variant=MG_SYMMETRIC,
+              relax=1.0):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    x = np.zeros_like(b)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0.0)
+    hist = np.zeros(maxit + 1)
+    rc = lib().oracle_coeff_pcg(dim, k, level, _p(c), mg_variant, relax, _p(b), _p(x), rtol,
+                                maxit, ctypes.byref(it), ctypes.byref(rel), _p(hist))
+    _chk(rc, "coeff_pcg")
+    return x, it.value, rel.value, hist[: it.value + 1]
+
+
+def coeff_gradient(dim, k, level, lam, u):
+    """dJ/dc_K = -lambda~_K^T A^1_K u_K (lambda's Dirichlet entries masked)."""
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    g = np.zeros(n_cells(dim, level))
+    _chk(lib().oracle_coeff_gradient(dim, k, level, _p(lam), _p(u), _p(g)), "coeff_gradient")
+    return g
+
+
+def coeff_problem(dim, k, level, seed=20261202, sigma=0.5, n_fd_cells=16):
+    """Coefficient exp(sigma g), target u_d and the finite-difference cells of
+    the variable-coefficient problem (SplitMix64 streams 2^32, 2^33, 2^34)."""
+    coef = np.zeros(n_cells(dim, level))
+    ud = np.zeros(n_dofs(dim, k, level))
+    cells = np.zeros(n_fd_cells, dtype=np.int64)
+    _chk(lib().oracle_coeff_problem(dim, k, level, seed, sigma, _p(coef), _p(ud), n_fd_cells,
+                                    cells.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))),
+         "coeff_problem")
+    return coef, ud, cells

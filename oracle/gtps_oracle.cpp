@@ -31,6 +31,12 @@
 //     valence weights (cuda_mg_transfer.template.cuh:48-369, 561-671)
 //   * V(1,1)-cycle (solver.cuh:1952-2001) with optional reversed colour order
 //     for post-smoothing, and MG-preconditioned CG.
+//   * variable-coefficient diffusion (no reference counterpart: the
+//     reference's MultigridSolvers ignores its coefficient, solver.cuh:1548):
+//     per-cell coefficient in the same cell loop, the transpose of the
+//     constrained operator, a vertex-patch smoother with the scaled
+//     constant-coefficient FD inverse as inexact local solve, coefficient
+//     coarsening by averaging, and dJ/dc_K = -lambda_K^T A^1_K u_K.
 //
 // Every function is pinned by tests/test_oracle_*.py against closed forms,
 // high-precision golden fixtures (tests/golden) and brute-force dense
@@ -415,22 +421,85 @@ namespace
   };
 
   // ------------------------------------------------------------- operator
-  // Cell-wise quadrature form: y_K = sum_d (D_d^T W N^T..) (D_d N..) u_K with
-  // JxW = h^dim w_q and J^{-1} = I/h; scattered into y.
-  void
-  laplace_vmult(const Grid &g, const double *x, double *y)
+  // Cell (cx, cy[, cz]) has the lexicographic cell index cx + n cy + n^2 cz.
+  inline long long
+  cell_index(const Grid &g, int cx, int cy, int cz)
   {
-    const Basis &b   = basis(g.k);
-    const int    n1d = g.k + 1;
-    const int    dim = g.dim;
-    const int    nl  = dim == 2 ? n1d * n1d : n1d * n1d * n1d;
-    const double scale = std::pow(g.h, dim - 2);
-    std::vector<double> wq(nl);
-    for (int l = 0; l < (dim == 3 ? n1d : 1); ++l)
-      for (int j = 0; j < n1d; ++j)
-        for (int i = 0; i < n1d; ++i)
-          wq[(l * n1d + j) * n1d + i] =
-            b.qw[i] * b.qw[j] * (dim == 3 ? b.qw[l] : 1.0) * scale;
+    return cx + (long long)g.n * cy + (long long)g.n * g.n * cz * (g.dim == 3);
+  }
+
+  // Cell-wise quadrature form of the reference-cell Laplacian: out =
+  // sum_d (D_d^T W N^T..)(D_d N..) u with JxW = h^dim w_q, J^{-1} = I/h.
+  struct CellOp
+  {
+    const Basis        &b;
+    int                 dim, n1d, nl;
+    std::vector<double> wq, t1, t2, gq, out;
+    int                 shape[3];
+    CellOp(const Grid &g)
+      : b(basis(g.k)), dim(g.dim), n1d(g.k + 1),
+        nl(g.dim == 2 ? (g.k + 1) * (g.k + 1) : (g.k + 1) * (g.k + 1) * (g.k + 1))
+    {
+      const double scale = std::pow(g.h, dim - 2);
+      wq.resize(nl);
+      t1.resize(nl);
+      t2.resize(nl);
+      gq.resize(nl);
+      out.resize(nl);
+      shape[0] = shape[1] = n1d;
+      shape[2]            = dim == 3 ? n1d : 1;
+      for (int l = 0; l < shape[2]; ++l)
+        for (int j = 0; j < n1d; ++j)
+          for (int i = 0; i < n1d; ++i)
+            wq[(l * n1d + j) * n1d + i] =
+              b.qw[i] * b.qw[j] * (dim == 3 ? b.qw[l] : 1.0) * scale;
+    }
+    // acc (+)= coef * A_K u
+    void
+    apply(const double *u, double *acc, double coef, bool add)
+    {
+      if (!add)
+        std::fill(acc, acc + nl, 0.0);
+      for (int d = 0; d < dim; ++d)
+        {
+          const double *src     = u;
+          double       *bufs[2] = {t1.data(), t2.data()};
+          int           nb      = 0;
+          for (int a = 0; a < dim; ++a)
+            {
+              double *dst = (a == dim - 1) ? gq.data() : bufs[nb++ & 1];
+              apply_axis(a == d ? b.D.data() : b.N.data(), n1d, n1d, src, dst,
+                         shape, a, false);
+              src = dst;
+            }
+          for (int q = 0; q < nl; ++q)
+            gq[q] *= wq[q];
+          src = gq.data();
+          nb  = 0;
+          for (int a = 0; a < dim; ++a)
+            {
+              double *dst = (a == dim - 1) ? out.data() : bufs[nb++ & 1];
+              apply_axis(a == d ? b.D.data() : b.N.data(), n1d, n1d, src, dst,
+                         shape, a, false, true);
+              src = dst;
+            }
+          for (int q = 0; q < nl; ++q)
+            acc[q] += coef * out[q];
+        }
+    }
+  };
+
+  // y = A x, A = sum_K coef_K A_K (coef == nullptr: all 1) scattered from
+  // the cells, then the Dirichlet rows. mode 0: identity rows y_B = x_B
+  // (laplace_operator.cuh:86-101, dealii/cuda_mf.template.cuh:494-508);
+  // mode 1: the transpose of that matrix, A^T x = [A_II x_I ; A_IB^T x_I +
+  // x_B] (boundary inputs masked in the cell products, full boundary rows).
+  void
+  laplace_vmult(const Grid &g, const double *x, double *y,
+                const double *coef = nullptr, int mode = 0)
+  {
+    const int n1d = g.k + 1;
+    const int dim = g.dim;
     std::fill(y, y + g.N, 0.0);
     const int ncol = 1 << dim;
     for (int col = 0; col < ncol; ++col)
@@ -439,8 +508,8 @@ namespace
         const int nz = dim == 3 ? g.n : 1;
 #pragma omp parallel
         {
-          std::vector<double> u(nl), t1(nl), t2(nl), gq(nl), out(nl), acc(nl);
-          const int shape[3] = {n1d, n1d, dim == 3 ? n1d : 1};
+          CellOp              op(g);
+          std::vector<double> u(op.nl), acc(op.nl);
 #pragma omp for collapse(2) schedule(static)
           for (int cz = pz * (dim == 3); cz < nz; cz += (dim == 3 ? 2 : 1))
             for (int cy = py; cy < g.n; cy += 2)
@@ -453,41 +522,17 @@ namespace
                                            g.N1 * ((long long)g.k * cy) +
                                            g.N1 * g.N1 *
                                              ((long long)g.k * cz * (dim == 3));
-                    for (int l = 0; l < shape[2]; ++l)
+                    for (int l = 0; l < op.shape[2]; ++l)
                       for (int j = 0; j < n1d; ++j)
                         for (int i = 0; i < n1d; ++i)
-                          u[(l * n1d + j) * n1d + i] =
-                            x[base + i + g.N1 * j + g.N1 * g.N1 * l];
-                    std::fill(acc.begin(), acc.end(), 0.0);
-                    for (int d = 0; d < dim; ++d)
-                      {
-                        // gradient component d at quadrature points
-                        const double *src = u.data();
-                        double       *bufs[2] = {t1.data(), t2.data()};
-                        int           nb      = 0;
-                        for (int a = 0; a < dim; ++a)
                           {
-                            double *dst = (a == dim - 1) ? gq.data() : bufs[nb++ & 1];
-                            apply_axis(a == d ? b.D.data() : b.N.data(), n1d,
-                                       n1d, src, dst, shape, a, false);
-                            src = dst;
+                            const long long gi = base + i + g.N1 * j + g.N1 * g.N1 * l;
+                            u[(l * n1d + j) * n1d + i] =
+                              (mode == 1 && g.on_boundary(gi)) ? 0.0 : x[gi];
                           }
-                        for (int q = 0; q < nl; ++q)
-                          gq[q] *= wq[q];
-                        // integrate: transpose of the same matrices
-                        src = gq.data();
-                        nb  = 0;
-                        for (int a = 0; a < dim; ++a)
-                          {
-                            double *dst = (a == dim - 1) ? out.data() : bufs[nb++ & 1];
-                            apply_axis(a == d ? b.D.data() : b.N.data(), n1d,
-                                       n1d, src, dst, shape, a, false, true);
-                            src = dst;
-                          }
-                        for (int q = 0; q < nl; ++q)
-                          acc[q] += out[q];
-                      }
-                    for (int l = 0; l < shape[2]; ++l)
+                    op.apply(u.data(), acc.data(),
+                             coef ? coef[cell_index(g, cx, cy, cz)] : 1.0, false);
+                    for (int l = 0; l < op.shape[2]; ++l)
                       for (int j = 0; j < n1d; ++j)
                         for (int i = 0; i < n1d; ++i)
                           y[base + i + g.N1 * j + g.N1 * g.N1 * l] +=
@@ -499,7 +544,7 @@ namespace
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < g.N; ++i)
       if (g.on_boundary(i))
-        y[i] = x[i];
+        y[i] = mode == 1 ? y[i] + x[i] : x[i];
   }
 
   // ------------------------------------------------------------- smoother
@@ -664,6 +709,157 @@ namespace
       }
   }
 
+  // ------------------------------------------- variable-coefficient smoother
+  // Multiplicative vertex-patch Schwarz for A = sum_K c_K A_K (no reference
+  // counterpart: the reference ignores its coefficient, solver.cuh:1548).
+  // The patch operator is no longer a Kronecker sum, so the local solve is
+  // the fast-diagonalisation inverse of the constant-coefficient patch
+  // operator scaled by the largest coefficient of the patch's 2^d cells,
+  // A~_P = cmax_P A^1_P >= A_P,II, applied in correction form
+  //   u_I <- u_I + relax A~_P^{-1} (b - A u)_I,
+  // with the local residual from the 2^d cells of the patch (the only cells
+  // that touch its interior DoFs).
+  void
+  smoother_step_coeff(const Grid &g, const double *coef, int variant,
+                      double relax, const double *b_, double *u)
+  {
+    const Basis &b      = basis(g.k);
+    const int    k      = g.k;
+    const int    n1d    = k + 1;
+    const int    np     = 2 * k + 1;
+    const int    ni     = 2 * k - 1;
+    const int    dim    = g.dim;
+    const int    npatch = g.n - 1;
+    const int    ncol   = 1 << dim;
+    for (int cc = 0; cc < ncol; ++cc)
+      {
+        const int col = variant == MULT_REV ? ncol - 1 - cc : cc;
+        const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+        const int nz = dim == 3 ? npatch : 1;
+        if (dim == 2 && pz == 1)
+          continue;
+#pragma omp parallel
+        {
+          CellOp              op(g);
+          const int           shi[3] = {ni, ni, dim == 3 ? ni : 1};
+          const int           nbox   = np * np * (dim == 3 ? np : 1);
+          std::vector<double> box(nbox), acc(nbox), ul(op.nl), al(op.nl),
+            delta(shi[0] * shi[1] * shi[2]), tmp(delta.size());
+#pragma omp for collapse(2) schedule(static)
+          for (int iz = (dim == 3 ? pz : 0); iz < nz; iz += (dim == 3 ? 2 : 1))
+            for (int iy = py; iy < npatch; iy += 2)
+              for (int ix = px; ix < npatch; ix += 2)
+                {
+                  const long long first =
+                    (long long)k * ix + g.N1 * ((long long)k * iy) +
+                    g.N1 * g.N1 * ((long long)k * iz * (dim == 3));
+                  auto gidx = [&](int i, int j, int l) {
+                    return first + i + g.N1 * j + g.N1 * g.N1 * l;
+                  };
+                  for (int l = 0; l < (dim == 3 ? np : 1); ++l)
+                    for (int j = 0; j < np; ++j)
+                      for (int i = 0; i < np; ++i)
+                        box[(l * np + j) * np + i] = u[gidx(i, j, l)];
+                  std::fill(acc.begin(), acc.end(), 0.0);
+                  double cmax = 0.0;
+                  for (int c = 0; c < (1 << dim); ++c)
+                    {
+                      const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+                      const double cK =
+                        coef[cell_index(g, ix + ox, iy + oy, iz + oz)];
+                      cmax = std::max(cmax, cK);
+                      for (int l = 0; l < op.shape[2]; ++l)
+                        for (int j = 0; j < n1d; ++j)
+                          for (int i = 0; i < n1d; ++i)
+                            ul[(l * n1d + j) * n1d + i] =
+                              box[((oz * k + l) * np + oy * k + j) * np + ox * k + i];
+                      op.apply(ul.data(), al.data(), cK, false);
+                      for (int l = 0; l < op.shape[2]; ++l)
+                        for (int j = 0; j < n1d; ++j)
+                          for (int i = 0; i < n1d; ++i)
+                            acc[((oz * k + l) * np + oy * k + j) * np + ox * k + i] +=
+                              al[(l * n1d + j) * n1d + i];
+                    }
+                  for (int l = 0; l < shi[2]; ++l)
+                    for (int j = 0; j < ni; ++j)
+                      for (int i = 0; i < ni; ++i)
+                        {
+                          const int lz = l + (dim == 3);
+                          delta[(l * ni + j) * ni + i] =
+                            b_[gidx(i + 1, j + 1, lz)] -
+                            acc[(lz * np + j + 1) * np + i + 1];
+                        }
+                  fd_inverse(g, b, delta.data(), tmp.data());
+                  for (int l = 0; l < shi[2]; ++l)
+                    for (int j = 0; j < ni; ++j)
+                      for (int i = 0; i < ni; ++i)
+                        u[gidx(i + 1, j + 1, l + (dim == 3))] +=
+                          relax * delta[(l * ni + j) * ni + i] / cmax;
+                }
+        }
+      }
+  }
+
+  // Coarse-level coefficient: mean of the 2^d children.
+  void
+  coarsen_coefficient(int dim, int nc, const double *fine, double *coarse)
+  {
+    const int nf = 2 * nc;
+    for (int cz = 0; cz < (dim == 3 ? nc : 1); ++cz)
+      for (int cy = 0; cy < nc; ++cy)
+        for (int cx = 0; cx < nc; ++cx)
+          {
+            double s = 0.0;
+            for (int c = 0; c < (1 << dim); ++c)
+              {
+                const int fx = 2 * cx + (c & 1), fy = 2 * cy + ((c >> 1) & 1),
+                          fz = 2 * cz + ((c >> 2) & 1);
+                s += fine[fx + (long long)nf * fy + (long long)nf * nf * fz * (dim == 3)];
+              }
+            coarse[cx + (long long)nc * cy + (long long)nc * nc * cz * (dim == 3)] =
+              s / (1 << dim);
+          }
+  }
+
+  // dJ/dc_K = -lambda~_K^T A^1_K u_K with lambda~ = lambda with its Dirichlet
+  // entries zeroed (the derivative of the interior rows of A(c) u).
+  void
+  coefficient_gradient(const Grid &g, const double *lambda, const double *u,
+                       double *grad)
+  {
+    const int n1d = g.k + 1;
+    const int dim = g.dim;
+    const int nz  = dim == 3 ? g.n : 1;
+#pragma omp parallel
+    {
+      CellOp              op(g);
+      std::vector<double> ul(op.nl), al(op.nl);
+#pragma omp for collapse(2) schedule(static)
+      for (int cz = 0; cz < nz; ++cz)
+        for (int cy = 0; cy < g.n; ++cy)
+          for (int cx = 0; cx < g.n; ++cx)
+            {
+              const long long base = (long long)g.k * cx + g.N1 * ((long long)g.k * cy) +
+                                     g.N1 * g.N1 * ((long long)g.k * cz * (dim == 3));
+              for (int l = 0; l < op.shape[2]; ++l)
+                for (int j = 0; j < n1d; ++j)
+                  for (int i = 0; i < n1d; ++i)
+                    ul[(l * n1d + j) * n1d + i] = u[base + i + g.N1 * j + g.N1 * g.N1 * l];
+              op.apply(ul.data(), al.data(), 1.0, false);
+              double s = 0.0;
+              for (int l = 0; l < op.shape[2]; ++l)
+                for (int j = 0; j < n1d; ++j)
+                  for (int i = 0; i < n1d; ++i)
+                    {
+                      const long long gi = base + i + g.N1 * j + g.N1 * g.N1 * l;
+                      if (!g.on_boundary(gi))
+                        s += lambda[gi] * al[(l * n1d + j) * n1d + i];
+                    }
+              grad[cell_index(g, cx, cy, cz)] = -s;
+            }
+    }
+  }
+
   // ------------------------------------------------------------- transfer
   // weight of fine local node q (0..2k) of coarse cell c (0..nc-1): 1/2 on
   // coarse-cell faces shared with a neighbour cell, else 1 (valence weights).
@@ -793,7 +989,9 @@ namespace
     double relax;
     std::vector<Grid> grids;
     mutable std::vector<std::vector<double>> d, u, t;
-    MG(int dim_, int k_, int maxl, int var, double rel)
+    std::vector<std::vector<double>> coef;  // per level; empty: constant 1
+    MG(int dim_, int k_, int maxl, int var, double rel,
+       const double *fine_coef = nullptr)
       : dim(dim_), k(k_), minlevel(1), maxlevel(maxl), variant(var), relax(rel)
     {
       for (int l = 0; l <= maxlevel; ++l)
@@ -807,6 +1005,29 @@ namespace
           u[l].assign(grids[l].N, 0.0);
           t[l].assign(grids[l].N, 0.0);
         }
+      if (fine_coef)
+        {
+          coef.resize(maxlevel + 1);
+          const long long nf = dim == 3 ? (1LL << (3 * maxlevel)) : (1LL << (2 * maxlevel));
+          coef[maxlevel].assign(fine_coef, fine_coef + nf);
+          for (int l = maxlevel - 1; l >= minlevel; --l)
+            {
+              const int nc = 1 << l;
+              coef[l].assign(dim == 3 ? (long long)nc * nc * nc : (long long)nc * nc, 0.0);
+              coarsen_coefficient(dim, nc, coef[l + 1].data(), coef[l].data());
+            }
+        }
+    }
+    const double *level_coef(int l) const
+    {
+      return coef.empty() ? nullptr : coef[l].data();
+    }
+    void smooth(int l, int var, double rel, const double *b, double *x) const
+    {
+      if (coef.empty())
+        smoother_step(grids[l], var, rel, b, x);
+      else
+        smoother_step_coeff(grids[l], coef[l].data(), var, rel, b, x);
     }
     int pre_variant() const { return variant == 2 ? ADDITIVE : MULT_FWD; }
     int post_variant() const
@@ -820,19 +1041,20 @@ namespace
       std::fill(u[l].begin(), u[l].end(), 0.0);
       if (l == minlevel)
         {
-          // single vertex patch: exact solve
-          smoother_step(g, MULT_FWD, 1.0, d[l].data(), u[l].data());
+          // single vertex patch: exact solve (one inexact local solve with a
+          // variable coefficient)
+          smooth(l, MULT_FWD, 1.0, d[l].data(), u[l].data());
           return;
         }
-      smoother_step(g, pre_variant(), relax, d[l].data(), u[l].data());
-      laplace_vmult(g, u[l].data(), t[l].data());
+      smooth(l, pre_variant(), relax, d[l].data(), u[l].data());
+      laplace_vmult(g, u[l].data(), t[l].data(), level_coef(l));
       for (long long i = 0; i < g.N; ++i)
         t[l][i] = d[l][i] - t[l][i];
       std::fill(d[l - 1].begin(), d[l - 1].end(), 0.0);
       transfer(g, false, d[l - 1].data(), nullptr, nullptr, t[l].data());
       vcycle(l - 1);
       transfer(g, true, nullptr, u[l].data(), u[l - 1].data(), nullptr);
-      smoother_step(g, post_variant(), relax, d[l].data(), u[l].data());
+      smooth(l, post_variant(), relax, d[l].data(), u[l].data());
     }
     void
     apply(const double *r, double *z) const
@@ -966,19 +1188,20 @@ extern "C"
   // taken as the boundary values of x (identity rows); stops when
   // ||r_k|| <= rtol ||r_0||. history (optional, maxit+1) gets ||r_k||.
   int
-  oracle_pcg(int dim, int k, int level, int mg_variant, double relax,
-             const double *b, double *x, double rtol, int maxit, int *iters,
-             double *relres, double *history)
+  oracle_coeff_pcg(int dim, int k, int level, const double *coef,
+                   int mg_variant, double relax, const double *b, double *x,
+                   double rtol, int maxit, int *iters, double *relres,
+                   double *history)
   {
     if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1)
       return 1;
     const Grid g(dim, k, level);
-    MG         mg(dim, k, level, mg_variant, relax);
+    MG         mg(dim, k, level, mg_variant, relax, coef);
     const long long N = g.N;
     std::vector<double> r(N), z(N), p(N), q(N);
     for (long long i = 0; i < N; ++i)
       x[i] = g.on_boundary(i) ? b[i] : 0.0;
-    laplace_vmult(g, x, q.data());
+    laplace_vmult(g, x, q.data(), coef);
     for (long long i = 0; i < N; ++i)
       r[i] = b[i] - q[i];
     const double r0 = std::sqrt(dot(N, r.data(), r.data()));
@@ -997,7 +1220,7 @@ extern "C"
     double rz = dot(N, r.data(), z.data());
     while (it < maxit)
       {
-        laplace_vmult(g, p.data(), q.data());
+        laplace_vmult(g, p.data(), q.data(), coef);
         const double alpha = rz / dot(N, p.data(), q.data());
 #pragma omp parallel for schedule(static)
         for (long long i = 0; i < N; ++i)
@@ -1022,6 +1245,118 @@ extern "C"
     *iters  = it;
     *relres = rn / r0;
     return std::isfinite(rn) ? 0 : 2;
+  }
+
+  int
+  oracle_pcg(int dim, int k, int level, int mg_variant, double relax,
+             const double *b, double *x, double rtol, int maxit, int *iters,
+             double *relres, double *history)
+  {
+    return oracle_coeff_pcg(dim, k, level, nullptr, mg_variant, relax, b, x,
+                            rtol, maxit, iters, relres, history);
+  }
+
+  // Variable-coefficient operator (coef per cell, lexicographic cells; NULL
+  // = 1). mode 0: Dirichlet identity rows; mode 1: transpose.
+  int
+  oracle_coeff_vmult(int dim, int k, int level, const double *coef,
+                     const double *x, double *y, int mode)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1 ||
+        mode < 0 || mode > 1)
+      return 1;
+    laplace_vmult(Grid(dim, k, level), x, y, coef, mode);
+    return 0;
+  }
+
+  // variant: 0 forward colours, 1 reverse colours
+  int
+  oracle_coeff_smoother_step(int dim, int k, int level, const double *coef,
+                             int variant, double relax, const double *b,
+                             double *u)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1 ||
+        variant < 0 || variant > 1 || !coef)
+      return 1;
+    smoother_step_coeff(Grid(dim, k, level), coef, variant, relax, b, u);
+    return 0;
+  }
+
+  int
+  oracle_coeff_coarsen(int dim, int coarse_level, const double *fine,
+                       double *coarse)
+  {
+    if ((dim != 2 && dim != 3) || coarse_level < 0)
+      return 1;
+    coarsen_coefficient(dim, 1 << coarse_level, fine, coarse);
+    return 0;
+  }
+
+  int
+  oracle_coeff_vcycle(int dim, int k, int level, const double *coef,
+                      int mg_variant, double relax, const double *r, double *z)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1 ||
+        mg_variant < 0 || mg_variant > 1)
+      return 1;
+    MG mg(dim, k, level, mg_variant, relax, coef);
+    mg.apply(r, z);
+    return 0;
+  }
+
+  int
+  oracle_coeff_gradient(int dim, int k, int level, const double *lambda,
+                        const double *u, double *grad)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1)
+      return 1;
+    coefficient_gradient(Grid(dim, k, level), lambda, u, grad);
+    return 0;
+  }
+
+  // Variable-coefficient problem data from the SplitMix64 generator
+  // u01(j) = (splitmix64(seed ^ j) >> 11) 2^-53:
+  //   coef_K = exp(sigma g_K), g_K = sqrt(-2 ln(1 - u01(2^32 + 2K)))
+  //            cos(2 pi u01(2^32 + 2K + 1))   (Box-Muller)
+  //   ud_i   = 0.01 u01(2^33 + i) (interior DoFs; 0 on the boundary)
+  //   cells  = the first n_cells distinct floor(u01(2^34 + j) n_cells_total)
+  int
+  oracle_coeff_problem(int dim, int k, int level, unsigned long long seed,
+                       double sigma, double *coef, double *ud, int n_cells,
+                       long long *cells)
+  {
+    const Grid g(dim, k, level);
+    const long long nc = dim == 3 ? (long long)g.n * g.n * g.n : (long long)g.n * g.n;
+    auto u01 = [&](uint64_t j) {
+      return (double)(splitmix64(seed ^ j) >> 11) * 0x1.0p-53;
+    };
+    if (coef)
+      for (long long K = 0; K < nc; ++K)
+        {
+          const double a = u01((1ULL << 32) + 2 * (uint64_t)K);
+          const double c = u01((1ULL << 32) + 2 * (uint64_t)K + 1);
+          const double gK = std::sqrt(-2.0 * std::log(1.0 - a)) * std::cos(2.0 * M_PI * c);
+          coef[K] = std::exp(sigma * gK);
+        }
+    if (ud)
+      for (long long i = 0; i < g.N; ++i)
+        ud[i] = g.on_boundary(i) ? 0.0 : 0.01 * u01((1ULL << 33) + (uint64_t)i);
+    if (cells)
+      {
+        int got = 0;
+        for (uint64_t j = 0; got < n_cells && j < (1ULL << 20); ++j)
+          {
+            const long long K = (long long)(u01((1ULL << 34) + j) * (double)nc);
+            bool dup = false;
+            for (int q = 0; q < got; ++q)
+              dup = dup || cells[q] == K;
+            if (!dup)
+              cells[got++] = K;
+          }
+        if (got < n_cells)
+          return 1;
+      }
+    return 0;
   }
 
   // b_i = 2*u01(i) - 1 with u01(i) = (splitmix64(seed ^ i) >> 11) 2^-53;
