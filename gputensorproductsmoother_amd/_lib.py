@@ -78,6 +78,7 @@ SIGNATURES = {
     "gtps_mg_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _pd]),
     "gtps_mg_matvec": (_i, [_vp, _vp, _vp]),
     "gtps_mg_set_coefficient": (_i, [_vp, _vp]),
+    "gtps_mg_fmg_solve": (_i, [_vp, _vp, ctypes.POINTER(_vp), _i]),
     "gtps_mg_matvec_transpose": (_i, [_vp, _vp, _vp]),
     "gtps_mg_coefficient_gradient": (_i, [_vp, _vp, _vp, _vp]),
     "gtps_mg_set_graph": (_i, [_vp, _i]),

@@ -412,6 +412,21 @@ class MultigridSolvers:
         flow to b and, when given, to the per-cell coefficient c."""
         return MGSolve.apply(self, b, coef, rtol, maxit)
 
+    def solve_fmg(self, rhs_levels, x: torch.Tensor | None = None,
+                  n_cycles: int = 1) -> torch.Tensor:
+        """Full multigrid (MultigridSolvers::solve): rhs_levels[l-1] is the
+        FP64 right-hand side of level l = 1..max_level."""
+        if len(rhs_levels) != self.max_level:
+            raise ValueError(f"need {self.max_level} right-hand sides (levels 1..max)")
+        for l, r in enumerate(rhs_levels, start=1):
+            _chk_vec(r, torch.float64, (self.fe_degree * (1 << l) + 1) ** self.dim)
+        if x is None:
+            x = torch.empty_like(rhs_levels[-1])
+        _chk_vec(x, torch.float64, self.n_dofs)
+        arr = (ctypes.c_void_p * self.max_level)(*[r.data_ptr() for r in rhs_levels])
+        L.check(L.load().gtps_mg_fmg_solve(self._h, _ptr(x), arr, n_cycles), "fmg_solve")
+        return x
+
     def set_graph(self, enable: bool = True):
         """Replay the per-iteration V-cycle sequence as one HIP graph
         (default) or launch its kernels one by one."""

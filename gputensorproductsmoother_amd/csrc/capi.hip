@@ -238,6 +238,16 @@ struct gtps_mg : LaunchTimer
   double                    *r = nullptr, *p = nullptr, *q = nullptr;
   hipEvent_t                 ev = nullptr;
 
+  // variable coefficient: FP64 per-level chain (index l; the finest entry
+  // is a copy of the caller's array), empty for c = 1
+  std::vector<double *>      coef64;
+  // full multigrid: FP64 operators and solutions per level, FP64 transfer
+  std::vector<gtps_level *>  lv64;
+  std::vector<bool>          own_lv64;
+  std::vector<double *>      sol;
+  gtps_transfer             *tr64     = nullptr;
+  bool                       own_tr64 = false;
+
   // the per-iteration preconditioning sequence (V-cycle, r.z, beta,
   // p update) replayed as one HIP graph; captured lazily on a private stream
   bool                       use_graph  = true;
@@ -321,9 +331,11 @@ struct gtps_mg : LaunchTimer
     return schedule == GTPS_MG_SYMMETRIC ? GTPS_MULT_REV : GTPS_MULT_FWD;
   }
 
-  // V-cycle on level l with defect dl and correction ul (level precision)
+  // V-cycle on level l with defect dl and correction ul (level precision);
+  // n_cycles > 1 repeats smoothing + coarse correction on the same defect
+  // (MultigridSolvers::v_cycle(level, my_n_cycles), solver.cuh:1950-2001)
   void
-  vcycle(int l, const void *dl, void *ul)
+  vcycle(int l, const void *dl, void *ul, int n_cycles = 1)
   {
     const gtps_level *L = lv[l];
     if (l == 1)
@@ -332,12 +344,19 @@ struct gtps_mg : LaunchTimer
         sweep(L, ul, dl, GTPS_MULT_FWD, 1.0, nullptr, nullptr, true);
         return;
       }
+    for (int c = 0; c < n_cycles; ++c)
+      cycle(l, L, dl, ul, c == 0);
+  }
+
+  void
+  cycle(int l, const gtps_level *L, const void *dl, void *ul, bool from_zero)
+  {
     // finest level: every colour launch is timed on its own; coarser
     // levels: whole sweeps
     const bool   fine = l == maxl;
     LaunchTimer *lt   = fine ? this : nullptr;
     long         e    = fine ? -1 : tic(GTPS_PROF_SMOOTH_COARSE);
-    sweep(L, ul, dl, pre_variant(), relax, t[l], lt, true);
+    sweep(L, ul, dl, pre_variant(), relax, t[l], lt, from_zero);
     toc(GTPS_PROF_SMOOTH_COARSE, e);
     e = fine ? tic(GTPS_PROF_RESIDUAL_FINE) : -1;
     launch_laplace(L, t[l], ul, dl, false);  // t = d - A u
@@ -447,8 +466,122 @@ struct gtps_mg : LaunchTimer
       vcycle(maxl, rin, zout ? (void *)zout : u[maxl]);
   }
 
+  // FP64 levels, solutions and transfer for full multigrid (lazily)
+  void
+  ensure_fmg()
+  {
+    if (!sol.empty())
+      return;
+    lv64.assign(maxl + 1, nullptr);
+    own_lv64.assign(maxl + 1, false);
+    sol.assign(maxl + 1, nullptr);
+    for (int l = 1; l <= maxl; ++l)
+      {
+        if (l == maxl)
+          lv64[l] = fine64;
+        else if (prec == GTPS_FP64)
+          lv64[l] = lv[l];
+        else
+          {
+            lv64[l]     = make_level(ctx, dim, k, l, GTPS_FP64);
+            own_lv64[l] = true;
+            if (!coef64.empty())
+              set_level_coefficient(lv64[l], coef64[l]);
+          }
+        sol[l] = (double *)dmalloc(sizeof(double) * lv64[l]->N);
+      }
+    if (maxl >= 2)
+      {
+        if (prec == GTPS_FP64)
+          tr64 = tr;
+        else
+          {
+            tr64            = new gtps_transfer();
+            own_tr64        = true;
+            tr64->ctx       = ctx;
+            tr64->dim       = dim;
+            tr64->k         = k;
+            tr64->min_level = 1;
+            tr64->max_level = maxl;
+            tr64->prec      = GTPS_FP64;
+            tr64->tab       = get_tables(ctx, k);
+            const size_t e  = transfer_tmp_elems(dim, k, maxl);
+            tr64->tmp1      = dmalloc(e * sizeof(double));
+            tr64->tmp2      = dmalloc(e * sizeof(double));
+          }
+      }
+  }
+
+  // full multigrid (MultigridSolvers::solve, solver.cuh:1784-1860)
+  void
+  fmg(double *x, const double *const *rhs, int n_cycles)
+  {
+    ensure_fmg();
+    hipStream_t s   = ctx->stream;
+    const bool  f32 = prec == GTPS_FP32;
+    // level 1: exact coarse solve, then one more smoothing step
+    const long long N1v = lv[1]->N;
+    if (f32)
+      launch_convert_f64_f32(s, N1v, rhs[0], (float *)d[1]);
+    else
+      GTPS_HIP(hipMemcpyAsync(d[1], rhs[0], sizeof(double) * N1v,
+                              hipMemcpyDeviceToDevice, s));
+    vcycle(1, d[1], u[1]);
+    sweep(lv[1], u[1], d[1], GTPS_MULT_FWD, 1.0, nullptr);
+    if (f32)
+      launch_convert_f32_f64(s, N1v, (const float *)u[1], sol[1]);
+    else
+      GTPS_HIP(hipMemcpyAsync(sol[1], u[1], sizeof(double) * N1v,
+                              hipMemcpyDeviceToDevice, s));
+    for (int l = 2; l <= maxl; ++l)
+      {
+        const gtps_level *L64 = lv64[l];
+        const long long   N   = L64->N;
+        // prolongate (coarse Dirichlet entries excluded), then impose this
+        // level's Dirichlet values
+        launch_fill(s, N, GTPS_FP64, sol[l], 0.0);
+        launch_prolongate_add(tr64, l, sol[l], sol[l - 1]);
+        launch_copy_boundary(L64, rhs[l - 1], sol[l]);
+        // FP64 residual into q (a finest-level-sized scratch vector)
+        launch_laplace(L64, q, sol[l], rhs[l - 1], false);
+        if (f32)
+          launch_convert_f64_f32(s, N, q, (float *)d[l]);
+        else
+          GTPS_HIP(hipMemcpyAsync(d[l], q, sizeof(double) * N,
+                                  hipMemcpyDeviceToDevice, s));
+        vcycle(l, d[l], u[l], n_cycles);
+        if (f32)
+          launch_add_f32_to_f64(s, N, (const float *)u[l], sol[l]);
+        else
+          launch_axpby(s, N, 1.0, (const double *)u[l], 1.0, sol[l]);
+      }
+    GTPS_HIP(hipMemcpyAsync(x, sol[maxl], sizeof(double) * lv64[maxl]->N,
+                            hipMemcpyDeviceToDevice, s));
+  }
+
+  void
+  free_coef64()
+  {
+    for (double *c : coef64)
+      dfree(c);
+    coef64.clear();
+  }
+
   ~gtps_mg()
   {
+    for (size_t l = 0; l < lv64.size(); ++l)
+      {
+        if (own_lv64[l])
+          delete lv64[l];
+        dfree(sol[l]);
+      }
+    if (own_tr64 && tr64)
+      {
+        dfree(tr64->tmp1);
+        dfree(tr64->tmp2);
+        delete tr64;
+      }
+    free_coef64();
     for (size_t l = 0; l < lv.size(); ++l)
       {
         dfree(d.size() > l ? d[l] : nullptr);
@@ -1119,18 +1252,25 @@ extern "C"
       set_device(mg->ctx);
       hipStream_t s = mg->ctx->stream;
       // kernels change: the captured preconditioning graph is stale
+      GTPS_HIP(hipStreamSynchronize(s));
       if (mg->dir_exec)
         {
-          GTPS_HIP(hipStreamSynchronize(s));
           (void)hipGraphExecDestroy(mg->dir_exec);
           mg->dir_exec = nullptr;
         }
+      auto each_level = [&](auto &&f) {
+        for (int l = 1; l <= mg->maxl; ++l)
+          f(mg->lv[l], l);
+        if (mg->own_fine64)
+          f(mg->fine64, mg->maxl);
+        for (size_t l = 0; l < mg->lv64.size(); ++l)
+          if (mg->own_lv64[l])
+            f(mg->lv64[l], (int)l);
+      };
       if (!coef)
         {
-          for (int l = 1; l <= mg->maxl; ++l)
-            clear_level_coefficient(mg->lv[l]);
-          if (mg->own_fine64)
-            clear_level_coefficient(mg->fine64);
+          each_level([](gtps_level *L, int) { clear_level_coefficient(L); });
+          mg->free_coef64();
           return;
         }
       GTPS_REQUIRE(mg->dim == 3, "the variable-coefficient operator is 3D");
@@ -1139,34 +1279,36 @@ extern "C"
       GTPS_REQUIRE(coeff_smoother_supported(mg->k, mg->prec),
                    "the variable-coefficient smoother supports degree <= 5 in "
                    "FP64 and <= 7 in FP32");
-      // FP64 chain of averaged coefficients, finest first
-      std::vector<double *> chain(mg->maxl + 1, nullptr);
-      try
+      // FP64 chain of averaged coefficients, owned by the hierarchy
+      if (mg->coef64.empty())
         {
-          for (int l = mg->maxl - 1; l >= 1; --l)
-            {
-              const long long nc = 1LL << (3 * l);
-              chain[l]           = (double *)dmalloc(sizeof(double) * nc);
-              launch_coarsen_coefficient(s, 1 << l,
-                                         l + 1 == mg->maxl ? coef : chain[l + 1],
-                                         chain[l]);
-            }
-          set_level_coefficient(mg->lv[mg->maxl], coef);
-          for (int l = 1; l < mg->maxl; ++l)
-            set_level_coefficient(mg->lv[l], chain[l]);
-          if (mg->own_fine64)
-            set_level_coefficient(mg->fine64, coef);
-          GTPS_HIP(hipStreamSynchronize(s));
+          mg->coef64.assign(mg->maxl + 1, nullptr);
+          for (int l = 1; l <= mg->maxl; ++l)
+            mg->coef64[l] = (double *)dmalloc(sizeof(double) * (1LL << (3 * l)));
         }
-      catch (...)
-        {
-          (void)hipStreamSynchronize(s);
-          for (double *c : chain)
-            dfree(c);
-          throw;
-        }
-      for (double *c : chain)
-        dfree(c);
+      GTPS_HIP(hipMemcpyAsync(mg->coef64[mg->maxl], coef,
+                              sizeof(double) * (1LL << (3 * mg->maxl)),
+                              hipMemcpyDeviceToDevice, s));
+      for (int l = mg->maxl - 1; l >= 1; --l)
+        launch_coarsen_coefficient(s, 1 << l, mg->coef64[l + 1], mg->coef64[l]);
+      each_level([&](gtps_level *L, int l) { set_level_coefficient(L, mg->coef64[l]); });
+      GTPS_HIP(hipStreamSynchronize(s));
+    });
+  }
+
+  int
+  gtps_mg_fmg_solve(gtps_mg *mg, double *x, const double *const *rhs,
+                    int n_cycles)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && x && rhs, "NULL argument");
+      GTPS_REQUIRE(n_cycles >= 1, "n_cycles must be >= 1");
+      for (int l = 1; l <= mg->maxl; ++l)
+        GTPS_REQUIRE(rhs[l - 1] != nullptr, "missing right-hand side of a level");
+      GTPS_REQUIRE((const void *)x != (const void *)rhs[mg->maxl - 1],
+                   "x and the finest right-hand side must differ");
+      set_device(mg->ctx);
+      mg->fmg(x, rhs, n_cycles);
     });
   }
 

@@ -204,6 +204,11 @@ namespace gtps
                            double *scalars, int slot);
   void launch_set_boundary(const gtps_level *lvl, void *v, double value);
   void launch_lift_boundary(const gtps_level *lvl, const double *b, double *x);
+  // dst_B = src_B on the Dirichlet DoFs, interior untouched
+  void launch_copy_boundary(const gtps_level *lvl, const void *src, void *dst);
+  // y += (double) x
+  void launch_add_f32_to_f64(hipStream_t s, long long n, const float *x,
+                             double *y);
 
   // CG building blocks (vector.hip); scalar slots are defined there
   void launch_cg_scalar(hipStream_t s, double *scalars, int op);

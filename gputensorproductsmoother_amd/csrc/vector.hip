@@ -170,6 +170,33 @@ namespace gtps
         }
     }
 
+    template <typename T>
+    __global__ void __launch_bounds__(VNT)
+    copy_boundary_kernel(int dim, int N1, long long n, const T *__restrict__ src,
+                         T *__restrict__ dst)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          const int  x  = (int)(i % N1);
+          const int  y  = (int)((i / N1) % N1);
+          const int  z  = (int)(i / ((long long)N1 * N1));
+          const bool bd = x == 0 || x == N1 - 1 || y == 0 || y == N1 - 1 ||
+                          (dim == 3 && (z == 0 || z == N1 - 1));
+          if (bd)
+            dst[i] = src[i];
+        }
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    add_f32_to_f64_kernel(long long n, const float *__restrict__ x,
+                          double *__restrict__ y)
+    {
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        y[i] += (double)x[i];
+    }
+
     __global__ void __launch_bounds__(VNT)
     lift_boundary_kernel(int dim, int N1, long long n,
                          const double *__restrict__ b, double *__restrict__ x)
@@ -291,6 +318,31 @@ namespace gtps
       hipLaunchKernelGGL(set_boundary_kernel<double>, dim3(grid_for(lvl->N)),
                          dim3(VNT), 0, s, lvl->dim, lvl->N1, lvl->N,
                          (double *)v, value);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_copy_boundary(const gtps_level *lvl, const void *src, void *dst)
+  {
+    hipStream_t s = lvl->ctx->stream;
+    if (lvl->prec == GTPS_FP32)
+      hipLaunchKernelGGL(copy_boundary_kernel<float>, dim3(grid_for(lvl->N)),
+                         dim3(VNT), 0, s, lvl->dim, lvl->N1, lvl->N,
+                         (const float *)src, (float *)dst);
+    else
+      hipLaunchKernelGGL(copy_boundary_kernel<double>, dim3(grid_for(lvl->N)),
+                         dim3(VNT), 0, s, lvl->dim, lvl->N1, lvl->N,
+                         (const double *)src, (double *)dst);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_add_f32_to_f64(hipStream_t s, long long n, const float *x, double *y)
+  {
+    if (n <= 0)
+      return;
+    hipLaunchKernelGGL(add_f32_to_f64_kernel, dim3(grid_for(n)), dim3(VNT), 0,
+                       s, n, x, y);
     GTPS_LAUNCH_CHECK();
   }
 

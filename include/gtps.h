@@ -231,6 +231,19 @@ int gtps_mg_matvec(gtps_mg *mg, double *dst, const double *src);
  * only.  No reference counterpart (see gtps_level_set_coefficient). */
 int gtps_mg_set_coefficient(gtps_mg *mg, const double *coef);
 
+/* Full multigrid, MultigridSolvers::solve (include/solver.cuh:1784-1860):
+ * exact level-1 solve plus one more smoothing step, then for each finer
+ * level: prolongate the coarser solution (FP64), impose the level's
+ * Dirichlet values (the boundary entries of its right-hand side), form the
+ * FP64 residual and add the correction of n_cycles V-cycles in the V-cycle
+ * precision.  rhs is a host array of max_level device pointers, rhs[l-1]
+ * the FP64 right-hand side of level l (the reference assembles the load
+ * vector on every level).  x (FP64, finest level) receives the solution.
+ * The finest level's residual goes through the hierarchy's CG scratch, so
+ * no solve may run concurrently on the same hierarchy. */
+int gtps_mg_fmg_solve(gtps_mg *mg, double *x, const double *const *rhs,
+                      int n_cycles);
+
 /* FP64 finest-level dst = A^T src (see gtps_laplace_vmult_transpose). */
 int gtps_mg_matvec_transpose(gtps_mg *mg, double *dst, const double *src);
 

@@ -57,6 +57,7 @@ def lib():
         L.oracle_coeff_coarsen.argtypes = [i, i, _dp, _dp]
         L.oracle_coeff_vcycle.argtypes = [i, i, i, _dp, i, d, _dp, _dp]
         L.oracle_coeff_gradient.argtypes = [i, i, i, _dp, _dp, _dp]
+        L.oracle_fmg.argtypes = [i, i, i, _dp, i, d, _dp, i, _dp]
         L.oracle_coeff_problem.argtypes = [i, i, i, ull, d, _dp, _dp, i,
                                            ctypes.POINTER(ll)]
         _lib = L
@@ -253,3 +254,17 @@ def coeff_problem(dim, k, level, seed=20261202, sigma=0.5, n_fd_cells=16):
                                     cells.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))),
          "coeff_problem")
     return coef, ud, cells
+
+
+def fmg(dim, k, level, rhs_levels, n_cycles=1, mg_variant=MG_SYMMETRIC, relax=1.0, coef=None):
+    """Full multigrid from level 1 to `level`; rhs_levels[l-1] is the
+    right-hand side of level l (its Dirichlet entries are the boundary
+    values)."""
+    assert len(rhs_levels) == level
+    allr = np.ascontiguousarray(np.concatenate([np.asarray(r, dtype=np.float64)
+                                                for r in rhs_levels]))
+    x = np.zeros(n_dofs(dim, k, level))
+    c = None if coef is None else np.ascontiguousarray(coef, dtype=np.float64)
+    _chk(lib().oracle_fmg(dim, k, level, None if c is None else _p(c), mg_variant, relax,
+                          _p(allr), n_cycles, _p(x)), "fmg")
+    return x

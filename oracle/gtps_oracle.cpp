@@ -1034,8 +1034,10 @@ namespace
     {
       return variant == 2 ? ADDITIVE : (variant == 0 ? MULT_REV : MULT_FWD);
     }
+    // n_cycles > 1 repeats smoothing + coarse correction on the same defect
+    // (MultigridSolvers::v_cycle(level, my_n_cycles), solver.cuh:1950-2001)
     void
-    vcycle(int l) const
+    vcycle(int l, int n_cycles = 1) const
     {
       const Grid &g = grids[l];
       std::fill(u[l].begin(), u[l].end(), 0.0);
@@ -1046,15 +1048,49 @@ namespace
           smooth(l, MULT_FWD, 1.0, d[l].data(), u[l].data());
           return;
         }
-      smooth(l, pre_variant(), relax, d[l].data(), u[l].data());
-      laplace_vmult(g, u[l].data(), t[l].data(), level_coef(l));
-      for (long long i = 0; i < g.N; ++i)
-        t[l][i] = d[l][i] - t[l][i];
-      std::fill(d[l - 1].begin(), d[l - 1].end(), 0.0);
-      transfer(g, false, d[l - 1].data(), nullptr, nullptr, t[l].data());
-      vcycle(l - 1);
-      transfer(g, true, nullptr, u[l].data(), u[l - 1].data(), nullptr);
-      smooth(l, post_variant(), relax, d[l].data(), u[l].data());
+      for (int c = 0; c < n_cycles; ++c)
+        {
+          smooth(l, pre_variant(), relax, d[l].data(), u[l].data());
+          laplace_vmult(g, u[l].data(), t[l].data(), level_coef(l));
+          for (long long i = 0; i < g.N; ++i)
+            t[l][i] = d[l][i] - t[l][i];
+          std::fill(d[l - 1].begin(), d[l - 1].end(), 0.0);
+          transfer(g, false, d[l - 1].data(), nullptr, nullptr, t[l].data());
+          vcycle(l - 1);
+          transfer(g, true, nullptr, u[l].data(), u[l - 1].data(), nullptr);
+          smooth(l, post_variant(), relax, d[l].data(), u[l].data());
+        }
+    }
+    // Full multigrid (MultigridSolvers::solve, solver.cuh:1784-1860): exact
+    // coarse solve plus one more smoothing step on level 1, then per finer
+    // level: prolongate the coarse solution, impose the Dirichlet values
+    // (boundary entries of that level's right-hand side), form the residual
+    // and add the correction of n_cycles V-cycles.  rhs[l] for l = 1..max.
+    void
+    fmg(const std::vector<const double *> &rhs, int n_cycles, double *x) const
+    {
+      std::vector<std::vector<double>> sol(maxlevel + 1);
+      const Grid &g1 = grids[minlevel];
+      std::copy(rhs[minlevel], rhs[minlevel] + g1.N, d[minlevel].begin());
+      vcycle(minlevel);
+      smooth(minlevel, MULT_FWD, 1.0, d[minlevel].data(), u[minlevel].data());
+      sol[minlevel] = u[minlevel];
+      for (int l = minlevel + 1; l <= maxlevel; ++l)
+        {
+          const Grid &g = grids[l];
+          sol[l].assign(g.N, 0.0);
+          transfer(g, true, nullptr, sol[l].data(), sol[l - 1].data(), nullptr);
+          for (long long i = 0; i < g.N; ++i)
+            if (g.on_boundary(i))
+              sol[l][i] = rhs[l][i];
+          laplace_vmult(g, sol[l].data(), t[l].data(), level_coef(l));
+          for (long long i = 0; i < g.N; ++i)
+            d[l][i] = rhs[l][i] - t[l][i];
+          vcycle(l, n_cycles);
+          for (long long i = 0; i < g.N; ++i)
+            sol[l][i] += u[l][i];
+        }
+      std::copy(sol[maxlevel].begin(), sol[maxlevel].end(), x);
     }
     void
     apply(const double *r, double *z) const
@@ -1301,6 +1337,26 @@ extern "C"
       return 1;
     MG mg(dim, k, level, mg_variant, relax, coef);
     mg.apply(r, z);
+    return 0;
+  }
+
+  // rhs_all: the right-hand sides of levels 1..level, concatenated
+  int
+  oracle_fmg(int dim, int k, int level, const double *coef, int mg_variant,
+             double relax, const double *rhs_all, int n_cycles, double *x)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || level < 1 ||
+        mg_variant < 0 || mg_variant > 2 || n_cycles < 1)
+      return 1;
+    MG mg(dim, k, level, mg_variant, relax, coef);
+    std::vector<const double *> rhs(level + 1, nullptr);
+    const double *p = rhs_all;
+    for (int l = 1; l <= level; ++l)
+      {
+        rhs[l] = p;
+        p += mg.grids[l].N;
+      }
+    mg.fmg(rhs, n_cycles, x);
     return 0;
   }
 

@@ -179,3 +179,31 @@ def test_graph_replays_match_eager(core, ctx, prec):
         assert it_g == it_e, rep
         assert h_g == h_e, rep
         assert torch.equal(x_g, x_e), rep
+
+
+# ------------------------------------------------------------ full multigrid
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dim,k,level,n_cycles,coef", [
+    (2, 2, 5, 1, False), (3, 2, 4, 1, False), (3, 3, 4, 2, False), (3, 2, 3, 1, True)])
+def test_fmg_matches_oracle(dim, k, level, n_cycles, coef, prec):
+    """GPU full multigrid (MultigridSolvers::solve) against the oracle's FMG
+    on the sine problem (per-level assembled right-hand sides)."""
+    from gputensorproductsmoother_amd import core
+    print("CASE fmg", dim, k, level, n_cycles, coef, prec, flush=True)
+    ctx = core.Context(0)
+    rhs = [O.rhs_assembled(dim, k, l, O.RHS_SINE) for l in range(1, level + 1)]
+    c = np.exp(0.5 * rng(77).standard_normal((1 << level) ** 3)) if coef else None
+    ref = O.fmg(dim, k, level, rhs, n_cycles=n_cycles, coef=c)
+    mg = core.MultigridSolvers(ctx, dim, k, level,
+                               torch.float64 if prec == "fp64" else torch.float32)
+    if coef:
+        mg.set_coefficient(torch.tensor(c, device="cuda:0"))
+    rd = [torch.tensor(r, device="cuda:0") for r in rhs]
+    x = mg.solve_fmg(rd, n_cycles=n_cycles)
+    torch.cuda.synchronize()
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    assert rel_err(x.cpu().numpy(), ref) < tol
+    if not coef:
+        e = O.l2_error_sine(dim, k, level, x.cpu().numpy())
+        e_ref = O.l2_error_sine(dim, k, level, ref)
+        assert e <= 1.01 * e_ref

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import boundary_mask, n_dofs, rng
+from helpers import boundary_mask, n_dofs, rel_err, rng
 
 
 def coords(dim, k, level):
@@ -152,3 +152,32 @@ def test_planted_mistake_is_caught():
     z_sym = O.vcycle(3, 3, 3, r, mg_variant=O.MG_SYMMETRIC)
     z_ref = O.vcycle(3, 3, 3, r, mg_variant=O.MG_REFERENCE_ORDER)
     assert np.abs(z_sym - z_ref).max() > 1e-3 * np.abs(z_sym).max()
+
+
+# ------------------------------------------------------------ full multigrid
+@pytest.mark.parametrize("dim,k,level", [(2, 2, 4), (3, 2, 3), (3, 3, 3)])
+def test_fmg_reaches_discretisation_error(dim, k, level):
+    """One FMG pass (n_cycles = 1) on the sine problem is as accurate as the
+    exact discrete solution (within 5%), the classical FMG property."""
+    rhs = [O.rhs_assembled(dim, k, l, O.RHS_SINE) for l in range(1, level + 1)]
+    x = O.fmg(dim, k, level, rhs)
+    xs, _, _, _ = O.pcg(dim, k, level, rhs[-1], rtol=1e-13)
+    e_fmg, e_ex = O.l2_error_sine(dim, k, level, x), O.l2_error_sine(dim, k, level, xs)
+    assert e_fmg <= 1.05 * e_ex
+
+
+@pytest.mark.parametrize("dim,k,level", [(2, 3, 3), (3, 2, 2)])
+def test_fmg_cycles_converge_to_discrete_solution(dim, k, level):
+    """With many V-cycles per level the FMG correction loop converges to the
+    discrete solution (it is an iteration on the right system)."""
+    rhs = [O.rhs_assembled(dim, k, l, O.RHS_CONST) for l in range(1, level + 1)]
+    x = O.fmg(dim, k, level, rhs, n_cycles=25)
+    xs, _, _, _ = O.pcg(dim, k, level, rhs[-1], rtol=1e-14)
+    assert rel_err(x, xs) < 1e-11
+
+
+def test_fmg_level_one_is_exact_solve():
+    rhs = [O.rhs_assembled(3, 3, 1, O.RHS_CONST)]
+    x = O.fmg(3, 3, 1, rhs)
+    r = rhs[0] - O.laplace_vmult(3, 3, 1, x)
+    assert np.abs(r).max() < 1e-13 * np.abs(rhs[0]).max()
