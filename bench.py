@@ -77,6 +77,218 @@ def measured_traffic(kernel: str, dim: int, k: int, level: int):
     return None
 
 
+def dominant_kernel_name(core, dim: int, k: int, level: int) -> str:
+    """Name of the FP32 smoother kernel a finest-level colour launch runs
+    with the current tuning (3D k <= 3: the patch-group kernel)."""
+    if dim == 3 and k <= 3:
+        n = 1 << level
+        patches = (n // 2) ** 3
+        npw = 1
+        if patches > core.get_tuning("group_min_patches"):
+            npw = core.get_tuning("patches_per_wave")
+        return f"patch3d_group_kernel<{k}, float, 0, {npw}>"
+    return f"patch3d_kernel<{k}, float, false>"
+
+
+def _best_ms(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) * 1e3)
+    return best
+
+
+def config_report(args):
+    """One JSON line per workload of the parity configurations (not the
+    contract line): iterations, time per solve, MDoF/s, and the relative L2
+    distance of each solve to a tight FP64 solve of the same system."""
+    import numpy as np
+    import torch
+    from gputensorproductsmoother_amd import core, problems
+
+    ctx = core.Context(0)
+    dev = "cuda:0"
+    reps = max(1, args.steps)
+
+    def rel_l2(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    def solve_pair(dim, k, level, b, rtol, schedule=core.MultigridSolvers.SYMMETRIC,
+                   coef=None, precs=("fp32", "fp64")):
+        ref_mg = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+        if coef is not None:
+            ref_mg.set_coefficient(coef)
+        xref, _, _, _ = ref_mg.solve_cg(b, rtol=1e-13, maxit=500, raise_on_failure=False)
+        xref = xref.clone()
+        del ref_mg
+        res = {}
+        n = b.numel()
+        for prec in precs:
+            mg = core.MultigridSolvers(ctx, dim, k, level,
+                                       torch.float32 if prec == "fp32" else torch.float64,
+                                       schedule)
+            if coef is not None:
+                mg.set_coefficient(coef)
+            x = torch.empty_like(b)
+            info = {}
+
+            def run():
+                _, it, rel, _ = mg.solve_cg(b, x, rtol=rtol, maxit=500)
+                info.update(iterations=it, relres=rel)
+            ms = _best_ms(run, reps)
+            res[prec] = {"iterations": info["iterations"], "relres": info["relres"],
+                         "ms_per_solve": round(ms, 3), "MDoF_s": round(n / ms / 1e3, 2),
+                         "rel_l2_vs_fp64_tight": rel_l2(x, xref)}
+            del mg
+        return res
+
+    def cell_mass(k):
+        op = core.LaplaceOperator(ctx, 3, k, 1, torch.float64)
+        return op.tensor_product_data()["cell_mass"]
+
+    for name in args.configs.split(","):
+        name = name.strip().lower()
+        if name == "c1":
+            dim, k, level = 2, 2, 7
+            b = torch.tensor(problems.rhs_constant(dim, k, level, cell_mass(k)), device=dev)
+            r = solve_pair(dim, k, level, b, 1e-10, core.MultigridSolvers.ADDITIVE,
+                           precs=("fp64",))
+            line = {"config": "C1", "workload": "2D Q2 Poisson f=1, 128^2 cells, additive "
+                    "vertex-patch smoother (relaxation 0.4), all FP64, CG to 1e-10",
+                    "n_dofs": b.numel(), **r}
+        elif name == "c2":
+            dim, k, level = 3, 3, 6
+            b = torch.tensor(problems.rhs_random(dim, k, level), device=dev)
+            r = solve_pair(dim, k, level, b, 1e-8)
+            line = {"config": "C2", "workload": "3D Q3 Poisson, 64^3 cells, random rhs, "
+                    "multiplicative smoother, CG to 1e-8", "n_dofs": b.numel(), **r}
+        elif name == "c3":
+            dim, k, level = 3, 7, 5
+            b = torch.tensor(problems.rhs_constant(dim, k, level, cell_mass(k)), device=dev)
+            r = solve_pair(dim, k, level, b, 1e-8)
+            line = {"config": "C3", "workload": "3D Q7 Poisson f=1, 32^3 cells, "
+                    "multiplicative smoother (LDS/FMA kernels; no MFMA kernel is built), "
+                    "CG to 1e-8", "n_dofs": b.numel(), "mfma_utilisation": None, **r}
+        elif name == "c4":
+            line = c4_report(ctx, core, problems, reps, solve_pair, rel_l2)
+        elif name == "c5":
+            line = {"config": "C5", "status": "not built: the SIPG-DG operator and DG "
+                    "vertex-patch smoother are outside this build so far"}
+        else:
+            raise SystemExit(f"unknown config {name}")
+        print(json.dumps(line), flush=True)
+
+
+def c4_report(ctx, core, problems, reps, solve_pair, rel_l2):
+    """Variable-coefficient adjoint workload: forward MG-CG, adjoint solve and
+    dJ/dc kernel through the autograd Function, checked against central
+    finite differences (h = 1e-6) of J on 16 cells with FP64 solves to 1e-14."""
+    import torch
+    dev = "cuda:0"
+    dim, k, level = 3, 4, 5
+    coef_h, ud_h, cells = problems.coeff_problem(dim, k, level)
+    op = core.LaplaceOperator(ctx, 3, k, 1, torch.float64)
+    b = torch.tensor(problems.rhs_constant(dim, k, level,
+                                           op.tensor_product_data()["cell_mass"]), device=dev)
+    coef = torch.tensor(coef_h, device=dev)
+    ud = torch.tensor(ud_h, device=dev)
+    r = solve_pair(dim, k, level, b, 1e-8, coef=coef)
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float32)
+    mask = mg.boundary_mask()
+
+    def J(u):
+        d = torch.where(mask, torch.zeros_like(u), u - ud)
+        return 0.5 * (d * d).sum()
+
+    def fwd_bwd(rtol):
+        c = coef.clone().requires_grad_(True)
+        u = mg.solve(b, c, rtol=rtol, maxit=500)
+        J(u).backward()
+        return c.grad
+
+    ms_fb = _best_ms(lambda: fwd_bwd(1e-8), reps)
+    grad = fwd_bwd(1e-12).cpu().numpy()
+    mg64 = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+
+    def u64(cv):
+        mg64.set_coefficient(cv)
+        u, _, _, _ = mg64.solve_cg(b, rtol=1e-14, maxit=500, raise_on_failure=False)
+        return u.clone()
+
+    # central difference J(c + h e_K) - J(c - h e_K), evaluated as
+    # sum (u+ - u-) ((u+ + u-)/2 - u_d) over the interior: the same number
+    # without the cancellation of two ~1e3-sized sums
+    h = 1e-6
+    dev_max = 0.0
+    for K in cells.tolist():
+        cp, cm = coef.clone(), coef.clone()
+        cp[K] += h
+        cm[K] -= h
+        up, um = u64(cp), u64(cm)
+        dj = torch.where(mask, torch.zeros_like(up), (up - um) * (0.5 * (up + um) - ud)).sum()
+        fd = dj.item() / (2 * h)
+        dev_max = max(dev_max, abs(grad[K] - fd) / abs(fd))
+    return {"config": "C4", "workload": "3D Q4 -div(c grad u) = 1, 32^3 cells, c = exp(0.5 g), "
+            "J = 1/2 ||u - u_d||^2, inexact-FD multiplicative smoother, CG to 1e-8",
+            "n_dofs": b.numel(), **r,
+            "forward_plus_adjoint_ms": round(ms_fb, 3),
+            "fd_check": {"cells": len(cells), "h": h,
+                         "max_rel_dev_adjoint_vs_fd": dev_max}}
+
+
+def traffic_summary(fetch_dir: str, write_dir: str, kernel: str, workload, out_path: str):
+    """Turn two rocprofv3 --pmc runs of `bench.py --kernels` (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) into HBM bytes per launch of `kernel`:
+    each counter is scaled by what it reads for the 256 MiB device copy of
+    the same run (known bytes read and written), and the result is written
+    to out_path (profiles/traffic.json)."""
+    import csv
+    import glob
+
+    def rows(d, counter):
+        path = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not path:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        with open(path[0]) as f:
+            return [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+
+    copy_bytes = 256 * 2 ** 20
+    res = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rs = rows(d, counter)
+        copies = [float(r["Counter_Value"]) for r in rs if "copyBuffer" in r["Kernel_Name"]]
+        if not copies:
+            raise SystemExit(f"no large copy dispatch in {d}")
+        scale = copy_bytes / (max(copies) * 1024.0)
+        vals = [float(r["Counter_Value"]) for r in rs
+                if "::" + kernel + "(" in r["Kernel_Name"]]
+        if not vals:
+            raise SystemExit(f"no dispatch of {kernel} in {d}")
+        res[counter] = {"kib_per_launch_raw": sum(vals) / len(vals), "scale": round(scale, 4),
+                        "launches": len(vals),
+                        "bytes_per_launch": sum(vals) / len(vals) * 1024.0 * scale}
+    total = res["FETCH_SIZE"]["bytes_per_launch"] + res["WRITE_SIZE"]["bytes_per_launch"]
+    entry = {"kernel": kernel, "workload": list(workload), "bytes_per_launch": int(round(total)),
+             "fetch": res["FETCH_SIZE"], "write": res["WRITE_SIZE"],
+             "calibration": "counter KiB x scale, scale = 256 MiB / counter of the "
+                            "256 MiB device copy of the same run"}
+    data = {"kernels": []}
+    if os.path.exists(out_path):
+        with open(out_path) as f:
+            data = json.load(f)
+    data["kernels"] = [e for e in data.get("kernels", [])
+                       if not (e.get("kernel") == kernel and e.get("workload") == list(workload))]
+    data["kernels"].append(entry)
+    with open(out_path, "w") as f:
+        json.dump(data, f, indent=1)
+    print(json.dumps(entry), flush=True)
+
+
 def cpu_baseline(args):
     """FP64 C++ oracle MG-PCG on the host cores (bounded sample)."""
     import oracle as O
@@ -200,9 +412,22 @@ def main():
     ap.add_argument("--repeat", type=int, default=10)
     ap.add_argument("--smoother-levels", action="store_true",
                     help="with --kernels: both smoother kernels on every level")
+    ap.add_argument("--configs", default="", metavar="c1,c2,c3,c4,c5",
+                    help="per-configuration report (iterations, times, FP32-vs-FP64 "
+                         "distance) instead of the contract line")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="library tuning knob (gtps_set_tuning) applied before the run")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic.json from two rocprofv3 --pmc runs of "
+                         "--kernels (no GPU work)")
+    ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2>")
     args = ap.parse_args()
+
+    if args.traffic_summary:
+        traffic_summary(args.traffic_summary[0], args.traffic_summary[1], args.traffic_kernel,
+                        (args.dim, args.degree, args.level),
+                        os.path.join(ROOT, "profiles", "traffic.json"))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
@@ -219,6 +444,9 @@ def main():
     if args.kernels:
         kernel_bench(args)
         return
+    if args.configs:
+        config_report(args)
+        return
 
     import torch
     from gputensorproductsmoother_amd import core
@@ -230,15 +458,17 @@ def main():
     b = torch.tensor(b_host, device="cuda:0")
     x = torch.empty_like(b)
 
-    def run(prec, profile_groups, steps, warmup):
+    def run(prec, steps, warmup, kernel_pass):
+        """Timed solves (HIP-graph replay of the V-cycle, no stage timers),
+        then, when kernel_pass, the same number of solves again with HIP
+        events around every finest-level sweep (kernel-by-kernel launches)
+        for the dominant kernel's mean launch duration."""
         t0 = time.perf_counter()
         mg = core.MultigridSolvers(ctx, dim, k, level, prec)
         torch.cuda.synchronize()
         setup_s = time.perf_counter() - t0
         for _ in range(warmup):
             mg.solve_cg(b, x, rtol=args.rtol, maxit=100)
-        mg.set_profiling(profile_groups)
-        mg.profile_reset()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         its = []
@@ -248,12 +478,18 @@ def main():
             its.append(it)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        prof = mg.profile()
-        mg.set_profiling(())
+        prof = None
+        if kernel_pass:
+            mg.set_profiling(("smooth_fine",))
+            mg.profile_reset()
+            for _ in range(steps):
+                mg.solve_cg(b, x, rtol=args.rtol, maxit=100)
+            prof = mg.profile()
+            mg.set_profiling(())
         return mg, dict(ms=dt / steps * 1e3, iterations=its, relres=rel, setup_s=setup_s,
                         profile=prof)
 
-    mg32, r32 = run(torch.float32, ("smooth_fine",), args.steps, args.warmup)
+    mg32, r32 = run(torch.float32, args.steps, args.warmup, True)
     value = N / (r32["ms"] / 1e3) / 1e6
 
     # dominant kernel: finest-level patch-smoother colour launch (FP32)
@@ -264,8 +500,7 @@ def main():
     bytes_per_launch = 4.0 * sum(vals) / len(vals)
     avg_launch_ms = tot_ms / max(cnt, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    kernel_name = (f"patch3d_single_kernel<{k}, float, 0>" if dim == 3 and k <= 3
-                   else f"patch3d_kernel<{k}, float, false>")
+    kernel_name = dominant_kernel_name(core, dim, k, level)
     traffic = measured_traffic(kernel_name, dim, k, level)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -278,7 +513,7 @@ def main():
                              "iterations": r32["iterations"][-1],
                              "relres": r32["relres"], "setup_s": round(r32["setup_s"], 3)}}
     if not args.no_fp64:
-        _, r64 = run(torch.float64, (), max(2, args.steps // 2), 1)
+        _, r64 = run(torch.float64, max(2, args.steps // 2), 1, False)
         v64 = N / (r64["ms"] / 1e3) / 1e6
         extra["fp64_everywhere"] = {"value": round(v64, 2), "ms_per_solve": round(r64["ms"], 3),
                                     "iterations": r64["iterations"][-1],

@@ -46,6 +46,7 @@ SIGNATURES = {
     "gtps_last_error": (ctypes.c_char_p, []),
     "gtps_version": (ctypes.c_char_p, []),
     "gtps_set_tuning": (_ll, [ctypes.c_char_p, _ll]),
+    "gtps_get_tuning": (_ll, [ctypes.c_char_p]),
     "gtps_level_create": (_i, [_vp, _i, _i, _i, _i, ctypes.POINTER(_vp)]),
     "gtps_level_destroy": (_i, [_vp]),
     "gtps_level_n_dofs": (_ll, [_vp]),

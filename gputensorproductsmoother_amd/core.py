@@ -46,6 +46,14 @@ def default_relaxation(dim: int, schedule: int) -> float:
     return 1.0
 
 
+def get_tuning(key: str) -> int:
+    """Current value of a tuning knob (see gtps_set_tuning)."""
+    v = L.load().gtps_get_tuning(key.encode())
+    if v < 0:
+        L.check(L.GTPS_EINVAL, "get_tuning")
+    return int(v)
+
+
 def set_tuning(key: str, value: int) -> int:
     """Set a process-wide tuning knob (see gtps_set_tuning in
     include/gtps.h); returns the previous value."""

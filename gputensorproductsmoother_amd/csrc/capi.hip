@@ -156,12 +156,13 @@ namespace
       }
   }
 
-  // brackets every colour launch of a sweep when a timer is attached
+  // brackets the colour launches of a sweep when a timer is attached (one
+  // event pair per sweep, credited to the number of launches it holds)
   struct LaunchTimer
   {
-    virtual long tic_launch()        = 0;
-    virtual void toc_launch(long e)  = 0;
-    virtual ~LaunchTimer()           = default;
+    virtual long tic_launch()                  = 0;
+    virtual void toc_launch(long e, int count) = 0;
+    virtual ~LaunchTimer()                     = default;
   };
 
   // one smoothing sweep; `resid` is level-sized scratch for the additive
@@ -186,14 +187,13 @@ namespace
                      "in FP64 and <= 7 in FP32");
         if (from_zero)
           launch_fill(lvl->ctx->stream, lvl->N, lvl->prec, u, 0.0);
+        const long e = timer ? timer->tic_launch() : -1;
         for (int cc = 0; cc < ncol; ++cc)
-          {
-            const int  c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
-            const long e = timer ? timer->tic_launch() : -1;
-            launch_patch_coeff_colour(lvl, u, b, c, relax);
-            if (timer)
-              timer->toc_launch(e);
-          }
+          launch_patch_coeff_colour(lvl, u, b,
+                                    variant == GTPS_MULT_REV ? ncol - 1 - cc : cc,
+                                    relax);
+        if (timer)
+          timer->toc_launch(e, ncol);
         return;
       }
     if (from_zero && variant != GTPS_MULT_FWD)
@@ -202,25 +202,23 @@ namespace
       {
         GTPS_REQUIRE(resid != nullptr, "additive sweep needs a residual buffer");
         launch_laplace(lvl, resid, u, b, false);
+        const long e = timer ? timer->tic_launch() : -1;
         for (int c = 0; c < ncol; ++c)
-          {
-            const long e = timer ? timer->tic_launch() : -1;
-            launch_patch_colour(lvl, u, resid, c, 1, relax);
-            if (timer)
-              timer->toc_launch(e);
-          }
+          launch_patch_colour(lvl, u, resid, c, 1, relax);
+        if (timer)
+          timer->toc_launch(e, ncol);
         return;
       }
+    const long e = timer ? timer->tic_launch() : -1;
     for (int cc = 0; cc < ncol; ++cc)
       {
-        const int  c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
-        const long e = timer ? timer->tic_launch() : -1;
+        const int c = variant == GTPS_MULT_REV ? ncol - 1 - cc : cc;
         launch_patch_colour(lvl, u, b, c,
                             (from_zero && variant == GTPS_MULT_FWD && cc == 0) ? 2 : 0,
                             relax);
-        if (timer)
-          timer->toc_launch(e);
       }
+    if (timer)
+      timer->toc_launch(e, ncol);
   }
 } // namespace
 
@@ -258,7 +256,13 @@ struct gtps_mg : LaunchTimer
   int                        prof_mask = 0;
   std::vector<hipEvent_t>    ev_pool;
   size_t                     ev_used = 0;
-  std::vector<std::pair<int, size_t>> pending;  // (group, index of start event)
+  struct Interval
+  {
+    int    group;
+    size_t start, stop;  // event indices
+    int    count;        // launches (or intervals) it stands for
+  };
+  std::vector<Interval>      pending;
   long long                  prof_count[GTPS_PROF_N_GROUPS] = {};
   double                     prof_ms[GTPS_PROF_N_GROUPS]    = {};
 
@@ -284,24 +288,24 @@ struct gtps_mg : LaunchTimer
     return (long)i;
   }
   void
-  toc(int group, long start)
+  toc(int group, long start, int count = 1)
   {
     if (start < 0)
       return;
+    const size_t stop = ev_used;
     GTPS_HIP(hipEventRecord(next_event(), ctx->stream));
-    pending.emplace_back(group, (size_t)start);
+    pending.push_back({group, (size_t)start, stop, count});
   }
   // after the stream is synchronised: fold pending intervals into totals
   void
   collect()
   {
-    for (const auto &pr : pending)
+    for (const auto &iv : pending)
       {
         float ms = 0.f;
-        GTPS_HIP(hipEventElapsedTime(&ms, ev_pool[pr.second],
-                                     ev_pool[pr.second + 1]));
-        prof_count[pr.first] += 1;
-        prof_ms[pr.first] += ms;
+        GTPS_HIP(hipEventElapsedTime(&ms, ev_pool[iv.start], ev_pool[iv.stop]));
+        prof_count[iv.group] += iv.count;
+        prof_ms[iv.group] += ms;
       }
     pending.clear();
     ev_used = 0;
@@ -313,9 +317,9 @@ struct gtps_mg : LaunchTimer
     return tic(GTPS_PROF_SMOOTH_FINE);
   }
   void
-  toc_launch(long e) override
+  toc_launch(long e, int count) override
   {
-    toc(GTPS_PROF_SMOOTH_FINE, e);
+    toc(GTPS_PROF_SMOOTH_FINE, e, count);
   }
 
   int
@@ -1166,6 +1170,25 @@ extern "C"
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });
     return rc == GTPS_OK ? prev : -1;
+  }
+
+  long long
+  gtps_get_tuning(const char *key)
+  {
+    long long v = -1;
+    int rc = guard([&] {
+      GTPS_REQUIRE(key != nullptr, "NULL key");
+      const std::string k(key);
+      if (k == "small_colour_patches")
+        v = tuning().small_colour_patches;
+      else if (k == "group_min_patches")
+        v = tuning().group_min_patches;
+      else if (k == "patches_per_wave")
+        v = tuning().patches_per_wave;
+      else
+        throw Error(GTPS_EINVAL, "unknown tuning key " + k);
+    });
+    return rc == GTPS_OK ? v : -1;
   }
 
   int

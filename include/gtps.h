@@ -81,6 +81,9 @@ const char *gtps_version(void);
  * scripts/ct_parameter.py.in:146-148). */
 long long gtps_set_tuning(const char *key, long long value);
 
+/* Current value of a tuning knob (-1 and gtps_last_error for an unknown key). */
+long long gtps_get_tuning(const char *key);
+
 /* ------------------------------------------------------ level / operator */
 /* One MG level with its matrix-free Laplace operator.
  * Replaces PSMF::LaplaceOperator<dim,k,Number,DoFLayout::Q>::initialize
@@ -266,7 +269,9 @@ int gtps_mg_set_graph(gtps_mg *mg, int enable);
  * reference's per-level timing buckets (MultigridSolvers::print_wall_times,
  * include/solver.cuh:1747-1771, filled at 1957-1999) without its per-stage
  * device synchronisations.  `mask` enables groups (bit g = group g):
- *   GTPS_PROF_SMOOTH_FINE   one patch-kernel (colour) launch, finest level
+ *   GTPS_PROF_SMOOTH_FINE   finest-level sweeps, one event pair per sweep,
+ *                           counted as its colour launches (so total/count
+ *                           is the mean colour-launch time)
  *   GTPS_PROF_SMOOTH_COARSE smoothing sweeps on the coarser levels
  *   GTPS_PROF_RESIDUAL_FINE t = d - A u on the finest level
  *   GTPS_PROF_TRANSFER      restriction and prolongation, all levels
