@@ -397,10 +397,8 @@ struct gtps_mg : LaunchTimer
     else
       vcycle(maxl, r, z64);
     toc(GTPS_PROF_VCYCLE, e);
-    launch_cg_scalar(s, sc, cg::OP_SHIFT);
     const int nb = launch_dot_partials(s, N, r, z64, z32, pt, ctx->ws.capacity);
-    launch_finalize_sum(s, pt, nb, sc, cg::RZ);
-    launch_cg_scalar(s, sc, cg::OP_BETA);
+    launch_finalize_op(s, pt, nb, sc, cg::RZ, cg::OP_SHIFT_BETA);
     launch_cg_update_p(s, N, p, z64, z32, sc);
   }
 
@@ -1405,24 +1403,31 @@ extern "C"
           return;
         }
       mg->direction();
-      double rn = r0;
+      double rn = r0, rn_prev = -1.0;
       while (it_done < maxit)
         {
           // q = A p with fused p.q, alpha = rz / pq
           const long e  = mg->tic(GTPS_PROF_CG_MATVEC);
           const int nbq = launch_laplace(mg->fine64, mg->q, mg->p, nullptr, true);
           mg->toc(GTPS_PROF_CG_MATVEC, e);
-          launch_finalize_sum(s, pt, nbq, sc, cg::PQ);
-          launch_cg_scalar(s, sc, cg::OP_ALPHA);
+          launch_finalize_op(s, pt, nbq, sc, cg::PQ, cg::OP_ALPHA);
           const int nbr =
             launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap);
           launch_finalize_sum(s, pt, nbr, sc, cg::RR);
           read_rr();
           ++it_done;
-          // enqueue the next preconditioning step while the host checks
-          if (it_done < maxit)
+          // Enqueue the next preconditioning step while the host checks,
+          // unless the residual history predicts that this iteration
+          // converges (then the step would only be waited for and dropped).
+          // The prediction assumes the last reduction factor again, with a
+          // margin of 4; a wrong guess costs one host round trip.
+          bool speculate = it_done < maxit;
+          if (speculate && rn_prev > 0.0 && rn < rn_prev)
+            speculate = rn * (rn / rn_prev) > 0.25 * rtol * r0;
+          if (speculate)
             mg->direction();
-          rn = wait_rr();
+          rn_prev = rn;
+          rn      = wait_rr();
           if (history)
             history[it_done] = rn;
           if (!std::isfinite(rn))
@@ -1435,6 +1440,8 @@ extern "C"
               converged = true;
               break;
             }
+          if (!speculate && it_done < maxit)
+            mg->direction();
         }
       rel = rn / r0;
       GTPS_HIP(hipStreamSynchronize(s));

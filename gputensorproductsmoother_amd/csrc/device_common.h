@@ -39,7 +39,18 @@ namespace gtps
     // 1 / (l_i + l_j + l_k), [k][j][i], and 1 / (l_i + l_j), [j][i]
     static constexpr int lamInv3 = bandK + K * W;
     static constexpr int lamInv2 = lamInv3 + NI * NI * NI;
-    static constexpr int size    = lamInv2 + NI * NI;
+    // Even/odd split of the eigenbasis.  The interior pencil is invariant
+    // under the reflection i -> NI-1-i, so every eigenvector is even or odd.
+    // With MH = K-1 the middle index, the K even vectors are stored by their
+    // rows 0..MH (SE[i*K+e]) and the K-1 odd ones by rows 0..MH-1
+    // (SO[i*(K-1)+o]); eigen coordinates are renumbered even first, then
+    // odd, and lamInv3p holds 1/(l_a+l_b+l_c) in that numbering [c][b][a].
+    static constexpr int MH      = K - 1;
+    static constexpr int NE = K, NO = K - 1;
+    static constexpr int SE      = lamInv2 + NI * NI;
+    static constexpr int SO      = SE + (MH + 1) * NE;
+    static constexpr int lamInv3p = SO + (MH > 0 ? MH * NO : 0);
+    static constexpr int size    = lamInv3p + NI * NI * NI;
   };
 
   // Column range of row r of a 2-cell patch matrix (block sparsity: rows

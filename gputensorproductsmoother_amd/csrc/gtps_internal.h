@@ -202,6 +202,9 @@ namespace gtps
   // sum partials[0..nb) into scalars[slot] (deterministic, one block)
   void launch_finalize_sum(hipStream_t s, const double *partials, int nb,
                            double *scalars, int slot);
+  // the same, followed by the scalar op (cg::OP_*) in the same kernel
+  void launch_finalize_op(hipStream_t s, const double *partials, int nb,
+                          double *scalars, int slot, int op);
   void launch_set_boundary(const gtps_level *lvl, void *v, double value);
   void launch_lift_boundary(const gtps_level *lvl, const double *b, double *x);
   // dst_B = src_B on the Dirichlet DoFs, interior untouched
@@ -230,5 +233,7 @@ namespace gtps
     constexpr int OP_BETA  = 1;  // beta = rz / rz_old
     constexpr int OP_SHIFT = 3;  // rz_old = rz
     constexpr int OP_INIT  = 4;  // rz = 1 (first direction: p = 0 + z)
+    // finalize_op only: rz_old = rz; rz = sum; beta = rz / rz_old
+    constexpr int OP_SHIFT_BETA = 5;
   } // namespace cg
 } // namespace gtps

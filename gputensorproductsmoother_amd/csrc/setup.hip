@@ -332,6 +332,42 @@ namespace gtps
               tab[TB::lamInv3 + (c * NI + b2) * NI + a] =
                 1.0 / (b.lam[c] + b.lam[b2] + b.lam[a]);
           }
+      // even/odd classification of the eigenvectors (column j of S):
+      // even if S[i][j] = S[NI-1-i][j], odd if S[i][j] = -S[NI-1-i][j]
+      std::vector<int> even, odd;
+      for (int j = 0; j < NI; ++j)
+        {
+          double de = 0.0, dodd = 0.0, nrm = 0.0;
+          for (int i = 0; i < NI; ++i)
+            {
+              const double a = b.S[i * NI + j], m = b.S[(NI - 1 - i) * NI + j];
+              de   = std::max(de, std::fabs(a - m));
+              dodd = std::max(dodd, std::fabs(a + m));
+              nrm  = std::max(nrm, std::fabs(a));
+            }
+          if (de <= 1e-10 * nrm)
+            even.push_back(j);
+          else if (dodd <= 1e-10 * nrm)
+            odd.push_back(j);
+          else
+            throw Error(GTPS_EHIP, "patch eigenvector " + std::to_string(j) +
+                                     " is neither even nor odd");
+        }
+      GTPS_REQUIRE((int)even.size() == TB::NE && (int)odd.size() == TB::NO,
+                   "unexpected even/odd split of the patch eigenbasis");
+      std::vector<int> perm(even);
+      perm.insert(perm.end(), odd.begin(), odd.end());
+      for (int i = 0; i <= TB::MH; ++i)
+        for (int e = 0; e < TB::NE; ++e)
+          tab[TB::SE + i * TB::NE + e] = b.S[i * NI + even[e]];
+      for (int i = 0; i < TB::MH; ++i)
+        for (int o = 0; o < TB::NO; ++o)
+          tab[TB::SO + i * TB::NO + o] = b.S[i * NI + odd[o]];
+      for (int c = 0; c < NI; ++c)
+        for (int b2 = 0; b2 < NI; ++b2)
+          for (int a = 0; a < NI; ++a)
+            tab[TB::lamInv3p + (c * NI + b2) * NI + a] =
+              1.0 / (b.lam[perm[c]] + b.lam[perm[b2]] + b.lam[perm[a]]);
       return tab;
     }
 

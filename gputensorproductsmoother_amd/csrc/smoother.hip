@@ -879,6 +879,69 @@ namespace gtps
     // The interior b-rows are loaded at the start so that every global load
     // is in flight at once.  Same MODE semantics as the wave kernel.
     // Needs (2k+1)^2 <= 64.
+    // Fast-diagonalisation transforms in the even/odd eigenbasis (tables
+    // Tab::SE, Tab::SO): t = S^T v with t numbered even vectors first, and
+    // its inverse v = S t.  The reflection symmetry of the eigenvectors
+    // halves the products: K(K) + (K-1)^2 multiply-adds and 2(K-1)
+    // additions instead of (2K-1)^2.
+    template <int K, typename T>
+    __device__ __forceinline__ void
+    fd_fwd(const T *SE, const T *SO, const T (&v)[2 * K - 1], T (&t)[2 * K - 1])
+    {
+      constexpr int NI = 2 * K - 1, MH = K - 1, NE = K, NO = K - 1;
+      T e[MH + 1], o[MH > 0 ? MH : 1];
+#pragma unroll
+      for (int i = 0; i < MH; ++i)
+        {
+          e[i] = v[i] + v[NI - 1 - i];
+          o[i] = v[i] - v[NI - 1 - i];
+        }
+      e[MH] = v[MH];
+#pragma unroll
+      for (int q = 0; q < NE; ++q)
+        {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i <= MH; ++i)
+            s += SE[i * NE + q] * e[i];
+          t[q] = s;
+        }
+#pragma unroll
+      for (int q = 0; q < NO; ++q)
+        {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i < MH; ++i)
+            s += SO[i * NO + q] * o[i];
+          t[NE + q] = s;
+        }
+    }
+
+    template <int K, typename T>
+    __device__ __forceinline__ void
+    fd_inv(const T *SE, const T *SO, const T (&t)[2 * K - 1], T (&v)[2 * K - 1])
+    {
+      constexpr int NI = 2 * K - 1, MH = K - 1, NE = K, NO = K - 1;
+#pragma unroll
+      for (int i = 0; i < MH; ++i)
+        {
+          T pe = 0, po = 0;
+#pragma unroll
+          for (int q = 0; q < NE; ++q)
+            pe += SE[i * NE + q] * t[q];
+#pragma unroll
+          for (int q = 0; q < NO; ++q)
+            po += SO[i * NO + q] * t[NE + q];
+          v[i]          = pe + po;
+          v[NI - 1 - i] = pe - po;
+        }
+      T pm = 0;
+#pragma unroll
+      for (int q = 0; q < NE; ++q)
+        pm += SE[MH * NE + q] * t[q];
+      v[MH] = pm;
+    }
+
     template <int K>
     struct SG
     {
@@ -928,8 +991,9 @@ namespace gtps
       __syncthreads();
       const T *pM   = tab + TB::patchM;
       const T *pK   = tab + TB::patchK;
-      const T *S    = tab + TB::S;
-      const T *linv = tab + TB::lamInv3;
+      const T *SE   = tab + TB::SE;
+      const T *SO   = tab + TB::SO;
+      const T *linv = tab + TB::lamInv3p;
 
       // interior right-hand side (or residual) rows of this lane's stage-3
       // tasks, loaded up front
@@ -954,40 +1018,120 @@ namespace gtps
         }
       if (MODE == 0)
         {
-          // stage 1: z-pencils of the skeleton, z-rows of M and K
+          // stage 1: z-pencils of the skeleton, z-rows of M and K.  When the
+          // pencils of the group need more than one pass, the 8K face
+          // pencils (ix or iy on the box face, all NP values on the
+          // skeleton) run first and the NI^2 interior pencils (only their
+          // two end values are skeleton) after them with the two-column
+          // form of the contraction.
+          auto zrows = [&](int lp, int pc, const T (&v)[NP]) {
+            T *sz = sm + lp * PER;
 #pragma unroll
-          for (int pp = 0; pp < P1; ++pp)
+            for (int zi = 0; zi < NI; ++zi)
+              {
+                const int row = zi + 1;
+                T         m = 0, kk = 0;
+#pragma unroll
+                for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                  {
+                    m += pM[row * NP + c] * v[c];
+                    kk += pK[row * NP + c] * v[c];
+                  }
+                sz[zi * NP * NP + pc]        = m;
+                sz[(NI + zi) * NP * NP + pc] = kk;
+              }
+          };
+          if constexpr (NPW * NP * NP <= 64)
             {
-              const int t = lane + 64 * pp;
-              if (t >= NPW * NP * NP)
-                continue;
-              const int       lp = t / (NP * NP), pc = t - lp * (NP * NP);
-              const long long f  = sfirst[lp];
-              if (f < 0)
-                continue;
-              const int  iy = pc / NP, ix = pc - iy * NP;
-              const bool face = ix == 0 || ix == NP - 1 || iy == 0 || iy == NP - 1;
-              T          v[NP];
-#pragma unroll
-              for (int iz = 0; iz < NP; ++iz)
+              const int t = lane;
+              if (t < NPW * NP * NP)
                 {
-                  const long long g = f + ix + (long long)iy * N1 + iz * P;
-                  v[iz] = (face || iz == 0 || iz == NP - 1) ? u[g] : T(0);
-                }
-              T *sz = sm + lp * PER;
-#pragma unroll
-              for (int zi = 0; zi < NI; ++zi)
-                {
-                  const int row = zi + 1;
-                  T         m = 0, kk = 0;
-#pragma unroll
-                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                  const int       lp = t / (NP * NP), pc = t - lp * (NP * NP);
+                  const long long f  = sfirst[lp];
+                  if (f >= 0)
                     {
-                      m += pM[row * NP + c] * v[c];
-                      kk += pK[row * NP + c] * v[c];
+                      const int  iy = pc / NP, ix = pc - iy * NP;
+                      const bool face = ix == 0 || ix == NP - 1 || iy == 0 || iy == NP - 1;
+                      T          v[NP];
+#pragma unroll
+                      for (int iz = 0; iz < NP; ++iz)
+                        {
+                          const long long g = f + ix + (long long)iy * N1 + iz * P;
+                          v[iz] = (face || iz == 0 || iz == NP - 1) ? u[g] : T(0);
+                        }
+                      zrows(lp, pc, v);
                     }
-                  sz[zi * NP * NP + pc]        = m;
-                  sz[(NI + zi) * NP * NP + pc] = kk;
+                }
+            }
+          else
+            {
+              constexpr int NF = NP * NP - NI * NI;
+#pragma unroll
+              for (int pp = 0; pp < (NPW * NF + 63) / 64; ++pp)
+                {
+                  const int t = lane + 64 * pp;
+                  if (t >= NPW * NF)
+                    continue;
+                  const int       lp = t / NF, fi = t - lp * NF;
+                  const long long f  = sfirst[lp];
+                  if (f < 0)
+                    continue;
+                  int ix, iy;
+                  if (fi < NP)
+                    {
+                      ix = fi;
+                      iy = 0;
+                    }
+                  else if (fi < 2 * NP)
+                    {
+                      ix = fi - NP;
+                      iy = NP - 1;
+                    }
+                  else
+                    {
+                      const int g2 = fi - 2 * NP;
+                      iy           = 1 + (g2 >> 1);
+                      ix           = (g2 & 1) ? NP - 1 : 0;
+                    }
+                  T v[NP];
+#pragma unroll
+                  for (int iz = 0; iz < NP; ++iz)
+                    v[iz] = u[f + ix + (long long)iy * N1 + iz * P];
+                  zrows(lp, iy * NP + ix, v);
+                }
+#pragma unroll
+              for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+                {
+                  const int t = lane + 64 * pp;
+                  if (t >= NPW * NI * NI)
+                    continue;
+                  const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
+                  const long long f  = sfirst[lp];
+                  if (f < 0)
+                    continue;
+                  const int       iy = 1 + r / NI, ix = 1 + r % NI;
+                  const long long g  = f + ix + (long long)iy * N1;
+                  const T         v0 = u[g], v1 = u[g + (NP - 1) * P];
+                  T              *sz = sm + lp * PER;
+                  const int       pc = iy * NP + ix;
+#pragma unroll
+                  for (int zi = 0; zi < NI; ++zi)
+                    {
+                      const int row = zi + 1;
+                      T         m = 0, kk = 0;
+                      if (prow_lo<K>(row) == 0)
+                        {
+                          m += pM[row * NP] * v0;
+                          kk += pK[row * NP] * v0;
+                        }
+                      if (prow_hi<K>(row) == NP - 1)
+                        {
+                          m += pM[row * NP + NP - 1] * v1;
+                          kk += pK[row * NP + NP - 1] * v1;
+                        }
+                      sz[zi * NP * NP + pc]        = m;
+                      sz[(NI + zi) * NP * NP + pc] = kk;
+                    }
                 }
             }
           __syncthreads();
@@ -1084,15 +1228,11 @@ namespace gtps
                 res[xi] = rb[pp][xi];
             }
           T *sr = sm + lp * PER;
+          T  tt[NI];
+          fd_fwd<K>(SE, SO, res, tt);
 #pragma unroll
           for (int j = 0; j < NI; ++j)
-            {
-              T s = 0;
-#pragma unroll
-              for (int i = 0; i < NI; ++i)
-                s += S[i * NI + j] * res[i];
-              sr[r * NI + j] = s;
-            }
+            sr[r * NI + j] = tt[j];
         }
       __syncthreads();
       // stage 4: S^T along y
@@ -1107,19 +1247,14 @@ namespace gtps
             continue;
           const int zi = r / NI, xi = r - zi * NI;
           T        *base = sm + lp * PER + zi * NI * NI + xi;
-          T         v[NI];
+          T         v[NI], tt[NI];
 #pragma unroll
           for (int i = 0; i < NI; ++i)
             v[i] = base[i * NI];
+          fd_fwd<K>(SE, SO, v, tt);
 #pragma unroll
           for (int j = 0; j < NI; ++j)
-            {
-              T s = 0;
-#pragma unroll
-              for (int i = 0; i < NI; ++i)
-                s += S[i * NI + j] * v[i];
-              base[j * NI] = s;
-            }
+            base[j * NI] = tt[j];
         }
       __syncthreads();
       // stage 5: S^T along z, eigenvalue scaling, S along z
@@ -1138,24 +1273,14 @@ namespace gtps
 #pragma unroll
           for (int i = 0; i < NI; ++i)
             v[i] = base[i * NI * NI];
+          fd_fwd<K>(SE, SO, v, tt);
 #pragma unroll
           for (int j = 0; j < NI; ++j)
-            {
-              T s = 0;
-#pragma unroll
-              for (int i = 0; i < NI; ++i)
-                s += S[i * NI + j] * v[i];
-              tt[j] = s * (linv[(j * NI + yi) * NI + xi] * inv_scale);
-            }
+            tt[j] *= linv[(j * NI + yi) * NI + xi] * inv_scale;
+          fd_inv<K>(SE, SO, tt, v);
 #pragma unroll
           for (int i = 0; i < NI; ++i)
-            {
-              T s = 0;
-#pragma unroll
-              for (int j = 0; j < NI; ++j)
-                s += S[i * NI + j] * tt[j];
-              base[i * NI * NI] = s;
-            }
+            base[i * NI * NI] = v[i];
         }
       __syncthreads();
       // stage 6: S along y
@@ -1170,19 +1295,14 @@ namespace gtps
             continue;
           const int zi = r / NI, xi = r - zi * NI;
           T        *base = sm + lp * PER + zi * NI * NI + xi;
-          T         v[NI];
+          T         v[NI], o[NI];
 #pragma unroll
           for (int j = 0; j < NI; ++j)
             v[j] = base[j * NI];
+          fd_inv<K>(SE, SO, v, o);
 #pragma unroll
           for (int i = 0; i < NI; ++i)
-            {
-              T s = 0;
-#pragma unroll
-              for (int j = 0; j < NI; ++j)
-                s += S[i * NI + j] * v[j];
-              base[i * NI] = s;
-            }
+            base[i * NI] = o[i];
         }
       __syncthreads();
       // stage 7: S along x, store
@@ -1197,22 +1317,19 @@ namespace gtps
           if (f < 0)
             continue;
           const int zi = r / NI, yi = r - zi * NI;
-          T         v[NI];
+          T         v[NI], o[NI];
 #pragma unroll
           for (int j = 0; j < NI; ++j)
             v[j] = sm[lp * PER + r * NI + j];
+          fd_inv<K>(SE, SO, v, o);
           const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
 #pragma unroll
           for (int i = 0; i < NI; ++i)
             {
-              T s = 0;
-#pragma unroll
-              for (int j = 0; j < NI; ++j)
-                s += S[i * NI + j] * v[j];
               if (MODE == 1)
-                u[g0 + i] += relax * s;
+                u[g0 + i] += relax * o[i];
               else
-                u[g0 + i] = relax * s;
+                u[g0 + i] = relax * o[i];
             }
         }
     }

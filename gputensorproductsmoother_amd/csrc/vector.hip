@@ -87,6 +87,32 @@ namespace gtps
         scalars[slot] = s;
     }
 
+    // sum of the partials into scalars[slot], then one scalar op; op < 0:
+    // none.  OP_SHIFT_BETA reads the old rz before it is replaced.
+    __global__ void __launch_bounds__(VNT)
+    finalize_op_kernel(const double *__restrict__ partials, int nb,
+                       double *__restrict__ sc, int slot, int op)
+    {
+      __shared__ double red[VNT / 64];
+      double            s = 0.0;
+      for (int i = threadIdx.x; i < nb; i += VNT)
+        s += partials[i];
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x != 0)
+        return;
+      if (op == cg::OP_SHIFT_BETA)
+        {
+          const double old = sc[cg::RZ];
+          sc[cg::RZ_OLD]   = old;
+          sc[cg::RZ]       = s;
+          sc[cg::BETA]     = s / old;
+          return;
+        }
+      sc[slot] = s;
+      if (op == cg::OP_ALPHA)
+        sc[cg::ALPHA] = sc[cg::RZ] / s;
+    }
+
     __global__ void
     cg_scalar_kernel(double *sc, int op)
     {
@@ -275,6 +301,15 @@ namespace gtps
   {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(VNT), 0, s, partials, nb,
                        scalars, slot);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_finalize_op(hipStream_t s, const double *partials, int nb,
+                     double *scalars, int slot, int op)
+  {
+    hipLaunchKernelGGL(finalize_op_kernel, dim3(1), dim3(VNT), 0, s, partials,
+                       nb, scalars, slot, op);
     GTPS_LAUNCH_CHECK();
   }
 

@@ -200,10 +200,18 @@ def test_autograd_adjoint_vs_finite_differences(core, ctx, prec):
     assert rel_err(gc, ref_gc) < 1e-8
     assert rel_err(gb[~bm], lam[~bm]) < 1e-8
 
+    mg64.set_coefficient(c0)
+    u0, _, _, _ = mg64.solve_cg(_dev(b0), rtol=1e-14, maxit=400, raise_on_failure=False)
+    u0 = u0.clone()
+
     def u64(bv, cv):
+        # u(c', b') = u0 + A(c')^{-1} (b' - A(c') u0): the correction is O(h),
+        # so the difference of two perturbed solutions keeps ~12 digits
         mg64.set_coefficient(cv)
-        uu, _, _, _ = mg64.solve_cg(bv, rtol=1e-14, maxit=400, raise_on_failure=False)
-        return uu.clone()
+        r = torch.empty_like(u0)
+        mg64.do_matvec(r, u0)
+        d, _, _, _ = mg64.solve_cg(bv - r, rtol=1e-12, maxit=400, raise_on_failure=False)
+        return u0 + d
 
     def central(up, um):
         # J(+) - J(-) = sum (u+ - u-) ((u+ + u-)/2 - u_d) over the interior,
