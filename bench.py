@@ -399,6 +399,20 @@ def kernel_bench(args):
                     timed(f"smoother_l{lv}_{'fp32' if s == 4 else 'fp64'}_{path}",
                           lambda: sm.step(u, b), s * sum(vals_l), len(vals_l))
                     core.set_tuning("small_colour_patches", prev)
+                yv = torch.empty_like(u)
+                for path, lim in (("box", 1 << 20), ("march", 0)):
+                    prev = core.set_tuning("box_vmult_max_n", lim)
+                    timed(f"residual_l{lv}_{'fp32' if s == 4 else 'fp64'}_{path}",
+                          lambda: op.residual(yv, b, u), 3 * s * op.n_dofs, 1)
+                    core.set_tuning("box_vmult_max_n", prev)
+                if lv >= 2:
+                    trl = core.MGTransferCUDA(ctx, dim, k, 1, lv, dt)
+                    ncl = (k * (1 << (lv - 1)) + 1) ** dim
+                    cl = torch.rand(ncl, generator=g, device="cuda:0", dtype=dt)
+                    timed(f"restrict_l{lv}_{'fp32' if s == 4 else 'fp64'}",
+                          lambda: trl.restrict_and_add(lv, cl, u), s * (op.n_dofs + 2 * ncl), 1)
+                    timed(f"prolongate_l{lv}_{'fp32' if s == 4 else 'fp64'}",
+                          lambda: trl.prolongate_and_add(lv, u, cl), s * (ncl + 2 * op.n_dofs), 1)
     print(json.dumps({"kernels": out, "workload": f"{dim}D Q{k} level {level}",
                       "repeat": R}), flush=True)
 

@@ -1164,6 +1164,18 @@ extern "C"
           prev                         = tuning().patches_per_wave;
           tuning().patches_per_wave    = (int)value;
         }
+      else if (k == "box_vmult_max_n")
+        {
+          GTPS_REQUIRE(value >= 0, "value must be >= 0");
+          prev                       = tuning().box_vmult_max_n;
+          tuning().box_vmult_max_n   = value;
+        }
+      else if (k == "row_staged")
+        {
+          GTPS_REQUIRE(value == 0 || value == 1, "row_staged must be 0 or 1");
+          prev                  = tuning().row_staged;
+          tuning().row_staged   = (int)value;
+        }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });
@@ -1183,6 +1195,10 @@ extern "C"
         v = tuning().group_min_patches;
       else if (k == "patches_per_wave")
         v = tuning().patches_per_wave;
+      else if (k == "box_vmult_max_n")
+        v = tuning().box_vmult_max_n;
+      else if (k == "row_staged")
+        v = tuning().row_staged;
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });

@@ -143,6 +143,15 @@ namespace gtps
     // pack patches_per_wave (1, 2 or 4) patches into each wavefront
     long long group_min_patches = 4096;
     int       patches_per_wave  = 2;
+    // 3D operator (k <= 4): levels with at most this many cells per
+    // direction use the one-round box kernel instead of z-marching
+    long long box_vmult_max_n = 16;
+    // patch-group smoother with row-staged global accesses (1) or the
+    // lane-pencil form (0).  Measured on MI355X at 64^3 Q3 FP32: 29.7 us
+    // (row-staged) against 24.8 us (lane form) per colour: the LDS staging
+    // and its extra barrier cost more than the fewer cache-line accesses
+    // save, so the lane form is the default.
+    int       row_staged = 0;
   };
   Tuning &tuning();
 

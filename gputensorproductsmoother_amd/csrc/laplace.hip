@@ -147,6 +147,30 @@ namespace gtps
             {
               const int       gz   = K * cz + p;
               const long long zoff = gz * P;
+              // right-hand side of this thread's pass-3 outputs, loaded now
+              // so that the loads overlap passes 1 and 2
+              // (not for k = 7 in FP64, where the registers would spill)
+              constexpr int  NT3 = (TCX * TY + NT - 1) / NT;
+              constexpr bool PFB = RESID && (K < 7 || sizeof(T) == 4);
+              T              rbv[NT3][PFB ? K : 1];
+              if constexpr (PFB)
+                {
+#pragma unroll
+                  for (int j3 = 0; j3 < NT3; ++j3)
+                    {
+                      const int task = tid + j3 * NT;
+                      const int cxl = task % TCX, ty = task / TCX;
+                      const int gyy = K * cy0 + ty;
+#pragma unroll
+                      for (int q = 0; q < K; ++q)
+                        {
+                          const int gxx = K * (cx0 + cxl) + q;
+                          const bool in = task < TCX * TY && gz != 0 && gxx >= 1 &&
+                                          gxx < N1 - 1 && gyy >= 1 && gyy < N1 - 1;
+                          rbv[j3][q] = in ? bv[gxx + (long long)gyy * N1 + zoff] : T(0);
+                        }
+                    }
+                }
               // pass 1: z-rows of M and K on every column of tile + halo
 #pragma unroll
               for (int c = 0; c < NCOL; ++c)
@@ -207,8 +231,12 @@ namespace gtps
                     }
                   __syncthreads();
                   // pass 3: x-rows; y = h^(d-2) (Kx C1 + Mx C2) on interior rows
-                  for (int task = tid; task < TCX * TY; task += NT)
+#pragma unroll
+                  for (int j3 = 0; j3 < NT3; ++j3)
                     {
+                      const int task = tid + j3 * NT;
+                      if (task >= TCX * TY)
+                        continue;
                       const int cxl = task % TCX, ty = task / TCX;
                       const int gy  = K * cy0 + ty;
                       T         c1[W], c2[W];
@@ -234,7 +262,11 @@ namespace gtps
                                 }
                               v *= scale;
                               const long long g  = gx + (long long)gy * N1 + zoff;
-                              const T         yv = RESID ? bv[g] - v : v;
+                              T               yv = v;
+                              if constexpr (PFB)
+                                yv = rbv[j3][q] - v;
+                              else if constexpr (RESID)
+                                yv = bv[g] - v;
                               y[g]               = yv;
                               if (DOT)
                                 dot += (double)sX0[(ty + K) * WX + K * cxl + q + K] *
@@ -300,6 +332,185 @@ namespace gtps
           const double s = block_sum<NT>(dot, sred);
           if (tid == 0)
             partial[linear_block_id()] = s;
+        }
+    }
+
+    // Box kernel for small levels: a work-group owns a B^3 block of DoFs,
+    // loads it with a halo of k in one round of loads (unrolled, all in
+    // flight together) and applies the z-, y- and x-rows of the assembled
+    // band matrices through LDS.  Latency-bound levels (a few thousand to a
+    // few hundred thousand DoFs) finish in one memory round trip instead of
+    // the serial plane chain of the z-marching kernel; the halo re-reads
+    // ((B+2k)^3 / B^3) come from L2.
+    // B is a multiple of k, so block origins sit on cell boundaries and the
+    // node position inside a cell (which selects the band coefficients) is
+    // a compile-time function of the unrolled output index.
+    template <int K>
+    struct VB
+    {
+      static constexpr int B = K <= 3 ? K * ((8 + K - 1) / K) : 4, NT = 256;
+      static constexpr int WB = B + 2 * K;
+    };
+
+    template <int K, typename T, bool RESID, bool DOT>
+    __global__ void __launch_bounds__(256)
+    laplace3d_box_kernel(const T *__restrict__ x, T *__restrict__ y,
+                         const T *__restrict__ bv, const T *__restrict__ tab,
+                         const int n, const T scale, double *__restrict__ partial)
+    {
+      using TB         = Tab<K>;
+      constexpr int B  = VB<K>::B, NT = VB<K>::NT, WB = VB<K>::WB;
+      constexpr int W  = 2 * K + 1;
+      // box (later reused for C1, C2), then A = Mz x and Bz = Kz x
+      constexpr int NBOX = WB * WB * WB;
+      constexpr int NCC  = WB * B * B;
+      constexpr int NBUF = NBOX > 2 * NCC ? NBOX : 2 * NCC;
+      __shared__ T      sBox[NBUF], sA[WB * WB * B], sB[WB * WB * B];
+      __shared__ double sred[NT / 64];
+      const T  *bM  = tab + TB::bandM;
+      const T  *bK  = tab + TB::bandK;
+      const int N1  = K * n + 1;
+      const long long P = (long long)N1 * N1;
+      const int tid = threadIdx.x;
+      const int nbx = (N1 + B - 1) / B;
+      const int L   = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int ox0 = (L % nbx) * B, oy0 = ((L / nbx) % nbx) * B,
+                oz0 = (L / (nbx * nbx)) * B;
+      {
+        constexpr int IT = (NBOX + NT - 1) / NT;
+        T             v[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i  = tid + j * NT;
+            const int bx = i % WB, by = (i / WB) % WB, bz = i / (WB * WB);
+            const int gx = ox0 - K + bx, gy = oy0 - K + by, gz = oz0 - K + bz;
+            const bool in = i < NBOX && gx >= 0 && gx < N1 && gy >= 0 &&
+                            gy < N1 && gz >= 0 && gz < N1;
+            v[j] = in ? x[gx + (long long)gy * N1 + gz * P] : T(0);
+          }
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i = tid + j * NT;
+            if (i < NBOX)
+              sBox[i] = v[j];
+          }
+      }
+      // the x-pass outputs of this thread (row (ly, lz) = tid < B*B): their
+      // input values and right-hand side, loaded with the box
+      constexpr int NRB = RESID ? B : 1;
+      T             xo[B], bo[NRB];
+      const int     rly = tid % B, rlz = tid / B;
+      const int     rgy = oy0 + rly, rgz = oz0 + rlz;
+      const bool    rrow = tid < B * B && rgy < N1 && rgz < N1;
+#pragma unroll
+      for (int lx = 0; lx < B; ++lx)
+        {
+          const int       gx = ox0 + lx;
+          const long long g  = gx + (long long)rgy * N1 + rgz * P;
+          xo[lx]             = (rrow && gx < N1) ? x[g] : T(0);
+          if constexpr (RESID)
+            bo[lx] = (rrow && gx < N1) ? bv[g] : T(0);
+        }
+      static_assert(B * B <= NT, "one x-pass row per thread");
+      __syncthreads();
+      // z-rows on every (x, y) column of the box, B output planes
+      for (int t = tid; t < WB * WB; t += NT)
+        {
+          T c[WB];
+#pragma unroll
+          for (int z = 0; z < WB; ++z)
+            c[z] = sBox[z * WB * WB + t];
+#pragma unroll
+          for (int lz = 0; lz < B; ++lz)
+            {
+              const int p = lz % K;
+              T         a = 0, b = 0;
+#pragma unroll
+              for (int o = -K; o <= K; ++o)
+                {
+                  const T m = bM[p * W + o + K], kk = bK[p * W + o + K];
+                  a += m * c[lz + K + o];
+                  b += kk * c[lz + K + o];
+                }
+              sA[lz * WB * WB + t] = a;
+              sB[lz * WB * WB + t] = b;
+            }
+        }
+      __syncthreads();
+      // y-rows on every (x, z) column: C1 = My A, C2 = Ky A + My Bz
+      T *sC1 = sBox, *sC2 = sBox + NCC;
+      for (int t = tid; t < WB * B; t += NT)
+        {
+          const int bx = t % WB, lz = t / WB;
+          T         a[WB], b[WB];
+#pragma unroll
+          for (int j = 0; j < WB; ++j)
+            {
+              a[j] = sA[(lz * WB + j) * WB + bx];
+              b[j] = sB[(lz * WB + j) * WB + bx];
+            }
+#pragma unroll
+          for (int ly = 0; ly < B; ++ly)
+            {
+              const int p = ly % K;
+              T         c1 = 0, c2 = 0;
+#pragma unroll
+              for (int o = -K; o <= K; ++o)
+                {
+                  const T m = bM[p * W + o + K], kk = bK[p * W + o + K];
+                  c1 += m * a[ly + K + o];
+                  c2 += kk * a[ly + K + o] + m * b[ly + K + o];
+                }
+              sC1[(lz * B + ly) * WB + bx] = c1;
+              sC2[(lz * B + ly) * WB + bx] = c2;
+            }
+        }
+      __syncthreads();
+      // x-rows per (y, z) row of the block, then the Dirichlet rows
+      double dot = 0.0;
+      if (rrow)
+        {
+          const int t  = tid;
+          const int gy = rgy, gz = rgz;
+          T c1[WB], c2[WB];
+#pragma unroll
+          for (int j = 0; j < WB; ++j)
+            {
+              c1[j] = sC1[t * WB + j];
+              c2[j] = sC2[t * WB + j];
+            }
+#pragma unroll
+          for (int lx = 0; lx < B; ++lx)
+            {
+              const int gx = ox0 + lx;
+              if (gx >= N1)
+                continue;
+              const int p = lx % K;
+              T         v = 0;
+#pragma unroll
+              for (int o = -K; o <= K; ++o)
+                v += bK[p * W + o + K] * c1[lx + K + o] +
+                     bM[p * W + o + K] * c2[lx + K + o];
+              const bool bd = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1 ||
+                              gz == 0 || gz == N1 - 1;
+              const long long g  = gx + (long long)gy * N1 + gz * P;
+              const T         xv = xo[lx];
+              v                  = bd ? xv : scale * v;
+              T yv               = v;
+              if constexpr (RESID)
+                yv = bo[lx] - v;
+              y[g] = yv;
+              if (DOT)
+                dot += (double)xv * (double)yv;
+            }
+        }
+      if (DOT)
+        {
+          const double s = block_sum<NT>(dot, sred);
+          if (tid == 0)
+            partial[blockIdx.x] = s;
         }
     }
 
@@ -429,6 +640,35 @@ namespace gtps
       const T   *xp    = reinterpret_cast<const T *>(x);
       T         *yp    = reinterpret_cast<T *>(y);
       const T   *bp    = reinterpret_cast<const T *>(b);
+      if constexpr (K <= 4)
+        if (lvl->dim == 3 && n <= tuning().box_vmult_max_n)
+          {
+            using C        = VB<K>;
+            const int nbx  = (lvl->N1 + C::B - 1) / C::B;
+            const int nb   = nbx * nbx * nbx;
+            if (fused_dot)
+              GTPS_REQUIRE(nb <= lvl->ctx->ws.capacity, "too many blocks for partials");
+#define GTPS_LB(R, D)                                                         \
+  hipLaunchKernelGGL((laplace3d_box_kernel<K, T, R, D>), dim3(nb), dim3(C::NT), \
+                     0, s, xp, yp, bp, tab, n, scale, part)
+            if (b)
+              {
+                if (fused_dot)
+                  GTPS_LB(true, true);
+                else
+                  GTPS_LB(true, false);
+              }
+            else
+              {
+                if (fused_dot)
+                  GTPS_LB(false, true);
+                else
+                  GTPS_LB(false, false);
+              }
+#undef GTPS_LB
+            GTPS_LAUNCH_CHECK();
+            return nb;
+          }
       if (lvl->dim == 3)
         {
           using C       = VT3<K>;

@@ -1334,6 +1334,455 @@ namespace gtps
         }
     }
 
+    // Row-staged patch-group kernel: the same stages as the patch-group
+    // kernel, but the NPW patches of a work-group are x-consecutive patches
+    // of one patch row, and every global access goes through contiguous
+    // x-runs of their union box: the skeleton box (width 2k NPW + 1), the
+    // interior right-hand side rows and the interior results are moved by
+    // lanes over (row, x), staged in LDS.  A row of the union box costs one
+    // or two cache-line accesses instead of one per lane-pencil.
+    template <int K, int NPW>
+    struct SR
+    {
+      static constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
+      static constexpr int UX = 2 * K * NPW + 1;   // union box width
+      static constexpr int BX = UX - 2;            // interior x-run width
+      static constexpr int ZS = 2 * NI * NP * NP;  // z-rows of M and K / FD
+      static constexpr int YS = 2 * NI * NI * NP;  // y-rows
+      static constexpr int US = NP * NP * UX;      // union skeleton box
+      // the box is dead after stage 1 and lives in the y-row region
+      static constexpr int YU = NPW * YS > US ? NPW * YS : US;
+      static constexpr int BS = NI * NI * BX;      // interior b rows
+    };
+
+    template <int K, typename T, int MODE, int NPW>
+    __global__ void __launch_bounds__(64)
+    patch3d_row_kernel(T *__restrict__ u, const T *__restrict__ src,
+                       const T *__restrict__ tab, const int n,
+                       const int colour, const T scale, const T inv_scale,
+                       const T relax)
+    {
+      using TB         = Tab<K>;
+      using C          = SR<K, NPW>;
+      constexpr int NP = C::NP, NI = C::NI, UX = C::UX, BX = C::BX;
+      constexpr int ZS = C::ZS, YS = C::YS;
+      __shared__ T sZ[NPW * ZS + 1], sYU[C::YU + 1], sB[C::BS + 1];
+      const int       N1   = K * n + 1;
+      const long long P    = (long long)N1 * N1;
+      const int       lane = threadIdx.x;
+      const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
+      const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py);
+      const int nbx = (npx + NPW - 1) / NPW;
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int bx = blk % nbx, qy = (blk / nbx) % npy, qz = blk / (nbx * npy);
+      const int qx0 = bx * NPW;
+      const int nph = min(NPW, npx - qx0);         // patches in this group
+      const int ux  = 2 * K * nph + 1, bxw = ux - 2;
+      const long long f0 = (long long)K * (2 * qx0 + px) +
+                           (long long)K * (2 * qy + py) * N1 +
+                           (long long)K * (2 * qz + pz) * P;
+      const T *pM   = tab + TB::patchM;
+      const T *pK   = tab + TB::patchK;
+      const T *SE   = tab + TB::SE;
+      const T *SO   = tab + TB::SO;
+      const T *linv = tab + TB::lamInv3p;
+      // position x (0..ux-1) of the union box lies inside some patch
+      // interior iff it is not a multiple of 2k
+      auto x_inner = [](int x) { return x % (2 * K) != 0; };
+
+      // phase L: interior b rows and (multiplicative) the skeleton box.
+      // Lanes form RPI row segments of XS lanes; a lane keeps its x and
+      // row-in-block for the whole phase, so every address is a per-lane
+      // base plus a compile-time multiple of the row strides.
+      constexpr int XS  = UX <= 8 ? 8 : (UX <= 16 ? 16 : (UX <= 32 ? 32 : 64));
+      constexpr int RPI = 64 / XS;
+      const int     lx  = lane % XS, lr = lane / XS;
+      const bool    xin = x_inner(lx);
+      {
+        constexpr int NBR = (NI + RPI - 1) / RPI;  // y-blocks per z
+        T             vb[NI][NBR];
+        const bool    bok = lx >= 1 && lx - 1 < bxw;
+#pragma unroll
+        for (int z = 1; z <= NI; ++z)
+#pragma unroll
+          for (int yb = 0; yb < NBR; ++yb)
+            {
+              const int y = 1 + yb * RPI + lr;
+              vb[z - 1][yb] = (bok && y <= NI)
+                                ? src[f0 + lx + (long long)y * N1 + z * P]
+                                : T(0);
+            }
+#pragma unroll
+        for (int z = 1; z <= NI; ++z)
+#pragma unroll
+          for (int yb = 0; yb < NBR; ++yb)
+            {
+              const int y = 1 + yb * RPI + lr;
+              if (bok && y <= NI)
+                sB[((z - 1) * NI + (y - 1)) * BX + lx - 1] = vb[z - 1][yb];
+            }
+      }
+      if (MODE == 0)
+        {
+          constexpr int NBR = (NP + RPI - 1) / RPI;
+          T             vu[NP][NBR];
+          const bool    uok = lx < ux;
+#pragma unroll
+          for (int z = 0; z < NP; ++z)
+#pragma unroll
+            for (int yb = 0; yb < NBR; ++yb)
+              {
+                const int  y     = yb * RPI + lr;
+                const bool inner = z > 0 && z < NP - 1 && y > 0 && y < NP - 1 && xin;
+                vu[z][yb] = (uok && y < NP && !inner)
+                              ? u[f0 + lx + (long long)y * N1 + z * P]
+                              : T(0);
+              }
+#pragma unroll
+          for (int z = 0; z < NP; ++z)
+#pragma unroll
+            for (int yb = 0; yb < NBR; ++yb)
+              {
+                const int y = yb * RPI + lr;
+                if (lx < UX && y < NP)
+                  sYU[(z * NP + y) * UX + lx] = vu[z][yb];
+              }
+        }
+      __syncthreads();
+      if (MODE == 0)
+        {
+          // stage 1: z-pencils of each patch box from the union box; when
+          // one pass does not hold them all, the 8k face pencils first, then
+          // the interior pencils in two-value form (their only skeleton
+          // values are the two ends)
+          auto zrows = [&](int lp, int pc, const T (&v)[NP]) {
+            T *sz = sZ + lp * ZS;
+#pragma unroll
+            for (int zi = 0; zi < NI; ++zi)
+              {
+                const int row = zi + 1;
+                T         m = 0, kk = 0;
+#pragma unroll
+                for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                  {
+                    m += pM[row * NP + c] * v[c];
+                    kk += pK[row * NP + c] * v[c];
+                  }
+                sz[zi * NP * NP + pc]        = m;
+                sz[(NI + zi) * NP * NP + pc] = kk;
+              }
+          };
+          if constexpr (NPW * NP * NP <= 64)
+            {
+              const int t = lane;
+              const int lp = t / (NP * NP), pc = t - lp * (NP * NP);
+              if (t < NPW * NP * NP && lp < nph)
+                {
+                  const int iy = pc / NP, ix = pc - iy * NP;
+                  const T  *col = sYU + iy * UX + 2 * K * lp + ix;
+                  T         v[NP];
+#pragma unroll
+                  for (int iz = 0; iz < NP; ++iz)
+                    v[iz] = col[iz * NP * UX];
+                  zrows(lp, pc, v);
+                }
+            }
+          else
+            {
+              constexpr int NF = NP * NP - NI * NI;
+#pragma unroll
+              for (int pp = 0; pp < (NPW * NF + 63) / 64; ++pp)
+                {
+                  const int t = lane + 64 * pp;
+                  const int lp = t / NF, fi = t - lp * NF;
+                  if (t >= NPW * NF || lp >= nph)
+                    continue;
+                  int ix, iy;
+                  if (fi < NP)
+                    {
+                      ix = fi;
+                      iy = 0;
+                    }
+                  else if (fi < 2 * NP)
+                    {
+                      ix = fi - NP;
+                      iy = NP - 1;
+                    }
+                  else
+                    {
+                      const int g2 = fi - 2 * NP;
+                      iy           = 1 + (g2 >> 1);
+                      ix           = (g2 & 1) ? NP - 1 : 0;
+                    }
+                  const T *col = sYU + iy * UX + 2 * K * lp + ix;
+                  T        v[NP];
+#pragma unroll
+                  for (int iz = 0; iz < NP; ++iz)
+                    v[iz] = col[iz * NP * UX];
+                  zrows(lp, iy * NP + ix, v);
+                }
+#pragma unroll
+              for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+                {
+                  const int t = lane + 64 * pp;
+                  const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+                  if (t >= NPW * NI * NI || lp >= nph)
+                    continue;
+                  const int iy = 1 + r / NI, ix = 1 + r % NI;
+                  const T  *col = sYU + iy * UX + 2 * K * lp + ix;
+                  const T   v0 = col[0], v1 = col[(NP - 1) * NP * UX];
+                  T        *sz = sZ + lp * ZS;
+                  const int pc = iy * NP + ix;
+#pragma unroll
+                  for (int zi = 0; zi < NI; ++zi)
+                    {
+                      const int row = zi + 1;
+                      T         m = 0, kk = 0;
+                      if (prow_lo<K>(row) == 0)
+                        {
+                          m += pM[row * NP] * v0;
+                          kk += pK[row * NP] * v0;
+                        }
+                      if (prow_hi<K>(row) == NP - 1)
+                        {
+                          m += pM[row * NP + NP - 1] * v1;
+                          kk += pK[row * NP + NP - 1] * v1;
+                        }
+                      sz[zi * NP * NP + pc]        = m;
+                      sz[(NI + zi) * NP * NP + pc] = kk;
+                    }
+                }
+            }
+          __syncthreads();
+          // stage 2: y-rows (the box region is dead now)
+#pragma unroll
+          for (int pp = 0; pp < (NPW * NI * NP + 63) / 64; ++pp)
+            {
+              const int t = lane + 64 * pp;
+              if (t >= NPW * NI * NP)
+                continue;
+              const int lp = t / (NI * NP), r = t - lp * (NI * NP);
+              if (lp >= nph)
+                continue;
+              const int zi = r / NP, ix = r - zi * NP;
+              const T  *sz = sZ + lp * ZS;
+              T        *sy = sYU + lp * YS;
+              T         a[NP], b[NP];
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                {
+                  a[iy] = sz[(zi * NP + iy) * NP + ix];
+                  b[iy] = sz[((NI + zi) * NP + iy) * NP + ix];
+                }
+#pragma unroll
+              for (int yi = 0; yi < NI; ++yi)
+                {
+                  const int row = yi + 1;
+                  T         c1 = 0, c2 = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      c1 += pM[row * NP + c] * a[c];
+                      c2 += pK[row * NP + c] * a[c];
+                      c2 += pM[row * NP + c] * b[c];
+                    }
+                  sy[(zi * NI + yi) * NP + ix]        = c1;
+                  sy[((NI + zi) * NI + yi) * NP + ix] = c2;
+                }
+            }
+          __syncthreads();
+        }
+      // stage 3: x-rows, residual, S^T along x -> FD buffer (z-row region)
+#pragma unroll
+      for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (lp >= nph)
+            continue;
+          const T *brow = sB + r * BX + 2 * K * lp;
+          T        res[NI];
+          if (MODE == 0)
+            {
+              const T *sy = sYU + lp * YS;
+              T        c1[NP], c2[NP];
+#pragma unroll
+              for (int ix = 0; ix < NP; ++ix)
+                {
+                  c1[ix] = sy[r * NP + ix];
+                  c2[ix] = sy[(NI * NI + r) * NP + ix];
+                }
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                {
+                  const int row = xi + 1;
+                  T         v   = 0;
+#pragma unroll
+                  for (int c = prow_lo<K>(row); c <= prow_hi<K>(row); ++c)
+                    {
+                      v += pK[row * NP + c] * c1[c];
+                      v += pM[row * NP + c] * c2[c];
+                    }
+                  res[xi] = brow[xi] - scale * v;
+                }
+            }
+          else
+            {
+#pragma unroll
+              for (int xi = 0; xi < NI; ++xi)
+                res[xi] = brow[xi];
+            }
+          T tt[NI];
+          fd_fwd<K>(SE, SO, res, tt);
+          T *sr = sZ + lp * ZS;
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            sr[r * NI + j] = tt[j];
+        }
+      __syncthreads();
+      // stage 4: S^T along y
+#pragma unroll
+      for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (lp >= nph)
+            continue;
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sZ + lp * ZS + zi * NI * NI + xi;
+          T         v[NI], tt[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI];
+          fd_fwd<K>(SE, SO, v, tt);
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            base[j * NI] = tt[j];
+        }
+      __syncthreads();
+      // stage 5: S^T along z, eigenvalue scaling, S along z
+#pragma unroll
+      for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (lp >= nph)
+            continue;
+          const int yi = r / NI, xi = r - yi * NI;
+          T        *base = sZ + lp * ZS + yi * NI + xi;
+          T         v[NI], tt[NI];
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            v[i] = base[i * NI * NI];
+          fd_fwd<K>(SE, SO, v, tt);
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            tt[j] *= linv[(j * NI + yi) * NI + xi] * inv_scale;
+          fd_inv<K>(SE, SO, tt, v);
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            base[i * NI * NI] = v[i];
+        }
+      __syncthreads();
+      // stage 6: S along y
+#pragma unroll
+      for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (lp >= nph)
+            continue;
+          const int zi = r / NI, xi = r - zi * NI;
+          T        *base = sZ + lp * ZS + zi * NI * NI + xi;
+          T         v[NI], o[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = base[j * NI];
+          fd_inv<K>(SE, SO, v, o);
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            base[i * NI] = o[i];
+        }
+      __syncthreads();
+      // stage 7: S along x, scaled result back into the line
+#pragma unroll
+      for (int pp = 0; pp < (NPW * NI * NI + 63) / 64; ++pp)
+        {
+          const int t = lane + 64 * pp;
+          if (t >= NPW * NI * NI)
+            continue;
+          const int lp = t / (NI * NI), r = t - lp * (NI * NI);
+          if (lp >= nph)
+            continue;
+          T *line = sZ + lp * ZS + r * NI;
+          T  v[NI], o[NI];
+#pragma unroll
+          for (int j = 0; j < NI; ++j)
+            v[j] = line[j];
+          fd_inv<K>(SE, SO, v, o);
+#pragma unroll
+          for (int i = 0; i < NI; ++i)
+            line[i] = relax * o[i];
+        }
+      __syncthreads();
+      // phase S: interior results by x-runs (x multiple of 2k: not interior)
+      {
+        constexpr int NBR = (NI + RPI - 1) / RPI;
+        const bool    sok = lx >= 1 && lx - 1 < bxw && xin;
+        const int     lps = (lx - 1) / (2 * K), xis = lx - 1 - 2 * K * lps;
+        T             old[MODE == 1 ? NI : 1][MODE == 1 ? NBR : 1];
+        if constexpr (MODE == 1)
+          {
+#pragma unroll
+            for (int z = 1; z <= NI; ++z)
+#pragma unroll
+              for (int yb = 0; yb < NBR; ++yb)
+                {
+                  const int y = 1 + yb * RPI + lr;
+                  old[z - 1][yb] = (sok && y <= NI)
+                                     ? u[f0 + lx + (long long)y * N1 + z * P]
+                                     : T(0);
+                }
+          }
+#pragma unroll
+        for (int z = 1; z <= NI; ++z)
+#pragma unroll
+          for (int yb = 0; yb < NBR; ++yb)
+            {
+              const int y = 1 + yb * RPI + lr;
+              if (!sok || y > NI)
+                continue;
+              const T val = sZ[lps * ZS + ((z - 1) * NI + (y - 1)) * NI + xis];
+              const long long g = f0 + lx + (long long)y * N1 + z * P;
+              if constexpr (MODE == 1)
+                u[g] = old[z - 1][yb] + val;
+              else
+                u[g] = val;
+            }
+      }
+      if (MODE == 2)
+        {
+          // the skeleton of the union box is zero (u = 0 on entry)
+          for (int e = lane; e < C::US; e += 64)
+            {
+              const int x = e % UX, row = e / UX;
+              const int y = row % NP, z = row / NP;
+              const bool inner = y > 0 && y < NP - 1 && z > 0 && z < NP - 1 &&
+                                 x_inner(x);
+              if (x < ux && !inner)
+                u[f0 + x + (long long)y * N1 + z * P] = T(0);
+            }
+        }
+    }
+
     long long
     small_colour_limit()
     {
@@ -1373,10 +1822,27 @@ namespace gtps
             // one patch per wave while a colour has too few patches to fill
             // the GPU several times over; packed groups beyond that
             const int npw = np <= tuning().group_min_patches ? 1 : tuning().patches_per_wave;
+            // row-staged form: groups never straddle a patch row
+            const int npx_ = (n - (colour & 1)) / 2, npy_ = (n - ((colour >> 1) & 1)) / 2,
+                      npz_ = (n - (colour >> 2)) / 2;
+            const bool rows = tuning().row_staged != 0;
+            auto grid_of = [&](int w) {
+              return rows ? (unsigned)((long long)((npx_ + w - 1) / w) * npy_ * npz_)
+                          : (unsigned)((np + w - 1) / w);
+            };
 #define GTPS_SG(M, W)                                                         \
-  hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W>),                      \
-                     dim3((unsigned)((np + W - 1) / W)), dim3(64), 0, s, up,  \
-                     sp, tab, n, colour, scale, inv, (T)relax)
+  do                                                                          \
+    {                                                                         \
+      if (rows)                                                               \
+        hipLaunchKernelGGL((patch3d_row_kernel<K, T, M, W>), dim3(grid_of(W)), \
+                           dim3(64), 0, s, up, sp, tab, n, colour, scale, inv,  \
+                           (T)relax);                                         \
+      else                                                                    \
+        hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W>),                \
+                           dim3(grid_of(W)), dim3(64), 0, s, up, sp, tab, n,   \
+                           colour, scale, inv, (T)relax);                     \
+    }                                                                         \
+  while (0)
 #define GTPS_SG_W(M)                                                          \
   do                                                                          \
     {                                                                         \

@@ -148,7 +148,7 @@ namespace gtps
       constexpr int CX  = TT<K>::CX, CY = TT<K>::CY, CZ = TT<K>::CZ, NT = 256;
       constexpr int N   = K + 1;
       constexpr int EX  = K * CX + 1, EY = K * CY + 1, EZ = K * CZ + 1;
-      constexpr int FX  = 2 * K * CX, FY = 2 * K * CY, FZ = 2 * K * CZ;
+      constexpr int FX  = 2 * K * CX, FY = 2 * K * CY;
       __shared__ T sC[EZ * EY * EX], sX[EZ * EY * FX], sY[EZ * FY * FX];
       const T  *E   = tab + TB::E;
       const int tid = threadIdx.x;
@@ -158,14 +158,28 @@ namespace gtps
                                      gridDim.x * gridDim.y * gridDim.z);
       const int c0x = (L % gridDim.x) * CX, c0y = ((L / gridDim.x) % gridDim.y) * CY,
                 c0z = (L / (gridDim.x * gridDim.y)) * CZ;
-      for (int i = tid; i < EZ * EY * EX; i += NT)
-        {
-          const int x = i % EX, y = (i / EX) % EY, z = i / (EX * EY);
-          const int gx = K * c0x + x, gy = K * c0y + y, gz = K * c0z + z;
-          const bool inner = gx > 0 && gx < Nc1 - 1 && gy > 0 && gy < Nc1 - 1 &&
-                             gz > 0 && gz < Nc1 - 1;
-          sC[i] = inner ? coarse[gx + (long long)gy * Nc1 + gz * Pc] : T(0);
-        }
+      {
+        // coarse block in one round of loads (unrolled), then to LDS
+        constexpr int TOT = EZ * EY * EX, IT = (TOT + NT - 1) / NT;
+        T             v[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i = tid + j * NT;
+            const int x = i % EX, y = (i / EX) % EY, z = i / (EX * EY);
+            const int gx = K * c0x + x, gy = K * c0y + y, gz = K * c0z + z;
+            const bool inner = i < TOT && gx > 0 && gx < Nc1 - 1 && gy > 0 &&
+                               gy < Nc1 - 1 && gz > 0 && gz < Nc1 - 1;
+            v[j] = inner ? coarse[gx + (long long)gy * Nc1 + gz * Pc] : T(0);
+          }
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i = tid + j * NT;
+            if (i < TOT)
+              sC[i] = v[j];
+          }
+      }
       __syncthreads();
       // x: one coarse row (z, y) per thread, all 2k CX fine outputs unrolled
       for (int t = tid; t < EZ * EY; t += NT)
@@ -220,6 +234,17 @@ namespace gtps
           for (int i = 0; i < EZ; ++i)
             r[i] = sY[(i * FY + yf) * FX + xf];
           T *out = fine + gx + (long long)gy * Nf1;
+          // all read-modify-write loads first, so that they are in flight
+          // together instead of one latency per output
+          T old[CZ * 2 * K];
+#pragma unroll
+          for (int cl = 0; cl < CZ; ++cl)
+#pragma unroll
+            for (int q = 0; q < 2 * K; ++q)
+              {
+                const int gz = 2 * K * (c0z + cl) + q;
+                old[cl * 2 * K + q] = gz < Nf1 - 1 ? out[gz * Pf] : T(0);
+              }
 #pragma unroll
           for (int cl = 0; cl < CZ; ++cl)
 #pragma unroll
@@ -232,7 +257,7 @@ namespace gtps
 #pragma unroll
                 for (int a = 0; a < N; ++a)
                   v += E[q * N + a] * r[K * cl + a];
-                out[gz * Pf] += v;
+                out[gz * Pf] = old[cl * 2 * K + q] + v;
               }
         }
     }
@@ -306,14 +331,28 @@ namespace gtps
                 c0z = (L / (gridDim.x * gridDim.y)) * CZ;
       const int w0x = 2 * K * c0x - 2 * K + 1, w0y = 2 * K * c0y - 2 * K + 1,
                 w0z = 2 * K * c0z - 2 * K + 1;
-      for (int i = tid; i < WZ * WY * WX; i += NT)
-        {
-          const int x = i % WX, y = (i / WX) % WY, z = i / (WX * WY);
-          const int gx = w0x + x, gy = w0y + y, gz = w0z + z;
-          const bool in = gx >= 0 && gx < Nf1 && gy >= 0 && gy < Nf1 && gz >= 0 &&
-                          gz < Nf1;
-          sF[i] = in ? fine[gx + (long long)gy * Nf1 + gz * Pf] : T(0);
-        }
+      {
+        // the whole window in one round of loads (unrolled), then to LDS
+        constexpr int TOT = WZ * WY * WX, IT = (TOT + NT - 1) / NT;
+        T             v[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i  = tid + j * NT;
+            const int x  = i % WX, y = (i / WX) % WY, z = i / (WX * WY);
+            const int gx = w0x + x, gy = w0y + y, gz = w0z + z;
+            const bool in = i < TOT && gx >= 0 && gx < Nf1 && gy >= 0 &&
+                            gy < Nf1 && gz >= 0 && gz < Nf1;
+            v[j] = in ? fine[gx + (long long)gy * Nf1 + gz * Pf] : T(0);
+          }
+#pragma unroll
+        for (int j = 0; j < IT; ++j)
+          {
+            const int i = tid + j * NT;
+            if (i < TOT)
+              sF[i] = v[j];
+          }
+      }
       __syncthreads();
       for (int t = tid; t < WZ * WY; t += NT)
         {
@@ -352,16 +391,20 @@ namespace gtps
             r[i] = sY[(i * OY + yo) * OX + xo];
           restrict_line<K, CZ>(E, r, c0z, o);
           T *out = coarse + gx + (long long)gy * Nc1;
+          T  old[OZ];
+#pragma unroll
+          for (int i = 0; i < OZ; ++i)
+            {
+              const int gz = K * c0z + i;
+              old[i] = (!STORE && gz > 0 && gz < Nc1 - 1) ? out[gz * Pc] : T(0);
+            }
 #pragma unroll
           for (int i = 0; i < OZ; ++i)
             {
               const int gz = K * c0z + i;
               if (gz <= 0 || gz >= Nc1 - 1)
                 continue;
-              if (STORE)
-                out[gz * Pc] = o[i];
-              else
-                out[gz * Pc] += o[i];
+              out[gz * Pc] = old[i] + o[i];
             }
         }
     }

@@ -74,15 +74,39 @@ namespace gtps
         partials[blockIdx.x] = s;
     }
 
+    // Block sum of partials[0..nb) (result in thread 0).  Each thread
+    // issues its loads 8 at a time into separate accumulators, so the load
+    // latencies overlap; the summation order is fixed for a given nb.
+    __device__ __forceinline__ double
+    sum_partials(const double *__restrict__ partials, int nb, double *red)
+    {
+      constexpr int U = 8;
+      double        acc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        acc[u] = 0.0;
+      for (int i0 = threadIdx.x; i0 < nb; i0 += VNT * U)
+        {
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            {
+              const int i = i0 + u * VNT;
+              acc[u] += i < nb ? partials[i] : 0.0;
+            }
+        }
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        s += acc[u];
+      return block_sum<VNT>(s, red);
+    }
+
     __global__ void __launch_bounds__(VNT)
     finalize_kernel(const double *__restrict__ partials, int nb,
                     double *__restrict__ scalars, int slot)
     {
       __shared__ double red[VNT / 64];
-      double            s = 0.0;
-      for (int i = threadIdx.x; i < nb; i += VNT)
-        s += partials[i];
-      s = block_sum<VNT>(s, red);
+      const double      s = sum_partials(partials, nb, red);
       if (threadIdx.x == 0)
         scalars[slot] = s;
     }
@@ -94,10 +118,7 @@ namespace gtps
                        double *__restrict__ sc, int slot, int op)
     {
       __shared__ double red[VNT / 64];
-      double            s = 0.0;
-      for (int i = threadIdx.x; i < nb; i += VNT)
-        s += partials[i];
-      s = block_sum<VNT>(s, red);
+      const double      s = sum_partials(partials, nb, red);
       if (threadIdx.x != 0)
         return;
       if (op == cg::OP_SHIFT_BETA)

@@ -75,7 +75,14 @@ const char *gtps_version(void);
  *   "group_min_patches"     colours with more patches than this pack
  *                           several patches into each wavefront (default
  *                           4096);
- *   "patches_per_wave"      how many (1, 2 or 4; default 2).
+ *   "patches_per_wave"      how many (1, 2 or 4; default 2);
+ *   "box_vmult_max_n"       3D operator with k <= 4 on levels with at most
+ *                           this many cells per direction: one-round box
+ *                           kernel instead of the z-marching one (default
+ *                           16);
+ *   "row_staged"            patch-group smoother with its global accesses
+ *                           staged through contiguous x-runs (1) or
+ *                           per-lane pencils and rows (0, default).
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */

@@ -181,11 +181,14 @@ def test_tensor_product_data(core, ctx):
 
 
 SMOOTHER_PATHS = {
-    # small_colour_patches, group_min_patches, patches_per_wave
-    "single": (1 << 40, 1 << 40, 2),
-    "group2": (1 << 40, 0, 2),
-    "group4": (1 << 40, 0, 4),
-    "multi": (0, 4096, 2),
+    # small_colour_patches, group_min_patches, patches_per_wave, row_staged
+    "single": (1 << 40, 1 << 40, 2, 1),
+    "group2": (1 << 40, 0, 2, 1),
+    "group4": (1 << 40, 0, 4, 1),
+    "single_lane": (1 << 40, 1 << 40, 2, 0),
+    "group2_lane": (1 << 40, 0, 2, 0),
+    "group4_lane": (1 << 40, 0, 4, 0),
+    "multi": (0, 4096, 2, 1),
 }
 
 
@@ -195,8 +198,9 @@ SMOOTHER_PATHS = {
 @pytest.mark.parametrize("dim,k,level", [(3, 1, 4), (3, 2, 3), (3, 3, 3), (3, 3, 4)])
 def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, path):
     """The patch-group kernel at 1, 2 and 4 patches per wavefront (coarse and
-    fine levels) and the multi-wave kernel agree with the oracle on the same
-    level; odd patch counts leave a partly empty last group."""
+    fine levels), with row-staged or per-lane global accesses, and the
+    multi-wave kernel agree with the oracle on the same level; odd patch
+    counts per row leave a partly empty last group."""
     _announce("smoother-path", dim, k, level, variant, prec, path)
     g = rng(level * 7 + k)
     N = n_dofs(dim, k, level)
@@ -207,7 +211,7 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
     ov, cv, relax = {"fwd": (O.MULT_FWD, core.PatchSmoother.MULTIPLICATIVE, 1.0),
                      "add": (O.ADDITIVE, core.PatchSmoother.ADDITIVE, 0.5)}[variant]
     ref = O.smoother_step(dim, k, level, b, u, variant=ov, relax=relax)
-    keys = ("small_colour_patches", "group_min_patches", "patches_per_wave")
+    keys = ("small_colour_patches", "group_min_patches", "patches_per_wave", "row_staged")
     prev = [core.set_tuning(kk, v) for kk, v in zip(keys, SMOOTHER_PATHS[path])]
     try:
         op = core.LaplaceOperator(ctx, dim, k, level, _dt(prec))
@@ -220,3 +224,32 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
             core.set_tuning(kk, v)
     tol = TOL[prec] * (20 if prec == "fp32" else 1)
     assert rel_err(ud.cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("path", ["box", "march"])
+@pytest.mark.parametrize("k,level", [(1, 3), (2, 3), (3, 2), (3, 4), (4, 3), (3, 5)])
+def test_vmult_box_and_march_paths(core, ctx, k, level, path, prec):
+    """The one-round box kernel (small levels, k <= 4) and the z-marching
+    kernel agree with the oracle on the same level (vmult and residual)."""
+    _announce("vmult-path", k, level, path, prec)
+    g = rng(level * 5 + k)
+    N = n_dofs(3, k, level)
+    x, b = g.uniform(-1, 1, (2, N))
+    if prec == "fp32":
+        x = x.astype(np.float32).astype(np.float64)
+        b = b.astype(np.float32).astype(np.float64)
+    prev = core.set_tuning("box_vmult_max_n", 1 << 20 if path == "box" else 0)
+    try:
+        op = core.LaplaceOperator(ctx, 3, k, level, _dt(prec))
+        xd, bd = _to_dev(x, prec), _to_dev(b, prec)
+        y = torch.full_like(xd, float("nan"))
+        r = torch.full_like(xd, float("nan"))
+        op.vmult(y, xd)
+        op.residual(r, bd, xd)
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("box_vmult_max_n", prev)
+    ref = O.laplace_vmult(3, k, level, x)
+    assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
+    assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec]
