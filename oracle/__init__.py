@@ -58,6 +58,19 @@ def lib():
         L.oracle_coeff_vcycle.argtypes = [i, i, i, _dp, i, d, _dp, _dp]
         L.oracle_coeff_gradient.argtypes = [i, i, i, _dp, _dp, _dp]
         L.oracle_fmg.argtypes = [i, i, i, _dp, i, d, _dp, i, _dp]
+        L.oracle_dg_n_dofs.argtypes = [i, i, i]
+        L.oracle_dg_n_dofs.restype = ll
+        L.oracle_dg_vmult.argtypes = [i, i, i, d, _dp, _dp]
+        L.oracle_dg_global_1d.argtypes = [i, i, d, _dp]
+        L.oracle_dg_smoother_step.argtypes = [i, i, i, d, i, d, _dp, _dp]
+        L.oracle_dg_prolongate_add.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_dg_restrict_add.argtypes = [i, i, i, _dp, _dp]
+        L.oracle_dg_vcycle.argtypes = [i, i, i, i, d, i, d, i, _dp, _dp]
+        L.oracle_dg_pcg.argtypes = [i, i, i, i, d, i, d, i, _dp, _dp, d, i,
+                                    ctypes.POINTER(i), _dp, _dp]
+        L.oracle_dg_rhs.argtypes = [i, i, i, i, _dp]
+        L.oracle_dg_l2_error_sine.argtypes = [i, i, i, _dp]
+        L.oracle_dg_l2_error_sine.restype = d
         L.oracle_coeff_problem.argtypes = [i, i, i, ull, d, _dp, _dp, i,
                                            ctypes.POINTER(ll)]
         _lib = L
@@ -268,3 +281,93 @@ def fmg(dim, k, level, rhs_levels, n_cycles=1, mg_variant=MG_SYMMETRIC, relax=1.
     _chk(lib().oracle_fmg(dim, k, level, None if c is None else _p(c), mg_variant, relax,
                           _p(allr), n_cycles, _p(x)), "fmg")
     return x
+
+
+# ---------------------------------------------------------------- SIPG-DG
+# DoFs on one tensor grid X = cx (k+1) + ix (n cells per direction), index
+# X + N1 Y + N1^2 Z; penalty sigma = pf / h.
+
+def dg_penalty(k):
+    """The penalty factor of the DG workload, 2.5 (k+1)^2."""
+    return 2.5 * (k + 1) ** 2
+
+
+def dg_n_dofs(dim, k, n):
+    return int(lib().oracle_dg_n_dofs(dim, k, n))
+
+
+def dg_vmult(dim, k, n, x, pf=None):
+    pf = dg_penalty(k) if pf is None else pf
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    _chk(lib().oracle_dg_vmult(dim, k, n, pf, _p(x), _p(y)), "dg_vmult")
+    return y
+
+
+def dg_global_1d(k, n, pf=None):
+    pf = dg_penalty(k) if pf is None else pf
+    N = n * (k + 1)
+    L = np.zeros((N, N))
+    _chk(lib().oracle_dg_global_1d(k, n, pf, _p(L)), "dg_global_1d")
+    return L
+
+
+def dg_smoother_step(dim, k, n, b, u, variant=MULT_FWD, relax=1.0, pf=None):
+    pf = dg_penalty(k) if pf is None else pf
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    u = np.array(u, dtype=np.float64, copy=True)
+    _chk(lib().oracle_dg_smoother_step(dim, k, n, pf, variant, relax, _p(b), _p(u)),
+         "dg_smoother")
+    return u
+
+
+def dg_prolongate_add(dim, k, n_fine, coarse, fine):
+    coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+    fine = np.array(fine, dtype=np.float64, copy=True)
+    _chk(lib().oracle_dg_prolongate_add(dim, k, n_fine, _p(coarse), _p(fine)), "dg_prolong")
+    return fine
+
+
+def dg_restrict_add(dim, k, n_fine, fine, coarse):
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    coarse = np.array(coarse, dtype=np.float64, copy=True)
+    _chk(lib().oracle_dg_restrict_add(dim, k, n_fine, _p(fine), _p(coarse)), "dg_restrict")
+    return coarse
+
+
+def dg_vcycle(dim, k, n0, max_level, r, pf=None, mg_variant=MG_SYMMETRIC, relax=1.0,
+              coarse_sweeps=2):
+    pf = dg_penalty(k) if pf is None else pf
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    _chk(lib().oracle_dg_vcycle(dim, k, n0, max_level, pf, mg_variant, relax, coarse_sweeps,
+                                _p(r), _p(z)), "dg_vcycle")
+    return z
+
+
+def dg_pcg(dim, k, n0, max_level, b, rtol=1e-8, maxit=200, pf=None, mg_variant=MG_SYMMETRIC,
+           relax=1.0, coarse_sweeps=2):
+    pf = dg_penalty(k) if pf is None else pf
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0.0)
+    hist = np.zeros(maxit + 1)
+    _chk(lib().oracle_dg_pcg(dim, k, n0, max_level, pf, mg_variant, relax, coarse_sweeps,
+                             _p(b), _p(x), rtol, maxit, ctypes.byref(it), ctypes.byref(rel),
+                             _p(hist)), "dg_pcg")
+    return x, it.value, rel.value, hist[: it.value + 1]
+
+
+DG_RHS_CONST, DG_RHS_SINE = 0, 1
+
+
+def dg_rhs(dim, k, n, kind=DG_RHS_SINE):
+    b = np.zeros(dg_n_dofs(dim, k, n))
+    _chk(lib().oracle_dg_rhs(dim, k, n, kind, _p(b)), "dg_rhs")
+    return b
+
+
+def dg_l2_error_sine(dim, k, n, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    return float(lib().oracle_dg_l2_error_sine(dim, k, n, _p(u)))

@@ -43,6 +43,7 @@
 // restatements on tiny grids.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1121,6 +1122,557 @@ namespace
   }
 } // namespace
 
+namespace
+{
+  // ============================================================== SIPG-DG
+  // Discontinuous Galerkin Q_k (FE_DGQ on Gauss-Lobatto points) on a
+  // uniform mesh of n cells per direction, DoFs stored as one tensor grid
+  // X = cx (k+1) + ix, index X + N1 Y + N1^2 Z with N1 = n (k+1).  The
+  // symmetric interior penalty form follows the reference's matrix-free DG
+  // test operator (tests/mf_dg.cu:63-274): cell term grad u . grad v; on an
+  // interior face (sigma [u] - {du/dn}) [v] - [u] {dv/dn} / 2 * 2; on a
+  // boundary face (2 sigma u - du/dn) v - u dv/dn, with sigma = pf / h.
+  // Here it is evaluated the way that test does it: cell and face
+  // quadrature loops with Gauss(k+1) points.
+  struct DGGrid
+  {
+    int       dim, k, n;
+    long long N1, N;
+    double    h, pf;
+    DGGrid(int dim_, int k_, int n_, double pf_)
+      : dim(dim_), k(k_), n(n_), pf(pf_)
+    {
+      N1 = (long long)n * (k + 1);
+      N  = dim == 3 ? N1 * N1 * N1 : N1 * N1;
+      h  = 1.0 / n;
+    }
+    long long
+    idx(long long X, long long Y, long long Z) const
+    {
+      return X + N1 * Y + N1 * N1 * Z;
+    }
+  };
+
+  // values and reference derivatives of the 1D basis at the cell ends
+  struct FaceBasis
+  {
+    std::vector<double> v0, v1, d0, d1;
+  };
+  FaceBasis
+  face_basis(int k)
+  {
+    const Basis &b = basis(k);
+    FaceBasis    f;
+    for (int i = 0; i <= k; ++i)
+      {
+        f.v0.push_back(lagrange(b.nodes, i, 0.0));
+        f.v1.push_back(lagrange(b.nodes, i, 1.0));
+        f.d0.push_back(lagrange_deriv(b.nodes, i, 0.0));
+        f.d1.push_back(lagrange_deriv(b.nodes, i, 1.0));
+      }
+    return f;
+  }
+
+  // y = A x (cell loop + interior-face loop + boundary-face loop)
+  void
+  dg_vmult(const DGGrid &g, const double *x, double *y)
+  {
+    const Basis    &b   = basis(g.k);
+    const FaceBasis fb  = face_basis(g.k);
+    const int       n1  = g.k + 1;
+    const int       dim = g.dim;
+    const int       nl  = dim == 3 ? n1 * n1 * n1 : n1 * n1;
+    const double    h   = g.h, sigma = g.pf / h;
+    const double    scale = std::pow(h, dim - 2);
+    std::fill(y, y + g.N, 0.0);
+    const int nz = dim == 3 ? g.n : 1;
+    auto gather = [&](int cx, int cy, int cz, const double *src, double *loc) {
+      for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+        for (int j = 0; j < n1; ++j)
+          for (int i = 0; i < n1; ++i)
+            loc[(l * n1 + j) * n1 + i] =
+              src[g.idx((long long)cx * n1 + i, (long long)cy * n1 + j,
+                        (long long)cz * n1 * (dim == 3) + l)];
+    };
+    auto scatter = [&](int cx, int cy, int cz, const double *loc) {
+      for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+        for (int j = 0; j < n1; ++j)
+          for (int i = 0; i < n1; ++i)
+            y[g.idx((long long)cx * n1 + i, (long long)cy * n1 + j,
+                    (long long)cz * n1 * (dim == 3) + l)] += loc[(l * n1 + j) * n1 + i];
+    };
+    // cell terms (quadrature of grad u . grad v), same as the CG cell loop
+    {
+      std::vector<double> wq(nl);
+      for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+        for (int j = 0; j < n1; ++j)
+          for (int i = 0; i < n1; ++i)
+            wq[(l * n1 + j) * n1 + i] =
+              b.qw[i] * b.qw[j] * (dim == 3 ? b.qw[l] : 1.0) * scale;
+      const int shape[3] = {n1, n1, dim == 3 ? n1 : 1};
+      std::vector<double> u(nl), t1(nl), t2(nl), gq(nl), out(nl), acc(nl);
+      for (int cz = 0; cz < nz; ++cz)
+        for (int cy = 0; cy < g.n; ++cy)
+          for (int cx = 0; cx < g.n; ++cx)
+            {
+              gather(cx, cy, cz, x, u.data());
+              std::fill(acc.begin(), acc.end(), 0.0);
+              for (int d = 0; d < dim; ++d)
+                {
+                  const double *src     = u.data();
+                  double       *bufs[2] = {t1.data(), t2.data()};
+                  int           nb      = 0;
+                  for (int a = 0; a < dim; ++a)
+                    {
+                      double *dst = (a == dim - 1) ? gq.data() : bufs[nb++ & 1];
+                      apply_axis(a == d ? b.D.data() : b.N.data(), n1, n1, src,
+                                 dst, shape, a, false);
+                      src = dst;
+                    }
+                  for (int q = 0; q < nl; ++q)
+                    gq[q] *= wq[q];
+                  src = gq.data();
+                  nb  = 0;
+                  for (int a = 0; a < dim; ++a)
+                    {
+                      double *dst = (a == dim - 1) ? out.data() : bufs[nb++ & 1];
+                      apply_axis(a == d ? b.D.data() : b.N.data(), n1, n1, src,
+                                 dst, shape, a, false, true);
+                      src = dst;
+                    }
+                  for (int q = 0; q < nl; ++q)
+                    acc[q] += out[q];
+                }
+              scatter(cx, cy, cz, acc.data());
+            }
+    }
+    // face terms: per face quadrature point (tensor Gauss points in the
+    // tangential directions), the values and normal derivatives of the two
+    // traces; basis function (i,j,l) = phi_i(x) phi_j(y) phi_l(z)
+    const int ntan = dim == 3 ? n1 * n1 : n1;  // tangential quad points
+    std::vector<double> um(nl), up(nl), om(nl), op(nl);
+    // value / normal-derivative of basis function `a` (local multi-index)
+    // of a cell at a face point: side s = 0 (face at xi_d = 0) or 1
+    auto eval = [&](int d, int s, const int *ai, const int *qt, double &val,
+                    double &dn) {
+      // ai: local 1D indices per direction; qt: tangential quad indices
+      val = 1.0;
+      dn  = 1.0;
+      int t = 0;
+      for (int a = 0; a < dim; ++a)
+        {
+          if (a == d)
+            {
+              val *= s ? fb.v1[ai[a]] : fb.v0[ai[a]];
+              dn *= (s ? fb.d1[ai[a]] : fb.d0[ai[a]]) / h;
+            }
+          else
+            {
+              const double nv = b.N[qt[t] * n1 + ai[a]];
+              val *= nv;
+              dn *= nv;
+              ++t;
+            }
+        }
+    };
+    auto tan_weight = [&](const int *qt) {
+      double w = 1.0;
+      for (int t = 0; t < dim - 1; ++t)
+        w *= b.qw[qt[t]] * h;
+      return w;
+    };
+    for (int d = 0; d < dim; ++d)
+      for (int cz = 0; cz < nz; ++cz)
+        for (int cy = 0; cy < g.n; ++cy)
+          for (int cx = 0; cx < g.n; ++cx)
+            {
+              const int c[3] = {cx, cy, cz};
+              // faces at the low side (interior with the lower neighbour or
+              // boundary) and at the high side (boundary only)
+              for (int side = 0; side < 2; ++side)
+                {
+                  const bool low   = side == 0;
+                  const bool bdry  = low ? c[d] == 0 : c[d] == g.n - 1;
+                  if (!low && !bdry)
+                    continue;  // interior high faces are the neighbour's low
+                  if (bdry)
+                    {
+                      // boundary face of this cell: outward normal -e_d
+                      // (low) or +e_d (high)
+                      const double sgn = low ? -1.0 : 1.0;
+                      gather(cx, cy, cz, x, um.data());
+                      std::fill(om.begin(), om.end(), 0.0);
+                      for (int q = 0; q < ntan; ++q)
+                        {
+                          const int qt[2] = {q % n1, q / n1};
+                          double    uval = 0, udn = 0;
+                          int       ai[3];
+                          for (int a = 0; a < nl; ++a)
+                            {
+                              ai[0] = a % n1;
+                              ai[1] = (a / n1) % n1;
+                              ai[2] = a / (n1 * n1);
+                              double v, dn;
+                              eval(d, low ? 0 : 1, ai, qt, v, dn);
+                              uval += um[a] * v;
+                              udn += um[a] * dn * sgn;
+                            }
+                          const double w  = tan_weight(qt);
+                          const double tv = (2.0 * sigma * uval - udn) * w;
+                          const double td = -uval * w;
+                          for (int a = 0; a < nl; ++a)
+                            {
+                              ai[0] = a % n1;
+                              ai[1] = (a / n1) % n1;
+                              ai[2] = a / (n1 * n1);
+                              double v, dn;
+                              eval(d, low ? 0 : 1, ai, qt, v, dn);
+                              om[a] += tv * v + td * dn * sgn;
+                            }
+                        }
+                      scatter(cx, cy, cz, om.data());
+                      if (low)
+                        continue;
+                    }
+                  if (!low || bdry)
+                    continue;
+                  // interior face between the lower neighbour (minus side,
+                  // its high face) and this cell (plus side, its low face);
+                  // normal +e_d
+                  int cm[3] = {cx, cy, cz};
+                  cm[d] -= 1;
+                  gather(cm[0], cm[1], cm[2], x, um.data());
+                  gather(cx, cy, cz, x, up.data());
+                  std::fill(om.begin(), om.end(), 0.0);
+                  std::fill(op.begin(), op.end(), 0.0);
+                  for (int q = 0; q < ntan; ++q)
+                    {
+                      const int qt[2] = {q % n1, q / n1};
+                      double    vm = 0, vp = 0, dm = 0, dp = 0;
+                      int       ai[3];
+                      for (int a = 0; a < nl; ++a)
+                        {
+                          ai[0] = a % n1;
+                          ai[1] = (a / n1) % n1;
+                          ai[2] = a / (n1 * n1);
+                          double v, dn;
+                          eval(d, 1, ai, qt, v, dn);
+                          vm += um[a] * v;
+                          dm += um[a] * dn;
+                          eval(d, 0, ai, qt, v, dn);
+                          vp += up[a] * v;
+                          dp += up[a] * dn;
+                        }
+                      const double w    = tan_weight(qt);
+                      const double jump = vm - vp;
+                      const double tv   = (jump * sigma - 0.5 * (dm + dp)) * w;
+                      const double td   = -0.5 * jump * w;
+                      for (int a = 0; a < nl; ++a)
+                        {
+                          ai[0] = a % n1;
+                          ai[1] = (a / n1) % n1;
+                          ai[2] = a / (n1 * n1);
+                          double v, dn;
+                          eval(d, 1, ai, qt, v, dn);
+                          om[a] += tv * v + td * dn;
+                          eval(d, 0, ai, qt, v, dn);
+                          op[a] += -tv * v + td * dn;
+                        }
+                    }
+                  scatter(cm[0], cm[1], cm[2], om.data());
+                  scatter(cx, cy, cz, op.data());
+                }
+            }
+  }
+
+  // 1D SIPG data of one direction: the global 1D matrix L (size n(k+1))
+  // of the unit-scaled form (cell stiffness + face terms with sigma = pf),
+  // from the face formulas of dg_vmult restricted to 1D; the cell mass M.
+  std::vector<double>
+  dg_global_L(int k, int n, double pf)
+  {
+    const Basis    &b  = basis(k);
+    const FaceBasis fb = face_basis(k);
+    const int       n1 = k + 1, N = n * n1;
+    std::vector<double> L((size_t)N * N, 0.0);
+    auto at = [&](int r, int c) -> double & { return L[(size_t)r * N + c]; };
+    for (int c = 0; c < n; ++c)
+      for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+          at(c * n1 + i, c * n1 + j) += b.K[i * n1 + j];
+    // interior faces between cells c (minus, its x = 1 end) and c + 1
+    for (int c = 0; c + 1 < n; ++c)
+      for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+          {
+            // test i / trial j on sides (m,m), (m,p), (p,m), (p,p)
+            const double tm[2] = {fb.v1[i], -fb.v0[i]};   // [v]
+            const double dtm[2] = {0.5 * fb.d1[i], 0.5 * fb.d0[i]};  // {v'}
+            const double sm[2] = {fb.v1[j], -fb.v0[j]};   // [u]
+            const double dsm[2] = {0.5 * fb.d1[j], 0.5 * fb.d0[j]};  // {u'}
+            for (int si = 0; si < 2; ++si)
+              for (int sj = 0; sj < 2; ++sj)
+                at((c + si) * n1 + i, (c + sj) * n1 + j) +=
+                  pf * sm[sj] * tm[si] - dsm[sj] * tm[si] - sm[sj] * dtm[si];
+          }
+    // boundary faces: left (normal -1) at cell 0, right (normal +1) at n-1
+    for (int i = 0; i < n1; ++i)
+      for (int j = 0; j < n1; ++j)
+        {
+          at(i, j) += 2 * pf * fb.v0[i] * fb.v0[j] + fb.d0[j] * fb.v0[i] +
+                      fb.v0[j] * fb.d0[i];
+          const int o = (n - 1) * n1;
+          at(o + i, o + j) += 2 * pf * fb.v1[i] * fb.v1[j] -
+                              fb.d1[j] * fb.v1[i] - fb.v1[j] * fb.d1[i];
+        }
+    return L;
+  }
+
+  // Patch data of one direction for the vertex patch on cells (c0, c0+1):
+  // restricted 1D matrices and their generalized eigenpairs.
+  struct DGPatch1D
+  {
+    std::vector<double> L, M, lam, S;  // (2(k+1))^2, S column j = vector j
+  };
+  DGPatch1D
+  dg_patch_1d(int k, int n, double pf, int c0)
+  {
+    const Basis &b  = basis(k);
+    const int    n1 = k + 1, np = 2 * n1, N = n * n1;
+    const std::vector<double> Lg = dg_global_L(k, n, pf);
+    DGPatch1D p;
+    p.L.assign(np * np, 0.0);
+    p.M.assign(np * np, 0.0);
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < np; ++j)
+        p.L[i * np + j] = Lg[(size_t)(c0 * n1 + i) * N + c0 * n1 + j];
+    for (int c = 0; c < 2; ++c)
+      for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+          p.M[(c * n1 + i) * np + c * n1 + j] = b.M[i * n1 + j];
+    p.lam.resize(np);
+    p.S.resize(np * np);
+    gen_eig(np, p.L.data(), p.M.data(), p.lam.data(), p.S.data());
+    return p;
+  }
+
+  // One multiplicative sweep over the 3^d colours of vertex patches
+  // (interior vertices v in 1..n-1 per direction, patch index p = v-1 with
+  // cells p, p+1; colour = p mod 3 per direction: DG patches two apart would
+  // be face neighbours, three apart they share nothing).  Local problem
+  // u_P = A_PP^{-1} (b_P - A_P,ext u_ext) with A_PP inverted by fast
+  // diagonalisation of the restricted 1D matrices (exact: A_PP is their
+  // Kronecker sum); b_P - A_P,ext u_ext comes from one operator application
+  // with the colour's patch DoFs zeroed.
+  void
+  dg_smoother_step(const DGGrid &g, int variant, double relax, const double *b_,
+                   double *u)
+  {
+    const int    k = g.k, n1 = k + 1, np = 2 * n1, dim = g.dim;
+    const int    npd = g.n - 1;  // patches per direction
+    const double scale = std::pow(g.h, dim - 2);
+    std::vector<DGPatch1D> P1(g.n);  // by first cell
+    for (int c0 = 0; c0 + 1 < g.n; ++c0)
+      P1[c0] = dg_patch_1d(k, g.n, g.pf, c0);
+    const int nloc = dim == 3 ? np * np * np : np * np;
+    const int ncol = dim == 3 ? 27 : 9;
+    std::vector<double> tmp(g.N), ax(g.N);
+    for (int cc = 0; cc < ncol; ++cc)
+      {
+        const int col = variant == MULT_REV ? ncol - 1 - cc : cc;
+        const int px = col % 3, py = (col / 3) % 3, pz = col / 9;
+        // residual with the patch DoFs of this colour zeroed: the patches
+        // of one colour are separated by a cell, so one global product
+        // gives every patch's b_P - A_P,ext u_ext
+        std::copy(u, u + g.N, tmp.begin());
+        std::vector<std::array<int, 3>> plist;
+        for (int iz = (dim == 3 ? pz : 0); iz < (dim == 3 ? npd : 1); iz += (dim == 3 ? 3 : 1))
+          for (int iy = py; iy < npd; iy += 3)
+            for (int ix = px; ix < npd; ix += 3)
+              plist.push_back({ix, iy, iz});
+        if (plist.empty())
+          continue;
+        auto pidx = [&](const std::array<int, 3> &p, int i, int j, int l) {
+          return g.idx((long long)p[0] * n1 + i, (long long)p[1] * n1 + j,
+                       dim == 3 ? (long long)p[2] * n1 + l : 0);
+        };
+        for (const auto &p : plist)
+          for (int l = 0; l < (dim == 3 ? np : 1); ++l)
+            for (int j = 0; j < np; ++j)
+              for (int i = 0; i < np; ++i)
+                tmp[pidx(p, i, j, l)] = 0.0;
+        dg_vmult(g, tmp.data(), ax.data());
+#pragma omp parallel
+        {
+          std::vector<double> r(nloc), t(nloc);
+#pragma omp for schedule(static)
+          for (size_t pi = 0; pi < plist.size(); ++pi)
+            {
+              const auto &p = plist[pi];
+              for (int l = 0; l < (dim == 3 ? np : 1); ++l)
+                for (int j = 0; j < np; ++j)
+                  for (int i = 0; i < np; ++i)
+                    {
+                      const long long gi = pidx(p, i, j, l);
+                      r[(l * np + j) * np + i] = b_[gi] - ax[gi];
+                    }
+              // S^T per direction, 1/(scale sum lambda), S per direction
+              const DGPatch1D *d1[3] = {&P1[p[0]], &P1[p[1]], &P1[dim == 3 ? p[2] : 0]};
+              const int shape[3] = {np, np, dim == 3 ? np : 1};
+              double *bufs[2] = {r.data(), t.data()};
+              int     cur     = 0;
+              for (int a = 0; a < dim; ++a)
+                {
+                  apply_axis(d1[a]->S.data(), np, np, bufs[cur], bufs[cur ^ 1],
+                             shape, a, false, true);
+                  cur ^= 1;
+                }
+              for (int l = 0; l < shape[2]; ++l)
+                for (int j = 0; j < np; ++j)
+                  for (int i = 0; i < np; ++i)
+                    {
+                      double lam = d1[0]->lam[i] + d1[1]->lam[j];
+                      if (dim == 3)
+                        lam += d1[2]->lam[l];
+                      bufs[cur][(l * np + j) * np + i] /= scale * lam;
+                    }
+              for (int a = 0; a < dim; ++a)
+                {
+                  apply_axis(d1[a]->S.data(), np, np, bufs[cur], bufs[cur ^ 1],
+                             shape, a, false, false);
+                  cur ^= 1;
+                }
+              for (int l = 0; l < shape[2]; ++l)
+                for (int j = 0; j < np; ++j)
+                  for (int i = 0; i < np; ++i)
+                    {
+                      const long long gi = pidx(p, i, j, l);
+                      u[gi] = (1.0 - relax) * u[gi] + relax * bufs[cur][(l * np + j) * np + i];
+                    }
+            }
+        }
+      }
+  }
+
+  // DG transfer: fine cell f of coarse cell f/2, child c = f % 2; the
+  // coarse polynomial interpolated at the child's nodes, x = (c + xi) / 2.
+  std::vector<double>
+  dg_child_embedding(int k)
+  {
+    const Basis        &b  = basis(k);
+    const int           n1 = k + 1;
+    std::vector<double> E(2 * n1 * n1);
+    for (int c = 0; c < 2; ++c)
+      for (int q = 0; q < n1; ++q)
+        for (int a = 0; a < n1; ++a)
+          E[(c * n1 + q) * n1 + a] = lagrange(b.nodes, a, 0.5 * (c + b.nodes[q]));
+    return E;
+  }
+
+  void
+  dg_transfer(const DGGrid &gf, bool prolongate, double *coarse, double *fine)
+  {
+    const int    k = gf.k, n1 = k + 1, dim = gf.dim;
+    const DGGrid gc(dim, k, gf.n / 2, gf.pf);
+    const std::vector<double> E = dg_child_embedding(k);
+    const int nz = dim == 3 ? gf.n : 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int fz = 0; fz < nz; ++fz)
+      for (int fy = 0; fy < gf.n; ++fy)
+        for (int fx = 0; fx < gf.n; ++fx)
+          {
+            const int f[3] = {fx, fy, fz};
+            for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+              for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n1; ++i)
+                  {
+                    const long long gfi = gf.idx((long long)fx * n1 + i, (long long)fy * n1 + j,
+                                                 dim == 3 ? (long long)fz * n1 + l : 0);
+                    const int q[3] = {i, j, l};
+                    double    acc = 0.0;
+                    for (int cl = 0; cl < (dim == 3 ? n1 : 1); ++cl)
+                      for (int cb = 0; cb < n1; ++cb)
+                        for (int ca = 0; ca < n1; ++ca)
+                          {
+                            const int a[3] = {ca, cb, cl};
+                            double    w   = 1.0;
+                            for (int d = 0; d < dim; ++d)
+                              w *= E[((f[d] & 1) * n1 + q[d]) * n1 + a[d]];
+                            const long long gci =
+                              gc.idx((long long)(fx / 2) * n1 + ca, (long long)(fy / 2) * n1 + cb,
+                                     dim == 3 ? (long long)(fz / 2) * n1 + cl : 0);
+                            if (prolongate)
+                              acc += w * coarse[gci];
+                            else
+                              {
+#pragma omp atomic
+                                coarse[gci] += w * fine[gfi];
+                              }
+                          }
+                    if (prolongate)
+                      fine[gfi] += acc;
+                  }
+          }
+  }
+
+  struct DGMG
+  {
+    int dim, k, n0, maxl, variant, coarse_sweeps;
+    double pf, relax;
+    std::vector<DGGrid> grids;
+    mutable std::vector<std::vector<double>> d, u, t;
+    DGMG(int dim_, int k_, int n0_, int maxl_, double pf_, int var, double rel, int cs)
+      : dim(dim_), k(k_), n0(n0_), maxl(maxl_), variant(var), coarse_sweeps(cs),
+        pf(pf_), relax(rel)
+    {
+      for (int l = 0; l <= maxl; ++l)
+        grids.emplace_back(dim, k, n0 << l, pf);
+      d.resize(maxl + 1);
+      u.resize(maxl + 1);
+      t.resize(maxl + 1);
+      for (int l = 0; l <= maxl; ++l)
+        {
+          d[l].assign(grids[l].N, 0.0);
+          u[l].assign(grids[l].N, 0.0);
+          t[l].assign(grids[l].N, 0.0);
+        }
+    }
+    void
+    vcycle(int l) const
+    {
+      const DGGrid &g = grids[l];
+      std::fill(u[l].begin(), u[l].end(), 0.0);
+      if (l == 0)
+        {
+          // coarsest level: symmetric sweep pairs from zero
+          for (int s = 0; s < coarse_sweeps; ++s)
+            {
+              dg_smoother_step(g, MULT_FWD, 1.0, d[l].data(), u[l].data());
+              dg_smoother_step(g, MULT_REV, 1.0, d[l].data(), u[l].data());
+            }
+          return;
+        }
+      dg_smoother_step(g, MULT_FWD, relax, d[l].data(), u[l].data());
+      dg_vmult(g, u[l].data(), t[l].data());
+      for (long long i = 0; i < g.N; ++i)
+        t[l][i] = d[l][i] - t[l][i];
+      std::fill(d[l - 1].begin(), d[l - 1].end(), 0.0);
+      dg_transfer(g, false, d[l - 1].data(), t[l].data());
+      vcycle(l - 1);
+      dg_transfer(g, true, u[l - 1].data(), u[l].data());
+      dg_smoother_step(g, variant == 0 ? MULT_REV : MULT_FWD, relax, d[l].data(),
+                       u[l].data());
+    }
+    void
+    apply(const double *r, double *z) const
+    {
+      std::copy(r, r + grids[maxl].N, d[maxl].begin());
+      vcycle(maxl);
+      std::copy(u[maxl].begin(), u[maxl].end(), z);
+    }
+  };
+} // namespace
+
 extern "C"
 {
   int
@@ -1413,6 +1965,212 @@ extern "C"
           return 1;
       }
     return 0;
+  }
+
+  // ---------------------------------------------------------- SIPG-DG
+  // n = cells per direction; pf = penalty factor (sigma = pf / h).
+  long long
+  oracle_dg_n_dofs(int dim, int k, int n)
+  {
+    return DGGrid(dim, k, n, 1.0).N;
+  }
+
+  int
+  oracle_dg_vmult(int dim, int k, int n, double pf, const double *x, double *y)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n < 1)
+      return 1;
+    dg_vmult(DGGrid(dim, k, n, pf), x, y);
+    return 0;
+  }
+
+  // global 1D SIPG matrix (n(k+1))^2 of the unit-scaled form
+  int
+  oracle_dg_global_1d(int k, int n, double pf, double *L)
+  {
+    if (k < 1 || k > MAXK || n < 1)
+      return 1;
+    const std::vector<double> v = dg_global_L(k, n, pf);
+    std::copy(v.begin(), v.end(), L);
+    return 0;
+  }
+
+  int
+  oracle_dg_smoother_step(int dim, int k, int n, double pf, int variant,
+                          double relax, const double *b, double *u)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n < 2 || variant < 0 ||
+        variant > 1)
+      return 1;
+    dg_smoother_step(DGGrid(dim, k, n, pf), variant, relax, b, u);
+    return 0;
+  }
+
+  // fine mesh of n_fine cells (even); coarse has n_fine/2
+  int
+  oracle_dg_prolongate_add(int dim, int k, int n_fine, const double *coarse,
+                           double *fine)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n_fine < 2 || n_fine % 2)
+      return 1;
+    dg_transfer(DGGrid(dim, k, n_fine, 1.0), true, const_cast<double *>(coarse), fine);
+    return 0;
+  }
+
+  int
+  oracle_dg_restrict_add(int dim, int k, int n_fine, const double *fine,
+                         double *coarse)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n_fine < 2 || n_fine % 2)
+      return 1;
+    dg_transfer(DGGrid(dim, k, n_fine, 1.0), false, coarse, const_cast<double *>(fine));
+    return 0;
+  }
+
+  int
+  oracle_dg_vcycle(int dim, int k, int n0, int max_level, double pf,
+                   int mg_variant, double relax, int coarse_sweeps,
+                   const double *r, double *z)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n0 < 2 || max_level < 0)
+      return 1;
+    DGMG mg(dim, k, n0, max_level, pf, mg_variant, relax, coarse_sweeps);
+    mg.apply(r, z);
+    return 0;
+  }
+
+  // PCG (no Dirichlet rows: boundary conditions are weak)
+  int
+  oracle_dg_pcg(int dim, int k, int n0, int max_level, double pf,
+                int mg_variant, double relax, int coarse_sweeps,
+                const double *b, double *x, double rtol, int maxit, int *iters,
+                double *relres, double *history)
+  {
+    if ((dim != 2 && dim != 3) || k < 1 || k > MAXK || n0 < 2 || max_level < 0)
+      return 1;
+    DGMG mg(dim, k, n0, max_level, pf, mg_variant, relax, coarse_sweeps);
+    const DGGrid &g = mg.grids[max_level];
+    const long long N = g.N;
+    std::vector<double> r(b, b + N), z(N), p(N), q(N);
+    std::fill(x, x + N, 0.0);
+    const double r0 = std::sqrt(dot(N, r.data(), r.data()));
+    if (history)
+      history[0] = r0;
+    int    it = 0;
+    double rn = r0;
+    if (r0 == 0.0)
+      {
+        *iters  = 0;
+        *relres = 0.0;
+        return 0;
+      }
+    mg.apply(r.data(), z.data());
+    std::copy(z.begin(), z.end(), p.begin());
+    double rz = dot(N, r.data(), z.data());
+    while (it < maxit)
+      {
+        dg_vmult(g, p.data(), q.data());
+        const double alpha = rz / dot(N, p.data(), q.data());
+        for (long long i = 0; i < N; ++i)
+          {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+          }
+        ++it;
+        rn = std::sqrt(dot(N, r.data(), r.data()));
+        if (history)
+          history[it] = rn;
+        if (!(rn > rtol * r0))
+          break;
+        mg.apply(r.data(), z.data());
+        const double rz_new = dot(N, r.data(), z.data());
+        const double beta   = rz_new / rz;
+        rz                  = rz_new;
+        for (long long i = 0; i < N; ++i)
+          p[i] = z[i] + beta * p[i];
+      }
+    *iters  = it;
+    *relres = rn / r0;
+    return std::isfinite(rn) ? 0 : 2;
+  }
+
+  // load vector of f = c prod_d sin(pi x_d) (c = dim pi^2: u = prod sin(pi
+  // x_d)), kind 1; or f = 1, kind 0; Gauss(k+3) per cell
+  int
+  oracle_dg_rhs(int dim, int k, int n, int kind, double *b)
+  {
+    const DGGrid g(dim, k, n, 1.0);
+    const Basis &bs = basis(k);
+    std::vector<double> qp, qw;
+    gauss_rule(k + 3, qp, qw);
+    const int n1 = k + 1, nq = k + 3;
+    std::vector<double> m(g.N1, 0.0);
+    for (int c = 0; c < n; ++c)
+      for (int i = 0; i < n1; ++i)
+        for (int q = 0; q < nq; ++q)
+          {
+            const double xq = (c + qp[q]) * g.h;
+            const double f1 = kind == 0 ? 1.0 : std::sin(M_PI * xq);
+            m[(long long)c * n1 + i] += qw[q] * g.h * f1 * lagrange(bs.nodes, i, qp[q]);
+          }
+    const double cst = kind == 0 ? 1.0 : dim * M_PI * M_PI;
+    for (long long i = 0; i < g.N; ++i)
+      {
+        const long long X = i % g.N1, Y = (i / g.N1) % g.N1, Z = i / (g.N1 * g.N1);
+        double v = cst * m[X] * m[Y];
+        if (dim == 3)
+          v *= m[Z];
+        b[i] = v;
+      }
+    return 0;
+  }
+
+  // L2 error of the DG function u against prod_d sin(pi x_d), Gauss(k+3)
+  double
+  oracle_dg_l2_error_sine(int dim, int k, int n, const double *u)
+  {
+    const DGGrid g(dim, k, n, 1.0);
+    const Basis &bs = basis(k);
+    std::vector<double> qp, qw;
+    gauss_rule(k + 3, qp, qw);
+    const int nq = k + 3, n1 = k + 1;
+    std::vector<double> phi(nq * n1);
+    for (int q = 0; q < nq; ++q)
+      for (int i = 0; i < n1; ++i)
+        phi[q * n1 + i] = lagrange(bs.nodes, i, qp[q]);
+    double err = 0.0;
+    const int nz = dim == 3 ? n : 1;
+#pragma omp parallel for collapse(2) reduction(+ : err) schedule(static)
+    for (int cz = 0; cz < nz; ++cz)
+      for (int cy = 0; cy < n; ++cy)
+        for (int cx = 0; cx < n; ++cx)
+          for (int qz = 0; qz < (dim == 3 ? nq : 1); ++qz)
+            for (int qy = 0; qy < nq; ++qy)
+              for (int qx = 0; qx < nq; ++qx)
+                {
+                  double uh = 0.0;
+                  for (int l = 0; l < (dim == 3 ? n1 : 1); ++l)
+                    for (int j = 0; j < n1; ++j)
+                      for (int i = 0; i < n1; ++i)
+                        {
+                          double w = phi[qx * n1 + i] * phi[qy * n1 + j];
+                          if (dim == 3)
+                            w *= phi[qz * n1 + l];
+                          uh += w * u[g.idx((long long)cx * n1 + i, (long long)cy * n1 + j,
+                                            dim == 3 ? (long long)cz * n1 + l : 0)];
+                        }
+                  const double x = (cx + qp[qx]) * g.h, y = (cy + qp[qy]) * g.h,
+                               z = (cz + qp[qz]) * g.h;
+                  double ue = std::sin(M_PI * x) * std::sin(M_PI * y);
+                  double w  = qw[qx] * qw[qy] * g.h * g.h;
+                  if (dim == 3)
+                    {
+                      ue *= std::sin(M_PI * z);
+                      w *= qw[qz] * g.h;
+                    }
+                  err += w * (uh - ue) * (uh - ue);
+                }
+    return std::sqrt(err);
   }
 
   // b_i = 2*u01(i) - 1 with u01(i) = (splitmix64(seed ^ i) >> 11) 2^-53;
