@@ -86,6 +86,20 @@ SIGNATURES = {
     "gtps_mg_set_profiling": (_i, [_vp, _i]),
     "gtps_mg_profile": (_i, [_vp, _i, ctypes.POINTER(_ll), _pd]),
     "gtps_mg_profile_reset": (_i, [_vp]),
+    "gtps_dg_level_create": (_i, [_vp, _i, _i, _i, _i, _d, ctypes.POINTER(_vp)]),
+    "gtps_dg_level_destroy": (_i, [_vp]),
+    "gtps_dg_level_n_dofs": (_ll, [_vp]),
+    "gtps_dg_vmult": (_i, [_vp, _vp, _vp]),
+    "gtps_dg_residual": (_i, [_vp, _vp, _vp, _vp]),
+    "gtps_dg_smoother_step": (_i, [_vp, _vp, _vp, _i, _d]),
+    "gtps_dg_prolongate_add": (_i, [_vp, _vp, _vp]),
+    "gtps_dg_restrict_add": (_i, [_vp, _vp, _vp]),
+    "gtps_dg_mg_create": (_i, [_vp, _i, _i, _i, _i, _i, _d, _i, ctypes.POINTER(_vp)]),
+    "gtps_dg_mg_destroy": (_i, [_vp]),
+    "gtps_dg_mg_n_dofs": (_ll, [_vp]),
+    "gtps_dg_mg_matvec": (_i, [_vp, _vp, _vp]),
+    "gtps_dg_mg_vcycle": (_i, [_vp, _vp, _vp]),
+    "gtps_dg_mg_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _pd]),
 }
 
 PROF_GROUPS = {

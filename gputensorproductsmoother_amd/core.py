@@ -520,3 +520,152 @@ class MGSolve(torch.autograd.Function):
             grad_c = torch.empty(solver.n_cells, dtype=torch.float64, device=g.device)
             solver.coefficient_gradient(grad_c, lam, u)
         return None, grad_b, grad_c, None, None
+
+
+# ------------------------------------------------------------------ SIPG-DG
+def dg_penalty(fe_degree: int) -> float:
+    """Default interior-penalty factor 2.5 (k+1)^2 (sigma = factor / h)."""
+    return 2.5 * (fe_degree + 1) ** 2
+
+
+class DGOperator:
+    """SIPG-DG Laplacian of one level: n_cells per direction, FE_DGQ(k), DoFs
+    on one tensor grid X = cx (k+1) + ix (x fastest), weak boundary
+    conditions (no Dirichlet rows)."""
+
+    def __init__(self, ctx: Context, dim: int, fe_degree: int, n_cells: int, dtype=FP64,
+                 penalty: float | None = None):
+        self.ctx, self.dim, self.fe_degree, self.n_cells = ctx, dim, fe_degree, n_cells
+        self.prec = _prec(dtype)
+        self.dtype = _DTYPES[self.prec]
+        self.penalty = dg_penalty(fe_degree) if penalty is None else penalty
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_dg_level_create(ctx.handle, dim, fe_degree, n_cells, self.prec,
+                                              self.penalty, ctypes.byref(h)), "dg_level_create")
+        self._h = h
+        self.n_dofs = int(L.load().gtps_dg_level_n_dofs(h))
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, self.dtype, self.n_dofs)
+        _chk_vec(src, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_dg_vmult(self._h, _ptr(dst), _ptr(src)), "dg_vmult")
+
+    def residual(self, dst: torch.Tensor, b: torch.Tensor, u: torch.Tensor):
+        for t in (dst, b, u):
+            _chk_vec(t, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_dg_residual(self._h, _ptr(dst), _ptr(b), _ptr(u)), "dg_residual")
+
+    def smoother_step(self, u: torch.Tensor, b: torch.Tensor, reverse: bool = False,
+                      relaxation: float = 1.0):
+        _chk_vec(u, self.dtype, self.n_dofs)
+        _chk_vec(b, self.dtype, self.n_dofs)
+        L.check(L.load().gtps_dg_smoother_step(
+            self._h, _ptr(u), _ptr(b), L.GTPS_MULT_REV if reverse else L.GTPS_MULT_FWD,
+            relaxation), "dg_smoother_step")
+
+    def _coarse_n(self):
+        n1 = (self.n_cells // 2) * (self.fe_degree + 1)
+        return n1 ** self.dim
+
+    def prolongate_and_add(self, fine: torch.Tensor, coarse: torch.Tensor):
+        """fine += P coarse (this operator is the fine level)."""
+        _chk_vec(fine, self.dtype, self.n_dofs)
+        _chk_vec(coarse, self.dtype, self._coarse_n())
+        L.check(L.load().gtps_dg_prolongate_add(self._h, _ptr(fine), _ptr(coarse)),
+                "dg_prolongate_add")
+
+    def restrict_and_add(self, coarse: torch.Tensor, fine: torch.Tensor):
+        _chk_vec(fine, self.dtype, self.n_dofs)
+        _chk_vec(coarse, self.dtype, self._coarse_n())
+        L.check(L.load().gtps_dg_restrict_add(self._h, _ptr(coarse), _ptr(fine)),
+                "dg_restrict_add")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_dg_level_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class DGMultigrid:
+    """SIPG-DG multigrid on n_coarse 2^l cells per direction (l = 0..max_level)
+    with the 27-colour vertex-patch smoother and FP64 MG-CG."""
+
+    def __init__(self, ctx: Context, dim: int, fe_degree: int, n_coarse: int, max_level: int,
+                 vcycle_dtype=FP32, penalty: float | None = None, coarse_sweeps: int = 2):
+        self.ctx, self.dim, self.fe_degree = ctx, dim, fe_degree
+        self.n_coarse, self.max_level = n_coarse, max_level
+        self.prec = _prec(vcycle_dtype)
+        self.penalty = dg_penalty(fe_degree) if penalty is None else penalty
+        h = ctypes.c_void_p()
+        L.check(L.load().gtps_dg_mg_create(ctx.handle, dim, fe_degree, n_coarse, max_level,
+                                           self.prec, self.penalty, coarse_sweeps,
+                                           ctypes.byref(h)), "dg_mg_create")
+        self._h = h
+        self.n_dofs = int(L.load().gtps_dg_mg_n_dofs(h))
+
+    def vmult(self, dst: torch.Tensor, src: torch.Tensor):
+        """Preconditioner z = V(r) (FP64 in and out)."""
+        _chk_vec(dst, torch.float64, self.n_dofs)
+        _chk_vec(src, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_dg_mg_vcycle(self._h, _ptr(dst), _ptr(src)), "dg_vcycle")
+
+    def do_matvec(self, dst: torch.Tensor, src: torch.Tensor):
+        _chk_vec(dst, torch.float64, self.n_dofs)
+        _chk_vec(src, torch.float64, self.n_dofs)
+        L.check(L.load().gtps_dg_mg_matvec(self._h, _ptr(dst), _ptr(src)), "dg_matvec")
+
+    def solve_cg(self, b: torch.Tensor, x: torch.Tensor | None = None, rtol: float = 1e-8,
+                 maxit: int = 200, raise_on_failure: bool = True):
+        _chk_vec(b, torch.float64, self.n_dofs)
+        if x is None:
+            x = torch.empty_like(b)
+        _chk_vec(x, torch.float64, self.n_dofs)
+        it = ctypes.c_int()
+        rel = ctypes.c_double()
+        hist = (ctypes.c_double * (maxit + 1))()
+        rc = L.load().gtps_dg_mg_pcg_solve(self._h, _ptr(x), _ptr(b), rtol, maxit,
+                                           ctypes.byref(it), ctypes.byref(rel), hist)
+        if rc != L.GTPS_OK and (raise_on_failure or rc != L.GTPS_ENOTCONV):
+            L.check(rc, "dg_pcg_solve")
+        return x, it.value, rel.value, list(hist[: it.value + 1])
+
+    def solve(self, B: torch.Tensor, rtol: float = 1e-10, maxit: int = 300) -> torch.Tensor:
+        """Differentiable solve of A U = B for one right-hand side (shape
+        (N,)) or several (shape (m, N), solved one after another); the
+        backward pass solves A G_B = G_U with the same MG-CG (A is
+        symmetric and has no constrained rows)."""
+        return DGSolve.apply(self, B, rtol, maxit)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                L.load().gtps_dg_mg_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class DGSolve(torch.autograd.Function):
+    """U = A^{-1} B row by row through DGMultigrid.solve_cg."""
+
+    @staticmethod
+    def _solve_rows(mg: "DGMultigrid", B: torch.Tensor, rtol, maxit):
+        rows = B.reshape(-1, B.shape[-1])
+        out = torch.empty_like(rows)
+        for i in range(rows.shape[0]):
+            mg.solve_cg(rows[i].contiguous(), out[i], rtol=rtol, maxit=maxit)
+        return out.reshape(B.shape)
+
+    @staticmethod
+    def forward(ctx, mg: "DGMultigrid", B: torch.Tensor, rtol: float, maxit: int):
+        ctx.mg, ctx.rtol, ctx.maxit = mg, rtol, maxit
+        return DGSolve._solve_rows(mg, B.detach().to(torch.float64).contiguous(), rtol, maxit)
+
+    @staticmethod
+    def backward(ctx, G: torch.Tensor):
+        gB = DGSolve._solve_rows(ctx.mg, G.detach().to(torch.float64).contiguous(), ctx.rtol,
+                                 ctx.maxit)
+        return None, gB, None, None

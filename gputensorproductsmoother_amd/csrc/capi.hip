@@ -613,6 +613,112 @@ struct gtps_mg : LaunchTimer
   }
 };
 
+namespace
+{
+  gtps_dg_level *
+  make_dg_level(gtps_ctx *ctx, int dim, int degree, int n, int precision,
+                double pf)
+  {
+    GTPS_REQUIRE(dim == 3, "the DG path is 3D");
+    GTPS_REQUIRE(degree >= 1 && degree <= MAX_DEGREE, "degree must be in 1..7");
+    GTPS_REQUIRE(n >= 2 && n <= 4096, "cells per direction must be in 2..4096");
+    GTPS_REQUIRE(precision == GTPS_FP32 || precision == GTPS_FP64,
+                 "precision must be GTPS_FP32 or GTPS_FP64");
+    GTPS_REQUIRE(std::isfinite(pf) && pf > 0.0, "penalty factor must be positive");
+    auto lvl   = std::make_unique<gtps_dg_level>();
+    lvl->ctx   = ctx;
+    lvl->dim   = dim;
+    lvl->k     = degree;
+    lvl->n     = n;
+    lvl->N1    = n * (degree + 1);
+    lvl->N     = (long long)lvl->N1 * lvl->N1 * lvl->N1;
+    lvl->prec  = precision;
+    lvl->h     = 1.0 / n;
+    lvl->scale = lvl->h;  // h^(dim-2)
+    lvl->pf    = pf;
+    dg_level_setup(lvl.get());
+    return lvl.release();
+  }
+} // namespace
+
+// SIPG-DG multigrid: levels 0..maxl with n0 2^l cells per direction;
+// V(1,1) with the 27-colour vertex-patch smoother (forward pre-, reverse
+// post-smoothing), coarse level solved approximately by symmetric sweep
+// pairs from zero; FP64 CG on the finest level.
+struct gtps_dg_mg
+{
+  gtps_ctx                     *ctx = nullptr;
+  int                           dim = 3, k = 1, n0 = 3, maxl = 0;
+  int                           prec = GTPS_FP32, coarse_sweeps = 2;
+  double                        pf = 1.0, relax = 1.0;
+  std::vector<gtps_dg_level *>  lv;
+  gtps_dg_level                *fine64     = nullptr;
+  bool                          own_fine64 = false;
+  std::vector<void *>           d, u, t;
+  double                       *r = nullptr, *p = nullptr, *q = nullptr;
+  hipEvent_t                    ev = nullptr;
+
+  void
+  vcycle(int l, const void *dl, void *ul)
+  {
+    const gtps_dg_level *L = lv[l];
+    launch_fill(ctx->stream, L->N, prec, ul, 0.0);
+    if (l == 0)
+      {
+        for (int s = 0; s < coarse_sweeps; ++s)
+          {
+            launch_dg_sweep(L, ul, dl, GTPS_MULT_FWD, 1.0);
+            launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, 1.0);
+          }
+        return;
+      }
+    launch_dg_sweep(L, ul, dl, GTPS_MULT_FWD, relax);
+    launch_dg_apply(L, t[l], ul, dl, false);  // t = d - A u
+    launch_dg_restrict(L, d[l - 1], t[l], true);
+    vcycle(l - 1, d[l - 1], u[l - 1]);
+    launch_dg_prolongate_add(L, ul, u[l - 1]);
+    launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, relax);
+  }
+
+  // z = V(r); rz_old = rz; rz = r.z; beta; p = z + beta p
+  void
+  direction()
+  {
+    hipStream_t     s   = ctx->stream;
+    const long long N   = fine64->N;
+    const bool      f32 = prec == GTPS_FP32;
+    double         *sc  = ctx->ws.scalars;
+    double         *pt  = ctx->ws.partials;
+    float          *z32 = f32 ? (float *)u[maxl] : nullptr;
+    double         *z64 = f32 ? nullptr : (double *)u[maxl];
+    if (f32)
+      vcycle(maxl, d[maxl], z32);
+    else
+      vcycle(maxl, r, z64);
+    const int nb = launch_dot_partials(s, N, r, z64, z32, pt, ctx->ws.capacity);
+    launch_finalize_op(s, pt, nb, sc, cg::RZ, cg::OP_SHIFT_BETA);
+    launch_cg_update_p(s, N, p, z64, z32, sc);
+  }
+
+  ~gtps_dg_mg()
+  {
+    for (size_t l = 0; l < lv.size(); ++l)
+      {
+        dfree(d.size() > l ? d[l] : nullptr);
+        dfree(u.size() > l ? u[l] : nullptr);
+        dfree(t.size() > l ? t[l] : nullptr);
+        delete lv[l];
+      }
+    if (own_fine64)
+      delete fine64;
+    dfree(r);
+    dfree(p);
+    dfree(q);
+    if (ev)
+      (void)hipEventDestroy(ev);
+  }
+};
+
 extern "C"
 {
   const char *
@@ -1462,6 +1568,294 @@ extern "C"
       rel = rn / r0;
       GTPS_HIP(hipStreamSynchronize(s));
       mg->collect();
+    });
+    if (iters)
+      *iters = it_done;
+    if (relres)
+      *relres = rel;
+    if (rc != GTPS_OK)
+      return rc;
+    if (nonfinite)
+      {
+        g_last_error = "non-finite residual in CG";
+        return GTPS_ENAN;
+      }
+    if (!converged)
+      {
+        g_last_error = "CG reached maxit without convergence";
+        return GTPS_ENOTCONV;
+      }
+    return GTPS_OK;
+  }
+
+  // ------------------------------------------------------------- SIPG-DG
+  int
+  gtps_dg_level_create(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                       int precision, double penalty, gtps_dg_level **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && out, "NULL argument");
+      set_device(ctx);
+      *out = make_dg_level(ctx, dim, degree, n_cells, precision, penalty);
+    });
+  }
+
+  int
+  gtps_dg_level_destroy(gtps_dg_level *lvl)
+  {
+    return guard([&] { delete lvl; });
+  }
+
+  long long
+  gtps_dg_level_n_dofs(const gtps_dg_level *lvl)
+  {
+    return lvl ? lvl->N : -1;
+  }
+
+  int
+  gtps_dg_vmult(gtps_dg_level *lvl, void *dst, const void *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && dst && src && dst != src, "bad argument");
+      set_device(lvl->ctx);
+      launch_dg_apply(lvl, dst, src, nullptr, false);
+    });
+  }
+
+  int
+  gtps_dg_residual(gtps_dg_level *lvl, void *dst, const void *b, const void *u)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && dst && b && u && dst != u, "bad argument");
+      set_device(lvl->ctx);
+      launch_dg_apply(lvl, dst, u, b, false);
+    });
+  }
+
+  int
+  gtps_dg_smoother_step(gtps_dg_level *lvl, void *u, const void *b, int variant,
+                        double relaxation)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(lvl && u && b && u != b, "bad argument");
+      GTPS_REQUIRE(variant == GTPS_MULT_FWD || variant == GTPS_MULT_REV,
+                   "the DG smoother is multiplicative (forward or reverse)");
+      GTPS_REQUIRE(dg_supported(lvl->k, lvl->prec),
+                   "the DG smoother supports degree <= 5 in FP64 and <= 7 in FP32");
+      set_device(lvl->ctx);
+      launch_dg_sweep(lvl, u, b, variant, relaxation);
+    });
+  }
+
+  int
+  gtps_dg_prolongate_add(gtps_dg_level *fine_lvl, void *fine, const void *coarse)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(fine_lvl && fine && coarse, "NULL argument");
+      set_device(fine_lvl->ctx);
+      launch_dg_prolongate_add(fine_lvl, fine, coarse);
+    });
+  }
+
+  int
+  gtps_dg_restrict_add(gtps_dg_level *fine_lvl, void *coarse, const void *fine)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(fine_lvl && fine && coarse, "NULL argument");
+      set_device(fine_lvl->ctx);
+      launch_dg_restrict(fine_lvl, coarse, fine, false);
+    });
+  }
+
+  int
+  gtps_dg_mg_create(gtps_ctx *ctx, int dim, int degree, int n_coarse,
+                    int max_level, int vcycle_precision, double penalty,
+                    int coarse_sweeps, gtps_dg_mg **out)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && out, "NULL argument");
+      GTPS_REQUIRE(max_level >= 0 && max_level <= 10, "max_level must be in 0..10");
+      GTPS_REQUIRE(coarse_sweeps >= 1 && coarse_sweeps <= 64,
+                   "coarse_sweeps must be in 1..64");
+      GTPS_REQUIRE(dg_supported(degree, vcycle_precision),
+                   "the DG smoother supports degree <= 5 in FP64 and <= 7 in FP32");
+      GTPS_REQUIRE((long long)n_coarse << max_level <= 4096, "mesh too large");
+      set_device(ctx);
+      auto mg           = std::make_unique<gtps_dg_mg>();
+      mg->ctx           = ctx;
+      mg->dim           = dim;
+      mg->k             = degree;
+      mg->n0            = n_coarse;
+      mg->maxl          = max_level;
+      mg->prec          = vcycle_precision;
+      mg->pf            = penalty;
+      mg->coarse_sweeps = coarse_sweeps;
+      mg->lv.assign(max_level + 1, nullptr);
+      mg->d.assign(max_level + 1, nullptr);
+      mg->u.assign(max_level + 1, nullptr);
+      mg->t.assign(max_level + 1, nullptr);
+      const size_t es = vcycle_precision == GTPS_FP32 ? sizeof(float) : sizeof(double);
+      for (int l = 0; l <= max_level; ++l)
+        {
+          mg->lv[l] = make_dg_level(ctx, dim, degree, n_coarse << l,
+                                    vcycle_precision, penalty);
+          mg->d[l]  = dmalloc(es * mg->lv[l]->N);
+          mg->u[l]  = dmalloc(es * mg->lv[l]->N);
+          mg->t[l]  = dmalloc(es * mg->lv[l]->N);
+        }
+      if (vcycle_precision == GTPS_FP64)
+        mg->fine64 = mg->lv[max_level];
+      else
+        {
+          mg->fine64 = make_dg_level(ctx, dim, degree, n_coarse << max_level,
+                                     GTPS_FP64, penalty);
+          mg->own_fine64 = true;
+        }
+      const long long N = mg->fine64->N;
+      mg->r = (double *)dmalloc(sizeof(double) * N);
+      mg->p = (double *)dmalloc(sizeof(double) * N);
+      mg->q = (double *)dmalloc(sizeof(double) * N);
+      GTPS_HIP(hipEventCreateWithFlags(&mg->ev, hipEventDisableTiming));
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+      *out = mg.release();
+    });
+  }
+
+  int
+  gtps_dg_mg_destroy(gtps_dg_mg *mg)
+  {
+    return guard([&] {
+      if (!mg)
+        return;
+      (void)hipSetDevice(mg->ctx->device);
+      (void)hipStreamSynchronize(mg->ctx->stream);
+      delete mg;
+    });
+  }
+
+  long long
+  gtps_dg_mg_n_dofs(const gtps_dg_mg *mg)
+  {
+    return mg ? mg->fine64->N : -1;
+  }
+
+  int
+  gtps_dg_mg_matvec(gtps_dg_mg *mg, double *dst, const double *src)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && dst && src && (const void *)dst != (const void *)src,
+                   "bad argument");
+      set_device(mg->ctx);
+      launch_dg_apply(mg->fine64, dst, src, nullptr, false);
+    });
+  }
+
+  int
+  gtps_dg_mg_vcycle(gtps_dg_mg *mg, double *z, const double *r)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && z && r && (const void *)z != (const void *)r, "bad argument");
+      set_device(mg->ctx);
+      const long long N = mg->fine64->N;
+      if (mg->prec == GTPS_FP32)
+        {
+          launch_convert_f64_f32(mg->ctx->stream, N, r, (float *)mg->d[mg->maxl]);
+          mg->vcycle(mg->maxl, mg->d[mg->maxl], mg->u[mg->maxl]);
+          launch_convert_f32_f64(mg->ctx->stream, N, (const float *)mg->u[mg->maxl], z);
+        }
+      else
+        mg->vcycle(mg->maxl, r, z);
+    });
+  }
+
+  int
+  gtps_dg_mg_pcg_solve(gtps_dg_mg *mg, double *x, const double *b, double rtol,
+                       int maxit, int *iters, double *relres, double *history)
+  {
+    int    it_done = 0;
+    double rel     = 0.0;
+    bool   converged = false, nonfinite = false;
+    int rc = guard([&] {
+      GTPS_REQUIRE(mg && x && b && (const void *)x != (const void *)b, "bad argument");
+      GTPS_REQUIRE(maxit >= 0 && rtol >= 0.0, "bad rtol / maxit");
+      set_device(mg->ctx);
+      gtps_ctx       *ctx = mg->ctx;
+      hipStream_t     s   = ctx->stream;
+      const long long N   = mg->fine64->N;
+      double         *sc  = ctx->ws.scalars;
+      double         *pt  = ctx->ws.partials;
+      const int       cap = ctx->ws.capacity;
+      const bool      f32 = mg->prec == GTPS_FP32;
+      float          *r32 = f32 ? (float *)mg->d[mg->maxl] : nullptr;
+      auto read_rr = [&]() {
+        GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, sc + cg::RR, sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        GTPS_HIP(hipEventRecord(mg->ev, s));
+      };
+      auto wait_rr = [&]() {
+        GTPS_HIP(hipEventSynchronize(mg->ev));
+        return std::sqrt(ctx->ws.pinned[0]);
+      };
+      // x0 = 0, r0 = b (no Dirichlet rows: boundary conditions are weak)
+      launch_fill(s, N, GTPS_FP64, x, 0.0);
+      GTPS_HIP(hipMemcpyAsync(mg->r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+      {
+        const int nb = launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap);
+        launch_finalize_sum(s, pt, nb, sc, cg::RR);
+      }
+      if (f32)
+        launch_convert_f64_f32(s, N, mg->r, r32);
+      launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
+      launch_cg_scalar(s, sc, cg::OP_INIT);
+      read_rr();
+      const double r0 = wait_rr();
+      if (history)
+        history[0] = r0;
+      if (!std::isfinite(r0))
+        {
+          nonfinite = true;
+          return;
+        }
+      if (r0 == 0.0)
+        {
+          converged = true;
+          return;
+        }
+      mg->direction();
+      double rn = r0, rn_prev = -1.0;
+      while (it_done < maxit)
+        {
+          const int nbq = launch_dg_apply(mg->fine64, mg->q, mg->p, nullptr, true);
+          launch_finalize_op(s, pt, nbq, sc, cg::PQ, cg::OP_ALPHA);
+          const int nbr =
+            launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap);
+          launch_finalize_sum(s, pt, nbr, sc, cg::RR);
+          read_rr();
+          ++it_done;
+          bool speculate = it_done < maxit;
+          if (speculate && rn_prev > 0.0 && rn < rn_prev)
+            speculate = rn * (rn / rn_prev) > 0.25 * rtol * r0;
+          if (speculate)
+            mg->direction();
+          rn_prev = rn;
+          rn      = wait_rr();
+          if (history)
+            history[it_done] = rn;
+          if (!std::isfinite(rn))
+            {
+              nonfinite = true;
+              break;
+            }
+          if (rn <= rtol * r0)
+            {
+              converged = true;
+              break;
+            }
+          if (!speculate && it_done < maxit)
+            mg->direction();
+        }
+      rel = rn / r0;
+      GTPS_HIP(hipStreamSynchronize(s));
     });
     if (iters)
       *iters = it_done;

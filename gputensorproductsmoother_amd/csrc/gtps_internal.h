@@ -120,6 +120,32 @@ struct gtps_smoother
   void       *resid    = nullptr;  // additive variant: global residual
 };
 
+// SIPG-DG level (3D): n cells per direction, DoFs on one tensor grid
+// X = cx (k+1) + ix, index X + N1 Y + N1^2 Z.
+struct gtps_dg_level
+{
+  gtps_ctx *ctx = nullptr;
+  int       dim = 3, k = 1, n = 2, N1 = 4;
+  long long N   = 0;
+  int       prec = GTPS_FP64;
+  double    h = 0.5, scale = 0.5, pf = 1.0;
+  void     *tab = nullptr;                     // packed DG table (prec)
+  void     *t[4] = {nullptr, nullptr, nullptr, nullptr};  // operator scratch
+  ~gtps_dg_level()
+  {
+    if (tab)
+      (void)hipFree(tab);
+    for (void *p : t)
+      if (p)
+        (void)hipFree(p);
+  }
+  size_t
+  elem() const
+  {
+    return prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
+  }
+};
+
 struct gtps_transfer
 {
   gtps_ctx *ctx = nullptr;
@@ -157,6 +183,15 @@ namespace gtps
 
   // setup.hip
   DegreeTables *get_tables(gtps_ctx *ctx, int k);
+  // values (and derivatives, der may be NULL) of the Lagrange basis on
+  // `nodes` at x
+  void lagrange_values(const std::vector<double> &nodes, double x,
+                       double *val, double *der);
+  // FP64 generalized eigenpairs K s = lam M s (n <= 16) on the GPU;
+  // S column j = eigenvector j, S^T M S = I, ascending eigenvalues
+  void gpu_gen_eig(gtps_ctx *ctx, int n, const std::vector<double> &K,
+                   const std::vector<double> &M, std::vector<double> &lam,
+                   std::vector<double> &S);
 
   // laplace.hip: y = A x (resid: y = b - A x); optional fused partial dot
   // sum_i x_i y_i written per block to ctx->ws.partials; returns #blocks.
@@ -178,6 +213,21 @@ namespace gtps
   void launch_coarsen_coefficient(hipStream_t s, int nc, const double *fine,
                                   double *coarse);
   bool coeff_smoother_supported(int k, int prec);
+
+  // dg.hip: SIPG-DG level set-up and kernels
+  void dg_level_setup(gtps_dg_level *lvl);  // tables + scratch
+  bool dg_supported(int k, int prec);
+  // y = A x (b != NULL: y = b - A x); fused partial dot x.y -> #blocks
+  int  launch_dg_apply(const gtps_dg_level *lvl, void *y, const void *x,
+                       const void *b, bool fused_dot);
+  // one sweep over the 27 colours (variant GTPS_MULT_FWD / _REV)
+  void launch_dg_sweep(const gtps_dg_level *lvl, void *u, const void *b,
+                       int variant, double relax);
+  // fine += P coarse / coarse (+)= P^T fine (store: coarse = P^T fine)
+  void launch_dg_prolongate_add(const gtps_dg_level *fine_lvl, void *fine,
+                                const void *coarse);
+  void launch_dg_restrict(const gtps_dg_level *fine_lvl, void *coarse,
+                          const void *fine, bool store);
 
   // smoother.hip: one colour pass. mode 0 multiplicative (src = b),
   // mode 1 additive (src = global residual), mode 2 multiplicative on

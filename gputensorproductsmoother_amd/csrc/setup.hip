@@ -432,6 +432,51 @@ namespace gtps
     return b;
   }
 
+  void
+  lagrange_values(const std::vector<double> &nodes, double x, double *val,
+                  double *der)
+  {
+    lagrange_eval(nodes, x, val, der);
+  }
+
+  void
+  gpu_gen_eig(gtps_ctx *ctx, int n, const std::vector<double> &K,
+              const std::vector<double> &M, std::vector<double> &lam,
+              std::vector<double> &S)
+  {
+    GTPS_REQUIRE(n >= 1 && n <= 16, "eigenproblem size must be in 1..16");
+    double *dbuf = nullptr;
+    GTPS_HIP(hipMalloc(&dbuf, sizeof(double) * (3 * n * n + n)));
+    double *dK = dbuf, *dM = dbuf + n * n, *dS = dbuf + 2 * n * n,
+           *dl = dbuf + 3 * n * n;
+    try
+      {
+        GTPS_HIP(hipMemcpyAsync(dK, K.data(), sizeof(double) * n * n,
+                                hipMemcpyHostToDevice, ctx->stream));
+        GTPS_HIP(hipMemcpyAsync(dM, M.data(), sizeof(double) * n * n,
+                                hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(gen_eig_kernel, dim3(1), dim3(64), 0, ctx->stream,
+                           n, dK, dM, dl, dS);
+        GTPS_LAUNCH_CHECK();
+        lam.resize(n);
+        S.resize(n * n);
+        GTPS_HIP(hipMemcpyAsync(lam.data(), dl, sizeof(double) * n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+        GTPS_HIP(hipMemcpyAsync(S.data(), dS, sizeof(double) * n * n,
+                                hipMemcpyDeviceToHost, ctx->stream));
+        GTPS_HIP(hipStreamSynchronize(ctx->stream));
+      }
+    catch (...)
+      {
+        (void)hipFree(dbuf);
+        throw;
+      }
+    GTPS_HIP(hipFree(dbuf));
+    for (double v : lam)
+      GTPS_REQUIRE(std::isfinite(v) && v > 0.0,
+                   "patch eigenproblem produced a non-positive eigenvalue");
+  }
+
   DegreeTables *
   get_tables(gtps_ctx *ctx, int k)
   {

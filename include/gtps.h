@@ -53,6 +53,8 @@ typedef struct gtps_level gtps_level;
 typedef struct gtps_smoother gtps_smoother;
 typedef struct gtps_transfer gtps_transfer;
 typedef struct gtps_mg gtps_mg;
+typedef struct gtps_dg_level gtps_dg_level;
+typedef struct gtps_dg_mg gtps_dg_mg;
 
 /* ---------------------------------------------------------------- context */
 /* Device selection and stream; replaces the reference's implicit CUDA
@@ -298,6 +300,55 @@ int gtps_mg_set_profiling(gtps_mg *mg, int mask);
 int gtps_mg_profile(gtps_mg *mg, int group, long long *count,
                     double *total_ms);
 int gtps_mg_profile_reset(gtps_mg *mg);
+
+/* ============================================================= SIPG-DG */
+/* Symmetric interior penalty DG Laplacian, FE_DGQ(k) on Gauss-Lobatto
+ * points, uniform mesh of n cells per direction on the unit cube (3D).  The
+ * form is the reference's matrix-free DG test operator (tests/mf_dg.cu:
+ * 63-274: cell term, interior faces (sigma [u] - {du/dn}) [v] - [u]{dv/dn},
+ * boundary faces (2 sigma u - du/dn) v - u dv/dn) with sigma = penalty / h;
+ * the reference uses penalty = k (k+1) there and has no DG smoother or
+ * solver, so every entry point below is an addition.  Vectors hold
+ * (n (k+1))^3 values on one tensor grid: X = cx (k+1) + ix, index
+ * X + N1 Y + N1^2 Z with N1 = n (k+1) (x fastest); there are no Dirichlet
+ * rows (boundary conditions are weak). */
+int gtps_dg_level_create(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                         int precision, double penalty, gtps_dg_level **out);
+int gtps_dg_level_destroy(gtps_dg_level *lvl);
+long long gtps_dg_level_n_dofs(const gtps_dg_level *lvl);
+/* dst = A src; dst = b - A u */
+int gtps_dg_vmult(gtps_dg_level *lvl, void *dst, const void *src);
+int gtps_dg_residual(gtps_dg_level *lvl, void *dst, const void *b,
+                     const void *u);
+/* One multiplicative sweep of the vertex-patch smoother (patch = the 2^3
+ * cells around an interior vertex, all their DoFs; 27 colours = patch index
+ * mod 3 per direction; exact local solves by fast diagonalisation).
+ * variant GTPS_MULT_FWD or GTPS_MULT_REV; u <- (1-w) u + w u_P per patch. */
+int gtps_dg_smoother_step(gtps_dg_level *lvl, void *u, const void *b,
+                          int variant, double relaxation);
+/* Cell-wise embedding from n/2 to n cells (fine_lvl is the fine level):
+ * fine += P coarse; coarse += P^T fine. */
+int gtps_dg_prolongate_add(gtps_dg_level *fine_lvl, void *fine,
+                           const void *coarse);
+int gtps_dg_restrict_add(gtps_dg_level *fine_lvl, void *coarse,
+                         const void *fine);
+
+/* DG multigrid on n_coarse 2^l cells per direction, l = 0..max_level:
+ * V(1,1) (forward pre-, reverse post-smoothing) in vcycle_precision, the
+ * coarsest level solved approximately by coarse_sweeps symmetric sweep pairs
+ * from zero; FP64 CG on the finest level. */
+int gtps_dg_mg_create(gtps_ctx *ctx, int dim, int degree, int n_coarse,
+                      int max_level, int vcycle_precision, double penalty,
+                      int coarse_sweeps, gtps_dg_mg **out);
+int gtps_dg_mg_destroy(gtps_dg_mg *mg);
+long long gtps_dg_mg_n_dofs(const gtps_dg_mg *mg);
+int gtps_dg_mg_matvec(gtps_dg_mg *mg, double *dst, const double *src);
+int gtps_dg_mg_vcycle(gtps_dg_mg *mg, double *z, const double *r);
+/* x = A^{-1} b by MG-preconditioned FP64 CG from x = 0 (same contract as
+ * gtps_mg_pcg_solve). */
+int gtps_dg_mg_pcg_solve(gtps_dg_mg *mg, double *x, const double *b,
+                         double rtol, int maxit, int *iterations,
+                         double *relres, double *history);
 
 #ifdef __cplusplus
 }
