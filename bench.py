@@ -177,11 +177,70 @@ def config_report(args):
         elif name == "c4":
             line = c4_report(ctx, core, problems, reps, solve_pair, rel_l2)
         elif name == "c5":
-            line = {"config": "C5", "status": "not built: the SIPG-DG operator and DG "
-                    "vertex-patch smoother are outside this build so far"}
+            line = c5_report(ctx, core, problems, reps, rel_l2)
         else:
             raise SystemExit(f"unknown config {name}")
         print(json.dumps(line), flush=True)
+
+
+def c5_report(ctx, core, problems, reps, rel_l2):
+    """SIPG-DG workload: Q4 DG on 24^3 cells (3^3 coarse mesh, 3 refinements),
+    penalty 2.5 (k+1)^2, random f, FP32 V-cycle levels or FP64 everywhere,
+    FP64 CG to 1e-8; then J = 1/2 sum_m ||U_m||^2 over 4 right-hand sides with
+    dJ/dB = A^{-1} U through the autograd Function, checked by a central
+    difference along a random direction (J is quadratic in B, so the
+    difference is exact up to the solver tolerance)."""
+    import torch
+    dev = "cuda:0"
+    k, n0, maxl = 4, 3, 3
+    n = n0 << maxl
+    B = torch.tensor(problems.rhs_dg_random(3, k, n, batch=4), device=dev)
+    b = B[0].contiguous()
+    ref = core.DGMultigrid(ctx, 3, k, n0, maxl, torch.float64)
+    xref, _, _, _ = ref.solve_cg(b, rtol=1e-13, maxit=500, raise_on_failure=False)
+    xref = xref.clone()
+    res = {}
+    for prec in ("fp32", "fp64"):
+        mg = core.DGMultigrid(ctx, 3, k, n0, maxl,
+                              torch.float32 if prec == "fp32" else torch.float64)
+        x = torch.empty_like(b)
+        info = {}
+
+        def run():
+            _, it, rel, _ = mg.solve_cg(b, x, rtol=1e-8, maxit=500)
+            info.update(iterations=it, relres=rel)
+        ms = _best_ms(run, reps)
+        res[prec] = {"iterations": info["iterations"], "relres": info["relres"],
+                     "ms_per_solve": round(ms, 3), "MDoF_s": round(b.numel() / ms / 1e3, 2),
+                     "rel_l2_vs_fp64_tight": rel_l2(x, xref)}
+        del mg
+    mg = core.DGMultigrid(ctx, 3, k, n0, maxl, torch.float32)
+
+    def fwd_bwd(rtol):
+        Bv = B.clone().requires_grad_(True)
+        U = mg.solve(Bv, rtol=rtol)
+        (0.5 * (U * U).sum()).backward()
+        return Bv.grad
+
+    ms_fb = _best_ms(lambda: fwd_bwd(1e-8), reps)
+    grad = fwd_bwd(1e-12)
+    D = torch.tensor(problems.rhs_dg_random(3, k, n, batch=4, seed=problems.SEED + 1),
+                     device=dev)
+    h = 1e-3
+
+    def J(Bv):
+        U = core.DGSolve._solve_rows(mg, Bv.contiguous(), 1e-12, 500)
+        return 0.5 * float((U * U).sum())
+
+    fd = (J(B + h * D) - J(B - h * D)) / (2 * h)
+    an = float((grad * D).sum())
+    return {"config": "C5", "workload": "3D SIPG-DG Q4 Laplace, 24^3 cells (3^3 coarse, "
+            "3 refinements), penalty 2.5 (k+1)^2, random f, 27-colour vertex-patch "
+            "smoother, CG to 1e-8", "n_dofs": b.numel(), **res,
+            "batched_adjoint": {"rhs": 4, "forward_plus_adjoint_ms": round(ms_fb, 3),
+                                "solves": "one after another (8 MG-CG solves)",
+                                "fd_direction_h": h,
+                                "rel_dev_adjoint_vs_fd": abs(an - fd) / abs(fd)}}
 
 
 def c4_report(ctx, core, problems, reps, solve_pair, rel_l2):

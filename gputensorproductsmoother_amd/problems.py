@@ -11,7 +11,10 @@ Every generator derives from SplitMix64, u01(j) = (splitmix64(seed ^ j) >>
 * ``coeff_problem``: per-cell coefficient c_K = exp(sigma g_K) with g_K
   standard normal by Box-Muller on stream 2^32, target u_d,i = 0.01 u01(2^33
   + i) on interior DoFs, and distinct finite-difference cells from stream
-  2^34 (the variable-coefficient adjoint problem).
+  2^34 (the variable-coefficient adjoint problem);
+* ``rhs_dg_random``: b_i = 2 u01(i + m 2^36) - 1 on every DoF of a DG
+  space (weak boundary conditions, no constrained rows), m = 0, 1, ... for
+  the right-hand sides of a batch.
 """
 from __future__ import annotations
 
@@ -92,3 +95,11 @@ def coeff_problem(dim: int, k: int, level: int, seed: int = SEED, sigma: float =
             cells.append(cell)
         j += 1
     return coef, ud, np.array(cells, dtype=np.int64)
+
+
+def rhs_dg_random(dim: int, k: int, n_cells: int, batch: int = 1,
+                  seed: int = SEED) -> np.ndarray:
+    """Random DG right-hand sides, shape (batch, (n_cells (k+1))^dim)."""
+    nd = (n_cells * (k + 1)) ** dim
+    i = np.arange(nd, dtype=np.uint64)
+    return np.stack([2.0 * u01(i + np.uint64(m << 36), seed) - 1.0 for m in range(batch)])
