@@ -30,6 +30,9 @@ GTPS_MG_SYMMETRIC = 0
 GTPS_MG_REFERENCE_ORDER = 1
 GTPS_MG_ADDITIVE = 2
 
+GTPS_LAYOUT_CG = 0
+GTPS_LAYOUT_DG = 1
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -100,6 +103,17 @@ SIGNATURES = {
     "gtps_dg_mg_matvec": (_i, [_vp, _vp, _vp]),
     "gtps_dg_mg_vcycle": (_i, [_vp, _vp, _vp]),
     "gtps_dg_mg_pcg_solve": (_i, [_vp, _vp, _vp, _d, _i, _pi, _pd, _pd]),
+    "gtps_assemble_rhs": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "gtps_rhs_sine": (_i, [_vp, _i, _i, _i, _d, _vp]),
+    "gtps_rhs_constant": (_i, [_vp, _i, _i, _i, _d, _vp]),
+    "gtps_dg_assemble_rhs": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "gtps_dg_rhs_sine": (_i, [_vp, _i, _i, _i, _d, _vp]),
+    "gtps_error_norms": (_i, [_vp, _i, _i, _i, _d, _i, _vp, _pd, _pd]),
+    "gtps_dg_error_norms": (_i, [_vp, _i, _i, _i, _d, _i, _vp, _pd, _pd]),
+    "gtps_vector_save": (_i, [_vp, ctypes.c_char_p, _vp, _i, _i, _i, _i, _i]),
+    "gtps_vector_info": (_i, [ctypes.c_char_p, _pi, _pi, _pi, _pi, _pi,
+                              ctypes.POINTER(_ll)]),
+    "gtps_vector_load": (_i, [_vp, ctypes.c_char_p, _vp, _ll, _i]),
 }
 
 PROF_GROUPS = {

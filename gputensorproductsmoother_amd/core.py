@@ -6,6 +6,10 @@ Class and method names follow the reference (PSMF namespace):
 * :class:`PatchSmoother`    - include/patch_smoother.cuh:86-124, 313-335
 * :class:`MGTransferCUDA`   - include/cuda_mg_transfer.cuh:57-358
 * :class:`MultigridSolvers` - include/solver.cuh:1541-2210
+* :func:`rhs_sine`, :func:`assemble_rhs` - LaplaceOperator::compute_residual
+  (include/laplace_operator.cuh:119-173) with apps/poisson.cu:76-87
+* :func:`error_norms` - LaplaceProblem::compute_error (apps/poisson.cu:353-382)
+* :class:`DGOperator`, :class:`DGMultigrid` - SIPG-DG (tests/mf_dg.cu:63-274)
 
 Vectors are 1D ``torch`` tensors on the context's device (float32 for FP32
 objects, float64 for FP64), lexicographic with boundary DoFs included.  All
@@ -669,3 +673,124 @@ class DGSolve(torch.autograd.Function):
         gB = DGSolve._solve_rows(ctx.mg, G.detach().to(torch.float64).contiguous(), ctx.rtol,
                                  ctx.maxit)
         return None, gB, None, None
+
+
+
+# ------------------------------------------------------------ post-processing
+def gauss_points(n_q: int):
+    """(points, weights) of the n_q-point Gauss-Legendre rule on [0, 1]."""
+    import numpy as np
+    x, w = np.polynomial.legendre.leggauss(n_q)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def quadrature_points_1d(fe_degree: int, n_cells: int):
+    """Coordinates of the QGauss(k+1) points along one direction, shape
+    (n_cells, k+1): f_q of :func:`assemble_rhs` is f at the tensor products
+    of these, ordered [cz][cy][cx][qz][qy][qx] (x fastest)."""
+    import numpy as np
+    q, _ = gauss_points(fe_degree + 1)
+    return (np.arange(n_cells)[:, None] + q[None, :]) / n_cells
+
+
+def _n_dofs(dim, k, n_cells, dg):
+    n1 = n_cells * (k + 1) if dg else k * n_cells + 1
+    return n1 ** dim
+
+
+def _rhs_call(fn, ctx, dim, k, mesh, dg, *args, out=None):
+    """mesh: 2^level cells (CG entry points take the level) or n_cells (DG)."""
+    N = _n_dofs(dim, k, mesh if dg else 1 << mesh, dg)
+    b = torch.empty(N, dtype=torch.float64, device=ctx.device) if out is None else out
+    _chk_vec(b, torch.float64, N)
+    L.check(fn(ctx.handle, dim, k, mesh, *args, _ptr(b)), fn.__name__)
+    return b
+
+
+def assemble_rhs(ctx: Context, dim: int, fe_degree: int, level: int, f_q: torch.Tensor,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """b_i = int f phi_i (QGauss(k+1)) on the Q_k space with 2^level cells per
+    direction from f at the quadrature points (float64 device tensor of
+    (2^level)^dim (k+1)^dim values); Dirichlet rows zero."""
+    n = 1 << level
+    _chk_vec(f_q, torch.float64, n ** dim * (fe_degree + 1) ** dim)
+    return _rhs_call(L.load().gtps_assemble_rhs, ctx, dim, fe_degree, level, False,
+                     _ptr(f_q), out=out)
+
+
+def rhs_sine(ctx: Context, dim: int, fe_degree: int, level: int, wave: float = 3.0,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+    """The reference Poisson problem's load vector, f = dim (w pi)^2 prod
+    sin(w pi x_d) (apps/poisson.cu:76-87), solution prod sin(w pi x_d)."""
+    return _rhs_call(L.load().gtps_rhs_sine, ctx, dim, fe_degree, level, False,
+                     ctypes.c_double(wave), out=out)
+
+
+def rhs_constant(ctx: Context, dim: int, fe_degree: int, level: int, value: float = 1.0,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    return _rhs_call(L.load().gtps_rhs_constant, ctx, dim, fe_degree, level, False,
+                     ctypes.c_double(value), out=out)
+
+
+def dg_assemble_rhs(ctx: Context, dim: int, fe_degree: int, n_cells: int, f_q: torch.Tensor,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+    _chk_vec(f_q, torch.float64, n_cells ** dim * (fe_degree + 1) ** dim)
+    return _rhs_call(L.load().gtps_dg_assemble_rhs, ctx, dim, fe_degree, n_cells, True,
+                     _ptr(f_q), out=out)
+
+
+def dg_rhs_sine(ctx: Context, dim: int, fe_degree: int, n_cells: int, wave: float = 1.0,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+    return _rhs_call(L.load().gtps_dg_rhs_sine, ctx, dim, fe_degree, n_cells, True,
+                     ctypes.c_double(wave), out=out)
+
+
+def _norms(fn, ctx, dim, k, mesh, dg, u, wave, n_q):
+    _chk_vec(u, torch.float64, _n_dofs(dim, k, mesh if dg else 1 << mesh, dg))
+    l2, h1 = ctypes.c_double(), ctypes.c_double()
+    L.check(fn(ctx.handle, dim, k, mesh, wave, k + 1 if n_q is None else n_q, _ptr(u),
+               ctypes.byref(l2), ctypes.byref(h1)), fn.__name__)
+    return l2.value, h1.value
+
+
+def error_norms(ctx: Context, dim: int, fe_degree: int, level: int, u: torch.Tensor,
+                wave: float = 3.0, n_q: int | None = None):
+    """(L2 error, H1-seminorm error) of u against prod sin(w pi x_d) with
+    QGauss(n_q), n_q = k+1 by default as in the reference."""
+    return _norms(L.load().gtps_error_norms, ctx, dim, fe_degree, level, False, u, wave, n_q)
+
+
+def dg_error_norms(ctx: Context, dim: int, fe_degree: int, n_cells: int, u: torch.Tensor,
+                   wave: float = 1.0, n_q: int | None = None):
+    return _norms(L.load().gtps_dg_error_norms, ctx, dim, fe_degree, n_cells, True, u, wave,
+                  n_q)
+
+
+def save_vector(ctx: Context, path: str, x: torch.Tensor, dim: int, fe_degree: int,
+                n_cells: int, layout: int = L.GTPS_LAYOUT_CG):
+    """Write a device vector with its mesh description (see gtps_vector_save)."""
+    prec = _prec(x.dtype)
+    _chk_vec(x, x.dtype, _n_dofs(dim, fe_degree, n_cells, layout == L.GTPS_LAYOUT_DG))
+    L.check(L.load().gtps_vector_save(ctx.handle, str(path).encode(), _ptr(x), prec, dim,
+                                      fe_degree, n_cells, layout), "vector_save")
+
+
+def vector_info(path: str) -> dict:
+    """Header of a solution file (no GPU needed)."""
+    vals = [ctypes.c_int() for _ in range(5)]
+    n = ctypes.c_longlong()
+    L.check(L.load().gtps_vector_info(str(path).encode(), *[ctypes.byref(v) for v in vals],
+                                      ctypes.byref(n)), "vector_info")
+    prec, dim, k, nc, layout = (v.value for v in vals)
+    return {"precision": prec, "dim": dim, "fe_degree": k, "n_cells": nc, "layout": layout,
+            "n": n.value, "dtype": _DTYPES[prec]}
+
+
+def load_vector(ctx: Context, path: str, out: torch.Tensor | None = None) -> torch.Tensor:
+    info = vector_info(path)
+    if out is None:
+        out = torch.empty(info["n"], dtype=info["dtype"], device=ctx.device)
+    _chk_vec(out, info["dtype"], info["n"])
+    L.check(L.load().gtps_vector_load(ctx.handle, str(path).encode(), _ptr(out), info["n"],
+                                      info["precision"]), "vector_load")
+    return out

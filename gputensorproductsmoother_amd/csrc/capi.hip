@@ -639,6 +639,89 @@ namespace
     dg_level_setup(lvl.get());
     return lvl.release();
   }
+
+  // solution-file header (64 bytes, little-endian hosts only)
+  struct VecHeader
+  {
+    char               magic[8];
+    int                precision, dim, degree, n_cells, layout, reserved;
+    long long          n;
+    unsigned long long hash;
+    unsigned char      pad[16];
+  };
+  static_assert(sizeof(VecHeader) == 64, "solution-file header must be 64 bytes");
+
+  unsigned long long
+  fnv1a(const unsigned char *p, size_t n)
+  {
+    unsigned long long h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < n; ++i)
+      {
+        h ^= p[i];
+        h *= 0x100000001b3ull;
+      }
+    return h;
+  }
+
+  long long
+  layout_dofs(int dim, int degree, int n_cells, int layout)
+  {
+    GTPS_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    GTPS_REQUIRE(degree >= 1 && degree <= MAX_DEGREE, "degree must be in 1..7");
+    GTPS_REQUIRE(n_cells >= 1 && n_cells <= 4096, "n_cells must be in 1..4096");
+    GTPS_REQUIRE(layout == GTPS_LAYOUT_CG || layout == GTPS_LAYOUT_DG, "bad layout");
+    const long long n1 = layout == GTPS_LAYOUT_CG ? (long long)degree * n_cells + 1
+                                                  : (long long)n_cells * (degree + 1);
+    return dim == 3 ? n1 * n1 * n1 : n1 * n1;
+  }
+
+  VecHeader
+  make_vec_header(int precision, int dim, int degree, int n_cells, int layout)
+  {
+    GTPS_REQUIRE(precision == GTPS_FP32 || precision == GTPS_FP64, "bad precision");
+    VecHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    std::memcpy(hd.magic, "GTPSVEC1", 8);
+    hd.precision = precision;
+    hd.dim       = dim;
+    hd.degree    = degree;
+    hd.n_cells   = n_cells;
+    hd.layout    = layout;
+    hd.n         = layout_dofs(dim, degree, n_cells, layout);
+    return hd;
+  }
+
+  // reads and validates the header; with data != NULL also the values
+  // (checked against the stored hash)
+  VecHeader
+  read_vec_header(const char *path, std::vector<unsigned char> *data)
+  {
+    FILE *f = std::fopen(path, "rb");
+    GTPS_REQUIRE(f != nullptr, std::string("cannot open: ") + path);
+    VecHeader hd;
+    const bool got = std::fread(&hd, sizeof hd, 1, f) == 1;
+    bool       ok  = got && std::memcmp(hd.magic, "GTPSVEC1", 8) == 0 &&
+               (hd.precision == GTPS_FP32 || hd.precision == GTPS_FP64) &&
+               hd.dim >= 2 && hd.dim <= 3 && hd.degree >= 1 &&
+               hd.degree <= MAX_DEGREE && hd.n_cells >= 1 && hd.n_cells <= 4096 &&
+               (hd.layout == GTPS_LAYOUT_CG || hd.layout == GTPS_LAYOUT_DG) &&
+               hd.n == layout_dofs(hd.dim, hd.degree, hd.n_cells, hd.layout);
+    if (ok && data)
+      {
+        const size_t bytes = (size_t)hd.n * (hd.precision == GTPS_FP32 ? 4 : 8);
+        data->resize(bytes);
+        ok = (bytes == 0 || std::fread(data->data(), bytes, 1, f) == 1) &&
+             std::fgetc(f) == EOF;
+        if (ok && fnv1a(data->data(), bytes) != hd.hash)
+          {
+            std::fclose(f);
+            throw Error(GTPS_EINVAL, std::string("data hash mismatch: ") + path);
+          }
+      }
+    std::fclose(f);
+    GTPS_REQUIRE(ok, std::string("not a valid solution file: ") + path);
+    return hd;
+  }
 } // namespace
 
 // SIPG-DG multigrid: levels 0..maxl with n0 2^l cells per direction;
@@ -1874,5 +1957,144 @@ extern "C"
         return GTPS_ENOTCONV;
       }
     return GTPS_OK;
+  }
+
+  // ------------------------------------------------------- post-processing
+  int
+  gtps_assemble_rhs(gtps_ctx *ctx, int dim, int degree, int level,
+                    const double *f_q, double *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && f_q && b, "NULL argument");
+      GTPS_REQUIRE(level >= 0 && level <= 12, "level must be in 0..12");
+      set_device(ctx);
+      assemble_rhs(ctx, dim, degree, 1 << level, false, 0, f_q, 0.0, 0.0, b);
+    });
+  }
+
+  int
+  gtps_rhs_sine(gtps_ctx *ctx, int dim, int degree, int level, double wave, double *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && b, "NULL argument");
+      GTPS_REQUIRE(level >= 0 && level <= 12, "level must be in 0..12");
+      GTPS_REQUIRE(std::isfinite(wave), "wave number must be finite");
+      set_device(ctx);
+      assemble_rhs(ctx, dim, degree, 1 << level, false, 1, nullptr, wave, 0.0, b);
+    });
+  }
+
+  int
+  gtps_rhs_constant(gtps_ctx *ctx, int dim, int degree, int level, double value,
+                    double *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && b, "NULL argument");
+      GTPS_REQUIRE(level >= 0 && level <= 12, "level must be in 0..12");
+      GTPS_REQUIRE(std::isfinite(value), "value must be finite");
+      set_device(ctx);
+      assemble_rhs(ctx, dim, degree, 1 << level, false, 2, nullptr, 0.0, value, b);
+    });
+  }
+
+  int
+  gtps_dg_assemble_rhs(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                       const double *f_q, double *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && f_q && b, "NULL argument");
+      set_device(ctx);
+      assemble_rhs(ctx, dim, degree, n_cells, true, 0, f_q, 0.0, 0.0, b);
+    });
+  }
+
+  int
+  gtps_dg_rhs_sine(gtps_ctx *ctx, int dim, int degree, int n_cells, double wave,
+                   double *b)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && b, "NULL argument");
+      GTPS_REQUIRE(std::isfinite(wave), "wave number must be finite");
+      set_device(ctx);
+      assemble_rhs(ctx, dim, degree, n_cells, true, 1, nullptr, wave, 0.0, b);
+    });
+  }
+
+  int
+  gtps_error_norms(gtps_ctx *ctx, int dim, int degree, int level, double wave,
+                   int n_q, const double *u, double *l2, double *h1)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && u && l2 && h1, "NULL argument");
+      GTPS_REQUIRE(level >= 0 && level <= 12, "level must be in 0..12");
+      set_device(ctx);
+      compute_error_norms(ctx, dim, degree, 1 << level, false, wave, n_q, u, l2, h1);
+    });
+  }
+
+  int
+  gtps_dg_error_norms(gtps_ctx *ctx, int dim, int degree, int n_cells, double wave,
+                      int n_q, const double *u, double *l2, double *h1)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && u && l2 && h1, "NULL argument");
+      set_device(ctx);
+      compute_error_norms(ctx, dim, degree, n_cells, true, wave, n_q, u, l2, h1);
+    });
+  }
+
+  int
+  gtps_vector_save(gtps_ctx *ctx, const char *path, const void *x, int precision,
+                   int dim, int degree, int n_cells, int layout)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && path && x, "NULL argument");
+      VecHeader hd = make_vec_header(precision, dim, degree, n_cells, layout);
+      const size_t bytes = (size_t)hd.n * (precision == GTPS_FP32 ? 4 : 8);
+      std::vector<unsigned char> data(bytes);
+      set_device(ctx);
+      GTPS_HIP(hipMemcpyAsync(data.data(), x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+      hd.hash = fnv1a(data.data(), bytes);
+      FILE *f = std::fopen(path, "wb");
+      GTPS_REQUIRE(f != nullptr, std::string("cannot open for writing: ") + path);
+      const bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1 &&
+                      (bytes == 0 || std::fwrite(data.data(), bytes, 1, f) == 1);
+      const bool closed = std::fclose(f) == 0;
+      GTPS_REQUIRE(ok && closed, std::string("write failed: ") + path);
+    });
+  }
+
+  int
+  gtps_vector_info(const char *path, int *precision, int *dim, int *degree,
+                   int *n_cells, int *layout, long long *n)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(path != nullptr, "NULL path");
+      const VecHeader hd = read_vec_header(path, nullptr);
+      if (precision) *precision = hd.precision;
+      if (dim) *dim = hd.dim;
+      if (degree) *degree = hd.degree;
+      if (n_cells) *n_cells = hd.n_cells;
+      if (layout) *layout = hd.layout;
+      if (n) *n = hd.n;
+    });
+  }
+
+  int
+  gtps_vector_load(gtps_ctx *ctx, const char *path, void *x, long long n,
+                   int precision)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && path && x, "NULL argument");
+      std::vector<unsigned char> data;
+      const VecHeader hd = read_vec_header(path, &data);
+      GTPS_REQUIRE(hd.n == n, "file holds a vector of another size");
+      GTPS_REQUIRE(hd.precision == precision, "file holds another precision");
+      set_device(ctx);
+      GTPS_HIP(hipMemcpyAsync(x, data.data(), data.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+      GTPS_HIP(hipStreamSynchronize(ctx->stream));
+    });
   }
 }

@@ -13,6 +13,7 @@
 namespace gtps
 {
   constexpr int MAX_DEGREE = 7;
+  constexpr int MAX_QUAD   = 16;  // quadrature points per direction (error norms)
 
   template <int K>
   struct Tab

@@ -187,6 +187,8 @@ namespace gtps
   // `nodes` at x
   void lagrange_values(const std::vector<double> &nodes, double x,
                        double *val, double *der);
+  // n-point Gauss-Legendre rule on [0, 1]
+  void gauss_rule(int n, std::vector<double> &p, std::vector<double> &w);
   // FP64 generalized eigenpairs K s = lam M s (n <= 16) on the GPU;
   // S column j = eigenvector j, S^T M S = I, ascending eigenvalues
   void gpu_gen_eig(gtps_ctx *ctx, int n, const std::vector<double> &K,
@@ -228,6 +230,16 @@ namespace gtps
                                 const void *coarse);
   void launch_dg_restrict(const gtps_dg_level *fine_lvl, void *coarse,
                           const void *fine, bool store);
+
+  // post.hip (FP64): error norms of u against prod sin(wave pi x_d) with
+  // QGauss(nq) (nq <= MAX_QUAD), and the load vector from f at the
+  // QGauss(k+1) points (mode 0: fq[cell][q]; 1: the sine problem's f;
+  // 2: constant fc); n cells per direction, CG or DG layout
+  void compute_error_norms(gtps_ctx *ctx, int dim, int k, int n, bool dg,
+                           double wave, int nq, const double *u, double *l2,
+                           double *h1);
+  void assemble_rhs(gtps_ctx *ctx, int dim, int k, int n, bool dg, int mode,
+                    const double *fq, double wave, double fc, double *b);
 
   // smoother.hip: one colour pass. mode 0 multiplicative (src = b),
   // mode 1 additive (src = global residual), mode 2 multiplicative on

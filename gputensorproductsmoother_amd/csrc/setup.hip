@@ -440,6 +440,12 @@ namespace gtps
   }
 
   void
+  gauss_rule(int n, std::vector<double> &p, std::vector<double> &w)
+  {
+    gauss_points(n, p, w);
+  }
+
+  void
   gpu_gen_eig(gtps_ctx *ctx, int n, const std::vector<double> &K,
               const std::vector<double> &M, std::vector<double> &lam,
               std::vector<double> &S)

@@ -350,6 +350,60 @@ int gtps_dg_mg_pcg_solve(gtps_dg_mg *mg, double *x, const double *b,
                          double rtol, int maxit, int *iterations,
                          double *relres, double *history);
 
+/* ====================================================== post-processing */
+/* FP64 device vectors; n_cells per direction (2^level for the CG layout).
+ *
+ * Load vector b_i = sum_cells int f phi_i with QGauss(k+1) per cell:
+ * replaces the host cell loop of LaplaceOperator::compute_residual
+ * (include/laplace_operator.cuh:119-173) with f given at the quadrature
+ * points, f_q[cell][q], cell = cx + n (cy + n cz), q = qx + (k+1) (qy +
+ * (k+1) qz), point ((c + xi_q) / n) per direction with xi_q the Gauss points
+ * on [0,1].  Dirichlet rows of the CG layout are set to zero (the reference
+ * leaves them; a CG residual needs them zero).  gtps_rhs_sine assembles the
+ * reference's RightHandSide f = dim (w pi)^2 prod sin(w pi x_d)
+ * (apps/poisson.cu:76-87, w = 3 there) without an f_q array;
+ * gtps_rhs_constant assembles f = value. */
+int gtps_assemble_rhs(gtps_ctx *ctx, int dim, int degree, int level,
+                      const double *f_q, double *b);
+int gtps_rhs_sine(gtps_ctx *ctx, int dim, int degree, int level, double wave,
+                  double *b);
+int gtps_rhs_constant(gtps_ctx *ctx, int dim, int degree, int level,
+                      double value, double *b);
+int gtps_dg_assemble_rhs(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                         const double *f_q, double *b);
+int gtps_dg_rhs_sine(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                     double wave, double *b);
+/* L2 norm and H1 seminorm (cell-wise gradient for DG) of u_h - u with
+ * u = prod sin(w pi x_d), QGauss(n_q) per cell, n_q in 1..16: replaces
+ * LaplaceProblem::compute_error (apps/poisson.cu:353-382: QGauss(k+1),
+ * VectorTools::integrate_difference L2_norm / H1_seminorm against Solution,
+ * apps/poisson.cu:44-74).  Blocks until the result is on the host. */
+int gtps_error_norms(gtps_ctx *ctx, int dim, int degree, int level,
+                     double wave, int n_q, const double *u, double *l2,
+                     double *h1);
+int gtps_dg_error_norms(gtps_ctx *ctx, int dim, int degree, int n_cells,
+                        double wave, int n_q, const double *u, double *l2,
+                        double *h1);
+
+/* Solution file: a 64-byte little-endian header (magic "GTPSVEC1", int32
+ * precision, dim, degree, n_cells, layout, 0; int64 n; uint64 FNV-1a hash
+ * of the data bytes; 16 zero bytes) followed by the n values.  The
+ * reference writes no solution files (its apps print norms and timings);
+ * these are additions for checkpointing a solve.  x is a device array of n
+ * = (k n_cells + 1)^dim (GTPS_LAYOUT_CG) or (n_cells (k+1))^dim
+ * (GTPS_LAYOUT_DG) values of the given precision.  gtps_vector_info reads
+ * the header only (no GPU); gtps_vector_load checks the header against n
+ * and precision and the hash against the data. */
+#define GTPS_LAYOUT_CG 0
+#define GTPS_LAYOUT_DG 1
+int gtps_vector_save(gtps_ctx *ctx, const char *path, const void *x,
+                     int precision, int dim, int degree, int n_cells,
+                     int layout);
+int gtps_vector_info(const char *path, int *precision, int *dim, int *degree,
+                     int *n_cells, int *layout, long long *n);
+int gtps_vector_load(gtps_ctx *ctx, const char *path, void *x, long long n,
+                     int precision);
+
 #ifdef __cplusplus
 }
 #endif

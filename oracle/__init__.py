@@ -71,6 +71,9 @@ def lib():
         L.oracle_dg_rhs.argtypes = [i, i, i, i, _dp]
         L.oracle_dg_l2_error_sine.argtypes = [i, i, i, _dp]
         L.oracle_dg_l2_error_sine.restype = d
+        L.oracle_assemble_rhs.argtypes = [i, i, i, i, _dp, _dp]
+        L.oracle_error_norms.argtypes = [i, i, i, i, d, i, _dp, ctypes.POINTER(d),
+                                         ctypes.POINTER(d)]
         L.oracle_coeff_problem.argtypes = [i, i, i, ull, d, _dp, _dp, i,
                                            ctypes.POINTER(ll)]
         _lib = L
@@ -371,3 +374,23 @@ def dg_rhs(dim, k, n, kind=DG_RHS_SINE):
 def dg_l2_error_sine(dim, k, n, u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     return float(lib().oracle_dg_l2_error_sine(dim, k, n, _p(u)))
+
+
+# ---------------------------------------------------------------- post-processing
+def assemble_rhs(dim, k, n_cells, fq, dg=False):
+    """b_i = int f phi_i from f at the QGauss(k+1) points, fq shaped
+    (n_cells^dim, (k+1)^dim) (cell and point lexicographic, x fastest)."""
+    fq = np.ascontiguousarray(fq, dtype=np.float64).ravel()
+    n1 = n_cells * (k + 1) if dg else k * n_cells + 1
+    b = np.zeros(n1 ** dim)
+    _chk(lib().oracle_assemble_rhs(dim, k, n_cells, int(dg), _p(fq), _p(b)), "assemble_rhs")
+    return b
+
+
+def error_norms(dim, k, n_cells, u, wave=3.0, n_q=None, dg=False):
+    """(L2, H1 seminorm) of u_h - prod sin(wave pi x_d), QGauss(n_q) per cell."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    l2, h1 = ctypes.c_double(), ctypes.c_double()
+    _chk(lib().oracle_error_norms(dim, k, n_cells, int(dg), wave, k + 1 if n_q is None else n_q,
+                                  _p(u), ctypes.byref(l2), ctypes.byref(h1)), "error_norms")
+    return l2.value, h1.value

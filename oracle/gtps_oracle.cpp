@@ -37,6 +37,9 @@
 //     constrained operator, a vertex-patch smoother with the scaled
 //     constant-coefficient FD inverse as inexact local solve, coefficient
 //     coarsening by averaging, and dJ/dc_K = -lambda_K^T A^1_K u_K.
+//   * post-processing: load-vector assembly from f at the Gauss points
+//     (laplace_operator.cuh:140-173) and the L2 / H1-seminorm error against
+//     prod sin(w pi x_d) (poisson.cu:44-74, 353-382), CG and DG layouts.
 //
 // Every function is pinned by tests/test_oracle_*.py against closed forms,
 // high-precision golden fixtures (tests/golden) and brute-force dense
@@ -2270,6 +2273,139 @@ extern "C"
                   err += w * (uh - ue) * (uh - ue);
                 }
     return std::sqrt(err);
+  }
+
+  // Load vector b_i = sum_cells int f phi_i with QGauss(nq = k+1) per cell
+  // (laplace_operator.cuh:140-173, the reference's compute_residual
+  // assembly), from f given at the quadrature points: fq[cell][q] with cell
+  // = cx + n (cy + n cz) and q = qx + nq (qy + nq qz), points
+  // ((c + xi_q) h).  CG (dg = 0): lexicographic Q_k DoFs, Dirichlet rows
+  // zeroed; DG (dg = 1): X = cx (k+1) + ix, no constrained rows.
+  int
+  oracle_assemble_rhs(int dim, int k, int n, int dg, const double *fq, double *b)
+  {
+    if (k < 1 || k > MAXK || n < 1 || (dim != 2 && dim != 3))
+      return 1;
+    const Basis &bs = basis(k);
+    const int    nq = k + 1, n1 = k + 1;
+    const long long N1 = dg ? (long long)n * n1 : (long long)k * n + 1;
+    const long long N  = dim == 3 ? N1 * N1 * N1 : N1 * N1;
+    const double    h  = 1.0 / n;
+    std::vector<double> phi(nq * n1);
+    for (int q = 0; q < nq; ++q)
+      for (int i = 0; i < n1; ++i)
+        phi[q * n1 + i] = lagrange(bs.nodes, i, bs.qp[q]);
+    std::fill(b, b + N, 0.0);
+    const int nz = dim == 3 ? n : 1, lz = dim == 3 ? n1 : 1, qz_n = dim == 3 ? nq : 1;
+    const long long nqd = dim == 3 ? (long long)nq * nq * nq : (long long)nq * nq;
+    // serial over cells: the scatter adds into shared DoFs
+    for (int cz = 0; cz < nz; ++cz)
+      for (int cy = 0; cy < n; ++cy)
+        for (int cx = 0; cx < n; ++cx)
+          {
+            const long long cell = cx + (long long)n * (cy + (long long)n * cz);
+            const double   *f    = fq + cell * nqd;
+            const int st = dg ? n1 : k;
+            for (int l = 0; l < lz; ++l)
+              for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n1; ++i)
+                  {
+                    double v = 0.0;
+                    for (int qz = 0; qz < qz_n; ++qz)
+                      for (int qy = 0; qy < nq; ++qy)
+                        for (int qx = 0; qx < nq; ++qx)
+                          {
+                            double w = bs.qw[qx] * bs.qw[qy] * h * h * phi[qx * n1 + i] *
+                                       phi[qy * n1 + j];
+                            if (dim == 3)
+                              w *= bs.qw[qz] * h * phi[qz * n1 + l];
+                            v += w * f[qx + nq * (qy + (long long)nq * qz)];
+                          }
+                    const long long X = (long long)st * cx + i, Y = (long long)st * cy + j,
+                                    Z = dim == 3 ? (long long)st * cz + l : 0;
+                    b[X + N1 * (Y + N1 * Z)] += v;
+                  }
+          }
+    if (!dg)
+      for (long long g = 0; g < N; ++g)
+        {
+          const long long X = g % N1, Y = (g / N1) % N1, Z = g / (N1 * N1);
+          bool bd = X == 0 || X == N1 - 1 || Y == 0 || Y == N1 - 1;
+          if (dim == 3)
+            bd = bd || Z == 0 || Z == N1 - 1;
+          if (bd)
+            b[g] = 0.0;
+        }
+    return 0;
+  }
+
+  // L2 norm and H1 seminorm (broken, cell by cell, for DG) of u_h - u with
+  // u = prod_d sin(wave pi x_d), QGauss(nq) per cell
+  // (poisson.cu:353-382: integrate_difference with QGauss(k+1) there).
+  int
+  oracle_error_norms(int dim, int k, int n, int dg, double wave, int nq, const double *u,
+                     double *l2, double *h1)
+  {
+    if (k < 1 || k > MAXK || n < 1 || nq < 1 || nq > 32 || (dim != 2 && dim != 3))
+      return 1;
+    const Basis &bs = basis(k);
+    const int    n1 = k + 1;
+    const long long N1 = dg ? (long long)n * n1 : (long long)k * n + 1;
+    const double    h  = 1.0 / n, a = wave * M_PI;
+    std::vector<double> qp, qw;
+    gauss_rule(nq, qp, qw);
+    std::vector<double> phi(nq * n1), dphi(nq * n1);
+    for (int q = 0; q < nq; ++q)
+      for (int i = 0; i < n1; ++i)
+        {
+          phi[q * n1 + i]  = lagrange(bs.nodes, i, qp[q]);
+          dphi[q * n1 + i] = lagrange_deriv(bs.nodes, i, qp[q]) / h;
+        }
+    const int nz = dim == 3 ? n : 1, lz = dim == 3 ? n1 : 1, qz_n = dim == 3 ? nq : 1;
+    const int st = dg ? n1 : k;
+    double e0 = 0.0, e1 = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : e0, e1) schedule(static)
+    for (int cz = 0; cz < nz; ++cz)
+      for (int cy = 0; cy < n; ++cy)
+        for (int cx = 0; cx < n; ++cx)
+          for (int qz = 0; qz < qz_n; ++qz)
+            for (int qy = 0; qy < nq; ++qy)
+              for (int qx = 0; qx < nq; ++qx)
+                {
+                  double uh = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+                  for (int l = 0; l < lz; ++l)
+                    for (int j = 0; j < n1; ++j)
+                      for (int i = 0; i < n1; ++i)
+                        {
+                          const long long X = (long long)st * cx + i,
+                                          Y = (long long)st * cy + j,
+                                          Z = dim == 3 ? (long long)st * cz + l : 0;
+                          const double v  = u[X + N1 * (Y + N1 * Z)];
+                          const double pz = dim == 3 ? phi[qz * n1 + l] : 1.0;
+                          const double dz = dim == 3 ? dphi[qz * n1 + l] : 0.0;
+                          uh += v * phi[qx * n1 + i] * phi[qy * n1 + j] * pz;
+                          gx += v * dphi[qx * n1 + i] * phi[qy * n1 + j] * pz;
+                          gy += v * phi[qx * n1 + i] * dphi[qy * n1 + j] * pz;
+                          gz += v * phi[qx * n1 + i] * phi[qy * n1 + j] * dz;
+                        }
+                  const double x = (cx + qp[qx]) * h, y = (cy + qp[qy]) * h,
+                               z = (cz + qp[qz]) * h;
+                  const double sx = std::sin(a * x), sy = std::sin(a * y),
+                               sz = dim == 3 ? std::sin(a * z) : 1.0;
+                  const double cxv = a * std::cos(a * x), cyv = a * std::cos(a * y),
+                               czv = dim == 3 ? a * std::cos(a * z) : 0.0;
+                  double w = qw[qx] * qw[qy] * h * h;
+                  if (dim == 3)
+                    w *= qw[qz] * h;
+                  const double du = uh - sx * sy * sz;
+                  const double dx = gx - cxv * sy * sz, dy = gy - sx * cyv * sz,
+                               dzv = gz - sx * sy * czv;
+                  e0 += w * du * du;
+                  e1 += w * (dx * dx + dy * dy + dzv * dzv);
+                }
+    *l2 = std::sqrt(e0);
+    *h1 = std::sqrt(e1);
+    return 0;
   }
 
   int

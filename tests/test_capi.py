@@ -54,3 +54,36 @@ def test_missing_library_fails_loudly(tmp_path):
     from gputensorproductsmoother_amd import _lib
     with pytest.raises(RuntimeError, match="not found"):
         _lib.load(str(tmp_path / "libgtps.so"))
+
+
+def _fnv1a(data: bytes) -> int:
+    h = 0xcbf29ce484222325
+    for byte in data:
+        h = ((h ^ byte) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_solution_file_header_without_gpu(tmp_path):
+    """gtps_vector_info parses the documented 64-byte header (written here
+    independently with struct) and rejects a wrong magic or size."""
+    import struct
+    from gputensorproductsmoother_amd import core, _lib as L
+    dim, k, nc = 2, 3, 4
+    n = (k * nc + 1) ** dim
+    data = struct.pack(f"<{n}d", *[0.5 * i for i in range(n)])
+    hdr = b"GTPSVEC1" + struct.pack("<6iqQ", L.GTPS_FP64, dim, k, nc, L.GTPS_LAYOUT_CG, 0, n,
+                                    _fnv1a(data)) + bytes(16)
+    assert len(hdr) == 64
+    good = tmp_path / "good.gtps"
+    good.write_bytes(hdr + data)
+    info = core.vector_info(good)
+    assert (info["dim"], info["fe_degree"], info["n_cells"], info["n"], info["layout"]) == \
+        (dim, k, nc, n, L.GTPS_LAYOUT_CG)
+    bad = tmp_path / "bad.gtps"
+    bad.write_bytes(b"GTPSVEC0" + hdr[8:] + data)
+    with pytest.raises(RuntimeError, match="not a valid solution file"):
+        core.vector_info(bad)
+    wrong_n = tmp_path / "wrong.gtps"
+    wrong_n.write_bytes(hdr[:32] + struct.pack("<q", n + 1) + hdr[40:] + data)
+    with pytest.raises(RuntimeError):
+        core.vector_info(wrong_n)
