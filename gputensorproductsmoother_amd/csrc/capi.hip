@@ -748,11 +748,7 @@ struct gtps_dg_mg
     launch_fill(ctx->stream, L->N, prec, ul, 0.0);
     if (l == 0)
       {
-        for (int s = 0; s < coarse_sweeps; ++s)
-          {
-            launch_dg_sweep(L, ul, dl, GTPS_MULT_FWD, 1.0);
-            launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, 1.0);
-          }
+        launch_dg_sweeps(L, ul, dl, GTPS_MULT_FWD, 2 * coarse_sweeps, 1.0);
         return;
       }
     launch_dg_sweep(L, ul, dl, GTPS_MULT_FWD, relax);

@@ -156,173 +156,243 @@ namespace gtps
         }
     }
 
+    // LDS plan of the patch kernel: r, t (patch box, ping-pong), NS face
+    // slabs and their intermediate (NS = 6: all faces in one phase; 2: one
+    // direction per phase; 1: one face per phase, whichever fits 64 KB), and
+    // the 1D tables of this patch (S and lambda per direction, M, L-, L+).
     template <int K, typename T>
     struct DP
     {
-      static constexpr int NN = K + 1, PD = 2 * NN, P3 = PD * PD * PD;
-      static constexpr int SL = PD * PD * NN;  // one slab
-      static constexpr size_t lds = sizeof(T) * (2 * P3 + 2 * SL);
-      static constexpr bool ok = lds <= 65536;
+      static constexpr int    NN = K + 1, PD = 2 * NN, P3 = PD * PD * PD;
+      static constexpr int    SL   = PD * PD * NN;  // one slab
+      static constexpr int    TABN = 3 * PD * PD + 3 * PD + 3 * NN * NN;
+      static constexpr size_t bytes(int ns)
+      {
+        return sizeof(T) * (2 * P3 + 2 * ns * SL + TABN);
+      }
+      static constexpr int    NS  = bytes(6) <= 65536 ? 6 : (bytes(2) <= 65536 ? 2 : 1);
+      // threads per patch: about one patch DoF per thread (the kernel is
+      // latency-bound: every phase is a dependent LDS pass)
+      static constexpr int    NT  = P3 >= 1000 ? 1024 : (P3 >= 500 ? 512 : 256);
+      static constexpr size_t lds = bytes(NS);
+      static constexpr bool   ok  = lds <= 65536;
     };
 
-    // A_P,ext u_ext for one direction D (0 x, 1 y, 2 z) and side (0: the
-    // neighbour cell below the patch, 1: above): r -= scale (M (x) M) (Lx u)
-    // on the patch cell next to that neighbour.  Patch arrays are [z][y][x]
-    // with PD per direction; the slab s[a][b][j] has j along D and (a, b)
-    // the other two directions (a the slower one).
-    template <int K, typename T, int D, int SIDE>
+    // One work-group per patch (2x2x2 cells around an interior vertex, all
+    // (2k+2)^3 DoFs), u_P <- (1-w) u_P + w A_PP^{-1} (b_P - A_P,ext u_ext):
+    //  1. b_P into r; the face-neighbour cell slabs of u into LDS;
+    //  2. per face, L-/L+ along the face normal, then the cell mass along the
+    //     two in-plane directions, gathered into r (one thread per patch DoF,
+    //     so the faces of different directions never race);
+    //  3. fast diagonalisation, ping-ponging r and t, the eigenvalue scaling
+    //     fused into the last forward pass and the last inverse pass written
+    //     straight to u.
+    // Patch arrays are [z][y][x] with PD per direction; slab f (direction
+    // D = f / 2, side f % 2: below / above the patch) is [a][b][j] with j
+    // along D and (a, b) the other two directions, a the slower one.
+    template <int K, typename T>
+    struct DPShared
+    {
+      using C = DP<K, T>;
+      T r[C::P3], t[C::P3], sa[C::NS * C::SL], sw[C::NS * C::SL];
+      T Ssh[3][C::PD * C::PD], lsh[3][C::PD], Msh[C::NN * C::NN], Lsh[2][C::NN * C::NN];
+    };
+
+    // the local solve of the patch with lower-left cell pc; ends with the
+    // patch written to u (callers separate dependent patches by a barrier)
+    template <int K, typename T>
     __device__ void
-    dg_patch_ext(const T *__restrict__ u, const T *tab, T *r, T *s, T *w,
-                 long long X0, long long Y0, long long Z0, int N1, T scale)
+    dg_patch_solve(T *__restrict__ u, const T *__restrict__ bvec,
+                   const T *__restrict__ tab, const int n, const int pc[3],
+                   const T scale, const T relax, DPShared<K, T> &sh)
     {
       using TB           = DTab<K>;
-      constexpr int NN   = TB::NN, PD = TB::PD, SL = PD * PD * NN;
-      const long long P  = (long long)N1 * N1;
-      const int       tid = threadIdx.x;
-      // strides of (a, b, j) in the global vector and the D-offset of the slab
-      const long long st[3] = {1, N1, P};
-      constexpr int   da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-      const long long sa = st[da], sb = st[db], sj = st[D];
-      const long long o0 = X0 + Y0 * N1 + Z0 * P +
-                           (SIDE == 0 ? -(long long)NN : (long long)PD) * sj;
-      for (int e = tid; e < SL; e += DNT)
-        {
-          const int j = e % NN, ab = e / NN, bb = ab % PD, aa = ab / PD;
-          s[e] = u[o0 + aa * sa + bb * sb + j * sj];
-        }
-      __syncthreads();
-      // w[a][b][i] = sum_j L?[i][j] s[a][b][j]
-      const T *Lo = tab + (SIDE == 0 ? TB::Lm : TB::Lp);
-      for (int e = tid; e < SL; e += DNT)
-        {
-          const int i = e % NN, ab = e / NN;
-          T         v = 0;
-#pragma unroll
-          for (int j = 0; j < NN; ++j)
-            v += Lo[i * NN + j] * s[ab * NN + j];
-          w[e] = v;
-        }
-      __syncthreads();
-      // M along b (cell-local)
-      const T *M = tab + TB::M;
-      for (int e = tid; e < SL; e += DNT)
-        {
-          const int i = e % NN, ab = e / NN, bb = ab % PD, aa = ab / PD;
-          const int cb = bb / NN, ib = bb - cb * NN;
-          T         v  = 0;
-#pragma unroll
-          for (int j = 0; j < NN; ++j)
-            v += M[ib * NN + j] * w[(aa * PD + cb * NN + j) * NN + i];
-          s[e] = v;
-        }
-      __syncthreads();
-      // M along a, then subtract into the patch cell next to the slab
-      for (int e = tid; e < SL; e += DNT)
-        {
-          const int i = e % NN, ab = e / NN, bb = ab % PD, aa = ab / PD;
-          const int ca = aa / NN, ia = aa - ca * NN;
-          T         v  = 0;
-#pragma unroll
-          for (int j = 0; j < NN; ++j)
-            v += M[ia * NN + j] * s[((ca * NN + j) * PD + bb) * NN + i];
-          const int pi = SIDE == 0 ? i : NN + i;  // patch index along D
-          int       c3[3];
-          c3[D]  = pi;
-          c3[da] = aa;
-          c3[db] = bb;
-          r[(c3[2] * PD + c3[1]) * PD + c3[0]] -= scale * v;
-        }
-      __syncthreads();
-    }
-
-    // one pass of the FD transform along direction D: in place on r with
-    // scratch t; TR: S^T (forward) or S (inverse)
-    template <int K, typename T, int D, bool TR>
-    __device__ void
-    dg_fd_pass(T *r, T *t, const T *S)
-    {
-      constexpr int PD = DTab<K>::PD, P3 = PD * PD * PD;
-      const int     tid = threadIdx.x;
-      constexpr int str = D == 0 ? 1 : (D == 1 ? PD : PD * PD);
-      for (int e = tid; e < P3; e += DNT)
-        {
-          const int c[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
-          const int q    = c[D];
-          const int base = e - q * str;
-          T         v    = 0;
-#pragma unroll
-          for (int j = 0; j < PD; ++j)
-            v += (TR ? S[j * PD + q] : S[q * PD + j]) * r[base + j * str];
-          t[e] = v;
-        }
-      __syncthreads();
-      for (int e = tid; e < P3; e += DNT)
-        r[e] = t[e];
-      __syncthreads();
-    }
-
-    template <int K, typename T>
-    __global__ void __launch_bounds__(DNT)
-    dg_patch_kernel(T *__restrict__ u, const T *__restrict__ bvec,
-                    const T *__restrict__ tab, const int n, const int colour,
-                    const T scale, const T relax)
-    {
-      using TB         = DTab<K>;
-      using C          = DP<K, T>;
-      constexpr int NN = C::NN, PD = C::PD, P3 = C::P3, SL = C::SL;
-      __shared__ T r[P3], t[P3], s[SL], w[SL];
+      using C            = DP<K, T>;
+      constexpr int NN   = C::NN, PD = C::PD, P3 = C::P3, SL = C::SL, NS = C::NS;
+      T *r = sh.r, *t = sh.t, *sa = sh.sa, *sw = sh.sw;
+      auto &Ssh = sh.Ssh;
+      auto &lsh = sh.lsh;
+      auto &Msh = sh.Msh;
+      auto &Lsh = sh.Lsh;
       const int       N1 = n * NN;
       const long long P  = (long long)N1 * N1;
-      const int cx = colour % 3, cy = (colour / 3) % 3, cz = colour / 9;
-      const int npx = (n - 1 - cx + 2) / 3, npy = (n - 1 - cy + 2) / 3;
-      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
-      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
-      const int px = cx + 3 * qx, py = cy + 3 * qy, pz = cz + 3 * qz;
-      const long long X0 = (long long)px * NN, Y0 = (long long)py * NN,
-                      Z0 = (long long)pz * NN;
-      const long long g0 = X0 + Y0 * N1 + Z0 * P;
-      const int tid = threadIdx.x;
-      for (int e = tid; e < P3; e += DNT)
+      const long long st[3] = {1, N1, P};
+      const long long g0    = (long long)pc[0] * NN + pc[1] * NN * st[1] + pc[2] * NN * st[2];
+      const int       tid   = threadIdx.x;
+      auto variant = [&](int p) { return (p == 0 ? 1 : 0) + (p + 2 == n ? 2 : 0); };
+      // face f exists if the neighbour cell slab lies inside the mesh
+      auto has_face = [&](int f) {
+        const int p = pc[f >> 1];
+        return (f & 1) ? p + 2 < n : p > 0;
+      };
+      // tables
+      for (int e = tid; e < 3 * PD * PD; e += C::NT)
+        {
+          const int d = e / (PD * PD);
+          Ssh[d][e - d * PD * PD] = tab[TB::S + variant(pc[d]) * PD * PD + e - d * PD * PD];
+        }
+      for (int e = tid; e < 3 * PD; e += C::NT)
+        {
+          const int d = e / PD;
+          lsh[d][e - d * PD] = tab[TB::lam + variant(pc[d]) * PD + e - d * PD];
+        }
+      for (int e = tid; e < NN * NN; e += C::NT)
+        {
+          Msh[e]    = tab[TB::M + e];
+          Lsh[0][e] = tab[TB::Lm + e];
+          Lsh[1][e] = tab[TB::Lp + e];
+        }
+      for (int e = tid; e < P3; e += C::NT)
         {
           const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
           r[e] = bvec[g0 + x + (long long)y * N1 + (long long)z * P];
         }
-      __syncthreads();
-      if (px > 0)
-        dg_patch_ext<K, T, 0, 0>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      if (px + 2 < n)
-        dg_patch_ext<K, T, 0, 1>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      if (py > 0)
-        dg_patch_ext<K, T, 1, 0>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      if (py + 2 < n)
-        dg_patch_ext<K, T, 1, 1>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      if (pz > 0)
-        dg_patch_ext<K, T, 2, 0>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      if (pz + 2 < n)
-        dg_patch_ext<K, T, 2, 1>(u, tab, r, s, w, X0, Y0, Z0, N1, scale);
-      auto variant = [&](int p) { return (p == 0 ? 1 : 0) + (p + 2 == n ? 2 : 0); };
-      const int vx = variant(px), vy = variant(py), vz = variant(pz);
-      const T  *Sx = tab + TB::S + vx * PD * PD, *Sy = tab + TB::S + vy * PD * PD,
-               *Sz = tab + TB::S + vz * PD * PD;
-      const T  *lx = tab + TB::lam + vx * PD, *ly = tab + TB::lam + vy * PD,
-               *lz = tab + TB::lam + vz * PD;
-      dg_fd_pass<K, T, 0, true>(r, t, Sx);
-      dg_fd_pass<K, T, 1, true>(r, t, Sy);
-      dg_fd_pass<K, T, 2, true>(r, t, Sz);
-      for (int e = tid; e < P3; e += DNT)
+      // 2. external coupling, NS faces per phase
+      for (int f0 = 0; f0 < 6; f0 += NS)
+        {
+          for (int e = tid; e < NS * SL; e += C::NT)
+            {
+              const int f = f0 + e / SL;
+              if (!has_face(f))
+                continue;
+              const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
+              const int l = e % SL, j = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
+              const long long o = g0 + ((f & 1) ? (long long)PD : -(long long)NN) * st[D];
+              sa[e] = u[o + aa * st[da] + bb * st[db] + j * st[D]];
+            }
+          __syncthreads();
+          // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]
+          for (int e = tid; e < NS * SL; e += C::NT)
+            {
+              const int f = f0 + e / SL;
+              if (!has_face(f))
+                continue;
+              const int l = e % SL, i = l % NN, base = e - i;
+              const T  *Lo = Lsh[f & 1] + i * NN;
+              T         v  = 0;
+#pragma unroll
+              for (int j = 0; j < NN; ++j)
+                v += Lo[j] * sa[base + j];
+              sw[e] = v;
+            }
+          __syncthreads();
+          // cell mass along b
+          for (int e = tid; e < NS * SL; e += C::NT)
+            {
+              const int f = f0 + e / SL;
+              if (!has_face(f))
+                continue;
+              const int l = e % SL, i = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
+              const int cb = bb / NN, ib = bb - cb * NN;
+              const T  *w0 = sw + (e - l) + (aa * PD + cb * NN) * NN + i;
+              T         v  = 0;
+#pragma unroll
+              for (int j = 0; j < NN; ++j)
+                v += Msh[ib * NN + j] * w0[j * NN];
+              sa[e] = v;
+            }
+          __syncthreads();
+          // cell mass along a, gathered per patch DoF over this phase's faces
+          for (int e = tid; e < P3; e += C::NT)
+            {
+              const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
+              T         acc   = 0;
+#pragma unroll
+              for (int ff = 0; ff < NS; ++ff)
+                {
+                  const int f = f0 + ff, D = f >> 1;
+                  const int pD = c3[D];
+                  if (!has_face(f) || ((f & 1) ? pD < NN : pD >= NN))
+                    continue;
+                  const int da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
+                  const int i = (f & 1) ? pD - NN : pD, aa = c3[da], bb = c3[db];
+                  const int ca = aa / NN, ia = aa - ca * NN;
+                  const T  *s0 = sa + ff * SL + ((ca * NN) * PD + bb) * NN + i;
+                  T         v  = 0;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    v += Msh[ia * NN + j] * s0[j * PD * NN];
+                  acc += v;
+                }
+              r[e] -= scale * acc;
+            }
+          __syncthreads();
+        }
+      // 3. fast diagonalisation: forward S^T along x, y, z, scale, inverse
+      auto pass = [&](const T *in, T *out, int D, bool tr, bool divide) {
+        const int str = D == 0 ? 1 : (D == 1 ? PD : PD * PD);
+        const T  *S   = Ssh[D];
+        for (int e = tid; e < P3; e += C::NT)
+          {
+            const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
+            const int qq    = c3[D];
+            const T  *line  = in + e - qq * str;
+            T         v     = 0;
+#pragma unroll
+            for (int j = 0; j < PD; ++j)
+              v += (tr ? S[j * PD + qq] : S[qq * PD + j]) * line[j * str];
+            if (divide)
+              v /= scale * (lsh[0][c3[0]] + lsh[1][c3[1]] + lsh[2][c3[2]]);
+            out[e] = v;
+          }
+        __syncthreads();
+      };
+      pass(r, t, 0, true, false);
+      pass(t, r, 1, true, false);
+      pass(r, t, 2, true, true);
+      pass(t, r, 2, false, false);
+      pass(r, t, 1, false, false);
+      for (int e = tid; e < P3; e += C::NT)
         {
           const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
-          r[e] /= scale * (lx[x] + ly[y] + lz[z]);
-        }
-      __syncthreads();
-      dg_fd_pass<K, T, 2, false>(r, t, Sz);
-      dg_fd_pass<K, T, 1, false>(r, t, Sy);
-      dg_fd_pass<K, T, 0, false>(r, t, Sx);
-      for (int e = tid; e < P3; e += DNT)
-        {
-          const int       x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
+          const T  *line = t + e - x;
+          T         v    = 0;
+#pragma unroll
+          for (int j = 0; j < PD; ++j)
+            v += Ssh[0][x * PD + j] * line[j];
           const long long g = g0 + x + (long long)y * N1 + (long long)z * P;
-          u[g]              = relax == T(1) ? r[e] : (T(1) - relax) * u[g] + relax * r[e];
+          u[g]              = relax == T(1) ? v : (T(1) - relax) * u[g] + relax * v;
+        }
+    }
+
+    // one colour: one work-group per patch of the colour
+    template <int K, typename T>
+    __global__ void __launch_bounds__((DP<K, T>::NT))
+    dg_patch_kernel(T *__restrict__ u, const T *__restrict__ bvec,
+                    const T *__restrict__ tab, const int n, const int colour,
+                    const T scale, const T relax)
+    {
+      __shared__ DPShared<K, T> sh;
+      const int cx = colour % 3, cy = (colour / 3) % 3, cz = colour / 9;
+      const int npx = (n - 1 - cx + 2) / 3, npy = (n - 1 - cy + 2) / 3;
+      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      const int pc[3] = {cx + 3 * qx, cy + 3 * qy, cz + 3 * qz};
+      dg_patch_solve<K, T>(u, bvec, tab, n, pc, scale, relax, sh);
+    }
+
+    // meshes of at most 4 cells per direction (at most one patch per
+    // colour): n_sweeps whole sweeps in one work-group, alternating forward
+    // and reverse colour order from `first`, patches one after another
+    template <int K, typename T>
+    __global__ void __launch_bounds__((DP<K, T>::NT))
+    dg_patch_seq_kernel(T *__restrict__ u, const T *__restrict__ bvec,
+                        const T *__restrict__ tab, const int n, const int first,
+                        const int n_sweeps, const T scale, const T relax)
+    {
+      __shared__ DPShared<K, T> sh;
+      for (int s = 0; s < n_sweeps; ++s)
+        {
+          const bool rev = ((first + s) & 1) != 0;
+          for (int cc = 0; cc < 27; ++cc)
+            {
+              const int c = rev ? 26 - cc : cc;
+              const int pc[3] = {c % 3, (c / 3) % 3, c / 9};
+              if (pc[0] >= n - 1 || pc[1] >= n - 1 || pc[2] >= n - 1)
+                continue;
+              dg_patch_solve<K, T>(u, bvec, tab, n, pc, scale, relax, sh);
+              __syncthreads();
+            }
         }
     }
 
@@ -531,25 +601,43 @@ namespace gtps
       return dot ? nbd : 0;
     }
 
+    // n_sweeps sweeps, alternating colour order from `first`
+    // (GTPS_MULT_FWD / _REV)
     template <int K, typename T>
     void
-    sweep_k(const gtps_dg_level *lvl, T *u, const T *b, int variant, double relax)
+    sweeps_k(const gtps_dg_level *lvl, T *u, const T *b, int first, int n_sweeps,
+             double relax)
     {
       if constexpr (DP<K, T>::ok)
         {
-          const int n = lvl->n;
-          for (int cc = 0; cc < 27; ++cc)
+          const int   n = lvl->n;
+          hipStream_t s = lvl->ctx->stream;
+          if (n <= 4)
             {
-              const int c = variant == GTPS_MULT_REV ? 26 - cc : cc;
-              const int cx = c % 3, cy = (c / 3) % 3, cz = c / 9;
-              const long long np = (long long)((n - 1 - cx + 2) / 3) *
-                                   ((n - 1 - cy + 2) / 3) * ((n - 1 - cz + 2) / 3);
-              if (np <= 0 || cx >= n - 1 || cy >= n - 1 || cz >= n - 1)
-                continue;
-              hipLaunchKernelGGL((dg_patch_kernel<K, T>), dim3((unsigned)np), dim3(DNT),
-                                 0, lvl->ctx->stream, u, b, (const T *)lvl->tab, n, c,
+              hipLaunchKernelGGL((dg_patch_seq_kernel<K, T>), dim3(1), dim3(DP<K, T>::NT), 0, s, u,
+                                 b, (const T *)lvl->tab, n,
+                                 first == GTPS_MULT_REV ? 1 : 0, n_sweeps,
                                  (T)lvl->scale, (T)relax);
               GTPS_LAUNCH_CHECK();
+              return;
+            }
+          for (int sw = 0; sw < n_sweeps; ++sw)
+            {
+              const bool rev = ((first == GTPS_MULT_REV ? 1 : 0) + sw) & 1;
+              for (int cc = 0; cc < 27; ++cc)
+                {
+                  const int c  = rev ? 26 - cc : cc;
+                  const int cx = c % 3, cy = (c / 3) % 3, cz = c / 9;
+                  if (cx >= n - 1 || cy >= n - 1 || cz >= n - 1)
+                    continue;
+                  const long long np = (long long)((n - 1 - cx + 2) / 3) *
+                                       ((n - 1 - cy + 2) / 3) * ((n - 1 - cz + 2) / 3);
+                  hipLaunchKernelGGL((dg_patch_kernel<K, T>), dim3((unsigned)np),
+                                     dim3(DP<K, T>::NT),
+                                     0, s, u, b, (const T *)lvl->tab, n, c, (T)lvl->scale,
+                                     (T)relax);
+                  GTPS_LAUNCH_CHECK();
+                }
             }
         }
       else
@@ -608,13 +696,24 @@ namespace gtps
   launch_dg_sweep(const gtps_dg_level *lvl, void *u, const void *b,
                   int variant, double relax)
   {
+    launch_dg_sweeps(lvl, u, b, variant, 1, relax);
+  }
+
+  void
+  launch_dg_sweeps(const gtps_dg_level *lvl, void *u, const void *b, int first,
+                   int n_sweeps, double relax)
+  {
     GTPS_REQUIRE(lvl->n >= 2, "the DG smoother needs at least 2 cells per direction");
+    GTPS_REQUIRE(n_sweeps >= 0, "negative sweep count");
+    if (n_sweeps == 0)
+      return;
+    const int variant = first;
     dispatch_k(lvl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (lvl->prec == GTPS_FP32)
-        sweep_k<K, float>(lvl, (float *)u, (const float *)b, variant, relax);
+        sweeps_k<K, float>(lvl, (float *)u, (const float *)b, variant, n_sweeps, relax);
       else
-        sweep_k<K, double>(lvl, (double *)u, (const double *)b, variant, relax);
+        sweeps_k<K, double>(lvl, (double *)u, (const double *)b, variant, n_sweeps, relax);
       return 0;
     });
   }

@@ -225,6 +225,10 @@ namespace gtps
   // one sweep over the 27 colours (variant GTPS_MULT_FWD / _REV)
   void launch_dg_sweep(const gtps_dg_level *lvl, void *u, const void *b,
                        int variant, double relax);
+  // n_sweeps sweeps alternating colour order from `first` (meshes of at
+  // most 4 cells per direction run them in one launch)
+  void launch_dg_sweeps(const gtps_dg_level *lvl, void *u, const void *b,
+                        int first, int n_sweeps, double relax);
   // fine += P coarse / coarse (+)= P^T fine (store: coarse = P^T fine)
   void launch_dg_prolongate_add(const gtps_dg_level *fine_lvl, void *fine,
                                 const void *coarse);
