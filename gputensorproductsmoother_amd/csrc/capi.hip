@@ -252,6 +252,12 @@ struct gtps_mg : LaunchTimer
   hipStream_t                cap_stream = nullptr;
   hipGraphExec_t             dir_exec   = nullptr;
 
+  // the V-cycle from level 2 down as a dense matrix (coarse.hip), built
+  // on first use and dropped when the operators change
+  void                      *dense2      = nullptr;
+  bool                       dense_valid = false;
+  bool                       dense_off   = false;  // while tabulating
+
   // stage timers (HIP events on the context stream)
   int                        prof_mask = 0;
   std::vector<hipEvent_t>    ev_pool;
@@ -342,6 +348,11 @@ struct gtps_mg : LaunchTimer
   vcycle(int l, const void *dl, void *ul, int n_cycles = 1)
   {
     const gtps_level *L = lv[l];
+    if (l == 2 && n_cycles == 1 && dense_valid && !dense_off && tuning().dense_coarse != 0)
+      {
+        launch_dense_apply(ctx->stream, dim, L->N1, prec, dense2, dl, ul);
+        return;
+      }
     if (l == 1)
       {
         // one vertex patch covers the domain: exact solve
@@ -440,6 +451,7 @@ struct gtps_mg : LaunchTimer
   void
   direction()
   {
+    ensure_dense();
     // stage timers need eager launches
     if (!use_graph || prof_mask != 0)
       {
@@ -455,6 +467,7 @@ struct gtps_mg : LaunchTimer
   void
   precondition(const double *rin, double *zout, bool convert_in)
   {
+    ensure_dense();
     const long long N = lv[maxl]->N;
     if (prec == GTPS_FP32)
       {
@@ -466,6 +479,67 @@ struct gtps_mg : LaunchTimer
       }
     else
       vcycle(maxl, rin, zout ? (void *)zout : u[maxl]);
+  }
+
+  bool
+  dense_eligible() const
+  {
+    // constant coefficient only: a coefficient typically changes between
+    // solves (adjoint loops), and tabulating costs one sub-cycle per column
+    return tuning().dense_coarse != 0 && maxl >= 3 && coef64.empty() &&
+           dense_coarse_size(dim, lv[2]->N1) <= MAX_DENSE_COARSE;
+  }
+
+  void
+  drop_dense()
+  {
+    dfree(dense2);
+    dense2      = nullptr;
+    dense_valid = false;
+  }
+
+  // tabulate V(level 2) column by column: unit defects in, the ordinary
+  // sub-cycle applied to each, interior values gathered into rows
+  void
+  ensure_dense()
+  {
+    if (dense_valid || !dense_eligible())
+      return;
+    hipStream_t     s  = ctx->stream;
+    const int       N1 = lv[2]->N1;
+    const long long N  = lv[2]->N;
+    const int       ni = dense_coarse_size(dim, N1);
+    const size_t    es = prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
+    void *D = dmalloc(es * N * ni), *U = nullptr;
+    const int saved_mask = prof_mask;
+    try
+      {
+        U = dmalloc(es * N * ni);
+        if (!dense2)
+          dense2 = dmalloc(es * (size_t)ni * ni);
+        launch_unit_defects(s, dim, N1, prec, D);
+        dense_off = true;
+        prof_mask = 0;
+        for (int j = 0; j < ni; ++j)
+          vcycle(2, (const char *)D + es * N * j, (char *)U + es * N * j);
+        dense_off = false;
+        prof_mask = saved_mask;
+        launch_gather_columns(s, dim, N1, prec, U, dense2);
+        GTPS_HIP(hipStreamSynchronize(s));
+      }
+    catch (...)
+      {
+        dense_off = false;
+        prof_mask = saved_mask;
+        (void)hipStreamSynchronize(s);
+        dfree(D);
+        dfree(U);
+        drop_dense();
+        throw;
+      }
+    dfree(D);
+    dfree(U);
+    dense_valid = true;
   }
 
   // FP64 levels, solutions and transfer for full multigrid (lazily)
@@ -519,6 +593,7 @@ struct gtps_mg : LaunchTimer
   fmg(double *x, const double *const *rhs, int n_cycles)
   {
     ensure_fmg();
+    ensure_dense();
     hipStream_t s   = ctx->stream;
     const bool  f32 = prec == GTPS_FP32;
     // level 1: exact coarse solve, then one more smoothing step
@@ -571,6 +646,7 @@ struct gtps_mg : LaunchTimer
 
   ~gtps_mg()
   {
+    drop_dense();
     for (size_t l = 0; l < lv64.size(); ++l)
       {
         if (own_lv64[l])
@@ -1361,6 +1437,12 @@ extern "C"
           prev                  = tuning().row_staged;
           tuning().row_staged   = (int)value;
         }
+      else if (k == "dense_coarse")
+        {
+          GTPS_REQUIRE(value == 0 || value == 1, "dense_coarse must be 0 or 1");
+          prev                  = tuning().dense_coarse;
+          tuning().dense_coarse = (int)value;
+        }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });
@@ -1384,6 +1466,8 @@ extern "C"
         v = tuning().box_vmult_max_n;
       else if (k == "row_staged")
         v = tuning().row_staged;
+      else if (k == "dense_coarse")
+        v = tuning().dense_coarse;
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });
@@ -1480,6 +1564,7 @@ extern "C"
           (void)hipGraphExecDestroy(mg->dir_exec);
           mg->dir_exec = nullptr;
         }
+      mg->drop_dense();
       auto each_level = [&](auto &&f) {
         for (int l = 1; l <= mg->maxl; ++l)
           f(mg->lv[l], l);

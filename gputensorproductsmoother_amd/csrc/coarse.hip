@@ -1,0 +1,176 @@
+// coarse.hip - the V-cycle from level 2 down as one dense matrix.
+//
+// Below level 3 the V-cycle (pre-smoothing from zero, residual,
+// restriction, the exact level-1 solve, prolongation, post-smoothing) is a
+// fixed linear map of the level-2 defect, zero on the Dirichlet DoFs.  On
+// these levels every kernel is latency-bound (a few microseconds each for a
+// few hundred to a few thousand DoFs, about twenty launches per V-cycle), so
+// the map is tabulated once at setup, column by column with the ordinary
+// V-cycle kernels applied to unit defects, and each V-cycle then replaces
+// the whole sub-cycle by one matrix-vector product over the interior DoFs.
+// The result is the same linear map, rounded differently.
+#include "gtps_internal.h"
+
+namespace gtps
+{
+  namespace
+  {
+    constexpr int CNT = 256;
+    constexpr int ROWS_PER_WAVE = 4;
+
+    // interior DoF j (x fastest over the (N1-2)^dim interior) -> level index
+    template <int DIM>
+    __device__ __forceinline__ long long
+    interior_index(int j, int N1)
+    {
+      const int ni1 = N1 - 2;
+      const int x = j % ni1 + 1, y = (j / ni1) % ni1 + 1;
+      const int z = DIM == 3 ? j / (ni1 * ni1) + 1 : 0;
+      return x + (long long)N1 * (y + (long long)N1 * z);
+    }
+
+    // D[j][*] = e_{interior j}: column j's unit defect
+    template <typename T, int DIM>
+    __global__ void __launch_bounds__(CNT)
+    unit_defects_kernel(T *__restrict__ D, const int N1, const long long N, const int ni)
+    {
+      for (long long e = (long long)blockIdx.x * CNT + threadIdx.x; e < (long long)ni * N;
+           e += (long long)gridDim.x * CNT)
+        {
+          const long long j = e / N, g = e - j * N;
+          D[e] = g == interior_index<DIM>((int)j, N1) ? T(1) : T(0);
+        }
+    }
+
+    // B[i][j] = U[j][interior i]
+    template <typename T, int DIM>
+    __global__ void __launch_bounds__(CNT)
+    gather_columns_kernel(const T *__restrict__ U, T *__restrict__ B, const int N1,
+                          const long long N, const int ni)
+    {
+      for (long long e = (long long)blockIdx.x * CNT + threadIdx.x; e < (long long)ni * ni;
+           e += (long long)gridDim.x * CNT)
+        {
+          const int i = (int)(e / ni), j = (int)(e - (long long)i * ni);
+          B[e]        = U[(long long)j * N + interior_index<DIM>(i, N1)];
+        }
+    }
+
+    // u = B d on the interior, 0 on the Dirichlet DoFs; one wavefront per
+    // ROWS_PER_WAVE rows, lanes across the columns (coalesced rows of B)
+    template <typename T, int DIM>
+    __global__ void __launch_bounds__(CNT)
+    dense_apply_kernel(const T *__restrict__ B, const T *__restrict__ d, T *__restrict__ u,
+                       const int N1, const long long N, const int ni)
+    {
+      __shared__ T sd[MAX_DENSE_COARSE];
+      const int    tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+      for (int j = tid; j < ni; j += CNT)
+        sd[j] = d[interior_index<DIM>(j, N1)];
+      // Dirichlet rows: zero (spread over the grid)
+      for (long long g = (long long)blockIdx.x * CNT + tid; g < N;
+           g += (long long)gridDim.x * CNT)
+        {
+          const long long x = g % N1, y = (g / N1) % N1, z = DIM == 3 ? g / ((long long)N1 * N1) : 1;
+          bool bd = x == 0 || x == N1 - 1 || y == 0 || y == N1 - 1;
+          if (DIM == 3)
+            bd = bd || z == 0 || z == N1 - 1;
+          if (bd)
+            u[g] = T(0);
+        }
+      __syncthreads();
+      const int row0 = (blockIdx.x * (CNT / 64) + wave) * ROWS_PER_WAVE;
+#pragma unroll
+      for (int rr = 0; rr < ROWS_PER_WAVE; ++rr)
+        {
+          const int i = row0 + rr;
+          if (i >= ni)
+            break;
+          const T *row = B + (long long)i * ni;
+          T        acc = 0;
+          for (int j = lane; j < ni; j += 64)
+            acc += row[j] * sd[j];
+          const double s = wave_sum((double)acc);
+          if (lane == 0)
+            u[interior_index<DIM>(i, N1)] = (T)s;
+        }
+    }
+
+    int
+    cgrid(long long work, int cap = 16384)
+    {
+      return (int)std::max<long long>(1, std::min<long long>((work + CNT - 1) / CNT, cap));
+    }
+
+    template <typename F>
+    void
+    by_type(int dim, int prec, F &&f)
+    {
+      if (prec == GTPS_FP32)
+        {
+          if (dim == 3)
+            f((float *)nullptr, std::integral_constant<int, 3>{});
+          else
+            f((float *)nullptr, std::integral_constant<int, 2>{});
+        }
+      else
+        {
+          if (dim == 3)
+            f((double *)nullptr, std::integral_constant<int, 3>{});
+          else
+            f((double *)nullptr, std::integral_constant<int, 2>{});
+        }
+    }
+  } // namespace
+
+  int
+  dense_coarse_size(int dim, int N1)
+  {
+    const int ni1 = N1 - 2;
+    return dim == 3 ? ni1 * ni1 * ni1 : ni1 * ni1;
+  }
+
+  void
+  launch_unit_defects(hipStream_t s, int dim, int N1, int prec, void *D)
+  {
+    const int       ni = dense_coarse_size(dim, N1);
+    const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
+    by_type(dim, prec, [&](auto tp, auto dd) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      hipLaunchKernelGGL((unit_defects_kernel<T, decltype(dd)::value>),
+                         dim3(cgrid((long long)ni * N)), dim3(CNT), 0, s, (T *)D, N1, N, ni);
+    });
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_gather_columns(hipStream_t s, int dim, int N1, int prec, const void *U, void *B)
+  {
+    const int       ni = dense_coarse_size(dim, N1);
+    const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
+    by_type(dim, prec, [&](auto tp, auto dd) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      hipLaunchKernelGGL((gather_columns_kernel<T, decltype(dd)::value>),
+                         dim3(cgrid((long long)ni * ni)), dim3(CNT), 0, s, (const T *)U,
+                         (T *)B, N1, N, ni);
+    });
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_dense_apply(hipStream_t s, int dim, int N1, int prec, const void *B, const void *d,
+                     void *u)
+  {
+    const int       ni = dense_coarse_size(dim, N1);
+    GTPS_REQUIRE(ni <= MAX_DENSE_COARSE, "dense coarse operator too large");
+    const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
+    const int rows_per_block = (CNT / 64) * ROWS_PER_WAVE;
+    const int nb             = (ni + rows_per_block - 1) / rows_per_block;
+    by_type(dim, prec, [&](auto tp, auto dd) {
+      using T = std::remove_pointer_t<decltype(tp)>;
+      hipLaunchKernelGGL((dense_apply_kernel<T, decltype(dd)::value>), dim3(nb), dim3(CNT), 0,
+                         s, (const T *)B, (const T *)d, (T *)u, N1, N, ni);
+    });
+    GTPS_LAUNCH_CHECK();
+  }
+} // namespace gtps

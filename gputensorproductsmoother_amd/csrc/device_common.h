@@ -14,6 +14,7 @@ namespace gtps
 {
   constexpr int MAX_DEGREE = 7;
   constexpr int MAX_QUAD   = 16;  // quadrature points per direction (error norms)
+  constexpr int MAX_DENSE_COARSE = 4096;  // interior DoFs of a tabulated level-2 V-cycle
 
   template <int K>
   struct Tab

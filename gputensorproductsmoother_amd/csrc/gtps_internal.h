@@ -178,6 +178,10 @@ namespace gtps
     // and its extra barrier cost more than the fewer cache-line accesses
     // save, so the lane form is the default.
     int       row_staged = 0;
+    // tabulate the V-cycle from level 2 down as a dense matrix (hierarchies
+    // of at least 3 levels whose level-2 interior has <= MAX_DENSE_COARSE
+    // DoFs)
+    int       dense_coarse = 1;
   };
   Tuning &tuning();
 
@@ -234,6 +238,15 @@ namespace gtps
                                 const void *coarse);
   void launch_dg_restrict(const gtps_dg_level *fine_lvl, void *coarse,
                           const void *fine, bool store);
+
+  // coarse.hip: the level-2 V-cycle as a dense matrix over the interior
+  // DoFs ((N1-2)^dim of them, N1 = DoFs per direction of level 2)
+  int  dense_coarse_size(int dim, int N1);
+  void launch_unit_defects(hipStream_t s, int dim, int N1, int prec, void *D);
+  void launch_gather_columns(hipStream_t s, int dim, int N1, int prec,
+                             const void *U, void *B);
+  void launch_dense_apply(hipStream_t s, int dim, int N1, int prec,
+                          const void *B, const void *d, void *u);
 
   // post.hip (FP64): error norms of u against prod sin(wave pi x_d) with
   // QGauss(nq) (nq <= MAX_QUAD), and the load vector from f at the

@@ -85,6 +85,15 @@ const char *gtps_version(void);
  *   "row_staged"            patch-group smoother with its global accesses
  *                           staged through contiguous x-runs (1) or
  *                           per-lane pencils and rows (0, default).
+ *   "dense_coarse"          1 (default): a multigrid hierarchy of at least 3
+ *                           levels tabulates its V-cycle from level 2 down
+ *                           (a fixed linear map of the level-2 defect) as a
+ *                           dense matrix over the level-2 interior DoFs on
+ *                           first use (at most 4096 of them: k <= 4 in 3D)
+ *                           and applies it with one launch (constant
+ *                           coefficient only); 0: the ordinary
+ *                           level-2 smoothing and level-1 solve.  Set it
+ *                           before the hierarchy's first solve.
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */

@@ -207,3 +207,59 @@ def test_fmg_matches_oracle(dim, k, level, n_cycles, coef, prec):
         e = O.l2_error_sine(dim, k, level, x.cpu().numpy())
         e_ref = O.l2_error_sine(dim, k, level, ref)
         assert e <= 1.01 * e_ref
+
+
+# ------------------------------------------------ tabulated level-2 V-cycle
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("dim,k,level,sched,coef", [
+    (2, 3, 4, "sym", False), (3, 1, 4, "ref", False), (3, 3, 3, "sym", False),
+    (3, 4, 3, "sym", False), (3, 2, 4, "add", False), (3, 2, 3, "sym", True)])
+def test_dense_coarse_matches_sub_cycle(core, ctx, dim, k, level, sched, coef, prec):
+    """The V-cycle with the level-2 sub-cycle tabulated as a dense matrix
+    equals the V-cycle that runs it kernel by kernel (FP64 1e-12, FP32
+    2e-5 relative), and both match the oracle (with a coefficient the
+    tabulation is off, so both runs take the kernel path)."""
+    _announce("dense-coarse", dim, k, level, sched, coef, prec)
+    s = SCHED[sched]
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    r = O.rhs_random(dim, k, level)
+    rd = torch.tensor(r, device="cuda:0")
+    c = np.exp(0.5 * rng(5).standard_normal((1 << level) ** 3)) if coef else None
+    out = {}
+    for dense in (1, 0):
+        prev = core.set_tuning("dense_coarse", dense)
+        try:
+            mg = core.MultigridSolvers(ctx, dim, k, level, dt, s[1], s[2])
+            if coef:
+                mg.set_coefficient(torch.tensor(c, device="cuda:0"))
+            z = torch.full_like(rd, float("nan"))
+            mg.vmult(z, rd)
+            torch.cuda.synchronize()
+            out[dense] = z.cpu().numpy()
+        finally:
+            core.set_tuning("dense_coarse", prev)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    assert rel_err(out[1], out[0]) < tol
+    ref = (O.coeff_vcycle(dim, k, level, c, r, mg_variant=s[0], relax=s[2]) if coef
+           else O.vcycle(dim, k, level, r, mg_variant=s[0], relax=s[2]))
+    assert rel_err(out[1], ref) < (1e-11 if prec == "fp64" else 1e-4)
+
+
+def test_dense_coarse_rebuilt_after_coefficient_change(core, ctx):
+    """Changing the coefficient drops the tabulated sub-cycle; the next
+    V-cycle matches the oracle for the new coefficient."""
+    _announce("dense-coarse-recoef")
+    dim, k, level = 3, 2, 3
+    r = O.rhs_random(dim, k, level)
+    rd = torch.tensor(r, device="cuda:0")
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+    z = torch.empty_like(rd)
+    mg.vmult(z, rd)
+    assert rel_err(z.cpu().numpy(), O.vcycle(dim, k, level, r)) < 1e-11
+    c = np.exp(0.5 * rng(9).standard_normal((1 << level) ** 3))
+    mg.set_coefficient(torch.tensor(c, device="cuda:0"))
+    mg.vmult(z, rd)
+    assert rel_err(z.cpu().numpy(), O.coeff_vcycle(dim, k, level, c, r)) < 1e-11
+    mg.set_coefficient(None)
+    mg.vmult(z, rd)
+    assert rel_err(z.cpu().numpy(), O.vcycle(dim, k, level, r)) < 1e-11
