@@ -1443,6 +1443,12 @@ extern "C"
           prev                  = tuning().dense_coarse;
           tuning().dense_coarse = (int)value;
         }
+      else if (k == "mfma_smoother")
+        {
+          GTPS_REQUIRE(value == 0 || value == 1, "mfma_smoother must be 0 or 1");
+          prev                   = tuning().mfma_smoother;
+          tuning().mfma_smoother = (int)value;
+        }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });
@@ -1468,6 +1474,8 @@ extern "C"
         v = tuning().row_staged;
       else if (k == "dense_coarse")
         v = tuning().dense_coarse;
+      else if (k == "mfma_smoother")
+        v = tuning().mfma_smoother;
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });

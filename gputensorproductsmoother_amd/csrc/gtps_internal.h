@@ -182,6 +182,8 @@ namespace gtps
     // of at least 3 levels whose level-2 interior has <= MAX_DENSE_COARSE
     // DoFs)
     int       dense_coarse = 1;
+    // 3D FP32 degree 4..7 multiplicative smoother on the matrix cores
+    int       mfma_smoother = 1;
   };
   Tuning &tuning();
 
@@ -238,6 +240,12 @@ namespace gtps
                                 const void *coarse);
   void launch_dg_restrict(const gtps_dg_level *fine_lvl, void *coarse,
                           const void *fine, bool store);
+
+  // smoother_mfma.hip: one multiplicative colour pass (b in src) with the
+  // 1D contractions on v_mfma_f32_16x16x4f32 (3D, FP32, k = 4..7)
+  bool mfma_smoother_supported(int dim, int k, int prec);
+  void launch_patch_colour_mfma(const gtps_level *lvl, void *u, const void *src,
+                                int colour, double relax);
 
   // coarse.hip: the level-2 V-cycle as a dense matrix over the interior
   // DoFs ((N1-2)^dim of them, N1 = DoFs per direction of level 2)

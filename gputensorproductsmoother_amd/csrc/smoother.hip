@@ -123,24 +123,40 @@ namespace gtps
 
       if (!ADD)
         {
-          // stage 0: skeleton of u (interior zeroed) and interior of b
-          for (int t = tid; t < NPB * NP * NP * NP; t += NT)
+          // stage 0: skeleton of u (interior zeroed) and interior of b; all
+          // loads of a thread issued before the first wait
+          constexpr int IT0 = (NPB * NP * NP * NP + NT - 1) / NT;
+          T             v0[IT0];
+#pragma unroll
+          for (int it = 0; it < IT0; ++it)
             {
-              const int       lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
-              const long long f  = sfirst[lp];
-              if (f < 0)
+              const int t  = tid + it * NT;
+              const int lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
+              const long long f = t < NPB * NP * NP * NP ? sfirst[lp] : -1;
+              const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
+              const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
+                                 iz > 0 && iz < NP - 1;
+              v0[it] = f >= 0 ? (inner ? src : u)[f + ix + (long long)iy * N1 + iz * P] : T(0);
+            }
+#pragma unroll
+          for (int it = 0; it < IT0; ++it)
+            {
+              const int t = tid + it * NT;
+              if (t >= NPB * NP * NP * NP)
+                continue;
+              const int lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
+              if (sfirst[lp] < 0)
                 continue;
               const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
-              const long long g = f + ix + (long long)iy * N1 + iz * P;
               const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
                                  iz > 0 && iz < NP - 1;
               if (inner)
                 {
                   sR1[lp * R1 + r] = T(0);
-                  sSB[lp * SB + ((iz - 1) * NI + iy - 1) * NI + ix - 1] = src[g];
+                  sSB[lp * SB + ((iz - 1) * NI + iy - 1) * NI + ix - 1] = v0[it];
                 }
               else
-                sR1[lp * R1 + r] = u[g];
+                sR1[lp * R1 + r] = v0[it];
             }
           __syncthreads();
           // stage 1: z-rows (interior rows only) of M and K
@@ -1891,6 +1907,12 @@ namespace gtps
           // no zero-skeleton form: clear u, then a multiplicative pass
           launch_fill(s, lvl->N, lvl->prec, u, 0.0);
           mode = 0;
+        }
+      if (mode == 0 && tuning().mfma_smoother != 0 &&
+          mfma_smoother_supported(lvl->dim, K, lvl->prec))
+        {
+          launch_patch_colour_mfma(lvl, u, src, colour, relax);
+          return;
         }
       if (lvl->dim == 3)
         {

@@ -94,6 +94,10 @@ const char *gtps_version(void);
  *                           coefficient only); 0: the ordinary
  *                           level-2 smoothing and level-1 solve.  Set it
  *                           before the hierarchy's first solve.
+ *   "mfma_smoother"         1 (default): 3D FP32 multiplicative smoothing of
+ *                           degree 4..7 runs its 1D contractions on the
+ *                           matrix cores (v_mfma_f32_16x16x4f32, patches
+ *                           padded to 16^3); 0: the LDS/FMA kernel.
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */

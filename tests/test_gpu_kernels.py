@@ -253,3 +253,32 @@ def test_vmult_box_and_march_paths(core, ctx, k, level, path, prec):
     ref = O.laplace_vmult(3, k, level, x)
     assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
     assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("mfma", [1, 0])
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("k,level", [(4, 2), (4, 3), (5, 2), (6, 2), (7, 1), (7, 2)])
+def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
+    """3D FP32 multiplicative smoothing of degree 4..7 on the matrix cores
+    (v_mfma_f32_16x16x4f32, patches padded to 16^3) and on the LDS/FMA
+    kernel agree with the oracle (FP32 tolerance x20, relaxation 0.8)."""
+    _announce("smoother-mfma", k, level, reverse, mfma)
+    g = rng(level * 11 + k)
+    N = n_dofs(3, k, level)
+    b, u = g.uniform(-1, 1, (2, N))
+    u = u.astype(np.float32).astype(np.float64)
+    b = b.astype(np.float32).astype(np.float64)
+    relax = 0.8
+    ref = O.smoother_step(3, k, level, b, u, variant=O.MULT_REV if reverse else O.MULT_FWD,
+                          relax=relax)
+    prev = core.set_tuning("mfma_smoother", mfma)
+    try:
+        op = core.LaplaceOperator(ctx, 3, k, level, torch.float32)
+        sm = core.PatchSmoother(op, core.PatchSmoother.MULTIPLICATIVE_REVERSE if reverse
+                                else core.PatchSmoother.MULTIPLICATIVE, relax)
+        ud = _to_dev(u, "fp32")
+        sm.step(ud, _to_dev(b, "fp32"))
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("mfma_smoother", prev)
+    assert rel_err(ud.cpu().numpy(), ref) < TOL["fp32"] * 20
