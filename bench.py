@@ -172,8 +172,9 @@ def config_report(args):
             b = torch.tensor(problems.rhs_constant(dim, k, level, cell_mass(k)), device=dev)
             r = solve_pair(dim, k, level, b, 1e-8)
             line = {"config": "C3", "workload": "3D Q7 Poisson f=1, 32^3 cells, "
-                    "multiplicative smoother (LDS/FMA kernels; no MFMA kernel is built), "
-                    "CG to 1e-8", "n_dofs": b.numel(), "mfma_utilisation": None, **r}
+                    "multiplicative smoother (FP32: 1D contractions on the matrix cores; "
+                    "FP64: LDS/FMA kernel), CG to 1e-8", "n_dofs": b.numel(),
+                    "mfma_utilisation": mfma_report(core, ctx, k, level, reps), **r}
         elif name == "c4":
             line = c4_report(ctx, core, problems, reps, solve_pair, rel_l2)
         elif name == "c5":
@@ -181,6 +182,47 @@ def config_report(args):
         else:
             raise SystemExit(f"unknown config {name}")
         print(json.dumps(line), flush=True)
+
+
+def mfma_flops_per_patch(k: int) -> int:
+    """FLOPs issued on the matrix cores per patch by the FP32 matrix-core
+    smoother (v_mfma_f32_16x16x4f32 = 16*16*4 multiply-adds): z pass 2
+    products on the NP non-zero tiles, y pass 3 and x pass 2 products plus
+    S^T on the 13 stored planes, S^T along y, S^T and S along z on the NI
+    interior tiles, S along y on 13 and along x on NI tiles; 4 k-steps
+    each."""
+    NP, NI, NZ = 2 * k + 1, 2 * k - 1, 13
+    mfma = 4 * (2 * NP + 3 * NZ + 3 * NZ + NZ + 2 * NI + NZ + NI)
+    return mfma * 16 * 16 * 4 * 2
+
+
+def mfma_report(core, ctx, k, level, reps):
+    """Finest-level colour pass of the FP32 smoother with the matrix-core
+    kernel and with the LDS/FMA kernel (eager launches, best of reps x 8
+    colours); MFMA utilisation = issued matrix-core FLOPs / time / the
+    157.3 TFLOP/s dense FP32 matrix peak."""
+    import torch
+    op = core.LaplaceOperator(ctx, 3, k, level, torch.float32)
+    g = torch.Generator(device="cuda:0").manual_seed(SEED)
+    u = torch.rand(op.n_dofs, generator=g, device="cuda:0")
+    b = torch.rand(op.n_dofs, generator=g, device="cuda:0")
+    sm = core.PatchSmoother(op)
+    n = 1 << level
+    patches = (n - 1) ** 3
+    res = {}
+    for mf in (1, 0):
+        prev = core.set_tuning("mfma_smoother", mf)
+        try:
+            ms = _best_ms(lambda: sm.step(u, b), max(3, reps))
+        finally:
+            core.set_tuning("mfma_smoother", prev)
+        res["mfma" if mf else "fma"] = ms * 1e3 / 8
+    fl = mfma_flops_per_patch(k) * patches
+    ach = fl / (res["mfma"] * 8 * 1e-6) / 1e12
+    return {"kernel": f"patch3d_mfma_kernel<{k}>", "us_per_colour_mfma": round(res["mfma"], 2),
+            "us_per_colour_lds_fma": round(res["fma"], 2), "issued_TFLOPs": round(ach, 1),
+            "peak_TFLOPs": 157.3, "frac": round(ach / 157.3, 3),
+            "note": "issued FLOPs include the zero padding of 13/15-wide lines to 16"}
 
 
 def c5_report(ctx, core, problems, reps, rel_l2):

@@ -242,23 +242,52 @@ namespace gtps
           Lsh[0][e] = tab[TB::Lm + e];
           Lsh[1][e] = tab[TB::Lp + e];
         }
-      for (int e = tid; e < P3; e += C::NT)
-        {
-          const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
-          r[e] = bvec[g0 + x + (long long)y * N1 + (long long)z * P];
-        }
+      // b_P and the slabs of one face group: every load of a thread issued
+      // before the first wait
+      constexpr int ITB = (P3 + C::NT - 1) / C::NT, ITS = (NS * SL + C::NT - 1) / C::NT;
+      auto slab_value = [&](int f0, int e) -> T {
+        const int f = f0 + e / SL;
+        if (e >= NS * SL || !has_face(f))
+          return T(0);
+        const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
+        const int l = e % SL, j = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
+        const long long o = g0 + ((f & 1) ? (long long)PD : -(long long)NN) * st[D];
+        return u[o + aa * st[da] + bb * st[db] + j * st[D]];
+      };
+      {
+        T vb[ITB], vs[ITS];
+#pragma unroll
+        for (int it = 0; it < ITB; ++it)
+          {
+            const int e = tid + it * C::NT;
+            const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
+            vb[it] = e < P3 ? bvec[g0 + x + (long long)y * N1 + (long long)z * P] : T(0);
+          }
+#pragma unroll
+        for (int it = 0; it < ITS; ++it)
+          vs[it] = slab_value(0, tid + it * C::NT);
+#pragma unroll
+        for (int it = 0; it < ITB; ++it)
+          if (tid + it * C::NT < P3)
+            r[tid + it * C::NT] = vb[it];
+#pragma unroll
+        for (int it = 0; it < ITS; ++it)
+          if (tid + it * C::NT < NS * SL)
+            sa[tid + it * C::NT] = vs[it];
+      }
       // 2. external coupling, NS faces per phase
       for (int f0 = 0; f0 < 6; f0 += NS)
         {
-          for (int e = tid; e < NS * SL; e += C::NT)
+          if (f0 > 0)
             {
-              const int f = f0 + e / SL;
-              if (!has_face(f))
-                continue;
-              const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-              const int l = e % SL, j = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
-              const long long o = g0 + ((f & 1) ? (long long)PD : -(long long)NN) * st[D];
-              sa[e] = u[o + aa * st[da] + bb * st[db] + j * st[D]];
+              T vs[ITS];
+#pragma unroll
+              for (int it = 0; it < ITS; ++it)
+                vs[it] = slab_value(f0, tid + it * C::NT);
+#pragma unroll
+              for (int it = 0; it < ITS; ++it)
+                if (tid + it * C::NT < NS * SL)
+                  sa[tid + it * C::NT] = vs[it];
             }
           __syncthreads();
           // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]

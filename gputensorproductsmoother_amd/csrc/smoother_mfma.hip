@@ -18,8 +18,7 @@
 // is exactly the B operand of a following contraction along the same
 // direction: x-residual -> S^T along x and the eigenvalue scaling -> S
 // along z are chained in registers without an LDS round trip.
-// LDS holds three 16^3 arrays, x fastest, z-plane stride 272 floats so that
-// the strided reads of a half-wavefront fall in distinct banks.
+// LDS holds three arrays of the 13 interior z-planes (16 x 16, x fastest).
 #include "gtps_internal.h"
 
 namespace gtps
@@ -29,13 +28,13 @@ namespace gtps
     typedef float floatx4 __attribute__((ext_vector_type(4)));
 
     constexpr int MF_NT = 256;  // 4 wavefronts, one patch
-    constexpr int ZS    = 272;  // z-plane stride of an LDS array
-    constexpr int ARR   = 16 * ZS;
+    constexpr int NZ    = 13;   // z-planes of an LDS array (interior rows)
+    constexpr int ARR   = NZ * 256;
 
     __device__ __forceinline__ int
     lds_index(int x, int y, int z)
     {
-      return x + 16 * y + ZS * z;
+      return x + 16 * y + 256 * z;
     }
 
     // element (j along D) of line n of tile t; lines of a tile: D = x: y = n,
@@ -52,7 +51,8 @@ namespace gtps
     }
 
     // acc += W (16x16, this lane's 4 values w[s] = W[l%16][4(l/16)+s])
-    // applied along D to the 16 lines of tile t of `in`
+    // applied along D to the 16 lines of tile t of `in`; along z only the
+    // NZ stored planes are read (the padded weights of the others are 0)
     template <int D>
     __device__ __forceinline__ floatx4
     contract(const float *in, int t, const float (&w)[4], floatx4 acc)
@@ -60,8 +60,11 @@ namespace gtps
       const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
 #pragma unroll
       for (int s = 0; s < 4; ++s)
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w[s], in[line_index<D>(t, n, 4 * kq + s)],
-                                                   acc, 0, 0, 0);
+        {
+          const int j = 4 * kq + s;
+          const float b = (D != 2 || j < NZ) ? in[line_index<D>(t, n, j)] : 0.f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(w[s], b, acc, 0, 0, 0);
+        }
       return acc;
     }
 
@@ -76,6 +79,7 @@ namespace gtps
       return acc;
     }
 
+    // rows 4(l/16)..+3 of line l%16; along z only the NZ stored planes
     template <int D>
     __device__ __forceinline__ void
     store_tile(float *out, int t, const floatx4 &v)
@@ -83,7 +87,8 @@ namespace gtps
       const int l = threadIdx.x & 63, n = l & 15, kq = l >> 4;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        out[line_index<D>(t, n, 4 * kq + r)] = v[r];
+        if (D != 2 || 4 * kq + r < NZ)
+          out[line_index<D>(t, n, 4 * kq + r)] = v[r];
     }
 
     // this lane's A-operand values of a padded 16x16 matrix given by f(i, j)
@@ -97,6 +102,12 @@ namespace gtps
         w[s] = f(i, 4 * kq + s);
     }
 
+    // Tiles of a stage: t = wave + 4 tt.  Every LDS array holds the NZ = 13
+    // interior z-planes (16 x 16 each, x fastest): the z pass reads the box
+    // straight from global memory into its B operands, so no array holds
+    // the 15 box planes, and 3 arrays (39 KiB) leave room for 4 work-groups
+    // per CU.  Tiles whose lines lie outside the interior are skipped (or
+    // stored as zeros where a later stage reads them).
     template <int K>
     __global__ void __launch_bounds__(MF_NT)
     patch3d_mfma_kernel(float *__restrict__ u, const float *__restrict__ bvec,
@@ -105,8 +116,8 @@ namespace gtps
     {
       using TB         = Tab<K>;
       constexpr int NP = TB::NP, NI = TB::NI;
-      static_assert(NP <= 16, "patch box must fit a 16-wide tile");
-      __shared__ float sP[ARR], sQ[ARR], sR[ARR];
+      static_assert(NP <= 16 && NI <= NZ, "patch box must fit a 16-wide tile");
+      __shared__ float sA[ARR], sB[ARR], sC[ARR];
 
       const int       N1   = K * n + 1;
       const long long P    = (long long)N1 * N1;
@@ -128,126 +139,146 @@ namespace gtps
       load_weights(wST, [&](int i, int j) { return i < NI && j < NI ? S[j * NI + i] : 0.f; });
       load_weights(wS, [&](int i, int j) { return i < NI && j < NI ? S[i * NI + j] : 0.f; });
 
-      // u on the box skeleton (interior zeroed), zero padding, and this
-      // lane's interior b values (consumed by the x pass): every load issued
-      // before the first wait
-      constexpr int IT = 16 * 16 * 16 / MF_NT;
-      float         uv[IT], bv[4][4];
+      // the z pass's B operands: u on the box skeleton (interior zeroed) at
+      // x = ln, y = t, z = 4(l/16) + s; and this lane's interior b values
+      // (x pass): every load issued before the first wait
+      float uz[4][4], bv[4][4];
 #pragma unroll
-      for (int it = 0; it < IT; ++it)
-        {
-          const int e = tid + it * MF_NT;
-          const int x = e & 15, y = (e >> 4) & 15, z = e >> 8;
-          const bool inner = x > 0 && x < NP - 1 && y > 0 && y < NP - 1 && z > 0 && z < NP - 1;
-          uv[it] = (x < NP && y < NP && z < NP && !inner)
-                     ? u[first + x + (long long)y * N1 + (long long)z * P]
-                     : 0.f;
-        }
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          {
+            const int  t = wave + 4 * tt, z = 4 * kq + s;
+            const bool inner = ln > 0 && ln < NP - 1 && t > 0 && t < NP - 1 && z > 0 &&
+                               z < NP - 1;
+            uz[tt][s] = (ln < NP && t < NP && z < NP && !inner)
+                          ? u[first + ln + (long long)t * N1 + (long long)z * P]
+                          : 0.f;
+          }
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           {
-            const int t = wave * 4 + tt, xi = 4 * kq + r;
+            const int t = wave + 4 * tt, xi = 4 * kq + r;
             bv[tt][r]   = (xi < NI && ln < NI && t < NI)
                             ? bvec[first + (xi + 1) + (long long)(ln + 1) * N1 +
                                    (long long)(t + 1) * P]
                             : 0.f;
           }
-#pragma unroll
-      for (int it = 0; it < IT; ++it)
-        {
-          const int e = tid + it * MF_NT;
-          sP[lds_index(e & 15, (e >> 4) & 15, e >> 8)] = uv[it];
-        }
-      __syncthreads();
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
-      // z pass: Q = Mz u, R = Kz u
+      // z pass (lines x = ln, y = t): A = Mz u, B = Kz u
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t = wave * 4 + tt;
-          store_tile<2>(sQ, t, contract<2>(sP, t, wM, zero));
-          store_tile<2>(sR, t, contract<2>(sP, t, wK, zero));
+          const int t = wave + 4 * tt;
+          floatx4   m = zero, k = zero;
+          if (t < NP)
+            {
+              floatx4 v = {uz[tt][0], uz[tt][1], uz[tt][2], uz[tt][3]};
+              m = contract_regs(v, wM, zero);
+              k = contract_regs(v, wK, zero);
+            }
+          store_tile<2>(sA, t, m);
+          store_tile<2>(sB, t, k);
         }
       __syncthreads();
-      // y pass: P = My Q, Q = Ky Q + My R (Q rewritten after every wave has
-      // read it)
+      // y pass (lines x = ln, z = t < NZ): C = My A, then A = Ky A + My B
+      // once every wave has read A
       floatx4 c2[4];
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t = wave * 4 + tt;
-          store_tile<1>(sP, t, contract<1>(sQ, t, wM, zero));
-          c2[tt] = contract<1>(sR, t, wM, contract<1>(sQ, t, wK, zero));
-        }
-      __syncthreads();
-#pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
-        store_tile<1>(sQ, wave * 4 + tt, c2[tt]);
-      __syncthreads();
-      // x pass: r = b_I - h (Kx P + Mx Q), then S^T along x (registers)
-#pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
-        {
-          const int t   = wave * 4 + tt;  // interior z
-          floatx4   acc = contract<0>(sQ, t, wM, contract<0>(sP, t, wK, zero));
-          floatx4   res;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+          const int t = wave + 4 * tt;
+          if (t < NZ)
             {
-              const int xi = 4 * kq + r;  // interior x; interior y = ln
-              res[r] = (xi < NI && ln < NI && t < NI) ? bv[tt][r] - scale * acc[r] : 0.f;
+              store_tile<1>(sC, t, contract<1>(sA, t, wM, zero));
+              c2[tt] = contract<1>(sB, t, wM, contract<1>(sA, t, wK, zero));
             }
-          store_tile<0>(sR, t, contract_regs(res, wST, zero));
         }
       __syncthreads();
-      // S^T along y
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t = wave * 4 + tt;
-          store_tile<1>(sP, t, contract<1>(sR, t, wST, zero));
+          const int t = wave + 4 * tt;
+          if (t < NZ)
+            store_tile<1>(sA, t, c2[tt]);
         }
       __syncthreads();
-      // S^T along z, eigenvalue scaling, S along z (registers)
+      // x pass (lines y = ln, z = t): r = b_I - h (Kx C + Mx A), then S^T
+      // along x in registers, into B
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t   = wave * 4 + tt;  // y (eigen index); x = ln
-          floatx4   acc = contract<2>(sP, t, wST, zero);
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+          const int t = wave + 4 * tt;
+          if (t < NZ)
             {
-              const int zi = 4 * kq + r;
-              acc[r] = (zi < NI && ln < NI && t < NI)
-                         ? acc[r] * linv[(zi * NI + t) * NI + ln] * inv_scale
-                         : 0.f;
+              const floatx4 acc = contract<0>(sA, t, wM, contract<0>(sC, t, wK, zero));
+              floatx4       res;
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                res[r] = (4 * kq + r < NI && ln < NI && t < NI) ? bv[tt][r] - scale * acc[r]
+                                                                 : 0.f;
+              store_tile<0>(sB, t, contract_regs(res, wST, zero));
             }
-          store_tile<2>(sQ, t, contract_regs(acc, wS, zero));
         }
       __syncthreads();
-      // S along y
+      // S^T along y (lines x = ln, z = t): B -> C
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t = wave * 4 + tt;
-          store_tile<1>(sR, t, contract<1>(sQ, t, wS, zero));
+          const int t = wave + 4 * tt;
+          if (t < NZ)
+            store_tile<1>(sC, t, contract<1>(sB, t, wST, zero));
         }
       __syncthreads();
-      // S along x, write the interior
+      // S^T along z (lines x = ln, y = t), eigenvalue scaling, S along z in
+      // registers: C -> A (rows y >= NI stored as zeros)
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
-          const int t   = wave * 4 + tt;  // interior z
-          floatx4   acc = contract<0>(sR, t, wS, zero);
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
+          const int t = wave + 4 * tt;
+          floatx4   out = zero;
+          if (t < NI)
             {
-              const int xi = 4 * kq + r;
-              if (xi < NI && ln < NI && t < NI)
-                u[first + (xi + 1) + (long long)(ln + 1) * N1 + (long long)(t + 1) * P] =
-                  relax * acc[r];
+              floatx4 acc = contract<2>(sC, t, wST, zero);
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                {
+                  const int zi = 4 * kq + r;
+                  acc[r] = (zi < NI && ln < NI) ? acc[r] * linv[(zi * NI + t) * NI + ln] * inv_scale
+                                                : 0.f;
+                }
+              out = contract_regs(acc, wS, zero);
+            }
+          store_tile<2>(sA, t, out);
+        }
+      __syncthreads();
+      // S along y (lines x = ln, z = t): A -> B
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+        {
+          const int t = wave + 4 * tt;
+          if (t < NZ)
+            store_tile<1>(sB, t, contract<1>(sA, t, wS, zero));
+        }
+      __syncthreads();
+      // S along x (lines y = ln, z = t), write the interior
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+        {
+          const int t = wave + 4 * tt;
+          if (t < NI)
+            {
+              const floatx4 acc = contract<0>(sB, t, wS, zero);
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                {
+                  const int xi = 4 * kq + r;
+                  if (xi < NI && ln < NI)
+                    u[first + (xi + 1) + (long long)(ln + 1) * N1 + (long long)(t + 1) * P] =
+                      relax * acc[r];
+                }
             }
         }
     }
