@@ -306,11 +306,33 @@ namespace gtps
           sc[tid] = coef[(c0x + ox) + (long long)n * (c0y + oy) +
                          (long long)n * n * (c0z + oz)];
         }
-      for (int t = tid; t < NP * NP * NP; t += CNT)
-        {
-          const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
-          sbox[t] = u[first + ix + (long long)iy * N1 + iz * P];
-        }
+      // the box of u and this thread's interior b values: every load issued
+      // before the first wait
+      constexpr int ITU = (NP * NP * NP + CNT - 1) / CNT, ITR = (NI * NI * NI + CNT - 1) / CNT;
+      T             bres[ITR];
+      {
+        T vu[ITU];
+#pragma unroll
+        for (int it = 0; it < ITU; ++it)
+          {
+            const int t  = tid + it * CNT;
+            const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
+            vu[it] = t < NP * NP * NP ? u[first + ix + (long long)iy * N1 + iz * P] : T(0);
+          }
+#pragma unroll
+        for (int it = 0; it < ITR; ++it)
+          {
+            const int t = tid + it * CNT;
+            const int i = t % NI, j = (t / NI) % NI, l = t / (NI * NI);
+            bres[it]    = t < NI * NI * NI
+                            ? bvec[first + (i + 1) + (long long)(j + 1) * N1 + (l + 1) * P]
+                            : T(0);
+          }
+#pragma unroll
+        for (int it = 0; it < ITU; ++it)
+          if (tid + it * CNT < NP * NP * NP)
+            sbox[tid + it * CNT] = vu[it];
+      }
       __syncthreads();
       // cell z-rows: task (cell, ix, iy)
       for (int t = tid; t < 8 * N2; t += CNT)
@@ -391,8 +413,12 @@ namespace gtps
         }
       __syncthreads();
       // interior residual r = b - sum of the cells containing the DoF
-      for (int t = tid; t < NI * NI * NI; t += CNT)
+#pragma unroll
+      for (int it = 0; it < ITR; ++it)
         {
+          const int t = tid + it * CNT;
+          if (t >= NI * NI * NI)
+            break;
           const int i = t % NI, j = (t / NI) % NI, l = t / (NI * NI);
           const int p[3] = {i + 1, j + 1, l + 1};
           int       co[3][2], lo[3][2], cnt[3];
@@ -428,8 +454,7 @@ namespace gtps
                   const int cell = co[0][c] + 2 * co[1][bb] + 4 * co[2][a];
                   s += sa[cell * N3 + (lo[2][a] * NN + lo[1][bb]) * NN + lo[0][c]];
                 }
-          const long long g = first + p[0] + (long long)p[1] * N1 + p[2] * P;
-          sr[t] = bvec[g] - s;
+          sr[t] = bres[it] - s;
         }
       T cmax = sc[0];
 #pragma unroll
