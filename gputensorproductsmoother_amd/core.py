@@ -610,6 +610,10 @@ class DGMultigrid:
         self._h = h
         self.n_dofs = int(L.load().gtps_dg_mg_n_dofs(h))
 
+    def set_graph(self, enable: bool):
+        """HIP-graph replay of the per-iteration preconditioning sequence."""
+        L.check(L.load().gtps_dg_mg_set_graph(self._h, int(enable)), "dg_set_graph")
+
     def vmult(self, dst: torch.Tensor, src: torch.Tensor):
         """Preconditioner z = V(r) (FP64 in and out)."""
         _chk_vec(dst, torch.float64, self.n_dofs)

@@ -816,6 +816,11 @@ struct gtps_dg_mg
   std::vector<void *>           d, u, t;
   double                       *r = nullptr, *p = nullptr, *q = nullptr;
   hipEvent_t                    ev = nullptr;
+  // the per-iteration preconditioning sequence replayed as one HIP graph
+  // (about 180 launches per V-cycle), captured lazily on a private stream
+  bool                          use_graph  = true;
+  hipStream_t                   cap_stream = nullptr;
+  hipGraphExec_t                dir_exec   = nullptr;
 
   void
   vcycle(int l, const void *dl, void *ul)
@@ -835,9 +840,56 @@ struct gtps_dg_mg
     launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, relax);
   }
 
-  // z = V(r); rz_old = rz; rz = r.z; beta; p = z + beta p
+  void
+  capture_direction()
+  {
+    if (!cap_stream)
+      GTPS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+    hipStream_t saved = ctx->stream;
+    ctx->stream       = cap_stream;
+    hipGraph_t graph  = nullptr;
+    try
+      {
+        GTPS_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+        enqueue_direction();
+        GTPS_HIP(hipStreamEndCapture(cap_stream, &graph));
+      }
+    catch (...)
+      {
+        ctx->stream = saved;
+        if (!graph)
+          {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cap_stream, &g);
+            if (g)
+              (void)hipGraphDestroy(g);
+          }
+        else
+          (void)hipGraphDestroy(graph);
+        throw;
+      }
+    ctx->stream  = saved;
+    hipError_t e = hipGraphInstantiate(&dir_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GTPS_HIP(e);
+  }
+
   void
   direction()
+  {
+    if (!use_graph)
+      {
+        enqueue_direction();
+        return;
+      }
+    if (!dir_exec)
+      capture_direction();
+    GTPS_HIP(hipGraphLaunch(dir_exec, ctx->stream));
+  }
+
+  // z = V(r); rz_old = rz; rz = r.z; beta; p = z + beta p
+  void
+  enqueue_direction()
   {
     hipStream_t     s   = ctx->stream;
     const long long N   = fine64->N;
@@ -857,6 +909,10 @@ struct gtps_dg_mg
 
   ~gtps_dg_mg()
   {
+    if (dir_exec)
+      (void)hipGraphExecDestroy(dir_exec);
+    if (cap_stream)
+      (void)hipStreamDestroy(cap_stream);
     for (size_t l = 0; l < lv.size(); ++l)
       {
         dfree(d.size() > l ? d[l] : nullptr);
@@ -1902,6 +1958,15 @@ extern "C"
       (void)hipSetDevice(mg->ctx->device);
       (void)hipStreamSynchronize(mg->ctx->stream);
       delete mg;
+    });
+  }
+
+  int
+  gtps_dg_mg_set_graph(gtps_dg_mg *mg, int enable)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg, "NULL argument");
+      mg->use_graph = enable != 0;
     });
   }
 

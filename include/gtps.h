@@ -354,6 +354,9 @@ int gtps_dg_mg_create(gtps_ctx *ctx, int dim, int degree, int n_coarse,
                       int max_level, int vcycle_precision, double penalty,
                       int coarse_sweeps, gtps_dg_mg **out);
 int gtps_dg_mg_destroy(gtps_dg_mg *mg);
+/* enable (default) / disable the HIP-graph replay of the per-iteration
+ * preconditioning sequence in gtps_dg_mg_pcg_solve */
+int gtps_dg_mg_set_graph(gtps_dg_mg *mg, int enable);
 long long gtps_dg_mg_n_dofs(const gtps_dg_mg *mg);
 int gtps_dg_mg_matvec(gtps_dg_mg *mg, double *dst, const double *src);
 int gtps_dg_mg_vcycle(gtps_dg_mg *mg, double *z, const double *r);
