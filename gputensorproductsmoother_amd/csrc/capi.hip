@@ -1499,6 +1499,12 @@ extern "C"
           prev                  = tuning().dense_coarse;
           tuning().dense_coarse = (int)value;
         }
+      else if (k == "vmult_zdiv")
+        {
+          GTPS_REQUIRE(value >= 1 && value <= (1 << 20), "vmult_zdiv must be in 1..2^20");
+          prev                = tuning().vmult_zdiv;
+          tuning().vmult_zdiv = value;
+        }
       else if (k == "mfma_smoother")
         {
           GTPS_REQUIRE(value == 0 || value == 1, "mfma_smoother must be 0 or 1");
@@ -1532,6 +1538,8 @@ extern "C"
         v = tuning().dense_coarse;
       else if (k == "mfma_smoother")
         v = tuning().mfma_smoother;
+      else if (k == "vmult_zdiv")
+        v = tuning().vmult_zdiv;
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });

@@ -184,6 +184,9 @@ namespace gtps
     int       dense_coarse = 1;
     // 3D FP32 degree 4..7 multiplicative smoother on the matrix cores
     int       mfma_smoother = 1;
+    // z-marching vmult: target work-group count divisor (cells per chunk =
+    // ceil(n tx ty / vmult_zdiv))
+    long long vmult_zdiv = 512;
   };
   Tuning &tuning();
 

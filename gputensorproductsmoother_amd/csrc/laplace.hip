@@ -675,7 +675,8 @@ namespace gtps
           const int tx  = (n + C::TCX - 1) / C::TCX;
           const int ty  = (n + C::TCY - 1) / C::TCY;
           // z-chunks: about two 512-thread work-groups per CU
-          int zc = (int)(((long long)n * tx * ty + 511) / 512);
+          const long long zdiv = tuning().vmult_zdiv > 0 ? tuning().vmult_zdiv : 512;
+          int zc = (int)(((long long)n * tx * ty + zdiv - 1) / zdiv);
           zc     = std::max(1, std::min(zc, n));
           const int tz = (n + zc - 1) / zc;
           const dim3 grid(tx, ty, tz);

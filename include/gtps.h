@@ -94,6 +94,8 @@ const char *gtps_version(void);
  *                           coefficient only); 0: the ordinary
  *                           level-2 smoothing and level-1 solve.  Set it
  *                           before the hierarchy's first solve.
+ *   "vmult_zdiv"            z-marching vmult work-group count target: z-chunks
+ *                           of ceil(n tx ty / value) cells (default 512);
  *   "mfma_smoother"         1 (default): 3D FP32 multiplicative smoothing of
  *                           degree 4..7 runs its 1D contractions on the
  *                           matrix cores (v_mfma_f32_16x16x4f32, patches
