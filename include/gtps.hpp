@@ -4,6 +4,8 @@
 //   PSMF::PatchSmoother     include/patch_smoother.cuh:86-124, 313-335
 //   PSMF::MGTransferCUDA    include/cuda_mg_transfer.cuh:57-358
 //   PSMF::MultigridSolvers  include/solver.cuh:1541-2210
+//   PSMF::LaplaceDGOperator, PSMF::DGMultigridSolvers  tests/mf_dg.cu:209-274
+//   gtps::rhs_sine / error_norms  apps/poisson.cu:44-87, 353-382
 // Vectors are raw device pointers (what deal.II's
 // LinearAlgebra::distributed::Vector<Number, MemorySpace::CUDA>::get_values()
 // returns), lexicographic with boundary DoFs.  Errors throw gtps::Error, as
@@ -228,10 +230,141 @@ namespace PSMF
     {
       gtps::check(gtps_mg_matvec(h.get(), dst, src));
     }
+    // full multigrid (solver.cuh:1784-1860): rhs[l-1] is level l's FP64
+    // right-hand side, l = 1..max_level
+    void
+    solve(double *x, const std::vector<const double *> &rhs, int n_cycles = 1) const
+    {
+      gtps::check(gtps_mg_fmg_solve(h.get(), x, rhs.data(), n_cycles));
+    }
+    // -div(c grad u) with one FP64 device value per finest cell (NULL: c = 1)
+    void
+    set_coefficient(const double *coef) const
+    {
+      gtps::check(gtps_mg_set_coefficient(h.get(), coef));
+    }
+    void
+    matvec_transpose(double *dst, const double *src) const
+    {
+      gtps::check(gtps_mg_matvec_transpose(h.get(), dst, src));
+    }
+    // grad[K] = -lambda_K^T A^1_K u_K (dJ/dc for the adjoint lambda)
+    void
+    coefficient_gradient(double *grad, const double *lambda, const double *u) const
+    {
+      gtps::check(gtps_mg_coefficient_gradient(h.get(), grad, lambda, u));
+    }
 
   private:
     std::shared_ptr<gtps_mg> h;
   };
+
+  // SIPG-DG Laplacian of one level (the operator of tests/mf_dg.cu:209-274
+  // on a uniform mesh; penalty = sigma h, k (k+1) there)
+  template <int dim, int fe_degree, typename Number>
+  class LaplaceDGOperator
+  {
+  public:
+    LaplaceDGOperator(const gtps::Context &ctx, int n_cells, double penalty)
+    {
+      gtps_dg_level *l = nullptr;
+      gtps::check(gtps_dg_level_create(ctx.get(), dim, fe_degree, n_cells,
+                                       gtps::precision_of<Number>(), penalty, &l));
+      h.reset(l, [](gtps_dg_level *p) { gtps_dg_level_destroy(p); });
+    }
+    long long
+    n_dofs() const
+    {
+      return gtps_dg_level_n_dofs(h.get());
+    }
+    void
+    vmult(Number *dst, const Number *src) const
+    {
+      gtps::check(gtps_dg_vmult(h.get(), dst, src));
+    }
+    void
+    smoother_step(Number *u, const Number *b, bool reverse = false,
+                  double relaxation = 1.0) const
+    {
+      gtps::check(gtps_dg_smoother_step(h.get(), u, b,
+                                        reverse ? GTPS_MULT_REV : GTPS_MULT_FWD,
+                                        relaxation));
+    }
+
+  private:
+    std::shared_ptr<gtps_dg_level> h;
+  };
+
+  // DG multigrid (n_coarse 2^l cells, l = 0..max_level) with FP64 MG-CG
+  template <int dim, int fe_degree, typename Number>
+  class DGMultigridSolvers
+  {
+  public:
+    DGMultigridSolvers(const gtps::Context &ctx, int n_coarse, int max_level,
+                       double penalty, int coarse_sweeps = 2)
+    {
+      gtps_dg_mg *m = nullptr;
+      gtps::check(gtps_dg_mg_create(ctx.get(), dim, fe_degree, n_coarse, max_level,
+                                    gtps::precision_of<Number>(), penalty,
+                                    coarse_sweeps, &m));
+      h.reset(m, [](gtps_dg_mg *p) { gtps_dg_mg_destroy(p); });
+    }
+    long long
+    n_dofs() const
+    {
+      return gtps_dg_mg_n_dofs(h.get());
+    }
+    int
+    solve_cg(double *x, const double *b, double rtol = 1e-8, int maxit = 200) const
+    {
+      int    it  = 0;
+      double rel = 0.;
+      gtps::check(gtps_dg_mg_pcg_solve(h.get(), x, b, rtol, maxit, &it, &rel, nullptr));
+      return it;
+    }
+
+  private:
+    std::shared_ptr<gtps_dg_mg> h;
+  };
 } // namespace PSMF
+
+namespace gtps
+{
+  // the reference's Poisson problem (apps/poisson.cu:44-87): load vector of
+  // f = dim (w pi)^2 prod sin(w pi x_d) on 2^level cells, and the L2 / H1
+  // errors against prod sin(w pi x_d) (compute_error, apps/poisson.cu:353-382)
+  inline void
+  rhs_sine(const Context &ctx, int dim, int degree, int level, double *b,
+           double wave = 3.0)
+  {
+    check(gtps_rhs_sine(ctx.get(), dim, degree, level, wave, b));
+  }
+  struct ErrorNorms
+  {
+    double l2 = 0., h1 = 0.;
+  };
+  inline ErrorNorms
+  error_norms(const Context &ctx, int dim, int degree, int level, const double *u,
+              double wave = 3.0, int n_q = 0)
+  {
+    ErrorNorms e;
+    check(gtps_error_norms(ctx.get(), dim, degree, level, wave,
+                           n_q > 0 ? n_q : degree + 1, u, &e.l2, &e.h1));
+    return e;
+  }
+  // solution files (64-byte header + values, FNV-1a checked on load)
+  inline void
+  save_vector(const Context &ctx, const std::string &path, const double *x, int dim,
+              int degree, int n_cells, int layout = GTPS_LAYOUT_CG)
+  {
+    check(gtps_vector_save(ctx.get(), path.c_str(), x, GTPS_FP64, dim, degree, n_cells,
+                           layout));
+  }
+  inline void
+  load_vector(const Context &ctx, const std::string &path, double *x, long long n)
+  {
+    check(gtps_vector_load(ctx.get(), path.c_str(), x, n, GTPS_FP64));
+  }
+} // namespace gtps
 
 #endif // GTPS_HPP

@@ -35,13 +35,19 @@ def _has_gpu():
 def test_facade_runs_or_fails_cleanly(exe):
     if _has_gpu():
         pytest.skip("covered by the gpu-marked run")
-    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    p = subprocess.run([exe, "/nonexistent/u.gtps"], capture_output=True, text=True,
+                       timeout=120)
     assert p.returncode == 3, (p.returncode, p.stdout, p.stderr)
     assert "gtps::Error" in p.stdout
 
 
 @pytest.mark.gpu
-def test_facade_solves_on_gpu(exe):
-    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
-    assert p.returncode == 0, (p.stdout, p.stderr)
-    assert "iterations" in p.stdout
+def test_facade_solves_on_gpu(exe, tmp_path):
+    """MG-CG and FMG on the reference's Poisson problem through the PSMF
+    facade, L2 order ~4 between 8^3 and 16^3 cells (oracle: 3.94), a
+    solution-file round trip and a DG solve."""
+    p = subprocess.run([exe, str(tmp_path / "u.gtps")], capture_output=True, text=True,
+                       timeout=120)
+    print(p.stdout)
+    assert p.returncode == 0, (p.returncode, p.stdout, p.stderr)
+    assert "iterations" in p.stdout and "dg iterations" in p.stdout
