@@ -821,11 +821,55 @@ struct gtps_dg_mg
   bool                          use_graph  = true;
   hipStream_t                   cap_stream = nullptr;
   hipGraphExec_t                dir_exec   = nullptr;
+  // the coarsest solve (symmetric sweep pairs from zero, a fixed linear
+  // map) tabulated as a dense matrix on first use (coarse.hip)
+  void                         *dense0       = nullptr;
+  bool                          dense0_valid = false;
+
+  void
+  ensure_dense0()
+  {
+    const gtps_dg_level *L = lv[0];
+    if (dense0_valid || tuning().dense_coarse == 0 || L->n > 4 || L->N > MAX_DENSE_COARSE)
+      return;
+    hipStream_t  s  = ctx->stream;
+    const int    N  = (int)L->N;
+    const size_t es = prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
+    void *D = dmalloc(es * N * N), *U = nullptr;
+    try
+      {
+        U = dmalloc(es * N * N);
+        if (!dense0)
+          dense0 = dmalloc(es * N * N);
+        launch_unit_defects_plain(s, N, prec, D);
+        launch_fill(s, (long long)N * N, prec, U, 0.0);
+        launch_dg_sweeps_batched(L, U, D, GTPS_MULT_FWD, 2 * coarse_sweeps, 1.0, N);
+        launch_transpose_plain(s, N, prec, U, dense0);
+        GTPS_HIP(hipStreamSynchronize(s));
+      }
+    catch (...)
+      {
+        (void)hipStreamSynchronize(s);
+        dfree(D);
+        dfree(U);
+        dfree(dense0);
+        dense0 = nullptr;
+        throw;
+      }
+    dfree(D);
+    dfree(U);
+    dense0_valid = true;
+  }
 
   void
   vcycle(int l, const void *dl, void *ul)
   {
     const gtps_dg_level *L = lv[l];
+    if (l == 0 && dense0_valid && tuning().dense_coarse != 0)
+      {
+        launch_dense_apply_plain(ctx->stream, (int)L->N, prec, dense0, dl, ul);
+        return;
+      }
     launch_fill(ctx->stream, L->N, prec, ul, 0.0);
     if (l == 0)
       {
@@ -877,6 +921,7 @@ struct gtps_dg_mg
   void
   direction()
   {
+    ensure_dense0();
     if (!use_graph)
       {
         enqueue_direction();
@@ -909,6 +954,7 @@ struct gtps_dg_mg
 
   ~gtps_dg_mg()
   {
+    dfree(dense0);
     if (dir_exec)
       (void)hipGraphExecDestroy(dir_exec);
     if (cap_stream)
@@ -2001,6 +2047,7 @@ extern "C"
     return guard([&] {
       GTPS_REQUIRE(mg && z && r && (const void *)z != (const void *)r, "bad argument");
       set_device(mg->ctx);
+      mg->ensure_dense0();
       const long long N = mg->fine64->N;
       if (mg->prec == GTPS_FP32)
         {

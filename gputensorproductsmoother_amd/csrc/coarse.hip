@@ -8,7 +8,10 @@
 // the map is tabulated once at setup, column by column with the ordinary
 // V-cycle kernels applied to unit defects, and each V-cycle then replaces
 // the whole sub-cycle by one matrix-vector product over the interior DoFs.
-// The result is the same linear map, rounded differently.
+// The result is the same linear map, rounded differently.  The DG
+// hierarchy tabulates its coarsest solve (symmetric sweep pairs from zero on
+// a mesh of at most 4 cells per direction) the same way, with the plain
+// forms below (no constrained rows), one column per work-group.
 #include "gtps_internal.h"
 
 namespace gtps
@@ -18,11 +21,14 @@ namespace gtps
     constexpr int CNT = 256;
     constexpr int ROWS_PER_WAVE = 4;
 
-    // interior DoF j (x fastest over the (N1-2)^dim interior) -> level index
+    // interior DoF j (x fastest over the (N1-2)^dim interior) -> level index;
+    // DIM = 0: no constrained rows, the identity
     template <int DIM>
     __device__ __forceinline__ long long
     interior_index(int j, int N1)
     {
+      if (DIM == 0)
+        return j;
       const int ni1 = N1 - 2;
       const int x = j % ni1 + 1, y = (j / ni1) % ni1 + 1;
       const int z = DIM == 3 ? j / (ni1 * ni1) + 1 : 0;
@@ -68,7 +74,7 @@ namespace gtps
       for (int j = tid; j < ni; j += CNT)
         sd[j] = d[interior_index<DIM>(j, N1)];
       // Dirichlet rows: zero (spread over the grid)
-      for (long long g = (long long)blockIdx.x * CNT + tid; g < N;
+      for (long long g = (long long)blockIdx.x * CNT + tid; DIM != 0 && g < N;
            g += (long long)gridDim.x * CNT)
         {
           const long long x = g % N1, y = (g / N1) % N1, z = DIM == 3 ? g / ((long long)N1 * N1) : 1;
@@ -122,6 +128,47 @@ namespace gtps
         }
     }
   } // namespace
+
+  void
+  launch_unit_defects_plain(hipStream_t s, int n, int prec, void *D)
+  {
+    if (prec == GTPS_FP32)
+      hipLaunchKernelGGL((unit_defects_kernel<float, 0>), dim3(cgrid((long long)n * n)),
+                         dim3(CNT), 0, s, (float *)D, 0, (long long)n, n);
+    else
+      hipLaunchKernelGGL((unit_defects_kernel<double, 0>), dim3(cgrid((long long)n * n)),
+                         dim3(CNT), 0, s, (double *)D, 0, (long long)n, n);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_transpose_plain(hipStream_t s, int n, int prec, const void *U, void *B)
+  {
+    if (prec == GTPS_FP32)
+      hipLaunchKernelGGL((gather_columns_kernel<float, 0>), dim3(cgrid((long long)n * n)),
+                         dim3(CNT), 0, s, (const float *)U, (float *)B, 0, (long long)n, n);
+    else
+      hipLaunchKernelGGL((gather_columns_kernel<double, 0>), dim3(cgrid((long long)n * n)),
+                         dim3(CNT), 0, s, (const double *)U, (double *)B, 0, (long long)n, n);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_dense_apply_plain(hipStream_t s, int n, int prec, const void *B, const void *d,
+                           void *u)
+  {
+    GTPS_REQUIRE(n <= MAX_DENSE_COARSE, "dense coarse operator too large");
+    const int rows_per_block = (CNT / 64) * ROWS_PER_WAVE;
+    const int nb             = (n + rows_per_block - 1) / rows_per_block;
+    if (prec == GTPS_FP32)
+      hipLaunchKernelGGL((dense_apply_kernel<float, 0>), dim3(nb), dim3(CNT), 0, s,
+                         (const float *)B, (const float *)d, (float *)u, 0, (long long)n, n);
+    else
+      hipLaunchKernelGGL((dense_apply_kernel<double, 0>), dim3(nb), dim3(CNT), 0, s,
+                         (const double *)B, (const double *)d, (double *)u, 0, (long long)n,
+                         n);
+    GTPS_LAUNCH_CHECK();
+  }
 
   int
   dense_coarse_size(int dim, int N1)

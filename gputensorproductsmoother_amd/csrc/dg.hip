@@ -402,14 +402,19 @@ namespace gtps
 
     // meshes of at most 4 cells per direction (at most one patch per
     // colour): n_sweeps whole sweeps in one work-group, alternating forward
-    // and reverse colour order from `first`, patches one after another
+    // and reverse colour order from `first`, patches one after another;
+    // launched with several work-groups, one independent column each
     template <int K, typename T>
     __global__ void __launch_bounds__((DP<K, T>::NT))
     dg_patch_seq_kernel(T *__restrict__ u, const T *__restrict__ bvec,
                         const T *__restrict__ tab, const int n, const int first,
-                        const int n_sweeps, const T scale, const T relax)
+                        const int n_sweeps, const T scale, const T relax,
+                        const long long column_stride)
     {
       __shared__ DPShared<K, T> sh;
+      // batched form: work-group j smooths column j of (u, b)
+      u += (long long)blockIdx.x * column_stride;
+      bvec += (long long)blockIdx.x * column_stride;
       for (int s = 0; s < n_sweeps; ++s)
         {
           const bool rev = ((first + s) & 1) != 0;
@@ -635,18 +640,19 @@ namespace gtps
     template <int K, typename T>
     void
     sweeps_k(const gtps_dg_level *lvl, T *u, const T *b, int first, int n_sweeps,
-             double relax)
+             double relax, int columns = 1)
     {
       if constexpr (DP<K, T>::ok)
         {
           const int   n = lvl->n;
           hipStream_t s = lvl->ctx->stream;
+          GTPS_REQUIRE(columns == 1 || n <= 4, "batched sweeps need at most 4 cells per direction");
           if (n <= 4)
             {
-              hipLaunchKernelGGL((dg_patch_seq_kernel<K, T>), dim3(1), dim3(DP<K, T>::NT), 0, s, u,
-                                 b, (const T *)lvl->tab, n,
+              hipLaunchKernelGGL((dg_patch_seq_kernel<K, T>), dim3(columns),
+                                 dim3(DP<K, T>::NT), 0, s, u, b, (const T *)lvl->tab, n,
                                  first == GTPS_MULT_REV ? 1 : 0, n_sweeps,
-                                 (T)lvl->scale, (T)relax);
+                                 (T)lvl->scale, (T)relax, lvl->N);
               GTPS_LAUNCH_CHECK();
               return;
             }
@@ -743,6 +749,23 @@ namespace gtps
         sweeps_k<K, float>(lvl, (float *)u, (const float *)b, variant, n_sweeps, relax);
       else
         sweeps_k<K, double>(lvl, (double *)u, (const double *)b, variant, n_sweeps, relax);
+      return 0;
+    });
+  }
+
+  void
+  launch_dg_sweeps_batched(const gtps_dg_level *lvl, void *u, const void *b, int first,
+                           int n_sweeps, double relax, int columns)
+  {
+    GTPS_REQUIRE(lvl->n >= 2 && lvl->n <= 4, "batched sweeps need 2..4 cells per direction");
+    GTPS_REQUIRE(columns >= 1 && n_sweeps >= 1, "bad batch");
+    dispatch_k(lvl->k, [&](auto kk) {
+      constexpr int K = decltype(kk)::value;
+      if (lvl->prec == GTPS_FP32)
+        sweeps_k<K, float>(lvl, (float *)u, (const float *)b, first, n_sweeps, relax, columns);
+      else
+        sweeps_k<K, double>(lvl, (double *)u, (const double *)b, first, n_sweeps, relax,
+                            columns);
       return 0;
     });
   }

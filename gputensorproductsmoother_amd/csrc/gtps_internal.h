@@ -167,7 +167,7 @@ namespace gtps
     long long small_colour_patches = 1LL << 62;
     // patch-group kernel: colours with more than group_min_patches patches
     // pack patches_per_wave (1, 2 or 4) patches into each wavefront
-    long long group_min_patches = 4096;
+    long long group_min_patches = 2048;
     int       patches_per_wave  = 2;
     // 3D operator (k <= 4): levels with at most this many cells per
     // direction use the one-round box kernel instead of z-marching
@@ -238,6 +238,10 @@ namespace gtps
   // most 4 cells per direction run them in one launch)
   void launch_dg_sweeps(const gtps_dg_level *lvl, void *u, const void *b,
                         int first, int n_sweeps, double relax);
+  // the same on `columns` independent (u, b) column pairs of lvl->N values
+  // each, one work-group per column (meshes of 2..4 cells per direction)
+  void launch_dg_sweeps_batched(const gtps_dg_level *lvl, void *u, const void *b,
+                                int first, int n_sweeps, double relax, int columns);
   // fine += P coarse / coarse (+)= P^T fine (store: coarse = P^T fine)
   void launch_dg_prolongate_add(const gtps_dg_level *fine_lvl, void *fine,
                                 const void *coarse);
@@ -258,6 +262,12 @@ namespace gtps
                              const void *U, void *B);
   void launch_dense_apply(hipStream_t s, int dim, int N1, int prec,
                           const void *B, const void *d, void *u);
+  // dense forms without Dirichlet rows (DG): D[j][i] = delta_ij over n
+  // DoFs, B = U^T, u = B d
+  void launch_unit_defects_plain(hipStream_t s, int n, int prec, void *D);
+  void launch_transpose_plain(hipStream_t s, int n, int prec, const void *U, void *B);
+  void launch_dense_apply_plain(hipStream_t s, int n, int prec, const void *B,
+                                const void *d, void *u);
 
   // post.hip (FP64): error norms of u against prod sin(wave pi x_d) with
   // QGauss(nq) (nq <= MAX_QUAD), and the load vector from f at the

@@ -76,7 +76,7 @@ const char *gtps_version(void);
  *                           k <= 3).
  *   "group_min_patches"     colours with more patches than this pack
  *                           several patches into each wavefront (default
- *                           4096);
+ *                           2048: the finest two levels at 64^3 Q3);
  *   "patches_per_wave"      how many (1, 2 or 4; default 2);
  *   "box_vmult_max_n"       3D operator with k <= 4 on levels with at most
  *                           this many cells per direction: one-round box

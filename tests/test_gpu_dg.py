@@ -295,3 +295,27 @@ def test_dg_graph_replays_match_eager(core, ctx, prec):
         assert np.isfinite(float((b * 2.0).sum().item()))
         assert it_g == it_e and h_g == h_e, rep
         assert torch.equal(x_g, x_e), rep
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("k,n0,maxl", [(2, 3, 1), (4, 3, 1), (1, 2, 2)])
+def test_dg_tabulated_coarse_solve(core, ctx, k, n0, maxl, prec):
+    """The coarsest DG solve tabulated as a dense matrix gives the same
+    V-cycle as the sweep kernels (FP64 1e-12, FP32 2e-5 relative) and as
+    the oracle."""
+    _announce("dg-dense-coarse", k, n0, maxl, prec)
+    dt = core.FP32 if prec == "fp32" else core.FP64
+    r = rng(40 + k).uniform(-1, 1, O.dg_n_dofs(3, k, n0 << maxl))
+    out = {}
+    for dense in (1, 0):
+        prev = core.set_tuning("dense_coarse", dense)
+        try:
+            mg = core.DGMultigrid(ctx, 3, k, n0, maxl, dt)
+            z = torch.empty(mg.n_dofs, dtype=torch.float64, device="cuda:0")
+            mg.vmult(z, _dev(r))
+            torch.cuda.synchronize()
+            out[dense] = _host(z)
+        finally:
+            core.set_tuning("dense_coarse", prev)
+    assert rel_err(out[1], out[0]) < (1e-12 if prec == "fp64" else 2e-5)
+    assert rel_err(out[1], O.dg_vcycle(3, k, n0, maxl, r)) < TOL_SMOOTH[prec]
