@@ -228,8 +228,9 @@ def mfma_report(core, ctx, k, level, reps):
 def c5_report(ctx, core, problems, reps, rel_l2):
     """SIPG-DG workload: Q4 DG on 24^3 cells (3^3 coarse mesh, 3 refinements),
     penalty 2.5 (k+1)^2, random f, FP32 V-cycle levels or FP64 everywhere,
-    FP64 CG to 1e-8; then J = 1/2 sum_m ||U_m||^2 over 4 right-hand sides with
-    dJ/dB = A^{-1} U through the autograd Function, checked by a central
+    FP64 CG to 1e-8; then J = 1/2 sum_m ||U_m||^2 over 4 right-hand sides
+    solved together (batched MG-CG) with dJ/dB = A^{-1} U through the
+    autograd Function, checked by a central
     difference along a random direction (J is quadratic in B, so the
     difference is exact up to the solver tolerance)."""
     import torch
@@ -280,7 +281,8 @@ def c5_report(ctx, core, problems, reps, rel_l2):
             "3 refinements), penalty 2.5 (k+1)^2, random f, 27-colour vertex-patch "
             "smoother, CG to 1e-8", "n_dofs": b.numel(), **res,
             "batched_adjoint": {"rhs": 4, "forward_plus_adjoint_ms": round(ms_fb, 3),
-                                "solves": "one after another (8 MG-CG solves)",
+                                "solves": "the 4 right-hand sides together: one batched "
+                                          "MG-CG forward, one for the adjoint",
                                 "fd_direction_h": h,
                                 "rel_dev_adjoint_vs_fd": abs(an - fd) / abs(fd)}}
 

@@ -100,6 +100,7 @@ SIGNATURES = {
     "gtps_dg_mg_create": (_i, [_vp, _i, _i, _i, _i, _i, _d, _i, ctypes.POINTER(_vp)]),
     "gtps_dg_mg_destroy": (_i, [_vp]),
     "gtps_dg_mg_set_graph": (_i, [_vp, _i]),
+    "gtps_dg_mg_pcg_solve_batch": (_i, [_vp, _vp, _vp, _i, _d, _i, _pi, _pd]),
     "gtps_dg_mg_n_dofs": (_ll, [_vp]),
     "gtps_dg_mg_matvec": (_i, [_vp, _vp, _vp]),
     "gtps_dg_mg_vcycle": (_i, [_vp, _vp, _vp]),

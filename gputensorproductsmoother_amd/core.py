@@ -640,11 +640,28 @@ class DGMultigrid:
             L.check(rc, "dg_pcg_solve")
         return x, it.value, rel.value, list(hist[: it.value + 1])
 
+    def solve_cg_batch(self, B: torch.Tensor, X: torch.Tensor | None = None,
+                       rtol: float = 1e-8, maxit: int = 200):
+        """Solve A X_j = B_j for the rows of B (shape (m, N), FP64, contiguous)
+        together; returns (X, iterations per row, relative residual per row)."""
+        if B.dim() != 2 or B.shape[1] != self.n_dofs:
+            raise ValueError(f"expected shape (m, {self.n_dofs}), got {tuple(B.shape)}")
+        _chk_vec(B.reshape(-1), torch.float64)
+        m = B.shape[0]
+        if X is None:
+            X = torch.empty_like(B)
+        _chk_vec(X.reshape(-1), torch.float64, B.numel())
+        its = (ctypes.c_int * m)()
+        rel = (ctypes.c_double * m)()
+        L.check(L.load().gtps_dg_mg_pcg_solve_batch(self._h, _ptr(X), _ptr(B), m, rtol, maxit,
+                                                    its, rel), "dg_pcg_solve_batch")
+        return X, list(its), list(rel)
+
     def solve(self, B: torch.Tensor, rtol: float = 1e-10, maxit: int = 300) -> torch.Tensor:
         """Differentiable solve of A U = B for one right-hand side (shape
-        (N,)) or several (shape (m, N), solved one after another); the
-        backward pass solves A G_B = G_U with the same MG-CG (A is
-        symmetric and has no constrained rows)."""
+        (N,)) or several (shape (m, N), solved together); the backward pass
+        solves A G_B = G_U with the same MG-CG (A is symmetric and has no
+        constrained rows)."""
         return DGSolve.apply(self, B, rtol, maxit)
 
     def __del__(self):
@@ -657,14 +674,17 @@ class DGMultigrid:
 
 
 class DGSolve(torch.autograd.Function):
-    """U = A^{-1} B row by row through DGMultigrid.solve_cg."""
+    """U = A^{-1} B; the rows of a batch are solved together
+    (DGMultigrid.solve_cg_batch), a single vector by DGMultigrid.solve_cg."""
 
     @staticmethod
     def _solve_rows(mg: "DGMultigrid", B: torch.Tensor, rtol, maxit):
-        rows = B.reshape(-1, B.shape[-1])
-        out = torch.empty_like(rows)
-        for i in range(rows.shape[0]):
-            mg.solve_cg(rows[i].contiguous(), out[i], rtol=rtol, maxit=maxit)
+        rows = B.reshape(-1, B.shape[-1]).contiguous()
+        if rows.shape[0] == 1:
+            out = torch.empty_like(rows)
+            mg.solve_cg(rows[0], out[0], rtol=rtol, maxit=maxit)
+        else:
+            out, _, _ = mg.solve_cg_batch(rows, rtol=rtol, maxit=maxit)
         return out.reshape(B.shape)
 
     @staticmethod

@@ -825,6 +825,119 @@ struct gtps_dg_mg
   // map) tabulated as a dense matrix on first use (coarse.hip)
   void                         *dense0       = nullptr;
   bool                          dense0_valid = false;
+  // batched solves: every level vector, r, p, q hold bcap contiguous
+  // vectors; per-vector CG scalars and host copies of the residual norms
+  int                           bcap    = 1;
+  double                       *bsc     = nullptr;
+  double                       *bpinned = nullptr;
+
+  // room for batches of nb right-hand sides (drops the captured graph)
+  void
+  ensure_batch(int nb)
+  {
+    if (nb <= bcap && bsc)
+      return;
+    hipStream_t s = ctx->stream;
+    GTPS_HIP(hipStreamSynchronize(s));
+    if (dir_exec)
+      {
+        (void)hipGraphExecDestroy(dir_exec);
+        dir_exec = nullptr;
+      }
+    const int    nc = std::max(nb, bcap);
+    const size_t es = prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
+    if (nc > bcap)
+      {
+        for (int l = 0; l <= maxl; ++l)
+          {
+            for (auto *vec : {&d, &u, &t})
+              {
+                dfree((*vec)[l]);
+                (*vec)[l] = nullptr;
+              }
+            d[l] = dmalloc(es * lv[l]->N * nc);
+            u[l] = dmalloc(es * lv[l]->N * nc);
+            t[l] = dmalloc(es * lv[l]->N * nc);
+            dg_level_reserve(lv[l], nc);
+          }
+        if (own_fine64)
+          dg_level_reserve(fine64, nc);
+        const long long N = fine64->N;
+        for (double **v : {&r, &p, &q})
+          {
+            dfree(*v);
+            *v = nullptr;
+            *v = (double *)dmalloc(sizeof(double) * N * nc);
+          }
+        bcap = nc;
+      }
+    dfree(bsc);
+    bsc = (double *)dmalloc(sizeof(double) * cg::N_SLOTS * bcap);
+    if (bpinned)
+      (void)hipHostFree(bpinned);
+    bpinned = nullptr;
+    GTPS_HIP(hipHostMalloc((void **)&bpinned, sizeof(double) * bcap, hipHostMallocDefault));
+  }
+
+  // V-cycle on nb contiguous vectors (the level's N apart)
+  void
+  vcycle_batch(int l, const void *dl, void *ul, int nb)
+  {
+    const gtps_dg_level *L = lv[l];
+    if (l == 0 && dense0_valid && tuning().dense_coarse != 0)
+      {
+        launch_dense_apply_plain(ctx->stream, (int)L->N, prec, dense0, dl, ul, nb);
+        return;
+      }
+    launch_fill(ctx->stream, L->N * nb, prec, ul, 0.0);
+    if (l == 0)
+      {
+        launch_dg_sweeps(L, ul, dl, GTPS_MULT_FWD, 2 * coarse_sweeps, 1.0, nb);
+        return;
+      }
+    launch_dg_sweep(L, ul, dl, GTPS_MULT_FWD, relax, nb);
+    launch_dg_apply(L, t[l], ul, dl, false, nb);  // t = d - A u
+    launch_dg_restrict(L, d[l - 1], t[l], true, nb);
+    vcycle_batch(l - 1, d[l - 1], u[l - 1], nb);
+    launch_dg_prolongate_add(L, ul, u[l - 1], nb);
+    launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, relax, nb);
+  }
+
+  // z_j = V(r_j); per vector: rz_old = rz; rz = r.z; beta; p = z + beta p
+  void
+  direction_batch(int nb)
+  {
+    hipStream_t     s   = ctx->stream;
+    const long long N   = fine64->N;
+    const bool      f32 = prec == GTPS_FP32;
+    double         *pt  = ctx->ws.partials;
+    if (f32)
+      vcycle_batch(maxl, d[maxl], u[maxl], nb);
+    else
+      vcycle_batch(maxl, r, u[maxl], nb);
+    for (int j = 0; j < nb; ++j)
+      {
+        const float  *z32 = f32 ? (const float *)u[maxl] + N * j : nullptr;
+        const double *z64 = f32 ? nullptr : (const double *)u[maxl] + N * j;
+        double       *sc  = bsc + cg::N_SLOTS * j;
+        const int nbk = launch_dot_partials(s, N, r + N * j, z64, z32, pt, ctx->ws.capacity);
+        launch_finalize_op(s, pt, nbk, sc, cg::RZ, cg::OP_SHIFT_BETA);
+        launch_cg_update_p(s, N, p + N * j, z64, z32, sc);
+      }
+  }
+
+  // residual norms of the batch to the host (one synchronisation)
+  void
+  read_norms(int nb, double *out)
+  {
+    hipStream_t s = ctx->stream;
+    for (int j = 0; j < nb; ++j)
+      GTPS_HIP(hipMemcpyAsync(bpinned + j, bsc + cg::N_SLOTS * j + cg::RR, sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    GTPS_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nb; ++j)
+      out[j] = std::sqrt(bpinned[j]);
+  }
 
   void
   ensure_dense0()
@@ -955,6 +1068,9 @@ struct gtps_dg_mg
   ~gtps_dg_mg()
   {
     dfree(dense0);
+    dfree(bsc);
+    if (bpinned)
+      (void)hipHostFree(bpinned);
     if (dir_exec)
       (void)hipGraphExecDestroy(dir_exec);
     if (cap_stream)
@@ -2012,6 +2128,91 @@ extern "C"
       (void)hipSetDevice(mg->ctx->device);
       (void)hipStreamSynchronize(mg->ctx->stream);
       delete mg;
+    });
+  }
+
+  int
+  gtps_dg_mg_pcg_solve_batch(gtps_dg_mg *mg, double *x, const double *b, int nrhs,
+                             double rtol, int maxit, int *iterations, double *relres)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(mg && x && b && (const void *)x != (const void *)b, "bad argument");
+      GTPS_REQUIRE(nrhs >= 1 && nrhs <= 64, "nrhs must be in 1..64");
+      GTPS_REQUIRE(maxit >= 0 && rtol >= 0.0, "bad rtol / maxit");
+      set_device(mg->ctx);
+      mg->ensure_batch(nrhs);
+      mg->ensure_dense0();
+      hipStream_t     s   = mg->ctx->stream;
+      const long long N   = mg->fine64->N;
+      const bool      f32 = mg->prec == GTPS_FP32;
+      double         *pt  = mg->ctx->ws.partials;
+      const int       cap = mg->ctx->ws.capacity;
+      launch_fill(s, N * nrhs, GTPS_FP64, x, 0.0);
+      GTPS_HIP(hipMemcpyAsync(mg->r, b, sizeof(double) * N * nrhs, hipMemcpyDeviceToDevice, s));
+      for (int j = 0; j < nrhs; ++j)
+        {
+          double *rj  = mg->r + N * j;
+          const int nbk = launch_dot_partials(s, N, rj, rj, nullptr, pt, cap);
+          launch_finalize_sum(s, pt, nbk, mg->bsc + cg::N_SLOTS * j, cg::RR);
+          launch_cg_scalar(s, mg->bsc + cg::N_SLOTS * j, cg::OP_INIT);
+        }
+      if (f32)
+        launch_convert_f64_f32(s, N * nrhs, mg->r, (float *)mg->d[mg->maxl]);
+      launch_fill(s, N * nrhs, GTPS_FP64, mg->p, 0.0);
+      std::vector<double> r0(nrhs), rn(nrhs);
+      std::vector<int>    itc(nrhs, -1);
+      mg->read_norms(nrhs, r0.data());
+      for (int j = 0; j < nrhs; ++j)
+        {
+          GTPS_REQUIRE(std::isfinite(r0[j]), "non-finite right-hand side");
+          if (r0[j] == 0.0)
+            itc[j] = 0;
+        }
+      auto all_done = [&]() {
+        for (int j = 0; j < nrhs; ++j)
+          if (itc[j] < 0)
+            return false;
+        return true;
+      };
+      int it = 0;
+      if (!all_done())
+        mg->direction_batch(nrhs);
+      while (!all_done() && it < maxit)
+        {
+          const int nbq = launch_dg_apply(mg->fine64, mg->q, mg->p, nullptr, true, nrhs);
+          for (int j = 0; j < nrhs; ++j)
+            launch_finalize_op(s, pt + (long long)nbq * j, nbq, mg->bsc + cg::N_SLOTS * j,
+                               cg::PQ, cg::OP_ALPHA);
+          for (int j = 0; j < nrhs; ++j)
+            {
+              float *r32 = f32 ? (float *)mg->d[mg->maxl] + N * j : nullptr;
+              const int nbr =
+                launch_cg_update_xr(s, N, x + N * j, mg->r + N * j, mg->p + N * j,
+                                    mg->q + N * j, r32, mg->bsc + cg::N_SLOTS * j, pt, cap);
+              launch_finalize_sum(s, pt, nbr, mg->bsc + cg::N_SLOTS * j, cg::RR);
+            }
+          ++it;
+          mg->read_norms(nrhs, rn.data());
+          for (int j = 0; j < nrhs; ++j)
+            {
+              if (!std::isfinite(rn[j]))
+                throw Error(GTPS_ENAN, "non-finite residual in batched CG");
+              if (itc[j] < 0 && rn[j] <= rtol * r0[j])
+                itc[j] = it;
+            }
+          if (!all_done() && it < maxit)
+            mg->direction_batch(nrhs);
+        }
+      for (int j = 0; j < nrhs; ++j)
+        {
+          if (iterations)
+            iterations[j] = itc[j] < 0 ? it : itc[j];
+          if (relres)
+            relres[j] = r0[j] > 0.0 ? (it > 0 ? rn[j] : r0[j]) / r0[j] : 0.0;
+        }
+      GTPS_HIP(hipStreamSynchronize(s));
+      if (!all_done())
+        throw Error(GTPS_ENOTCONV, "batched CG reached maxit without convergence");
     });
   }
 

@@ -71,6 +71,9 @@ namespace gtps
     {
       __shared__ T sd[MAX_DENSE_COARSE];
       const int    tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+      // batch: blockIdx.y selects one of several contiguous vectors
+      d += (long long)blockIdx.y * N;
+      u += (long long)blockIdx.y * N;
       for (int j = tid; j < ni; j += CNT)
         sd[j] = d[interior_index<DIM>(j, N1)];
       // Dirichlet rows: zero (spread over the grid)
@@ -155,16 +158,17 @@ namespace gtps
 
   void
   launch_dense_apply_plain(hipStream_t s, int n, int prec, const void *B, const void *d,
-                           void *u)
+                           void *u, int nvec)
   {
     GTPS_REQUIRE(n <= MAX_DENSE_COARSE, "dense coarse operator too large");
+    GTPS_REQUIRE(nvec >= 1, "bad batch");
     const int rows_per_block = (CNT / 64) * ROWS_PER_WAVE;
-    const int nb             = (n + rows_per_block - 1) / rows_per_block;
+    const dim3 grid((n + rows_per_block - 1) / rows_per_block, nvec);
     if (prec == GTPS_FP32)
-      hipLaunchKernelGGL((dense_apply_kernel<float, 0>), dim3(nb), dim3(CNT), 0, s,
+      hipLaunchKernelGGL((dense_apply_kernel<float, 0>), grid, dim3(CNT), 0, s,
                          (const float *)B, (const float *)d, (float *)u, 0, (long long)n, n);
     else
-      hipLaunchKernelGGL((dense_apply_kernel<double, 0>), dim3(nb), dim3(CNT), 0, s,
+      hipLaunchKernelGGL((dense_apply_kernel<double, 0>), grid, dim3(CNT), 0, s,
                          (const double *)B, (const double *)d, (double *)u, 0, (long long)n,
                          n);
     GTPS_LAUNCH_CHECK();

@@ -119,6 +119,21 @@ namespace gtps
       __shared__ double red[DNT / 64];
       double            dot = 0.0;
       const long long   P   = (long long)N1 * N1;
+      // batch: blockIdx.y selects one of several contiguous vectors
+      const long long   bo  = (long long)blockIdx.y * N;
+      i1 += bo;
+      o1 += bo;
+      if (PASS != 0)
+        i2 += bo;
+      if (PASS != 2)
+        o2 += bo;
+      if (RESID)
+        b += bo;
+      if (DOT)
+        {
+          x0 += bo;
+          partials += (long long)blockIdx.y * gridDim.x;
+        }
       for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < N;
            g += (long long)gridDim.x * DNT)
         {
@@ -392,6 +407,9 @@ namespace gtps
                     const T scale, const T relax)
     {
       __shared__ DPShared<K, T> sh;
+      const long long NNN = (long long)n * (K + 1);
+      u += (long long)blockIdx.y * NNN * NNN * NNN;
+      bvec += (long long)blockIdx.y * NNN * NNN * NNN;
       const int cx = colour % 3, cy = (colour / 3) % 3, cz = colour / 9;
       const int npx = (n - 1 - cx + 2) / 3, npy = (n - 1 - cy + 2) / 3;
       const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
@@ -442,6 +460,8 @@ namespace gtps
       const int       N1f = nf * NN, N1c = (nf / 2) * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
       const T        *E  = tab + TB::E;
+      fine += (long long)blockIdx.y * Nf;
+      coarse += (long long)blockIdx.y * Pc * N1c;
       for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < Nf;
            g += (long long)gridDim.x * DNT)
         {
@@ -480,6 +500,8 @@ namespace gtps
       const int       N1f = nf * NN, N1c = (nf / 2) * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
       const T        *E  = tab + TB::E;
+      fine += (long long)blockIdx.y * Pf * N1f;
+      coarse += (long long)blockIdx.y * Nc;
       for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < Nc;
            g += (long long)gridDim.x * DNT)
         {
@@ -602,24 +624,25 @@ namespace gtps
 
     template <int K, typename T>
     int
-    apply_k(const gtps_dg_level *lvl, T *y, const T *x, const T *b, bool dot)
+    apply_k(const gtps_dg_level *lvl, T *y, const T *x, const T *b, bool dot, int nvec)
     {
+      GTPS_REQUIRE(nvec >= 1 && nvec <= lvl->tcap, "DG operator scratch too small for the batch");
       hipStream_t s   = lvl->ctx->stream;
       const T    *tab = (const T *)lvl->tab;
       T *t0 = (T *)lvl->t[0], *t1 = (T *)lvl->t[1], *t2 = (T *)lvl->t[2],
         *t3 = (T *)lvl->t[3];
-      const int  nb = dgrid(lvl->N);
-      const int  nbd = dgrid(lvl->N, std::min(lvl->ctx->ws.capacity, 2048));
+      const dim3 nb(dgrid(lvl->N), nvec);
+      const dim3 nbd(dgrid(lvl->N, std::min(lvl->ctx->ws.capacity / nvec, 2048)), nvec);
       double    *pt = lvl->ctx->ws.partials;
       const T    sc = (T)lvl->scale;
-      hipLaunchKernelGGL((dg_op_kernel<K, T, 0, false, false>), dim3(nb), dim3(DNT), 0,
+      hipLaunchKernelGGL((dg_op_kernel<K, T, 0, false, false>), nb, dim3(DNT), 0,
                          s, x, nullptr, t0, t1, tab, nullptr, nullptr, lvl->n,
                          lvl->N1, lvl->N, sc, pt);
-      hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), dim3(nb), dim3(DNT), 0,
+      hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), nb, dim3(DNT), 0,
                          s, t0, t1, t2, t3, tab, nullptr, nullptr, lvl->n,
                          lvl->N1, lvl->N, sc, pt);
 #define GTPS_DX(R, D, G)                                                      \
-  hipLaunchKernelGGL((dg_op_kernel<K, T, 2, R, D>), dim3(G), dim3(DNT), 0, s, \
+  hipLaunchKernelGGL((dg_op_kernel<K, T, 2, R, D>), G, dim3(DNT), 0, s,       \
                      t2, t3, y, nullptr, tab, b, x, lvl->n, lvl->N1, lvl->N,  \
                      sc, pt)
       if (b && dot)
@@ -632,7 +655,7 @@ namespace gtps
         GTPS_DX(false, false, nb);
 #undef GTPS_DX
       GTPS_LAUNCH_CHECK();
-      return dot ? nbd : 0;
+      return dot ? (int)nbd.x : 0;
     }
 
     // n_sweeps sweeps, alternating colour order from `first`
@@ -640,13 +663,15 @@ namespace gtps
     template <int K, typename T>
     void
     sweeps_k(const gtps_dg_level *lvl, T *u, const T *b, int first, int n_sweeps,
-             double relax, int columns = 1)
+             double relax, int columns = 1, int nvec = 1)
     {
       if constexpr (DP<K, T>::ok)
         {
           const int   n = lvl->n;
           hipStream_t s = lvl->ctx->stream;
           GTPS_REQUIRE(columns == 1 || n <= 4, "batched sweeps need at most 4 cells per direction");
+          if (n <= 4 && nvec > 1)
+            columns = nvec;  // independent vectors: one work-group each
           if (n <= 4)
             {
               hipLaunchKernelGGL((dg_patch_seq_kernel<K, T>), dim3(columns),
@@ -667,7 +692,7 @@ namespace gtps
                     continue;
                   const long long np = (long long)((n - 1 - cx + 2) / 3) *
                                        ((n - 1 - cy + 2) / 3) * ((n - 1 - cz + 2) / 3);
-                  hipLaunchKernelGGL((dg_patch_kernel<K, T>), dim3((unsigned)np),
+                  hipLaunchKernelGGL((dg_patch_kernel<K, T>), dim3((unsigned)np, nvec),
                                      dim3(DP<K, T>::NT),
                                      0, s, u, b, (const T *)lvl->tab, n, c, (T)lvl->scale,
                                      (T)relax);
@@ -713,30 +738,47 @@ namespace gtps
       GTPS_HIP(hipMalloc(&p, lvl->elem() * lvl->N));
   }
 
+  void
+  dg_level_reserve(gtps_dg_level *lvl, int nb)
+  {
+    if (nb <= lvl->tcap)
+      return;
+    GTPS_HIP(hipStreamSynchronize(lvl->ctx->stream));
+    for (void *&p : lvl->t)
+      {
+        (void)hipFree(p);
+        p = nullptr;
+      }
+    lvl->tcap = 0;
+    for (void *&p : lvl->t)
+      GTPS_HIP(hipMalloc(&p, lvl->elem() * lvl->N * nb));
+    lvl->tcap = nb;
+  }
+
   int
   launch_dg_apply(const gtps_dg_level *lvl, void *y, const void *x,
-                  const void *b, bool fused_dot)
+                  const void *b, bool fused_dot, int nvec)
   {
     return dispatch_k(lvl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (lvl->prec == GTPS_FP32)
         return apply_k<K, float>(lvl, (float *)y, (const float *)x,
-                                 (const float *)b, fused_dot);
+                                 (const float *)b, fused_dot, nvec);
       return apply_k<K, double>(lvl, (double *)y, (const double *)x,
-                                (const double *)b, fused_dot);
+                                (const double *)b, fused_dot, nvec);
     });
   }
 
   void
   launch_dg_sweep(const gtps_dg_level *lvl, void *u, const void *b,
-                  int variant, double relax)
+                  int variant, double relax, int nvec)
   {
-    launch_dg_sweeps(lvl, u, b, variant, 1, relax);
+    launch_dg_sweeps(lvl, u, b, variant, 1, relax, nvec);
   }
 
   void
   launch_dg_sweeps(const gtps_dg_level *lvl, void *u, const void *b, int first,
-                   int n_sweeps, double relax)
+                   int n_sweeps, double relax, int nvec)
   {
     GTPS_REQUIRE(lvl->n >= 2, "the DG smoother needs at least 2 cells per direction");
     GTPS_REQUIRE(n_sweeps >= 0, "negative sweep count");
@@ -746,9 +788,11 @@ namespace gtps
     dispatch_k(lvl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (lvl->prec == GTPS_FP32)
-        sweeps_k<K, float>(lvl, (float *)u, (const float *)b, variant, n_sweeps, relax);
+        sweeps_k<K, float>(lvl, (float *)u, (const float *)b, variant, n_sweeps, relax, 1,
+                           nvec);
       else
-        sweeps_k<K, double>(lvl, (double *)u, (const double *)b, variant, n_sweeps, relax);
+        sweeps_k<K, double>(lvl, (double *)u, (const double *)b, variant, n_sweeps, relax, 1,
+                            nvec);
       return 0;
     });
   }
@@ -771,17 +815,17 @@ namespace gtps
   }
 
   void
-  launch_dg_prolongate_add(const gtps_dg_level *fl, void *fine, const void *coarse)
+  launch_dg_prolongate_add(const gtps_dg_level *fl, void *fine, const void *coarse, int nvec)
   {
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
     dispatch_k(fl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (fl->prec == GTPS_FP32)
-        hipLaunchKernelGGL((dg_prolongate_kernel<K, float>), dim3(dgrid(fl->N)),
+        hipLaunchKernelGGL((dg_prolongate_kernel<K, float>), dim3(dgrid(fl->N), nvec),
                            dim3(DNT), 0, fl->ctx->stream, (const float *)coarse,
                            (float *)fine, (const float *)fl->tab, fl->n, fl->N);
       else
-        hipLaunchKernelGGL((dg_prolongate_kernel<K, double>), dim3(dgrid(fl->N)),
+        hipLaunchKernelGGL((dg_prolongate_kernel<K, double>), dim3(dgrid(fl->N), nvec),
                            dim3(DNT), 0, fl->ctx->stream, (const double *)coarse,
                            (double *)fine, (const double *)fl->tab, fl->n, fl->N);
       GTPS_LAUNCH_CHECK();
@@ -791,7 +835,7 @@ namespace gtps
 
   void
   launch_dg_restrict(const gtps_dg_level *fl, void *coarse, const void *fine,
-                     bool store)
+                     bool store, int nvec)
   {
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
     const long long N1c = (long long)(fl->n / 2) * (fl->k + 1);
@@ -801,22 +845,22 @@ namespace gtps
       if (fl->prec == GTPS_FP32)
         {
           if (store)
-            hipLaunchKernelGGL((dg_restrict_kernel<K, float, true>), dim3(dgrid(Nc)),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, float, true>), dim3(dgrid(Nc), nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const float *)fine,
                                (float *)coarse, (const float *)fl->tab, fl->n, Nc);
           else
-            hipLaunchKernelGGL((dg_restrict_kernel<K, float, false>), dim3(dgrid(Nc)),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, float, false>), dim3(dgrid(Nc), nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const float *)fine,
                                (float *)coarse, (const float *)fl->tab, fl->n, Nc);
         }
       else
         {
           if (store)
-            hipLaunchKernelGGL((dg_restrict_kernel<K, double, true>), dim3(dgrid(Nc)),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, double, true>), dim3(dgrid(Nc), nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const double *)fine,
                                (double *)coarse, (const double *)fl->tab, fl->n, Nc);
           else
-            hipLaunchKernelGGL((dg_restrict_kernel<K, double, false>), dim3(dgrid(Nc)),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, double, false>), dim3(dgrid(Nc), nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const double *)fine,
                                (double *)coarse, (const double *)fl->tab, fl->n, Nc);
         }

@@ -131,6 +131,7 @@ struct gtps_dg_level
   double    h = 0.5, scale = 0.5, pf = 1.0;
   void     *tab = nullptr;                     // packed DG table (prec)
   void     *t[4] = {nullptr, nullptr, nullptr, nullptr};  // operator scratch
+  int       tcap = 1;  // vectors each operator scratch buffer holds
   ~gtps_dg_level()
   {
     if (tab)
@@ -227,26 +228,31 @@ namespace gtps
 
   // dg.hip: SIPG-DG level set-up and kernels
   void dg_level_setup(gtps_dg_level *lvl);  // tables + scratch
+  // operator scratch for batches of nb vectors
+  void dg_level_reserve(gtps_dg_level *lvl, int nb);
   bool dg_supported(int k, int prec);
-  // y = A x (b != NULL: y = b - A x); fused partial dot x.y -> #blocks
+  // Every DG launcher takes a batch of nb contiguous vectors (stride lvl->N
+  // on the level, the coarse level's N for coarse operands); one launch.
+  // y = A x (b != NULL: y = b - A x); fused partial dot x.y -> #blocks per
+  // vector (vector j's partials at j * #blocks)
   int  launch_dg_apply(const gtps_dg_level *lvl, void *y, const void *x,
-                       const void *b, bool fused_dot);
+                       const void *b, bool fused_dot, int nb = 1);
   // one sweep over the 27 colours (variant GTPS_MULT_FWD / _REV)
   void launch_dg_sweep(const gtps_dg_level *lvl, void *u, const void *b,
-                       int variant, double relax);
+                       int variant, double relax, int nb = 1);
   // n_sweeps sweeps alternating colour order from `first` (meshes of at
   // most 4 cells per direction run them in one launch)
   void launch_dg_sweeps(const gtps_dg_level *lvl, void *u, const void *b,
-                        int first, int n_sweeps, double relax);
+                        int first, int n_sweeps, double relax, int nb = 1);
   // the same on `columns` independent (u, b) column pairs of lvl->N values
   // each, one work-group per column (meshes of 2..4 cells per direction)
   void launch_dg_sweeps_batched(const gtps_dg_level *lvl, void *u, const void *b,
                                 int first, int n_sweeps, double relax, int columns);
   // fine += P coarse / coarse (+)= P^T fine (store: coarse = P^T fine)
   void launch_dg_prolongate_add(const gtps_dg_level *fine_lvl, void *fine,
-                                const void *coarse);
+                                const void *coarse, int nb = 1);
   void launch_dg_restrict(const gtps_dg_level *fine_lvl, void *coarse,
-                          const void *fine, bool store);
+                          const void *fine, bool store, int nb = 1);
 
   // smoother_mfma.hip: one multiplicative colour pass (b in src) with the
   // 1D contractions on v_mfma_f32_16x16x4f32 (3D, FP32, k = 4..7)
@@ -267,7 +273,7 @@ namespace gtps
   void launch_unit_defects_plain(hipStream_t s, int n, int prec, void *D);
   void launch_transpose_plain(hipStream_t s, int n, int prec, const void *U, void *B);
   void launch_dense_apply_plain(hipStream_t s, int n, int prec, const void *B,
-                                const void *d, void *u);
+                                const void *d, void *u, int nb = 1);
 
   // post.hip (FP64): error norms of u against prod sin(wave pi x_d) with
   // QGauss(nq) (nq <= MAX_QUAD), and the load vector from f at the

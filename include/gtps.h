@@ -359,6 +359,17 @@ int gtps_dg_mg_destroy(gtps_dg_mg *mg);
 /* enable (default) / disable the HIP-graph replay of the per-iteration
  * preconditioning sequence in gtps_dg_mg_pcg_solve */
 int gtps_dg_mg_set_graph(gtps_dg_mg *mg, int enable);
+/* nrhs systems A x_j = b_j solved together (x, b: nrhs contiguous FP64
+ * vectors of gtps_dg_mg_n_dofs values): every V-cycle kernel runs once on
+ * the whole batch (a second grid dimension), the CG scalars are per vector,
+ * and the loop runs until every vector reaches rtol (iterations[j]: the
+ * first iteration at which vector j did; relres[j] its final relative
+ * residual).  The first call with a larger batch reallocates the
+ * hierarchy's vectors.  No reference counterpart (the reference has no DG
+ * solver); this serves the batched adjoint of the DG workload. */
+int gtps_dg_mg_pcg_solve_batch(gtps_dg_mg *mg, double *x, const double *b,
+                               int nrhs, double rtol, int maxit,
+                               int *iterations, double *relres);
 long long gtps_dg_mg_n_dofs(const gtps_dg_mg *mg);
 int gtps_dg_mg_matvec(gtps_dg_mg *mg, double *dst, const double *src);
 int gtps_dg_mg_vcycle(gtps_dg_mg *mg, double *z, const double *r);
