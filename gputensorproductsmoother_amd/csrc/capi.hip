@@ -1661,9 +1661,15 @@ extern "C"
           prev                  = tuning().dense_coarse;
           tuning().dense_coarse = (int)value;
         }
+      else if (k == "vmult_tile")
+        {
+          GTPS_REQUIRE(value >= 0 && value <= 3, "vmult_tile must be 0, 1, 2 or 3");
+          prev                = tuning().vmult_tile;
+          tuning().vmult_tile = (int)value;
+        }
       else if (k == "vmult_zdiv")
         {
-          GTPS_REQUIRE(value >= 1 && value <= (1 << 20), "vmult_zdiv must be in 1..2^20");
+          GTPS_REQUIRE(value >= 0 && value <= (1 << 20), "vmult_zdiv must be in 0..2^20");
           prev                = tuning().vmult_zdiv;
           tuning().vmult_zdiv = value;
         }
@@ -1702,6 +1708,8 @@ extern "C"
         v = tuning().mfma_smoother;
       else if (k == "vmult_zdiv")
         v = tuning().vmult_zdiv;
+      else if (k == "vmult_tile")
+        v = tuning().vmult_tile;
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
     });

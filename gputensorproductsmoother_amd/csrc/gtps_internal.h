@@ -186,8 +186,11 @@ namespace gtps
     // 3D FP32 degree 4..7 multiplicative smoother on the matrix cores
     int       mfma_smoother = 1;
     // z-marching vmult: target work-group count divisor (cells per chunk =
-    // ceil(n tx ty / vmult_zdiv))
-    long long vmult_zdiv = 512;
+    // ceil(n tx ty / vmult_zdiv)); 0: by precision
+    long long vmult_zdiv = 0;
+    // z-marching vmult tile for k = 3: 0 = 16x8 cells / 512 threads,
+    // 1 = 8x8 / 256, 2 = 16x8 / 256; 3: by precision
+    int       vmult_tile = 3;
   };
   Tuning &tuning();
 

@@ -64,17 +64,33 @@ namespace gtps
       static constexpr int TCX = 8, TCY = 2, NT = 512;
     };
 
-    template <int K, typename T, bool RESID, bool DOT>
-    __global__ void __launch_bounds__(VT3<K>::NT)
+    // alternative tiles of the metric degree (tuning knob vmult_tile)
+    template <int K, int CFG>
+    struct VT3C : VT3<K>
+    {
+    };
+    template <>
+    struct VT3C<3, 1>
+    {
+      static constexpr int TCX = 8, TCY = 8, NT = 256;
+    };
+    template <>
+    struct VT3C<3, 2>
+    {
+      static constexpr int TCX = 16, TCY = 8, NT = 256;
+    };
+
+    template <int K, typename T, bool RESID, bool DOT, int CFG = 0>
+    __global__ void __launch_bounds__((VT3C<K, CFG>::NT))
     laplace3d_kernel(const T *__restrict__ x, T *__restrict__ y,
                      const T *__restrict__ bv, const T *__restrict__ tab,
                      const int n, const int zc, const T scale,
                      double *__restrict__ partial)
     {
       using TB               = Tab<K>;
-      constexpr int TCX      = VT3<K>::TCX;
-      constexpr int TCY      = VT3<K>::TCY;
-      constexpr int NT       = VT3<K>::NT;
+      constexpr int TCX      = VT3C<K, CFG>::TCX;
+      constexpr int TCY      = VT3C<K, CFG>::TCY;
+      constexpr int NT       = VT3C<K, CFG>::NT;
       constexpr int W        = 2 * K + 1;
       constexpr int WX       = K * (TCX + 2) + 1;
       constexpr int WY       = K * (TCY + 2) + 1;
@@ -671,38 +687,56 @@ namespace gtps
           }
       if (lvl->dim == 3)
         {
-          using C       = VT3<K>;
-          const int tx  = (n + C::TCX - 1) / C::TCX;
-          const int ty  = (n + C::TCY - 1) / C::TCY;
-          // z-chunks: about two 512-thread work-groups per CU
-          const long long zdiv = tuning().vmult_zdiv > 0 ? tuning().vmult_zdiv : 512;
-          int zc = (int)(((long long)n * tx * ty + zdiv - 1) / zdiv);
-          zc     = std::max(1, std::min(zc, n));
-          const int tz = (n + zc - 1) / zc;
-          const dim3 grid(tx, ty, tz);
-          const int  nb = tx * ty * tz;
-          if (fused_dot)
-            GTPS_REQUIRE(nb <= lvl->ctx->ws.capacity, "too many blocks for partials");
-#define GTPS_L3(R, D)                                                         \
-  hipLaunchKernelGGL((laplace3d_kernel<K, T, R, D>), grid, dim3(C::NT), 0, s, \
+          auto go = [&](auto cfg) -> int {
+            constexpr int CFG = decltype(cfg)::value;
+            using C       = VT3C<K, CFG>;
+            const int tx  = (n + C::TCX - 1) / C::TCX;
+            const int ty  = (n + C::TCY - 1) / C::TCY;
+            // z-chunks: about vmult_zdiv work-groups in total (0: 512 in
+            // FP32, 2048 in FP64, from a sweep at 64^3 Q3)
+            const long long zdiv = tuning().vmult_zdiv > 0 ? tuning().vmult_zdiv
+                                   : sizeof(T) == 4        ? 512
+                                                           : 2048;
+            int zc = (int)(((long long)n * tx * ty + zdiv - 1) / zdiv);
+            zc     = std::max(1, std::min(zc, n));
+            const int tz = (n + zc - 1) / zc;
+            const dim3 grid(tx, ty, tz);
+            const int  nb = tx * ty * tz;
+            if (fused_dot)
+              GTPS_REQUIRE(nb <= lvl->ctx->ws.capacity, "too many blocks for partials");
+#define GTPS_L3(R, D)                                                              \
+  hipLaunchKernelGGL((laplace3d_kernel<K, T, R, D, CFG>), grid, dim3(C::NT), 0, s, \
                      xp, yp, bp, tab, n, zc, scale, part)
-          if (b)
-            {
-              if (fused_dot)
-                GTPS_L3(true, true);
-              else
-                GTPS_L3(true, false);
-            }
-          else
-            {
-              if (fused_dot)
-                GTPS_L3(false, true);
-              else
-                GTPS_L3(false, false);
-            }
+            if (b)
+              {
+                if (fused_dot)
+                  GTPS_L3(true, true);
+                else
+                  GTPS_L3(true, false);
+              }
+            else
+              {
+                if (fused_dot)
+                  GTPS_L3(false, true);
+                else
+                  GTPS_L3(false, false);
+              }
 #undef GTPS_L3
-          GTPS_LAUNCH_CHECK();
-          return nb;
+            GTPS_LAUNCH_CHECK();
+            return nb;
+          };
+          if constexpr (K == 3)
+            {
+              // 3: 16x8 cells / 512 threads in FP32, 8x8 / 256 in FP64
+              const int tile = tuning().vmult_tile < 3 ? tuning().vmult_tile
+                               : sizeof(T) == 4         ? 0
+                                                        : 1;
+              if (tile == 1)
+                return go(std::integral_constant<int, 1>{});
+              if (tile == 2)
+                return go(std::integral_constant<int, 2>{});
+            }
+          return go(std::integral_constant<int, 0>{});
         }
       using C      = VT2<K>;
       const dim3 grid((n + C::TCX - 1) / C::TCX, (n + C::TCY - 1) / C::TCY);

@@ -282,3 +282,34 @@ def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
     finally:
         core.set_tuning("mfma_smoother", prev)
     assert rel_err(ud.cpu().numpy(), ref) < TOL["fp32"] * 20
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("level", [3, 5])
+def test_vmult_tiles(core, ctx, level, tile, prec):
+    """The three z-marching tiles of the k = 3 vmult (16x8 cells / 512
+    threads, 8x8 / 256, 16x8 / 256) agree with the oracle, vmult and
+    residual, with a partial last tile (level 3: 8 cells per direction)."""
+    _announce("vmult-tile", level, tile, prec)
+    k = 3
+    g = rng(level * 13 + tile)
+    N = n_dofs(3, k, level)
+    x, b = g.uniform(-1, 1, (2, N))
+    if prec == "fp32":
+        x = x.astype(np.float32).astype(np.float64)
+        b = b.astype(np.float32).astype(np.float64)
+    prev = [core.set_tuning("vmult_tile", tile), core.set_tuning("box_vmult_max_n", 0)]
+    try:
+        op = core.LaplaceOperator(ctx, 3, k, level, _dt(prec))
+        y = torch.empty(N, dtype=_dt(prec), device="cuda:0")
+        op.vmult(y, _to_dev(x, prec))
+        r = torch.empty_like(y)
+        op.residual(r, _to_dev(b, prec), _to_dev(x, prec))
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("vmult_tile", prev[0])
+        core.set_tuning("box_vmult_max_n", prev[1])
+    ref = O.laplace_vmult(3, k, level, x)
+    assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
+    assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec] * 10
