@@ -115,6 +115,29 @@ namespace gtps
     return s;
   }
 
+  // division by an invariant divisor d >= 1 for 0 <= n < 2^31:
+  // n / d = (umulhi(n, m) + n) >> s with s = ceil(log2 d),
+  // m = floor(2^32 (2^s - d) / d) + 1 (set up on the host by make_fastdiv)
+  struct FastDiv
+  {
+    unsigned d, m;
+    int      s;
+  };
+  inline FastDiv
+  make_fastdiv(unsigned d)
+  {
+    int s = 0;
+    while ((1ull << s) < d)
+      ++s;
+    const unsigned long long m = ((1ull << 32) * ((1ull << s) - d)) / d + 1;
+    return FastDiv{d, (unsigned)m, s};
+  }
+  __device__ __forceinline__ unsigned
+  fastdiv(unsigned n, const FastDiv &f)
+  {
+    return (__umulhi(n, f.m) + n) >> f.s;
+  }
+
   __device__ __forceinline__ int
   linear_block_id()
   {

@@ -973,7 +973,7 @@ namespace gtps
     patch3d_group_kernel(T *__restrict__ u, const T *__restrict__ src,
                          const T *__restrict__ tab, const int n,
                          const int colour, const T scale, const T inv_scale,
-                         const T relax)
+                         const T relax, const FastDiv div_x, const FastDiv div_xy)
     {
       using TB         = Tab<K>;
       using C          = SG<K>;
@@ -983,7 +983,6 @@ namespace gtps
       constexpr int P2 = (NPW * NI * NP + 63) / 64;
       constexpr int P3 = (NPW * NI * NI + 63) / 64;
       __shared__ T      sm[NPW * PER];
-      __shared__ long long sfirst[NPW];
       const int       N1   = K * n + 1;
       const long long P    = (long long)N1 * N1;
       const int       lane = threadIdx.x;
@@ -991,20 +990,30 @@ namespace gtps
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
                 npz = n_patches_1d(n, pz);
       const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
-      if (lane < NPW)
+      // patch origins: wave-uniform arithmetic, divisions by the patch
+      // counts as multiply-high + shift (divisors set up on the host)
+      long long fst[NPW];
+#pragma unroll
+      for (int p = 0; p < NPW; ++p)
         {
-          const int q = blk * NPW + lane;
-          if (q < npx * npy * npz)
-            {
-              const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
-              sfirst[lane] = (long long)K * (2 * qx + px) +
-                             (long long)K * (2 * qy + py) * N1 +
-                             (long long)K * (2 * qz + pz) * P;
-            }
-          else
-            sfirst[lane] = -1;
+          const int q  = blk * NPW + p;
+          const int qz = (int)fastdiv((unsigned)q, div_xy);
+          const int rq = q - qz * npx * npy;
+          const int qy = (int)fastdiv((unsigned)rq, div_x);
+          const int qx = rq - qy * npx;
+          fst[p] = q < npx * npy * npz ? (long long)K * (2 * qx + px) +
+                                            (long long)K * (2 * qy + py) * N1 +
+                                            (long long)K * (2 * qz + pz) * P
+                                      : -1;
         }
-      __syncthreads();
+      auto sfirst_of = [&](int lp) -> long long {
+        long long f = fst[0];
+#pragma unroll
+        for (int p = 1; p < NPW; ++p)
+          if (lp == p)
+            f = fst[p];
+        return f;
+      };
       const T *pM   = tab + TB::patchM;
       const T *pK   = tab + TB::patchK;
       const T *SE   = tab + TB::SE;
@@ -1021,7 +1030,7 @@ namespace gtps
           if (t < NPW * NI * NI)
             {
               const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-              const long long f  = sfirst[lp];
+              const long long f  = sfirst_of(lp);
               const int       zi = r / NI, yi = r - zi * NI;
               if (f >= 0)
                 {
@@ -1063,7 +1072,7 @@ namespace gtps
               if (t < NPW * NP * NP)
                 {
                   const int       lp = t / (NP * NP), pc = t - lp * (NP * NP);
-                  const long long f  = sfirst[lp];
+                  const long long f  = sfirst_of(lp);
                   if (f >= 0)
                     {
                       const int  iy = pc / NP, ix = pc - iy * NP;
@@ -1089,7 +1098,7 @@ namespace gtps
                   if (t >= NPW * NF)
                     continue;
                   const int       lp = t / NF, fi = t - lp * NF;
-                  const long long f  = sfirst[lp];
+                  const long long f  = sfirst_of(lp);
                   if (f < 0)
                     continue;
                   int ix, iy;
@@ -1122,7 +1131,7 @@ namespace gtps
                   if (t >= NPW * NI * NI)
                     continue;
                   const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-                  const long long f  = sfirst[lp];
+                  const long long f  = sfirst_of(lp);
                   if (f < 0)
                     continue;
                   const int       iy = 1 + r / NI, ix = 1 + r % NI;
@@ -1159,7 +1168,7 @@ namespace gtps
               if (t >= NPW * NI * NP)
                 continue;
               const int lp = t / (NI * NP), r = t - lp * (NI * NP);
-              if (sfirst[lp] < 0)
+              if (sfirst_of(lp) < 0)
                 continue;
               const int zi = r / NP, ix = r - zi * NP;
               const T  *sz = sm + lp * PER;
@@ -1194,7 +1203,7 @@ namespace gtps
           for (int t = lane; t < NPW * NP * NP * NP; t += 64)
             {
               const int       lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
-              const long long f  = sfirst[lp];
+              const long long f  = sfirst_of(lp);
               const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
               const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
                                  iz > 0 && iz < NP - 1;
@@ -1210,7 +1219,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int lp = t / (NI * NI), r = t - lp * (NI * NI);
-          if (sfirst[lp] < 0)
+          if (sfirst_of(lp) < 0)
             continue;
           T res[NI];
           if (MODE == 0)
@@ -1259,7 +1268,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int lp = t / (NI * NI), r = t - lp * (NI * NI);
-          if (sfirst[lp] < 0)
+          if (sfirst_of(lp) < 0)
             continue;
           const int zi = r / NI, xi = r - zi * NI;
           T        *base = sm + lp * PER + zi * NI * NI + xi;
@@ -1281,7 +1290,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int lp = t / (NI * NI), r = t - lp * (NI * NI);
-          if (sfirst[lp] < 0)
+          if (sfirst_of(lp) < 0)
             continue;
           const int yi = r / NI, xi = r - yi * NI;
           T        *base = sm + lp * PER + yi * NI + xi;
@@ -1307,7 +1316,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int lp = t / (NI * NI), r = t - lp * (NI * NI);
-          if (sfirst[lp] < 0)
+          if (sfirst_of(lp) < 0)
             continue;
           const int zi = r / NI, xi = r - zi * NI;
           T        *base = sm + lp * PER + zi * NI * NI + xi;
@@ -1329,7 +1338,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-          const long long f  = sfirst[lp];
+          const long long f  = sfirst_of(lp);
           if (f < 0)
             continue;
           const int zi = r / NI, yi = r - zi * NI;
@@ -1842,6 +1851,8 @@ namespace gtps
             const int npx_ = (n - (colour & 1)) / 2, npy_ = (n - ((colour >> 1) & 1)) / 2,
                       npz_ = (n - (colour >> 2)) / 2;
             const bool rows = tuning().row_staged != 0;
+            const FastDiv dx  = make_fastdiv((unsigned)std::max(npx_, 1));
+            const FastDiv dxy = make_fastdiv((unsigned)std::max(npx_ * npy_, 1));
             auto grid_of = [&](int w) {
               return rows ? (unsigned)((long long)((npx_ + w - 1) / w) * npy_ * npz_)
                           : (unsigned)((np + w - 1) / w);
@@ -1856,7 +1867,7 @@ namespace gtps
       else                                                                    \
         hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W>),                \
                            dim3(grid_of(W)), dim3(64), 0, s, up, sp, tab, n,   \
-                           colour, scale, inv, (T)relax);                     \
+                           colour, scale, inv, (T)relax, dx, dxy);            \
     }                                                                         \
   while (0)
 #define GTPS_SG_W(M)                                                          \
