@@ -984,7 +984,7 @@ namespace gtps
       constexpr int P3 = (NPW * NI * NI + 63) / 64;
       __shared__ T      sm[NPW * PER];
       const int       N1   = K * n + 1;
-      const long long P    = (long long)N1 * N1;
+      const int       P    = N1 * N1;  // level sizes < 2^31 (host check)
       const int       lane = threadIdx.x;
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
@@ -992,7 +992,7 @@ namespace gtps
       const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
       // patch origins: wave-uniform arithmetic, divisions by the patch
       // counts as multiply-high + shift (divisors set up on the host)
-      long long fst[NPW];
+      int fst[NPW];
 #pragma unroll
       for (int p = 0; p < NPW; ++p)
         {
@@ -1001,13 +1001,12 @@ namespace gtps
           const int rq = q - qz * npx * npy;
           const int qy = (int)fastdiv((unsigned)rq, div_x);
           const int qx = rq - qy * npx;
-          fst[p] = q < npx * npy * npz ? (long long)K * (2 * qx + px) +
-                                            (long long)K * (2 * qy + py) * N1 +
-                                            (long long)K * (2 * qz + pz) * P
-                                      : -1;
+          fst[p] = q < npx * npy * npz
+                     ? K * (2 * qx + px) + K * (2 * qy + py) * N1 + K * (2 * qz + pz) * P
+                     : -1;
         }
-      auto sfirst_of = [&](int lp) -> long long {
-        long long f = fst[0];
+      auto sfirst_of = [&](int lp) -> int {
+        int f = fst[0];
 #pragma unroll
         for (int p = 1; p < NPW; ++p)
           if (lp == p)
@@ -1030,11 +1029,11 @@ namespace gtps
           if (t < NPW * NI * NI)
             {
               const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-              const long long f  = sfirst_of(lp);
+              const int       f  = sfirst_of(lp);
               const int       zi = r / NI, yi = r - zi * NI;
               if (f >= 0)
                 {
-                  const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+                  const int       g0 = f + 1 + (yi + 1) * N1 + (zi + 1) * P;
 #pragma unroll
                   for (int xi = 0; xi < NI; ++xi)
                     rb[pp][xi] = src[g0 + xi];
@@ -1072,7 +1071,7 @@ namespace gtps
               if (t < NPW * NP * NP)
                 {
                   const int       lp = t / (NP * NP), pc = t - lp * (NP * NP);
-                  const long long f  = sfirst_of(lp);
+                  const int       f  = sfirst_of(lp);
                   if (f >= 0)
                     {
                       const int  iy = pc / NP, ix = pc - iy * NP;
@@ -1081,7 +1080,7 @@ namespace gtps
 #pragma unroll
                       for (int iz = 0; iz < NP; ++iz)
                         {
-                          const long long g = f + ix + (long long)iy * N1 + iz * P;
+                          const int       g = f + ix + iy * N1 + iz * P;
                           v[iz] = (face || iz == 0 || iz == NP - 1) ? u[g] : T(0);
                         }
                       zrows(lp, pc, v);
@@ -1098,7 +1097,7 @@ namespace gtps
                   if (t >= NPW * NF)
                     continue;
                   const int       lp = t / NF, fi = t - lp * NF;
-                  const long long f  = sfirst_of(lp);
+                  const int       f  = sfirst_of(lp);
                   if (f < 0)
                     continue;
                   int ix, iy;
@@ -1121,7 +1120,7 @@ namespace gtps
                   T v[NP];
 #pragma unroll
                   for (int iz = 0; iz < NP; ++iz)
-                    v[iz] = u[f + ix + (long long)iy * N1 + iz * P];
+                    v[iz] = u[f + ix + iy * N1 + iz * P];
                   zrows(lp, iy * NP + ix, v);
                 }
 #pragma unroll
@@ -1131,11 +1130,11 @@ namespace gtps
                   if (t >= NPW * NI * NI)
                     continue;
                   const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-                  const long long f  = sfirst_of(lp);
+                  const int       f  = sfirst_of(lp);
                   if (f < 0)
                     continue;
                   const int       iy = 1 + r / NI, ix = 1 + r % NI;
-                  const long long g  = f + ix + (long long)iy * N1;
+                  const int       g  = f + ix + iy * N1;
                   const T         v0 = u[g], v1 = u[g + (NP - 1) * P];
                   T              *sz = sm + lp * PER;
                   const int       pc = iy * NP + ix;
@@ -1203,12 +1202,12 @@ namespace gtps
           for (int t = lane; t < NPW * NP * NP * NP; t += 64)
             {
               const int       lp = t / (NP * NP * NP), r = t - lp * (NP * NP * NP);
-              const long long f  = sfirst_of(lp);
+              const int       f  = sfirst_of(lp);
               const int ix = r % NP, iy = (r / NP) % NP, iz = r / (NP * NP);
               const bool inner = ix > 0 && ix < NP - 1 && iy > 0 && iy < NP - 1 &&
                                  iz > 0 && iz < NP - 1;
               if (f >= 0 && !inner)
-                u[f + ix + (long long)iy * N1 + iz * P] = T(0);
+                u[f + ix + iy * N1 + iz * P] = T(0);
             }
         }
       // stage 3: x-rows, residual, S^T along x -> FD buffer (z-row region)
@@ -1338,7 +1337,7 @@ namespace gtps
           if (t >= NPW * NI * NI)
             continue;
           const int       lp = t / (NI * NI), r = t - lp * (NI * NI);
-          const long long f  = sfirst_of(lp);
+          const int       f  = sfirst_of(lp);
           if (f < 0)
             continue;
           const int zi = r / NI, yi = r - zi * NI;
@@ -1347,7 +1346,7 @@ namespace gtps
           for (int j = 0; j < NI; ++j)
             v[j] = sm[lp * PER + r * NI + j];
           fd_inv<K>(SE, SO, v, o);
-          const long long g0 = f + 1 + (long long)(yi + 1) * N1 + (zi + 1) * P;
+          const int       g0 = f + 1 + (yi + 1) * N1 + (zi + 1) * P;
 #pragma unroll
           for (int i = 0; i < NI; ++i)
             {
@@ -1841,7 +1840,7 @@ namespace gtps
       T       *up = reinterpret_cast<T *>(u);
       const T *sp = reinterpret_cast<const T *>(src);
       if constexpr (2 * K + 1 <= 8)
-        if (lvl->dim == 3 && np <= small_colour_limit())
+        if (lvl->dim == 3 && np <= small_colour_limit() && lvl->N < (1LL << 31))
           {
             const T   scale = (T)lvl->scale, inv = (T)(1.0 / lvl->scale);
             // one patch per wave while a colour has too few patches to fill
