@@ -516,7 +516,7 @@ struct gtps_mg : LaunchTimer
       {
         U = dmalloc(es * N * ni);
         if (!dense2)
-          dense2 = dmalloc(es * (size_t)ni * ni);
+          dense2 = dmalloc(es * (size_t)ni * dense_ld(ni));
         launch_unit_defects(s, dim, N1, prec, D);
         dense_off = true;
         prof_mask = 0;
@@ -953,7 +953,7 @@ struct gtps_dg_mg
       {
         U = dmalloc(es * N * N);
         if (!dense0)
-          dense0 = dmalloc(es * N * N);
+          dense0 = dmalloc(es * (size_t)N * dense_ld(N));
         launch_unit_defects_plain(s, N, prec, D);
         launch_fill(s, (long long)N * N, prec, U, 0.0);
         launch_dg_sweeps_batched(L, U, D, GTPS_MULT_FWD, 2 * coarse_sweeps, 1.0, N);

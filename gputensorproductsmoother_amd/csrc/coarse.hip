@@ -19,7 +19,8 @@ namespace gtps
   namespace
   {
     constexpr int CNT = 256;
-    constexpr int ROWS_PER_WAVE = 4;
+    // loads in flight per lane in the row loop of dense_apply_kernel
+    constexpr int ROW_BATCH = 8;
 
     // interior DoF j (x fastest over the (N1-2)^dim interior) -> level index;
     // DIM = 0: no constrained rows, the identity
@@ -48,34 +49,63 @@ namespace gtps
         }
     }
 
-    // B[i][j] = U[j][interior i]
+    // B[i][j] = U[j][interior i] for j < ni, 0 in the padding columns
+    // ni <= j < ld (rows of ld entries, 16-byte aligned)
     template <typename T, int DIM>
     __global__ void __launch_bounds__(CNT)
     gather_columns_kernel(const T *__restrict__ U, T *__restrict__ B, const int N1,
-                          const long long N, const int ni)
+                          const long long N, const int ni, const int ld)
     {
-      for (long long e = (long long)blockIdx.x * CNT + threadIdx.x; e < (long long)ni * ni;
+      for (long long e = (long long)blockIdx.x * CNT + threadIdx.x; e < (long long)ni * ld;
            e += (long long)gridDim.x * CNT)
         {
-          const int i = (int)(e / ni), j = (int)(e - (long long)i * ni);
-          B[e]        = U[(long long)j * N + interior_index<DIM>(i, N1)];
+          const int i = (int)(e / ld), j = (int)(e - (long long)i * ld);
+          B[e]        = j < ni ? U[(long long)j * N + interior_index<DIM>(i, N1)] : T(0);
         }
     }
 
-    // u = B d on the interior, 0 on the Dirichlet DoFs; one wavefront per
-    // ROWS_PER_WAVE rows, lanes across the columns (coalesced rows of B)
+    template <typename T>
+    struct Vec16;
+    template <>
+    struct Vec16<float>
+    {
+      using type = float4;
+      __device__ static float
+      dot(const float4 a, const float4 b)
+      {
+        return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+      }
+    };
+    template <>
+    struct Vec16<double>
+    {
+      using type = double2;
+      __device__ static double
+      dot(const double2 a, const double2 b)
+      {
+        return a.x * b.x + a.y * b.y;
+      }
+    };
+
+    // u = B d on the interior, 0 on the Dirichlet DoFs.  One wavefront per
+    // row: the lanes read the row in 16-byte pieces, ROW_BATCH of them in
+    // flight per lane before the first product, so a row of up to
+    // 64 * ROW_BATCH pieces costs one memory round trip; d (gathered to the
+    // interior numbering) sits in LDS.
     template <typename T, int DIM>
     __global__ void __launch_bounds__(CNT)
     dense_apply_kernel(const T *__restrict__ B, const T *__restrict__ d, T *__restrict__ u,
-                       const int N1, const long long N, const int ni)
+                       const int N1, const long long N, const int ni, const int ld)
     {
-      __shared__ T sd[MAX_DENSE_COARSE];
-      const int    tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+      using V  = typename Vec16<T>::type;
+      constexpr int PER = 16 / sizeof(T);
+      __shared__ __align__(16) T sd[MAX_DENSE_COARSE];
+      const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
       // batch: blockIdx.y selects one of several contiguous vectors
       d += (long long)blockIdx.y * N;
       u += (long long)blockIdx.y * N;
-      for (int j = tid; j < ni; j += CNT)
-        sd[j] = d[interior_index<DIM>(j, N1)];
+      for (int j = tid; j < ld; j += CNT)
+        sd[j] = j < ni ? d[interior_index<DIM>(j, N1)] : T(0);
       // Dirichlet rows: zero (spread over the grid)
       for (long long g = (long long)blockIdx.x * CNT + tid; DIM != 0 && g < N;
            g += (long long)gridDim.x * CNT)
@@ -88,21 +118,34 @@ namespace gtps
             u[g] = T(0);
         }
       __syncthreads();
-      const int row0 = (blockIdx.x * (CNT / 64) + wave) * ROWS_PER_WAVE;
-#pragma unroll
-      for (int rr = 0; rr < ROWS_PER_WAVE; ++rr)
+      const int i = blockIdx.x * (CNT / 64) + wave;
+      if (i >= ni)
+        return;
+      const V  *row = reinterpret_cast<const V *>(B + (long long)i * ld);
+      const V  *sv  = reinterpret_cast<const V *>(sd);
+      const int nv  = ld / PER;
+      T         acc = 0;
+      for (int base = 0; base < nv; base += 64 * ROW_BATCH)
         {
-          const int i = row0 + rr;
-          if (i >= ni)
-            break;
-          const T *row = B + (long long)i * ni;
-          T        acc = 0;
-          for (int j = lane; j < ni; j += 64)
-            acc += row[j] * sd[j];
-          const double s = wave_sum((double)acc);
-          if (lane == 0)
-            u[interior_index<DIM>(i, N1)] = (T)s;
+          V r[ROW_BATCH];
+#pragma unroll
+          for (int q = 0; q < ROW_BATCH; ++q)
+            {
+              const int j = base + q * 64 + lane;
+              if (j < nv)
+                r[q] = row[j];
+            }
+#pragma unroll
+          for (int q = 0; q < ROW_BATCH; ++q)
+            {
+              const int j = base + q * 64 + lane;
+              if (j < nv)
+                acc += Vec16<T>::dot(r[q], sv[j]);
+            }
         }
+      const double s = wave_sum((double)acc);
+      if (lane == 0)
+        u[interior_index<DIM>(i, N1)] = (T)s;
     }
 
     int
@@ -132,6 +175,12 @@ namespace gtps
     }
   } // namespace
 
+  int
+  dense_ld(int n)
+  {
+    return (n + 3) & ~3;
+  }
+
   void
   launch_unit_defects_plain(hipStream_t s, int n, int prec, void *D)
   {
@@ -147,12 +196,15 @@ namespace gtps
   void
   launch_transpose_plain(hipStream_t s, int n, int prec, const void *U, void *B)
   {
+    const int ld = dense_ld(n);
     if (prec == GTPS_FP32)
-      hipLaunchKernelGGL((gather_columns_kernel<float, 0>), dim3(cgrid((long long)n * n)),
-                         dim3(CNT), 0, s, (const float *)U, (float *)B, 0, (long long)n, n);
+      hipLaunchKernelGGL((gather_columns_kernel<float, 0>), dim3(cgrid((long long)n * ld)),
+                         dim3(CNT), 0, s, (const float *)U, (float *)B, 0, (long long)n, n,
+                         ld);
     else
-      hipLaunchKernelGGL((gather_columns_kernel<double, 0>), dim3(cgrid((long long)n * n)),
-                         dim3(CNT), 0, s, (const double *)U, (double *)B, 0, (long long)n, n);
+      hipLaunchKernelGGL((gather_columns_kernel<double, 0>), dim3(cgrid((long long)n * ld)),
+                         dim3(CNT), 0, s, (const double *)U, (double *)B, 0, (long long)n, n,
+                         ld);
     GTPS_LAUNCH_CHECK();
   }
 
@@ -160,17 +212,18 @@ namespace gtps
   launch_dense_apply_plain(hipStream_t s, int n, int prec, const void *B, const void *d,
                            void *u, int nvec)
   {
-    GTPS_REQUIRE(n <= MAX_DENSE_COARSE, "dense coarse operator too large");
+    GTPS_REQUIRE(n >= 1 && dense_ld(n) <= MAX_DENSE_COARSE, "dense coarse operator too large");
     GTPS_REQUIRE(nvec >= 1, "bad batch");
-    const int rows_per_block = (CNT / 64) * ROWS_PER_WAVE;
-    const dim3 grid((n + rows_per_block - 1) / rows_per_block, nvec);
+    const int  ld = dense_ld(n);
+    const dim3 grid((n + CNT / 64 - 1) / (CNT / 64), nvec);
     if (prec == GTPS_FP32)
       hipLaunchKernelGGL((dense_apply_kernel<float, 0>), grid, dim3(CNT), 0, s,
-                         (const float *)B, (const float *)d, (float *)u, 0, (long long)n, n);
+                         (const float *)B, (const float *)d, (float *)u, 0, (long long)n, n,
+                         ld);
     else
       hipLaunchKernelGGL((dense_apply_kernel<double, 0>), grid, dim3(CNT), 0, s,
                          (const double *)B, (const double *)d, (double *)u, 0, (long long)n,
-                         n);
+                         n, ld);
     GTPS_LAUNCH_CHECK();
   }
 
@@ -197,13 +250,13 @@ namespace gtps
   void
   launch_gather_columns(hipStream_t s, int dim, int N1, int prec, const void *U, void *B)
   {
-    const int       ni = dense_coarse_size(dim, N1);
+    const int       ni = dense_coarse_size(dim, N1), ld = dense_ld(ni);
     const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
     by_type(dim, prec, [&](auto tp, auto dd) {
       using T = std::remove_pointer_t<decltype(tp)>;
       hipLaunchKernelGGL((gather_columns_kernel<T, decltype(dd)::value>),
-                         dim3(cgrid((long long)ni * ni)), dim3(CNT), 0, s, (const T *)U,
-                         (T *)B, N1, N, ni);
+                         dim3(cgrid((long long)ni * ld)), dim3(CNT), 0, s, (const T *)U,
+                         (T *)B, N1, N, ni, ld);
     });
     GTPS_LAUNCH_CHECK();
   }
@@ -212,15 +265,14 @@ namespace gtps
   launch_dense_apply(hipStream_t s, int dim, int N1, int prec, const void *B, const void *d,
                      void *u)
   {
-    const int       ni = dense_coarse_size(dim, N1);
-    GTPS_REQUIRE(ni <= MAX_DENSE_COARSE, "dense coarse operator too large");
+    const int ni = dense_coarse_size(dim, N1), ld = dense_ld(ni);
+    GTPS_REQUIRE(ni >= 1 && ld <= MAX_DENSE_COARSE, "dense coarse operator too large");
     const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
-    const int rows_per_block = (CNT / 64) * ROWS_PER_WAVE;
-    const int nb             = (ni + rows_per_block - 1) / rows_per_block;
+    const int       nb = (ni + CNT / 64 - 1) / (CNT / 64);
     by_type(dim, prec, [&](auto tp, auto dd) {
       using T = std::remove_pointer_t<decltype(tp)>;
       hipLaunchKernelGGL((dense_apply_kernel<T, decltype(dd)::value>), dim3(nb), dim3(CNT), 0,
-                         s, (const T *)B, (const T *)d, (T *)u, N1, N, ni);
+                         s, (const T *)B, (const T *)d, (T *)u, N1, N, ni, ld);
     });
     GTPS_LAUNCH_CHECK();
   }

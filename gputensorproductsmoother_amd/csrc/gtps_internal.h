@@ -266,6 +266,9 @@ namespace gtps
   // coarse.hip: the level-2 V-cycle as a dense matrix over the interior
   // DoFs ((N1-2)^dim of them, N1 = DoFs per direction of level 2)
   int  dense_coarse_size(int dim, int N1);
+  // row length of the stored matrices: n rounded up to a multiple of 4
+  // (16-byte rows; the padding columns hold zeros)
+  int  dense_ld(int n);
   void launch_unit_defects(hipStream_t s, int dim, int N1, int prec, void *D);
   void launch_gather_columns(hipStream_t s, int dim, int N1, int prec,
                              const void *U, void *B);
