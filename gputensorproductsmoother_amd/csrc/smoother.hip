@@ -964,8 +964,14 @@ namespace gtps
       static constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
       static constexpr int ZS = 2 * NI * NP * NP;  // z-rows of M and K
       static constexpr int YS = 2 * NI * NI * NP;  // y-rows (C1, C2)
-      // FD work buffer (NI^3) reuses the z-row region after stage 2
-      static constexpr int PER = (ZS + YS) | 1;
+      // The y-rows live at the end of the z-row region (stage 2 reads all
+      // of its z-row inputs before its first store) and the FD work buffer
+      // (NI^3) at its start, clear of the y-rows that stage 3 reads: one
+      // region per patch, which keeps LDS from limiting the wavefronts per
+      // SIMD.
+      static constexpr int YOFF = ZS - YS;
+      static constexpr int PER  = ZS | 1;
+      static_assert(NI * NI * NI <= YOFF, "FD buffer overlaps the y-rows");
     };
 
     template <int K, typename T, int MODE, int NPW>
@@ -1159,7 +1165,27 @@ namespace gtps
                 }
             }
           __syncthreads();
-          // stage 2: y-rows, C1 = My Mz u, C2 = Ky Mz u + My Kz u
+          // stage 2: y-rows, C1 = My Mz u, C2 = Ky Mz u + My Kz u.  Every
+          // pass reads its z-rows before the first y-row is stored (the
+          // y-rows overwrite the end of the z-row region).
+          T a2[P2][NP], b2[P2][NP];
+#pragma unroll
+          for (int pp = 0; pp < P2; ++pp)
+            {
+              const int t = lane + 64 * pp;
+              if (t >= NPW * NI * NP)
+                continue;
+              const int lp = t / (NI * NP), r = t - lp * (NI * NP);
+              const int zi = r / NP, ix = r - zi * NP;
+              const T  *sz = sm + lp * PER;
+#pragma unroll
+              for (int iy = 0; iy < NP; ++iy)
+                {
+                  a2[pp][iy] = sz[(zi * NP + iy) * NP + ix];
+                  b2[pp][iy] = sz[((NI + zi) * NP + iy) * NP + ix];
+                }
+            }
+          __syncthreads();
 #pragma unroll
           for (int pp = 0; pp < P2; ++pp)
             {
@@ -1170,15 +1196,9 @@ namespace gtps
               if (sfirst_of(lp) < 0)
                 continue;
               const int zi = r / NP, ix = r - zi * NP;
-              const T  *sz = sm + lp * PER;
-              T        *sy = sm + lp * PER + ZS;
-              T         a[NP], b[NP];
-#pragma unroll
-              for (int iy = 0; iy < NP; ++iy)
-                {
-                  a[iy] = sz[(zi * NP + iy) * NP + ix];
-                  b[iy] = sz[((NI + zi) * NP + iy) * NP + ix];
-                }
+              T        *sy = sm + lp * PER + C::YOFF;
+              const T (&a)[NP] = a2[pp];
+              const T (&b)[NP] = b2[pp];
 #pragma unroll
               for (int yi = 0; yi < NI; ++yi)
                 {
@@ -1223,7 +1243,7 @@ namespace gtps
           T res[NI];
           if (MODE == 0)
             {
-              const T *sy = sm + lp * PER + ZS;
+              const T *sy = sm + lp * PER + C::YOFF;
               T        c1[NP], c2[NP];
 #pragma unroll
               for (int ix = 0; ix < NP; ++ix)
