@@ -958,12 +958,19 @@ namespace gtps
       v[MH] = pm;
     }
 
-    template <int K>
+    template <int K, typename T>
     struct SG
     {
       static constexpr int NP = 2 * K + 1, NI = 2 * K - 1;
-      static constexpr int ZS = 2 * NI * NP * NP;  // z-rows of M and K
-      static constexpr int YS = 2 * NI * NI * NP;  // y-rows (C1, C2)
+      // z-rows of M and K: 2 NI slabs of NP^2, ZST apart.  Stage 2 reads
+      // slab offsets zi ZST + ix across the lanes: k = 3 in FP32 pads the
+      // slab from 49 to 57 words (25 mod 32), which spreads the 35 (zi, ix)
+      // lanes of a patch over distinct banks but for 3 (49 = 17 mod 32
+      // gives 2- and 3-way conflicts); FP64 keeps 49 (its occupancy is set
+      // by LDS).
+      static constexpr int ZST = NP * NP + (K == 3 && sizeof(T) == 4 ? 8 : 0);
+      static constexpr int ZS  = 2 * NI * ZST;
+      static constexpr int YS  = 2 * NI * NI * NP;  // y-rows (C1, C2)
       // The y-rows live at the end of the z-row region (stage 2 reads all
       // of its z-row inputs before its first store) and the FD work buffer
       // (NI^3) at its start, clear of the y-rows that stage 3 reads: one
@@ -982,8 +989,8 @@ namespace gtps
                          const T relax, const FastDiv div_x, const FastDiv div_xy)
     {
       using TB         = Tab<K>;
-      using C          = SG<K>;
-      constexpr int NP = C::NP, NI = C::NI, PER = C::PER, ZS = C::ZS;
+      using C          = SG<K, T>;
+      constexpr int NP = C::NP, NI = C::NI, PER = C::PER, ZS = C::ZS, ZST = C::ZST;
       static_assert(NP * NP <= 64, "one lane per z-pencil");
       constexpr int P1 = (NPW * NP * NP + 63) / 64;  // passes per stage
       constexpr int P2 = (NPW * NI * NP + 63) / 64;
@@ -1067,8 +1074,8 @@ namespace gtps
                     m += pM[row * NP + c] * v[c];
                     kk += pK[row * NP + c] * v[c];
                   }
-                sz[zi * NP * NP + pc]        = m;
-                sz[(NI + zi) * NP * NP + pc] = kk;
+                sz[zi * ZST + pc]        = m;
+                sz[(NI + zi) * ZST + pc] = kk;
               }
           };
           if constexpr (NPW * NP * NP <= 64)
@@ -1159,8 +1166,8 @@ namespace gtps
                           m += pM[row * NP + NP - 1] * v1;
                           kk += pK[row * NP + NP - 1] * v1;
                         }
-                      sz[zi * NP * NP + pc]        = m;
-                      sz[(NI + zi) * NP * NP + pc] = kk;
+                      sz[zi * ZST + pc]        = m;
+                      sz[(NI + zi) * ZST + pc] = kk;
                     }
                 }
             }
@@ -1181,8 +1188,8 @@ namespace gtps
 #pragma unroll
               for (int iy = 0; iy < NP; ++iy)
                 {
-                  a2[pp][iy] = sz[(zi * NP + iy) * NP + ix];
-                  b2[pp][iy] = sz[((NI + zi) * NP + iy) * NP + ix];
+                  a2[pp][iy] = sz[zi * ZST + iy * NP + ix];
+                  b2[pp][iy] = sz[(NI + zi) * ZST + iy * NP + ix];
                 }
             }
           __syncthreads();
