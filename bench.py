@@ -431,6 +431,27 @@ def cpu_baseline(args):
             "kind": "port", "sample": sample}
 
 
+def cpu_baseline_one_core(args):
+    """The same FP64 oracle MG-PCG on one host core, on a bounded sample: the
+    solve one level below the workload (one eighth of the DoFs in 3D)."""
+    import oracle as O
+    prev = O.set_threads(0)
+    O.set_threads(1)
+    try:
+        lvl = args.level - 1
+        b = O.rhs_random(args.dim, args.degree, lvl, SEED)
+        t0 = time.perf_counter()
+        _, it, _, _ = O.pcg(args.dim, args.degree, lvl, b, rtol=args.rtol, maxit=100)
+        t = time.perf_counter() - t0
+    finally:
+        O.set_threads(prev)
+    n = len(b)
+    return {"value": round(n / t / 1e6, 3), "unit": "MDoF/s", "cores": 1,
+            "kind": "port",
+            "sample": f"level {lvl} ({n} DoFs) full FP64 MG-PCG solve, {it} iterations, "
+                      f"{t:.1f} s"}
+
+
 def kernel_bench(args):
     """Per-kernel timing of the level-`level` building blocks (not the
     contract line): finest smoother sweep FP32/FP64, vmult FP64 and FP32
@@ -660,6 +681,8 @@ def main():
         mg32.set_profiling(())
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    if cpu is not None:
+        extra["cpu_baseline_one_core"] = cpu_baseline_one_core(args)
 
     line = {
         "metric": "MDoF/s of MG-CG to 1e-8 rel, FP32-smoother vs FP64, 3D Q3 Poisson, 1 GPU",
