@@ -79,6 +79,13 @@ namespace gtps
     {
       static constexpr int TCX = 16, TCY = 8, NT = 256;
     };
+    // k = 7: 4x4 cells (the 8x2 tile of VT3<7> holds 5 columns of a
+    // 15-plane window per thread: 256 VGPRs and spills in FP64)
+    template <>
+    struct VT3C<7, 2>
+    {
+      static constexpr int TCX = 4, TCY = 4, NT = 512;
+    };
 
     template <int K, typename T, bool RESID, bool DOT, int CFG = 0>
     __global__ void __launch_bounds__((VT3C<K, CFG>::NT))
@@ -693,9 +700,11 @@ namespace gtps
             const int tx  = (n + C::TCX - 1) / C::TCX;
             const int ty  = (n + C::TCY - 1) / C::TCY;
             // z-chunks: about vmult_zdiv work-groups in total (0: 512 in
-            // FP32, 2048 in FP64, from a sweep at 64^3 Q3)
+            // FP32, 2048 in FP64 from a sweep at 64^3 Q3; 512 and 256 for
+            // k = 7 from a sweep at 32^3 Q7)
             const long long zdiv = tuning().vmult_zdiv > 0 ? tuning().vmult_zdiv
                                    : sizeof(T) == 4        ? 512
+                                   : K == 7                ? 256
                                                            : 2048;
             int zc = (int)(((long long)n * tx * ty + zdiv - 1) / zdiv);
             zc     = std::max(1, std::min(zc, n));
@@ -734,6 +743,13 @@ namespace gtps
               if (tile == 1)
                 return go(std::integral_constant<int, 1>{});
               if (tile == 2)
+                return go(std::integral_constant<int, 2>{});
+            }
+          if constexpr (K == 7)
+            {
+              // 3 (auto) and 2: 4x4 cells / 512 threads; 0, 1: 8x2 / 512
+              const int tile = tuning().vmult_tile;
+              if (tile == 2 || tile == 3)
                 return go(std::integral_constant<int, 2>{});
             }
           return go(std::integral_constant<int, 0>{});

@@ -313,3 +313,33 @@ def test_vmult_tiles(core, ctx, level, tile, prec):
     ref = O.laplace_vmult(3, k, level, x)
     assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
     assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec] * 10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tile", [0, 2])
+@pytest.mark.parametrize("level", [2, 3])
+def test_vmult_tiles_k7(core, ctx, level, tile, prec):
+    """The two z-marching tiles of the k = 7 vmult (8x2 cells and 4x4 cells,
+    512 threads) agree with the oracle, vmult and residual; level 2 (4 cells
+    per direction) leaves the 8x2 tile partial in x."""
+    _announce("vmult-tile-k7", level, tile, prec)
+    k = 7
+    g = rng(level * 17 + tile)
+    N = n_dofs(3, k, level)
+    x, b = g.uniform(-1, 1, (2, N))
+    if prec == "fp32":
+        x = x.astype(np.float32).astype(np.float64)
+        b = b.astype(np.float32).astype(np.float64)
+    prev = core.set_tuning("vmult_tile", tile)
+    try:
+        op = core.LaplaceOperator(ctx, 3, k, level, _dt(prec))
+        y = torch.empty(N, dtype=_dt(prec), device="cuda:0")
+        op.vmult(y, _to_dev(x, prec))
+        r = torch.empty_like(y)
+        op.residual(r, _to_dev(b, prec), _to_dev(x, prec))
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("vmult_tile", prev)
+    ref = O.laplace_vmult(3, k, level, x)
+    assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
+    assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec] * 10
