@@ -106,7 +106,16 @@ namespace gtps
     }
 
     // PASS 0 (z): o1 = M x, o2 = L x;  PASS 1 (y): o1 = M i1, o2 = L i1 + M i2
-    // PASS 2 (x): y = scale (L i1 + M i2), residual / dot forms
+    // PASS 2 (x): y = scale (L i1 + M i2), residual / dot forms.
+    // Passes 0 and 1: one thread per cell line along the pass direction (the
+    // NN DoFs of one cell), the cell and its two neighbour cells loaded once
+    // for all NN outputs, lanes along x (coalesced; the cell index, and with
+    // it the variant of the diagonal block of L, is uniform across a
+    // wavefront except where it crosses a plane).  Pass 2 runs along x
+    // itself: one thread per DoF with dg_L_line / dg_M_line, so that the
+    // lanes stay coalesced (a line per thread there reads strided rows).  An
+    // output of L sums the diagonal block, then the block of the previous
+    // cell, then that of the next, in both forms.
     template <int K, typename T, int PASS, bool RESID, bool DOT>
     __global__ void __launch_bounds__(DNT)
     dg_op_kernel(const T *__restrict__ i1, const T *__restrict__ i2,
@@ -116,6 +125,8 @@ namespace gtps
                  const long long N, const T scale,
                  double *__restrict__ partials)
     {
+      using TB         = DTab<K>;
+      constexpr int NN = TB::NN;
       __shared__ double red[DNT / 64];
       double            dot = 0.0;
       const long long   P   = (long long)N1 * N1;
@@ -134,26 +145,13 @@ namespace gtps
           x0 += bo;
           partials += (long long)blockIdx.y * gridDim.x;
         }
-      for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < N;
-           g += (long long)gridDim.x * DNT)
+      if constexpr (PASS == 2)
         {
-          const int X = (int)(g % N1), Y = (int)((g / N1) % N1), Z = (int)(g / P);
-          if (PASS == 0)
+          for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < N;
+               g += (long long)gridDim.x * DNT)
             {
-              const long long base = X + (long long)Y * N1;
-              o1[g] = dg_M_line<K>(tab, i1, base, P, Z);
-              o2[g] = dg_L_line<K>(tab, i1, base, P, Z, n);
-            }
-          else if (PASS == 1)
-            {
-              const long long base = X + (long long)Z * P;
-              o1[g] = dg_M_line<K>(tab, i1, base, N1, Y);
-              o2[g] = dg_L_line<K>(tab, i1, base, N1, Y, n) +
-                      dg_M_line<K>(tab, i2, base, N1, Y);
-            }
-          else
-            {
-              const long long base = (long long)Y * N1 + (long long)Z * P;
+              const int       X    = (int)(g % N1);
+              const long long base = g - X;
               T v = scale * (dg_L_line<K>(tab, i1, base, 1, X, n) +
                              dg_M_line<K>(tab, i2, base, 1, X));
               if (RESID)
@@ -162,6 +160,97 @@ namespace gtps
               if (DOT)
                 dot += (double)x0[g] * (double)v;
             }
+        }
+      const long long st = PASS == 0 ? P : (long long)N1;
+      const long long NL = PASS == 2 ? 0 : N / NN;
+      for (long long q = (long long)blockIdx.x * DNT + threadIdx.x; q < NL;
+           q += (long long)gridDim.x * DNT)
+        {
+          int       c;
+          long long base;
+          if (PASS == 0)
+            {
+              // lanes along x, lines along z
+              const long long r = q / N1;
+              base = q - r * N1 + (r % N1) * N1;
+              c    = (int)(r / N1);
+            }
+          else if (PASS == 1)
+            {
+              // lanes along x, lines along y
+              const long long r = q / N1;
+              base = q - r * N1 + (r / n) * P;
+              c    = (int)(r % n);
+            }
+          else
+            {
+              base = 0;
+              c    = 0;
+            }
+          const long long c0 = base + (long long)c * NN * st;
+          T a[NN], pm[NN], pp[NN], bb[NN];
+#pragma unroll
+          for (int j = 0; j < NN; ++j)
+            {
+              a[j]  = i1[c0 + j * st];
+              pm[j] = c > 0 ? i1[c0 + (j - NN) * st] : T(0);
+              pp[j] = c < n - 1 ? i1[c0 + (j + NN) * st] : T(0);
+              bb[j] = PASS != 0 ? i2[c0 + j * st] : T(0);
+            }
+          auto emit = [&](auto variant) {
+            constexpr int V  = decltype(variant)::value;
+            const T      *Lc = tab + TB::Lc + V * NN * NN;
+            const T      *Lm = tab + TB::Lm, *Lp = tab + TB::Lp, *M = tab + TB::M;
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+              {
+                T l = 0;
+#pragma unroll
+                for (int j = 0; j < NN; ++j)
+                  l += Lc[i * NN + j] * a[j];
+                if (c > 0)
+                  {
+#pragma unroll
+                    for (int j = 0; j < NN; ++j)
+                      l += Lm[i * NN + j] * pm[j];
+                  }
+                if (c < n - 1)
+                  {
+#pragma unroll
+                    for (int j = 0; j < NN; ++j)
+                      l += Lp[i * NN + j] * pp[j];
+                  }
+                const long long g = c0 + i * st;
+                if (PASS == 0)
+                  {
+                    T m = 0;
+#pragma unroll
+                    for (int j = 0; j < NN; ++j)
+                      m += M[i * NN + j] * a[j];
+                    o1[g] = m;
+                    o2[g] = l;
+                  }
+                else
+                  {
+                    T mb = 0, m = 0;
+#pragma unroll
+                    for (int j = 0; j < NN; ++j)
+                      mb += M[i * NN + j] * bb[j];
+#pragma unroll
+                    for (int j = 0; j < NN; ++j)
+                      m += M[i * NN + j] * a[j];
+                    o1[g] = m;
+                    o2[g] = l + mb;
+                  }
+              }
+          };
+          // diagonal block variant as dg_cell_variant: first, last, interior
+          if (c == 0)
+            emit(std::integral_constant<int, 1>{});
+          else if (c == n - 1)
+            emit(std::integral_constant<int, 2>{});
+          else
+            emit(std::integral_constant<int, 0>{});
         }
       if (DOT)
         {
@@ -631,14 +720,16 @@ namespace gtps
       const T    *tab = (const T *)lvl->tab;
       T *t0 = (T *)lvl->t[0], *t1 = (T *)lvl->t[1], *t2 = (T *)lvl->t[2],
         *t3 = (T *)lvl->t[3];
+      // passes 0 and 1: one thread per cell line of K+1 DoFs; pass 2: per DoF
+      const dim3 nbl(dgrid(lvl->N / (K + 1)), nvec);
       const dim3 nb(dgrid(lvl->N), nvec);
       const dim3 nbd(dgrid(lvl->N, std::min(lvl->ctx->ws.capacity / nvec, 2048)), nvec);
       double    *pt = lvl->ctx->ws.partials;
       const T    sc = (T)lvl->scale;
-      hipLaunchKernelGGL((dg_op_kernel<K, T, 0, false, false>), nb, dim3(DNT), 0,
+      hipLaunchKernelGGL((dg_op_kernel<K, T, 0, false, false>), nbl, dim3(DNT), 0,
                          s, x, nullptr, t0, t1, tab, nullptr, nullptr, lvl->n,
                          lvl->N1, lvl->N, sc, pt);
-      hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), nb, dim3(DNT), 0,
+      hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), nbl, dim3(DNT), 0,
                          s, t0, t1, t2, t3, tab, nullptr, nullptr, lvl->n,
                          lvl->N1, lvl->N, sc, pt);
 #define GTPS_DX(R, D, G)                                                      \
