@@ -260,6 +260,101 @@ namespace gtps
         }
     }
 
+    // Pass 2 (x) with the rows staged in LDS: a work-group takes RB whole
+    // x-rows (contiguous in memory) of i1 and i2 with coalesced loads, then
+    // every output reads its cell and neighbour cells, and the 1D blocks of
+    // its node, from LDS, in the sums of dg_L_line / dg_M_line.  Rows of at
+    // most DG_ROW_CAP values.
+    constexpr int DG_ROW_CAP = 2048;
+
+    template <int K, typename T, bool RESID, bool DOT>
+    __global__ void __launch_bounds__(DNT)
+    dg_op_x_kernel(const T *__restrict__ i1, const T *__restrict__ i2,
+                   T *__restrict__ o1, const T *__restrict__ tab,
+                   const T *__restrict__ b, const T *__restrict__ x0, const int n,
+                   const int N1, const long long N, const int RB, const T scale,
+                   double *__restrict__ partials)
+    {
+      using TB         = DTab<K>;
+      constexpr int NN = TB::NN;
+      // M, L-, L+ and the three diagonal blocks of L (they lead the table)
+      constexpr int NTAB = TB::Lc + 3 * NN * NN;
+      __shared__ T      s1[DG_ROW_CAP], s2[DG_ROW_CAP], st[NTAB];
+      __shared__ double red[DNT / 64];
+      double            dot = 0.0;
+      for (int e = threadIdx.x; e < NTAB; e += DNT)
+        st[e] = tab[e];
+      const long long   bo  = (long long)blockIdx.y * N;
+      i1 += bo;
+      i2 += bo;
+      o1 += bo;
+      if (RESID)
+        b += bo;
+      if (DOT)
+        {
+          x0 += bo;
+          partials += (long long)blockIdx.y * gridDim.x;
+        }
+      const long long rows = N / N1;
+      const int       E    = RB * N1;
+      for (long long r0 = (long long)blockIdx.x * RB; r0 < rows;
+           r0 += (long long)gridDim.x * RB)
+        {
+          const long long g0 = r0 * N1;
+          const int       ne = N - g0 < E ? (int)(N - g0) : E;
+          __syncthreads();  // the previous round's reads are done
+          for (int e = threadIdx.x; e < ne; e += DNT)
+            {
+              s1[e] = i1[g0 + e];
+              s2[e] = i2[g0 + e];
+            }
+          __syncthreads();
+          for (int e = threadIdx.x; e < ne; e += DNT)
+            {
+              const int X = e % N1, c = X / NN, i = X - c * NN;
+              const T  *in1 = s1 + (e - X) + c * NN;
+              const T  *in2 = s2 + (e - X) + c * NN;
+              const T  *Lc  = st + TB::Lc + dg_cell_variant(c, n) * NN * NN + i * NN;
+              T         l   = 0;
+#pragma unroll
+              for (int j = 0; j < NN; ++j)
+                l += Lc[j] * in1[j];
+              if (c > 0)
+                {
+                  const T *Lm = st + TB::Lm + i * NN;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    l += Lm[j] * in1[j - NN];
+                }
+              if (c < n - 1)
+                {
+                  const T *Lp = st + TB::Lp + i * NN;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    l += Lp[j] * in1[j + NN];
+                }
+              const T *M = st + TB::M + i * NN;
+              T        m = 0;
+#pragma unroll
+              for (int j = 0; j < NN; ++j)
+                m += M[j] * in2[j];
+              const long long g = g0 + e;
+              T               v = scale * (l + m);
+              if (RESID)
+                v = b[g] - v;
+              o1[g] = v;
+              if (DOT)
+                dot += (double)x0[g] * (double)v;
+            }
+        }
+      if (DOT)
+        {
+          dot = block_sum<DNT>(dot, red);
+          if (threadIdx.x == 0)
+            partials[blockIdx.x] = dot;
+        }
+    }
+
     // LDS plan of the patch kernel: r, t (patch box, ping-pong), NS face
     // slabs and their intermediate (NS = 6: all faces in one phase; 2: one
     // direction per phase; 1: one face per phase, whichever fits 64 KB), and
@@ -732,10 +827,24 @@ namespace gtps
       hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), nbl, dim3(DNT), 0,
                          s, t0, t1, t2, t3, tab, nullptr, nullptr, lvl->n,
                          lvl->N1, lvl->N, sc, pt);
-#define GTPS_DX(R, D, G)                                                      \
-  hipLaunchKernelGGL((dg_op_kernel<K, T, 2, R, D>), G, dim3(DNT), 0, s,       \
-                     t2, t3, y, nullptr, tab, b, x, lvl->n, lvl->N1, lvl->N,  \
-                     sc, pt)
+      // x pass: whole rows staged in LDS when a row fits, else per DoF
+      const bool rows_fit = lvl->N1 <= DG_ROW_CAP;
+      const int  RB       = std::max(1, 1024 / lvl->N1);
+      const dim3 nbr(dgrid((lvl->N / lvl->N1 + RB - 1) / RB * DNT), nvec);
+      const dim3 nbrd(dgrid((lvl->N / lvl->N1 + RB - 1) / RB * DNT,
+                            std::min(lvl->ctx->ws.capacity / nvec, 2048)),
+                      nvec);
+#define GTPS_DX(R, D, G)                                                             \
+  do                                                                                 \
+    {                                                                                \
+      if (rows_fit)                                                                  \
+        hipLaunchKernelGGL((dg_op_x_kernel<K, T, R, D>), D ? nbrd : nbr, dim3(DNT), 0, \
+                           s, t2, t3, y, tab, b, x, lvl->n, lvl->N1, lvl->N, RB, sc, pt); \
+      else                                                                           \
+        hipLaunchKernelGGL((dg_op_kernel<K, T, 2, R, D>), G, dim3(DNT), 0, s, t2, t3, \
+                           y, nullptr, tab, b, x, lvl->n, lvl->N1, lvl->N, sc, pt);  \
+    }                                                                                \
+  while (0)
       if (b && dot)
         GTPS_DX(true, true, nbd);
       else if (b)
@@ -746,7 +855,7 @@ namespace gtps
         GTPS_DX(false, false, nb);
 #undef GTPS_DX
       GTPS_LAUNCH_CHECK();
-      return dot ? (int)nbd.x : 0;
+      return dot ? (int)(rows_fit ? nbrd.x : nbd.x) : 0;
     }
 
     // n_sweeps sweeps, alternating colour order from `first`
