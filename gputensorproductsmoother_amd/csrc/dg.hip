@@ -632,7 +632,19 @@ namespace gtps
         }
     }
 
-    // fine += P coarse: one thread per fine DoF
+    // Transfer, one work-group per coarse cell: its 8 children form one
+    // (2k+2)^3 brick of fine DoFs, and the embedding is the tensor product of
+    // the 1D map Pm[xf][a] = E[xf][a] (fine node xf = child * (k+1) + q of the
+    // brick, coarse node a), applied in three LDS passes.
+    template <int K, typename T>
+    struct DT
+    {
+      static constexpr int NN = K + 1, FB = 2 * NN;  // brick width
+      static constexpr int F3 = FB * FB * FB, C3 = NN * NN * NN;
+      static constexpr int S1 = NN * FB * FB, S2 = NN * NN * FB;
+    };
+
+    // fine += P coarse
     template <int K, typename T>
     __global__ void __launch_bounds__(DNT)
     dg_prolongate_kernel(const T *__restrict__ coarse, T *__restrict__ fine,
@@ -640,39 +652,64 @@ namespace gtps
                          const long long Nf)
     {
       using TB          = DTab<K>;
-      constexpr int NN  = TB::NN;
-      const int       N1f = nf * NN, N1c = (nf / 2) * NN;
+      using C           = DT<K, T>;
+      constexpr int NN  = C::NN, FB = C::FB;
+      __shared__ T    sc[C::C3], s1[C::S2], s2[C::S1], sE[FB * NN];
+      const int       tid = threadIdx.x;
+      const int       ncd = nf / 2;  // coarse cells per direction
+      const int       N1f = nf * NN, N1c = ncd * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
-      const T        *E  = tab + TB::E;
       fine += (long long)blockIdx.y * Nf;
       coarse += (long long)blockIdx.y * Pc * N1c;
-      for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < Nf;
-           g += (long long)gridDim.x * DNT)
+      const int cc  = blockIdx.x;
+      const int cxc = cc % ncd, cyc = (cc / ncd) % ncd, czc = cc / (ncd * ncd);
+      const T  *c0  = coarse + (long long)cxc * NN + (long long)cyc * NN * N1c +
+                    (long long)czc * NN * Pc;
+      T        *f0  = fine + (long long)cxc * FB + (long long)cyc * FB * N1f +
+                    (long long)czc * FB * Pf;
+      for (int e = tid; e < FB * NN; e += DNT)
+        sE[e] = tab[TB::E + e];
+      for (int e = tid; e < C::C3; e += DNT)
         {
-          const int X = (int)(g % N1f), Y = (int)((g / N1f) % N1f), Z = (int)(g / Pf);
-          const int fx = X / NN, fy = Y / NN, fz = Z / NN;
-          const T  *ex = E + ((fx & 1) * NN + X - fx * NN) * NN;
-          const T  *ey = E + ((fy & 1) * NN + Y - fy * NN) * NN;
-          const T  *ez = E + ((fz & 1) * NN + Z - fz * NN) * NN;
-          const T  *c0 = coarse + (long long)(fx / 2) * NN + (long long)(fy / 2) * NN * N1c +
-                        (long long)(fz / 2) * NN * Pc;
-          T v = 0;
+          const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
+          sc[e] = c0[a + (long long)b * N1c + (long long)c * Pc];
+        }
+      __syncthreads();
+      // x: s1[c][b][xf] = sum_a E[xf][a] sc[c][b][a]
+      for (int e = tid; e < C::S2; e += DNT)
+        {
+          const int xf = e % FB, cb = e / FB;
+          T         v  = 0;
 #pragma unroll
-          for (int cz = 0; cz < NN; ++cz)
+          for (int a = 0; a < NN; ++a)
+            v += sE[xf * NN + a] * sc[cb * NN + a];
+          s1[e] = v;
+        }
+      __syncthreads();
+      // y: s2[c][yf][xf] = sum_b E[yf][b] s1[c][b][xf]
+      for (int e = tid; e < C::S1; e += DNT)
+        {
+          const int xf = e % FB, yf = (e / FB) % FB, c = e / (FB * FB);
+          T         v  = 0;
 #pragma unroll
-            for (int cy = 0; cy < NN; ++cy)
-              {
-                T a = 0;
+          for (int b = 0; b < NN; ++b)
+            v += sE[yf * NN + b] * s1[(c * NN + b) * FB + xf];
+          s2[e] = v;
+        }
+      __syncthreads();
+      // z: fine[zf][yf][xf] += sum_c E[zf][c] s2[c][yf][xf]
+      for (int e = tid; e < C::F3; e += DNT)
+        {
+          const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
+          T         v  = 0;
 #pragma unroll
-                for (int cx = 0; cx < NN; ++cx)
-                  a += ex[cx] * c0[cx + (long long)cy * N1c + (long long)cz * Pc];
-                v += ez[cz] * ey[cy] * a;
-              }
-          fine[g] += v;
+          for (int c = 0; c < NN; ++c)
+            v += sE[zf * NN + c] * s2[(c * FB + yf) * FB + xf];
+          f0[xf + (long long)yf * N1f + (long long)zf * Pf] += v;
         }
     }
 
-    // coarse (+)= P^T fine: one thread per coarse DoF
+    // coarse (+)= P^T fine
     template <int K, typename T, bool STORE>
     __global__ void __launch_bounds__(DNT)
     dg_restrict_kernel(const T *__restrict__ fine, T *__restrict__ coarse,
@@ -680,40 +717,61 @@ namespace gtps
                        const long long Nc)
     {
       using TB          = DTab<K>;
-      constexpr int NN  = TB::NN;
-      const int       N1f = nf * NN, N1c = (nf / 2) * NN;
+      using C           = DT<K, T>;
+      constexpr int NN  = C::NN, FB = C::FB;
+      __shared__ T    sf[C::F3], s1[C::S1], s2[C::S2], sE[FB * NN];
+      const int       tid = threadIdx.x;
+      const int       ncd = nf / 2;
+      const int       N1f = nf * NN, N1c = ncd * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
-      const T        *E  = tab + TB::E;
       fine += (long long)blockIdx.y * Pf * N1f;
       coarse += (long long)blockIdx.y * Nc;
-      for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < Nc;
-           g += (long long)gridDim.x * DNT)
+      const int cc  = blockIdx.x;
+      const int cxc = cc % ncd, cyc = (cc / ncd) % ncd, czc = cc / (ncd * ncd);
+      T        *c0  = coarse + (long long)cxc * NN + (long long)cyc * NN * N1c +
+                    (long long)czc * NN * Pc;
+      const T  *f0  = fine + (long long)cxc * FB + (long long)cyc * FB * N1f +
+                     (long long)czc * FB * Pf;
+      for (int e = tid; e < FB * NN; e += DNT)
+        sE[e] = tab[TB::E + e];
+      for (int e = tid; e < C::F3; e += DNT)
         {
-          const int X = (int)(g % N1c), Y = (int)((g / N1c) % N1c), Z = (int)(g / Pc);
-          const int cxc = X / NN, cyc = Y / NN, czc = Z / NN;
-          const int ax = X - cxc * NN, ay = Y - cyc * NN, az = Z - czc * NN;
-          T         v  = 0;
-          for (int chz = 0; chz < 2; ++chz)
-            for (int chy = 0; chy < 2; ++chy)
-              for (int chx = 0; chx < 2; ++chx)
-                {
-                  const T *f0 = fine + (long long)(2 * cxc + chx) * NN +
-                                (long long)(2 * cyc + chy) * NN * N1f +
-                                (long long)(2 * czc + chz) * NN * Pf;
+          const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
+          sf[e] = f0[xf + (long long)yf * N1f + (long long)zf * Pf];
+        }
+      __syncthreads();
+      // x: s1[zf][yf][a] = sum_xf E[xf][a] sf[zf][yf][xf]
+      for (int e = tid; e < C::S1; e += DNT)
+        {
+          const int a = e % NN, r = e / NN;
+          T         v = 0;
 #pragma unroll
-                  for (int qz = 0; qz < NN; ++qz)
+          for (int xf = 0; xf < FB; ++xf)
+            v += sE[xf * NN + a] * sf[r * FB + xf];
+          s1[e] = v;
+        }
+      __syncthreads();
+      // y: s2[zf][b][a] = sum_yf E[yf][b] s1[zf][yf][a]
+      for (int e = tid; e < C::S2; e += DNT)
+        {
+          const int a = e % NN, b = (e / NN) % NN, zf = e / (NN * NN);
+          T         v = 0;
 #pragma unroll
-                    for (int qy = 0; qy < NN; ++qy)
-                      {
-                        T a = 0;
+          for (int yf = 0; yf < FB; ++yf)
+            v += sE[yf * NN + b] * s1[(zf * FB + yf) * NN + a];
+          s2[e] = v;
+        }
+      __syncthreads();
+      // z: coarse[c][b][a] (+)= sum_zf E[zf][c] s2[zf][b][a]
+      for (int e = tid; e < C::C3; e += DNT)
+        {
+          const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
+          T         v = 0;
 #pragma unroll
-                        for (int qx = 0; qx < NN; ++qx)
-                          a += E[(chx * NN + qx) * NN + ax] *
-                               f0[qx + (long long)qy * N1f + (long long)qz * Pf];
-                        v += E[(chz * NN + qz) * NN + az] * E[(chy * NN + qy) * NN + ay] * a;
-                      }
-                }
-          coarse[g] = STORE ? v : coarse[g] + v;
+          for (int zf = 0; zf < FB; ++zf)
+            v += sE[zf * NN + c] * s2[(zf * NN + b) * NN + a];
+          T *o = c0 + a + (long long)b * N1c + (long long)c * Pc;
+          *o   = STORE ? v : *o + v;
         }
     }
 
@@ -1018,14 +1076,16 @@ namespace gtps
   launch_dg_prolongate_add(const gtps_dg_level *fl, void *fine, const void *coarse, int nvec)
   {
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
+    // one work-group per coarse cell
+    const unsigned ncc = (unsigned)((fl->n / 2) * (fl->n / 2) * (fl->n / 2));
     dispatch_k(fl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (fl->prec == GTPS_FP32)
-        hipLaunchKernelGGL((dg_prolongate_kernel<K, float>), dim3(dgrid(fl->N), nvec),
+        hipLaunchKernelGGL((dg_prolongate_kernel<K, float>), dim3(ncc, nvec),
                            dim3(DNT), 0, fl->ctx->stream, (const float *)coarse,
                            (float *)fine, (const float *)fl->tab, fl->n, fl->N);
       else
-        hipLaunchKernelGGL((dg_prolongate_kernel<K, double>), dim3(dgrid(fl->N), nvec),
+        hipLaunchKernelGGL((dg_prolongate_kernel<K, double>), dim3(ncc, nvec),
                            dim3(DNT), 0, fl->ctx->stream, (const double *)coarse,
                            (double *)fine, (const double *)fl->tab, fl->n, fl->N);
       GTPS_LAUNCH_CHECK();
@@ -1040,27 +1100,29 @@ namespace gtps
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
     const long long N1c = (long long)(fl->n / 2) * (fl->k + 1);
     const long long Nc  = N1c * N1c * N1c;
+    // one work-group per coarse cell
+    const unsigned ncc = (unsigned)((fl->n / 2) * (fl->n / 2) * (fl->n / 2));
     dispatch_k(fl->k, [&](auto kk) {
       constexpr int K = decltype(kk)::value;
       if (fl->prec == GTPS_FP32)
         {
           if (store)
-            hipLaunchKernelGGL((dg_restrict_kernel<K, float, true>), dim3(dgrid(Nc), nvec),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, float, true>), dim3(ncc, nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const float *)fine,
                                (float *)coarse, (const float *)fl->tab, fl->n, Nc);
           else
-            hipLaunchKernelGGL((dg_restrict_kernel<K, float, false>), dim3(dgrid(Nc), nvec),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, float, false>), dim3(ncc, nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const float *)fine,
                                (float *)coarse, (const float *)fl->tab, fl->n, Nc);
         }
       else
         {
           if (store)
-            hipLaunchKernelGGL((dg_restrict_kernel<K, double, true>), dim3(dgrid(Nc), nvec),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, double, true>), dim3(ncc, nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const double *)fine,
                                (double *)coarse, (const double *)fl->tab, fl->n, Nc);
           else
-            hipLaunchKernelGGL((dg_restrict_kernel<K, double, false>), dim3(dgrid(Nc), nvec),
+            hipLaunchKernelGGL((dg_restrict_kernel<K, double, false>), dim3(ncc, nvec),
                                dim3(DNT), 0, fl->ctx->stream, (const double *)fine,
                                (double *)coarse, (const double *)fl->tab, fl->n, Nc);
         }
