@@ -303,8 +303,11 @@ namespace gtps
     template <int K>
     struct RT
     {
-      static constexpr int CX = K <= 2 ? 4 : 2, CY = K <= 2 ? 2 : (K <= 4 ? 2 : 1),
-                           CZ = K <= 2 ? 2 : (K <= 4 ? 2 : 1);
+      // coarse cells per work-group (k = 4: one, the fine window of 15^3
+      // nodes then takes 18 KB of LDS in FP32, 36 KB in FP64)
+      static constexpr int CX = K <= 2 ? 4 : (K <= 3 ? 2 : 1),
+                           CY = K <= 2 ? 2 : (K <= 3 ? 2 : 1),
+                           CZ = K <= 2 ? 2 : (K <= 3 ? 2 : 1);
     };
 
     // STORE: coarse = P^T fine on interior coarse nodes (Dirichlet entries
@@ -447,9 +450,9 @@ namespace gtps
           GTPS_LAUNCH_CHECK();
           return;
         }
-      // the restriction window outgrows 64 KiB of LDS beyond k = 3: those
+      // the restriction window outgrows 64 KiB of LDS beyond k = 4: those
       // degrees keep the three separable passes below
-      if constexpr (K <= 3)
+      if constexpr (K <= 4)
         if (dim == 3)
           {
             using C = RT<K>;
