@@ -990,10 +990,9 @@ namespace gtps
     {
       using TB         = Tab<K>;
       using C          = SG<K, T>;
-      constexpr int NP = C::NP, NI = C::NI, PER = C::PER, ZS = C::ZS, ZST = C::ZST;
+      constexpr int NP = C::NP, NI = C::NI, PER = C::PER, ZST = C::ZST;
       static_assert(NP * NP <= 64, "one lane per z-pencil");
-      constexpr int P1 = (NPW * NP * NP + 63) / 64;  // passes per stage
-      constexpr int P2 = (NPW * NI * NP + 63) / 64;
+      constexpr int P2 = (NPW * NI * NP + 63) / 64;  // passes per stage
       constexpr int P3 = (NPW * NI * NI + 63) / 64;
       __shared__ T      sm[NPW * PER];
       const int       N1   = K * n + 1;
