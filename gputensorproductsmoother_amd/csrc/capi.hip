@@ -251,6 +251,7 @@ struct gtps_mg : LaunchTimer
   bool                       use_graph  = true;
   hipStream_t                cap_stream = nullptr;
   hipGraphExec_t             dir_exec   = nullptr;
+  unsigned long long         dir_gen    = 0;  // tuning generation at capture
 
   // the V-cycle from level 2 down as a dense matrix (coarse.hip), built
   // on first use and dropped when the operators change
@@ -413,6 +414,19 @@ struct gtps_mg : LaunchTimer
     launch_cg_update_p(s, N, p, z64, z32, sc);
   }
 
+  // a graph captured before the last gtps_set_tuning may replay other
+  // kernels than an eager launch would: capture it again
+  void
+  drop_stale_graph()
+  {
+    if (dir_exec && dir_gen != tuning().generation)
+      {
+        GTPS_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(dir_exec);
+        dir_exec = nullptr;
+      }
+  }
+
   void
   capture_direction()
   {
@@ -446,6 +460,7 @@ struct gtps_mg : LaunchTimer
     hipError_t e = hipGraphInstantiate(&dir_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     GTPS_HIP(e);
+    dir_gen = tuning().generation;
   }
 
   void
@@ -458,6 +473,7 @@ struct gtps_mg : LaunchTimer
         enqueue_direction();
         return;
       }
+    drop_stale_graph();
     if (!dir_exec)
       capture_direction();
     GTPS_HIP(hipGraphLaunch(dir_exec, ctx->stream));
@@ -821,6 +837,7 @@ struct gtps_dg_mg
   bool                          use_graph  = true;
   hipStream_t                   cap_stream = nullptr;
   hipGraphExec_t                dir_exec   = nullptr;
+  unsigned long long            dir_gen    = 0;  // tuning generation at capture
   // the coarsest solve (symmetric sweep pairs from zero, a fixed linear
   // map) tabulated as a dense matrix on first use (coarse.hip)
   void                         *dense0       = nullptr;
@@ -905,7 +922,9 @@ struct gtps_dg_mg
 
   // z_j = V(r_j); per vector: rz_old = rz; rz = r.z; beta; p = z + beta p
   void
-  direction_batch(int nb)
+  // z = V(r) for the whole batch, then rz, beta and p = z + beta p for the
+  // rows still iterating (itc[j] >= 0: converged, left frozen)
+  direction_batch(int nb, const std::vector<int> &itc)
   {
     hipStream_t     s   = ctx->stream;
     const long long N   = fine64->N;
@@ -917,6 +936,8 @@ struct gtps_dg_mg
       vcycle_batch(maxl, r, u[maxl], nb);
     for (int j = 0; j < nb; ++j)
       {
+        if (itc[j] >= 0)
+          continue;
         const float  *z32 = f32 ? (const float *)u[maxl] + N * j : nullptr;
         const double *z64 = f32 ? nullptr : (const double *)u[maxl] + N * j;
         double       *sc  = bsc + cg::N_SLOTS * j;
@@ -997,6 +1018,19 @@ struct gtps_dg_mg
     launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, relax);
   }
 
+  // a graph captured before the last gtps_set_tuning may replay other
+  // kernels than an eager launch would: capture it again
+  void
+  drop_stale_graph()
+  {
+    if (dir_exec && dir_gen != tuning().generation)
+      {
+        GTPS_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(dir_exec);
+        dir_exec = nullptr;
+      }
+  }
+
   void
   capture_direction()
   {
@@ -1029,6 +1063,7 @@ struct gtps_dg_mg
     hipError_t e = hipGraphInstantiate(&dir_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     GTPS_HIP(e);
+    dir_gen = tuning().generation;
   }
 
   void
@@ -1040,6 +1075,7 @@ struct gtps_dg_mg
         enqueue_direction();
         return;
       }
+    drop_stale_graph();
     if (!dir_exec)
       capture_direction();
     GTPS_HIP(hipGraphLaunch(dir_exec, ctx->stream));
@@ -1425,6 +1461,8 @@ extern "C"
   {
     return guard([&] {
       GTPS_REQUIRE(ctx && (x || n == 0) && n >= 0, "bad argument");
+      GTPS_REQUIRE(precision == GTPS_FP32 || precision == GTPS_FP64,
+                   "precision must be GTPS_FP32 or GTPS_FP64");
       set_device(ctx);
       launch_fill(ctx->stream, n, precision, x, value);
     });
@@ -1681,6 +1719,7 @@ extern "C"
         }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
+      ++tuning().generation;
     });
     return rc == GTPS_OK ? prev : -1;
   }
@@ -2172,7 +2211,8 @@ extern "C"
       mg->read_norms(nrhs, r0.data());
       for (int j = 0; j < nrhs; ++j)
         {
-          GTPS_REQUIRE(std::isfinite(r0[j]), "non-finite right-hand side");
+          if (!std::isfinite(r0[j]))
+            throw Error(GTPS_ENAN, "non-finite right-hand side in batched CG");
           if (r0[j] == 0.0)
             itc[j] = 0;
         }
@@ -2182,17 +2222,24 @@ extern "C"
             return false;
         return true;
       };
+      // a row stops changing once it has converged (or had a zero right-hand
+      // side): its x, r, p and CG scalars are left as they are, so each row
+      // is the single-vector solve of that row; the batched kernels still run
+      // over every row
       int it = 0;
       if (!all_done())
-        mg->direction_batch(nrhs);
+        mg->direction_batch(nrhs, itc);
       while (!all_done() && it < maxit)
         {
           const int nbq = launch_dg_apply(mg->fine64, mg->q, mg->p, nullptr, true, nrhs);
           for (int j = 0; j < nrhs; ++j)
-            launch_finalize_op(s, pt + (long long)nbq * j, nbq, mg->bsc + cg::N_SLOTS * j,
-                               cg::PQ, cg::OP_ALPHA);
+            if (itc[j] < 0)
+              launch_finalize_op(s, pt + (long long)nbq * j, nbq, mg->bsc + cg::N_SLOTS * j,
+                                 cg::PQ, cg::OP_ALPHA);
           for (int j = 0; j < nrhs; ++j)
             {
+              if (itc[j] >= 0)
+                continue;
               float *r32 = f32 ? (float *)mg->d[mg->maxl] + N * j : nullptr;
               const int nbr =
                 launch_cg_update_xr(s, N, x + N * j, mg->r + N * j, mg->p + N * j,
@@ -2209,7 +2256,7 @@ extern "C"
                 itc[j] = it;
             }
           if (!all_done() && it < maxit)
-            mg->direction_batch(nrhs);
+            mg->direction_batch(nrhs, itc);
         }
       for (int j = 0; j < nrhs; ++j)
         {

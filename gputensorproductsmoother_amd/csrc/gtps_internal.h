@@ -191,6 +191,9 @@ namespace gtps
     // z-marching vmult tile for k = 3: 0 = 16x8 cells / 512 threads,
     // 1 = 8x8 / 256, 2 = 16x8 / 256; 3: by precision
     int       vmult_tile = 3;
+    // bumped by every successful gtps_set_tuning: HIP graphs captured under
+    // an older generation are re-captured before their next replay
+    unsigned long long generation = 0;
   };
   Tuning &tuning();
 

@@ -101,6 +101,15 @@ namespace gtps
       return block_sum<VNT>(s, red);
     }
 
+    // CG step quotients: a zero denominator only arises when the residual
+    // (and with it z, p and q) is exactly zero, where the step is 0; without
+    // the guard 0/0 would turn a zero right-hand side of a batch into NaN.
+    __device__ __forceinline__ double
+    quotient(double num, double den)
+    {
+      return den != 0.0 ? num / den : 0.0;
+    }
+
     __global__ void __launch_bounds__(VNT)
     finalize_kernel(const double *__restrict__ partials, int nb,
                     double *__restrict__ scalars, int slot)
@@ -126,12 +135,12 @@ namespace gtps
           const double old = sc[cg::RZ];
           sc[cg::RZ_OLD]   = old;
           sc[cg::RZ]       = s;
-          sc[cg::BETA]     = s / old;
+          sc[cg::BETA]     = quotient(s, old);
           return;
         }
       sc[slot] = s;
       if (op == cg::OP_ALPHA)
-        sc[cg::ALPHA] = sc[cg::RZ] / s;
+        sc[cg::ALPHA] = quotient(sc[cg::RZ], s);
     }
 
     __global__ void
@@ -142,10 +151,10 @@ namespace gtps
       switch (op)
         {
           case cg::OP_ALPHA:
-            sc[cg::ALPHA] = sc[cg::RZ] / sc[cg::PQ];
+            sc[cg::ALPHA] = quotient(sc[cg::RZ], sc[cg::PQ]);
             break;
           case cg::OP_BETA:
-            sc[cg::BETA] = sc[cg::RZ] / sc[cg::RZ_OLD];
+            sc[cg::BETA] = quotient(sc[cg::RZ], sc[cg::RZ_OLD]);
             break;
           case cg::OP_SHIFT:
             sc[cg::RZ_OLD] = sc[cg::RZ];

@@ -67,7 +67,11 @@ void *gtps_ctx_stream(gtps_ctx *ctx);
 const char *gtps_last_error(void);
 const char *gtps_version(void);
 /* Process-wide tuning knobs; returns the previous value, or -1 (with
- * gtps_last_error set) for an unknown key or invalid value.  Keys:
+ * gtps_last_error set) for an unknown key or invalid value.  The knobs are
+ * shared by every context and thread without locking: change them only
+ * while no other thread is inside the library.  They act on the next
+ * launch; a solver's replayed HIP graph is captured again after any change.
+ * Keys:
  *   "small_colour_patches"  smoother colours with at most this many patches
  *                           run the single-patch (latency-bound, coarse
  *                           level) kernel, larger ones the multi-patch kernel
@@ -92,8 +96,7 @@ const char *gtps_version(void);
  *                           first use (at most 4096 of them: k <= 4 in 3D)
  *                           and applies it with one launch (constant
  *                           coefficient only); 0: the ordinary
- *                           level-2 smoothing and level-1 solve.  Set it
- *                           before the hierarchy's first solve.
+ *                           level-2 smoothing and level-1 solve.
  *   "vmult_zdiv"            z-marching vmult work-group count target: z-chunks
  *                           of ceil(n tx ty / value) cells (default 0: 512
  *                           in FP32, 2048 in FP64, 256 in FP64 at k = 7);

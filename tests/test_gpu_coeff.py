@@ -169,11 +169,13 @@ def _J(u, ud, mask):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_autograd_adjoint_vs_finite_differences(core, ctx, prec):
+@pytest.mark.parametrize("k,level", [(2, 3), (4, 3)])
+def test_autograd_adjoint_vs_finite_differences(core, ctx, k, level, prec):
     """dJ/dc and dJ/db from MGSolve.backward (V-cycle in `prec`) against
     central differences (h = 1e-6) of J evaluated with FP64-everywhere solves
-    to 1e-14, and against the oracle's adjoint (3D Q2, 8^3 cells)."""
-    k, level = 2, 3
+    to 1e-14, and against the oracle's adjoint (3D Q2 and Q4 on 8^3 cells;
+    the Q4 32^3 workload is checked against central differences in
+    test_gpu_configs.py)."""
     _announce("autograd", k, level, prec)
     rtol = 1e-13 if prec == "fp64" else 1e-12
     mg = core.MultigridSolvers(ctx, 3, k, level, _dt(prec))

@@ -1,0 +1,297 @@
+"""Each parity configuration run at its own workload size on the GPU.
+
+For every configuration the FP32-V-cycle (mixed-precision) and the
+FP64-everywhere MG-CG are solved to the configuration's tolerance, and the
+test asserts
+
+* the relative residual and the iteration count of each form (FP64 equal to
+  the stated count, the FP32 V-cycle at most one more),
+* the relative L2 distance ``||x - x_ref|| / ||x_ref||`` below 1e-8, where
+  ``x_ref`` is a tightly converged FP64 solution: the oracle's own FP64 PCG
+  to 1e-13 where it finishes in seconds (C1, C2), else the GPU's FP64 solve
+  to 1e-12 whose true residual the oracle's operator confirms at full size
+  (C3, C4, C5; the GPU FP64 path itself is compared with the oracle's
+  solver at smaller sizes in test_gpu_mg.py, test_gpu_coeff.py and
+  test_gpu_dg.py),
+* the true residual of the mixed-precision solution, computed with the
+  oracle's operator at full size (rounding puts a floor of about 1e-12
+  under such a residual at these sizes: a perturbation of one unit in the
+  last place of a smooth field changes A x by 5e-14 (C4) to 1.6e-12 (C3)
+  relative, so the tight references are checked to 1e-11),
+
+plus each configuration's own extra: the matrix-core and LDS/FMA smoothers
+for C3, the adjoint gradient against central differences (h = 1e-6, the
+16 cells from stream 2^34) for C4, and the batched 4-right-hand-side
+adjoint for C5.  The reference reports iterations and residual reduction
+for every run (apps/poisson.cu:246-286, 337-349).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_err
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-8
+
+
+@pytest.fixture(scope="module")
+def core():
+    from gputensorproductsmoother_amd import core as C
+    return C
+
+
+@pytest.fixture(scope="module")
+def problems():
+    from gputensorproductsmoother_amd import problems as P
+    return P
+
+
+@pytest.fixture(scope="module")
+def ctx(core):
+    return core.Context(0)
+
+
+def _announce(*args):
+    print("CASE", *args, flush=True)
+
+
+def _dev(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64, device="cuda:0")
+
+
+def _rel_l2(a, b):
+    a = a.detach().double().cpu().numpy() if isinstance(a, torch.Tensor) else a
+    b = b.detach().double().cpu().numpy() if isinstance(b, torch.Tensor) else b
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _true_relres(b, Ax):
+    return float(np.linalg.norm(b - Ax) / np.linalg.norm(b))
+
+
+def _solve_forms(core, ctx, dim, k, level, bd, rtol, it64, schedule=None, coef=None,
+                 x_ref=None, precs=("fp32", "fp64")):
+    """Solve with each V-cycle precision; assert residual, iterations and
+    the distance to x_ref; return the solutions."""
+    out = {}
+    for prec in precs:
+        dt = torch.float32 if prec == "fp32" else torch.float64
+        args = (ctx, dim, k, level, dt) + ((schedule,) if schedule is not None else ())
+        mg = core.MultigridSolvers(*args)
+        if coef is not None:
+            mg.set_coefficient(coef)
+        x, it, rel, hist = mg.solve_cg(bd, rtol=rtol, maxit=200)
+        x = x.clone()
+        d = _rel_l2(x, x_ref) if x_ref is not None else float("nan")
+        print(f"{prec}: iterations {it} relres {rel:.3e} rel-L2 to FP64 tight {d:.3e}",
+              flush=True)
+        assert rel <= rtol and len(hist) == it + 1
+        if prec == "fp64":
+            assert it == it64
+        else:
+            assert it64 <= it <= it64 + 1
+        if x_ref is not None:
+            assert d < PARITY
+        out[prec] = x
+        del mg
+    return out
+
+
+def test_c1_workload(core, ctx):
+    """C1: 2D Q2 on 128^2 cells, f = 1, additive vertex-patch smoother
+    (relaxation 0.4), all FP64, CG to 1e-10: 12 iterations, like the
+    oracle; solution within 1e-8 of the oracle's FP64 solve to 1e-13."""
+    dim, k, level = 2, 2, 7
+    _announce("C1", dim, k, level)
+    b = O.rhs_assembled(dim, k, level, O.RHS_CONST)
+    x_ref, _, rel_ref, _ = O.pcg(dim, k, level, b, rtol=1e-13, maxit=300,
+                                 mg_variant=O.MG_ADDITIVE, relax=0.4)
+    assert rel_ref <= 1e-13
+    xs = _solve_forms(core, ctx, dim, k, level, _dev(b), 1e-10, 12,
+                      schedule=core.MultigridSolvers.ADDITIVE, x_ref=x_ref, precs=("fp64",))
+    x = xs["fp64"].cpu().numpy()
+    assert _true_relres(b, O.laplace_vmult(dim, k, level, x)) <= 1.1e-10
+
+
+def test_c2_workload(core, ctx):
+    """C2 (the metric configuration): 3D Q3 on 64^3 cells (N = 193^3),
+    random right-hand side, FP32 V-cycle and FP64-everywhere MG-CG to 1e-8:
+    5 iterations each, both within 1e-8 relative L2 of the oracle's FP64
+    PCG solution to 1e-13; the GPU FP64 operator equals the oracle's at
+    full size."""
+    dim, k, level = 3, 3, 6
+    _announce("C2", dim, k, level)
+    b = O.rhs_random(dim, k, level)
+    bd = _dev(b)
+    x_ref, it_ref, rel_ref, _ = O.pcg(dim, k, level, b, rtol=1e-13, maxit=100)
+    print(f"oracle FP64 PCG to 1e-13: {it_ref} iterations", flush=True)
+    assert rel_ref <= 1e-13
+    xs = _solve_forms(core, ctx, dim, k, level, bd, 1e-8, 5, x_ref=x_ref)
+    Ax = O.laplace_vmult(dim, k, level, xs["fp32"].cpu().numpy())
+    assert _true_relres(b, Ax) <= 1.1e-8
+    op = core.LaplaceOperator(ctx, dim, k, level, torch.float64)
+    y = torch.empty_like(bd)
+    op.vmult(y, bd)
+    assert rel_err(y.cpu().numpy(), O.laplace_vmult(dim, k, level, b)) < 1e-12
+
+
+def test_c3_workload(core, ctx):
+    """C3: 3D Q7 on 32^3 cells (N = 225^3), f = 1, CG to 1e-8: the FP32
+    V-cycle with the matrix-core smoother and with the LDS/FMA smoother,
+    and FP64 everywhere, 5 iterations each, within 1e-8 of a GPU FP64 solve
+    to 1e-12; true residuals and the full-size operator from the oracle."""
+    dim, k, level = 3, 7, 5
+    _announce("C3", dim, k, level)
+    b = O.rhs_assembled(dim, k, level, O.RHS_CONST)
+    bd = _dev(b)
+    ref = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+    x_ref, _, rel_ref, _ = ref.solve_cg(bd, rtol=1e-12, maxit=200)
+    x_ref = x_ref.cpu().numpy()
+    del ref
+    assert rel_ref <= 1e-12
+    assert _true_relres(b, O.laplace_vmult(dim, k, level, x_ref)) <= 1e-11
+    sols = {}
+    for mf in (1, 0):
+        prev = core.set_tuning("mfma_smoother", mf)
+        try:
+            _announce("C3 FP32 smoother", "matrix cores" if mf else "LDS/FMA")
+            sols[mf] = _solve_forms(core, ctx, dim, k, level, bd, 1e-8, 5, x_ref=x_ref,
+                                    precs=("fp32",))["fp32"]
+        finally:
+            core.set_tuning("mfma_smoother", prev)
+    _solve_forms(core, ctx, dim, k, level, bd, 1e-8, 5, x_ref=x_ref, precs=("fp64",))
+    for mf, x in sols.items():
+        Ax = O.laplace_vmult(dim, k, level, x.cpu().numpy())
+        assert _true_relres(b, Ax) <= 1.1e-8, mf
+    # the two FP32 smoothers are the same operator up to FP32 round-off
+    assert _rel_l2(sols[1], sols[0]) < PARITY
+
+
+def test_c4_workload(core, ctx, problems):
+    """C4: -div(c grad u) = 1, Q4 on 32^3 cells (N = 129^3), c = exp(0.5 g)
+    per cell; MG-CG to 1e-8 in both forms (19 iterations) within 1e-8 of a
+    GPU FP64 solve to 1e-12 (oracle residual at full size); then
+    J = 1/2 ||u - u_d||^2 through the autograd Function (FP32 V-cycle), whose
+    dJ/dc matches central differences (h = 1e-6) on the 16 cells drawn from
+    stream 2^34 within 1e-6 relative."""
+    dim, k, level = 3, 4, 5
+    _announce("C4", dim, k, level)
+    coef_h, ud_h, cells = problems.coeff_problem(dim, k, level)
+    assert len(cells) == 16
+    b = O.rhs_assembled(dim, k, level, O.RHS_CONST)
+    bd, coef, ud = _dev(b), _dev(coef_h), _dev(ud_h)
+    mg64 = core.MultigridSolvers(ctx, dim, k, level, torch.float64)
+    mg64.set_coefficient(coef)
+    x_ref, _, rel_ref, _ = mg64.solve_cg(bd, rtol=1e-12, maxit=400)
+    x_ref = x_ref.clone()
+    assert rel_ref <= 1e-12
+    xr = x_ref.cpu().numpy()
+    assert _true_relres(b, O.coeff_vmult(dim, k, level, coef_h, xr)) <= 1e-11
+    xs = _solve_forms(core, ctx, dim, k, level, bd, 1e-8, 19, coef=coef, x_ref=xr)
+    Ax = O.coeff_vmult(dim, k, level, coef_h, xs["fp32"].cpu().numpy())
+    assert _true_relres(b, Ax) <= 1.1e-8
+
+    mg = core.MultigridSolvers(ctx, dim, k, level, torch.float32)
+    mask = mg.boundary_mask()
+    c = coef.clone().requires_grad_(True)
+    u = mg.solve(bd, c, rtol=1e-12, maxit=400)
+    J = 0.5 * (torch.where(mask, torch.zeros_like(u), u - ud) ** 2).sum()
+    J.backward()
+    grad = c.grad.cpu().numpy()
+    assert np.all(np.isfinite(grad))
+
+    mg64.set_coefficient(coef)
+    u0, _, _, _ = mg64.solve_cg(bd, rtol=1e-14, maxit=500, raise_on_failure=False)
+    u0 = u0.clone()
+
+    def u_of(cv):
+        # u(c') = u0 + A(c')^{-1}(b - A(c') u0): the correction is O(h) and
+        # solved to 1e-12 relative, so u(c+) - u(c-) keeps ~12 digits
+        mg64.set_coefficient(cv)
+        r = torch.empty_like(u0)
+        mg64.do_matvec(r, u0)
+        d, _, _, _ = mg64.solve_cg(bd - r, rtol=1e-12, maxit=500, raise_on_failure=False)
+        return u0 + d
+
+    h = 1e-6
+    worst = 0.0
+    for K in cells.tolist():
+        cp, cm = coef.clone(), coef.clone()
+        cp[K] += h
+        cm[K] -= h
+        up, um = u_of(cp), u_of(cm)
+        # J(+) - J(-) = sum (u+ - u-)((u+ + u-)/2 - u_d) over the interior
+        dj = torch.where(mask, torch.zeros_like(up), (up - um) * (0.5 * (up + um) - ud)).sum()
+        fd = dj.item() / (2 * h)
+        dev = abs(grad[K] - fd) / abs(fd)
+        worst = max(worst, dev)
+        assert dev <= 1e-6, (K, grad[K], fd)
+    print(f"C4 adjoint vs central differences, 16 cells: worst relative deviation "
+          f"{worst:.2e}", flush=True)
+
+
+def test_c5_workload(core, ctx, problems):
+    """C5: SIPG-DG Q4 on 24^3 cells (3^3 coarse mesh, 3 refinements,
+    N = 120^3), penalty 2.5 (k+1)^2, random f, CG to 1e-8 with FP32 levels
+    and FP64 everywhere (4 iterations each) within 1e-8 of a GPU FP64 solve
+    to 1e-12 (oracle residual and operator at full size); then the batched
+    adjoint of J = 1/2 sum_m ||U_m||^2 for 4 right-hand sides solved
+    together, against a central difference along a random direction (J is
+    quadratic in B, so the difference is exact up to the solves) and against
+    single FP64 solves of A G_m = U_m."""
+    k, n0, maxl = 4, 3, 3
+    n = n0 << maxl
+    _announce("C5", k, n)
+    Bh = problems.rhs_dg_random(3, k, n, batch=4)
+    b = Bh[0]
+    bd = _dev(b)
+    ref = core.DGMultigrid(ctx, 3, k, n0, maxl, torch.float64)
+    x_ref, _, rel_ref, _ = ref.solve_cg(bd, rtol=1e-12, maxit=200)
+    x_ref = x_ref.clone()
+    assert rel_ref <= 1e-12
+    xr = x_ref.cpu().numpy()
+    assert _true_relres(b, O.dg_vmult(3, k, n, xr)) <= 1e-11
+    y = torch.empty_like(bd)
+    ref.do_matvec(y, bd)
+    assert rel_err(y.cpu().numpy(), O.dg_vmult(3, k, n, b)) < 1e-12
+    sols = {}
+    for prec in ("fp32", "fp64"):
+        mg = core.DGMultigrid(ctx, 3, k, n0, maxl,
+                              torch.float32 if prec == "fp32" else torch.float64)
+        x, it, rel, hist = mg.solve_cg(bd, rtol=1e-8)
+        d = _rel_l2(x, xr)
+        print(f"{prec}: iterations {it} relres {rel:.3e} rel-L2 to FP64 tight {d:.3e}",
+              flush=True)
+        assert rel <= 1e-8 and len(hist) == it + 1
+        assert it == 4 if prec == "fp64" else 4 <= it <= 5
+        assert d < PARITY
+        sols[prec] = x.cpu().numpy()
+        del mg
+    assert _true_relres(b, O.dg_vmult(3, k, n, sols["fp32"])) <= 1.1e-8
+
+    mg = core.DGMultigrid(ctx, 3, k, n0, maxl, torch.float32)
+    B = _dev(Bh).requires_grad_(True)
+    U = mg.solve(B, rtol=1e-12)
+    (0.5 * (U * U).sum()).backward()
+    G = B.grad.detach()
+    assert torch.isfinite(G).all()
+    Ud = U.detach()
+    for m in range(4):
+        g_ref, _, _, _ = ref.solve_cg(Ud[m].contiguous(), rtol=1e-12, maxit=200)
+        assert _rel_l2(G[m], g_ref) < PARITY, m
+    D = _dev(problems.rhs_dg_random(3, k, n, batch=4, seed=problems.SEED + 1))
+    hstep = 1e-3
+
+    def J(Bv):
+        X, its, rel = mg.solve_cg_batch(Bv.contiguous(), rtol=1e-12, maxit=200)
+        assert max(rel) <= 1e-12
+        return 0.5 * float((X * X).sum())
+
+    fd = (J(Ud.new_tensor(Bh) + hstep * D) - J(Ud.new_tensor(Bh) - hstep * D)) / (2 * hstep)
+    an = float((G * D).sum())
+    dev = abs(an - fd) / abs(fd)
+    print(f"C5 batched adjoint vs central difference: {dev:.2e}", flush=True)
+    assert dev <= 1e-6

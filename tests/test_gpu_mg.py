@@ -1,11 +1,11 @@
-"""GPU V-cycle and MG-preconditioned CG against the FP64 CPU oracle, plus
-size-independent properties at the metric configuration (3D Q3, 64^3 cells).
+"""GPU V-cycle and MG-preconditioned CG against the FP64 CPU oracle (the
+configurations' full-size workloads are in test_gpu_configs.py).
 """
 import numpy as np
 import pytest
 
 import oracle as O
-from helpers import boundary_mask, n_dofs, rel_err, rng
+from helpers import boundary_mask, n_dofs, rel_err, rel_l2, rng
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -83,28 +83,41 @@ def test_vcycle_symmetric(core, ctx):
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("dim,k,level,rtol", [(2, 2, 5, 1e-10), (3, 3, 3, 1e-8),
                                                (3, 3, 4, 1e-8), (3, 2, 4, 1e-8),
-                                               (3, 4, 3, 1e-8)])
+                                               (3, 4, 3, 1e-8), (3, 7, 2, 1e-8),
+                                               (3, 7, 3, 1e-8)])
 def test_pcg_against_oracle(core, ctx, dim, k, level, rtol, prec):
+    """MG-PCG against the oracle's PCG on the same right-hand side.  At the
+    workload tolerance: the FP64 form takes the oracle's iterations and its
+    iterate equals the oracle's to 1e-10 relative L2, the FP32 V-cycle takes
+    at most one more, and the true residual (oracle operator) is below the
+    tolerance.  Solved again to 1e-12, both forms are within 1e-8 relative
+    L2 of the oracle's solution to 1e-13 (a 1e-8 residual alone does not
+    bound the error to 1e-8: the oracle's own iterate at 1e-8 is 1.2e-8
+    from its tight solution at 3D Q2 level 4 and 3.7e-8 at Q7 level 2)."""
     _announce("pcg", dim, k, level, prec)
     b = O.rhs_random(dim, k, level)
     x_ref, it_ref, rel_ref, _ = O.pcg(dim, k, level, b, rtol=1e-13, maxit=200)
-    _, it64, _, _ = O.pcg(dim, k, level, b, rtol=rtol)
+    assert rel_ref <= 1e-13
+    x64, it64, _, _ = O.pcg(dim, k, level, b, rtol=rtol)
     mg = core.MultigridSolvers(ctx, dim, k, level,
                                torch.float64 if prec == "fp64" else torch.float32)
     bd = torch.tensor(b, device="cuda:0")
     x, it, rel, hist = mg.solve_cg(bd, rtol=rtol, maxit=100)
     x = x.cpu().numpy()
-    assert rel <= rtol
-    # true residual from the oracle operator
+    assert rel <= rtol and len(hist) == it + 1
     r = b - O.laplace_vmult(dim, k, level, x)
-    assert np.linalg.norm(r) <= 2 * rtol * np.linalg.norm(b)
-    # FP64 form takes the oracle's iteration count; the FP32 V-cycle at most one more
+    assert np.linalg.norm(r) <= 1.01 * rtol * np.linalg.norm(b)
     if prec == "fp64":
         assert it == it64
+        assert rel_l2(x, x64) < 1e-10
     else:
-        assert it <= it64 + 1
-    # error against the tightly converged FP64 oracle solution
-    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 100 * rtol
+        assert it64 <= it <= it64 + 1
+    xt, itt, relt, _ = mg.solve_cg(bd, rtol=1e-12, maxit=100)
+    assert relt <= 1e-12
+    d = rel_l2(xt.cpu().numpy(), x_ref)
+    print(f"iterations {it} (oracle {it64}); to 1e-12: {itt}, rel-L2 to oracle {d:.2e}",
+          flush=True)
+    assert d < 1e-8
 
 
 @pytest.mark.parametrize("dim,k,level,relax,max_it", [(2, 2, 5, 0.4, 14), (2, 2, 7, 0.4, 14),
@@ -123,37 +136,6 @@ def test_pcg_additive(core, ctx, dim, k, level, relax, max_it):
     assert rel <= 1e-10
     assert it == it_ref and it <= max_it
     assert rel_err(x.cpu().numpy(), x_ref) < 1e-8
-
-
-def test_pcg_metric_config(core, ctx):
-    """3D Q3 on 64^3 cells (N = 193^3): FP32-V-cycle and FP64 solves both
-    reach 1e-8, their solutions agree, and the true FP64 residual of the
-    mixed-precision solution (GPU FP64 operator) is below 1e-8."""
-    dim, k, level = 3, 3, 6
-    _announce("pcg-metric", dim, k, level)
-    b = O.rhs_random(dim, k, level)
-    bd = torch.tensor(b, device="cuda:0")
-    sols = {}
-    for dt in (torch.float32, torch.float64):
-        mg = core.MultigridSolvers(ctx, dim, k, level, dt)
-        x, it, rel, hist = mg.solve_cg(bd, rtol=1e-8, maxit=50)
-        assert rel <= 1e-8
-        assert it <= 12
-        sols[dt] = (x.clone(), mg)
-        print("precision", dt, "iterations", it, "relres", rel, flush=True)
-    x32, mg32 = sols[torch.float32]
-    x64, _ = sols[torch.float64]
-    r = torch.empty_like(bd)
-    mg32.do_matvec(r, x32)
-    r = bd - r
-    assert (r.norm() / bd.norm()).item() < 1e-8
-    d = ((x32 - x64).norm() / x64.norm()).item()
-    assert d < 1e-6
-    # GPU operator against the oracle at full size (one matvec, FP64)
-    y_ref = O.laplace_vmult(dim, k, level, b)
-    y = torch.empty_like(bd)
-    mg32.do_matvec(y, bd)
-    assert rel_err(y.cpu().numpy(), y_ref) < 1e-12
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
