@@ -72,6 +72,7 @@ SIGNATURES = {
     "gtps_vec_axpby": (_i, [_vp, _ll, _d, _vp, _d, _vp]),
     "gtps_vec_dot": (_i, [_vp, _ll, _vp, _vp, _pd]),
     "gtps_vec_nrm2": (_i, [_vp, _ll, _vp, _pd]),
+    "gtps_vec_dot_noalloc": (_i, [_vp, _ll, _vp, _vp, _vp]),
     "gtps_vec_convert_f64_f32": (_i, [_vp, _ll, _vp, _vp]),
     "gtps_vec_convert_f32_f64": (_i, [_vp, _ll, _vp, _vp]),
     "gtps_vec_set_boundary": (_i, [_vp, _vp, _d]),

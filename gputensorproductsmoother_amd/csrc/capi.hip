@@ -9,11 +9,20 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gtps_internal.h"
 
 using namespace gtps;
+
+// Partial sums of gtps_vec_dot_noalloc: a file-scope device array of
+// GTPS_NOALLOC_PARTIALS doubles (16 KiB), one copy on every GPU the library
+// runs on, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): it
+// uses the per-device address that hipGetSymbolAddress returns once per
+// device when the first context on that device is created.
+static __device__ __align__(256) double scratch[GTPS_NOALLOC_PARTIALS];
 
 gtps::Tuning &
 gtps::tuning()
@@ -62,6 +71,44 @@ namespace
   set_device(const gtps_ctx *ctx)
   {
     GTPS_HIP(hipSetDevice(ctx->device));
+  }
+
+  // device address of `scratch` per device (set up by gtps_ctx_create with
+  // that device current)
+  // (a failed lookup leaves the pointer NULL and keeps the HIP error for
+  // gtps_vec_dot_noalloc to report; the context itself does not need it)
+  constexpr int MAX_SCRATCH_DEVICES = 64;
+  std::mutex    g_scratch_mutex;
+  double       *g_scratch_ptr[MAX_SCRATCH_DEVICES] = {};
+  hipError_t    g_scratch_err[MAX_SCRATCH_DEVICES]  = {};
+
+  void
+  scratch_setup(int device)
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= MAX_SCRATCH_DEVICES || g_scratch_ptr[device])
+      return;
+    void            *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      {
+        (void)hipGetLastError();
+        g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        return;
+      }
+    g_scratch_ptr[device] = (double *)p;
+  }
+
+  double *
+  scratch_of(int device)
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    GTPS_REQUIRE(device >= 0 && device < MAX_SCRATCH_DEVICES,
+                 "no scratch array on devices with index >= 64");
+    if (!g_scratch_ptr[device])
+      throw Error(GTPS_EHIP, std::string("scratch array address unavailable: ") +
+                               hipGetErrorString(g_scratch_err[device]));
+    return g_scratch_ptr[device];
   }
 
   void *
@@ -1151,6 +1198,7 @@ extern "C"
       GTPS_HIP(hipGetDeviceCount(&ndev));
       GTPS_REQUIRE(device >= 0 && device < ndev, "no such device");
       GTPS_HIP(hipSetDevice(device));
+      scratch_setup(device);
       auto ctx    = std::make_unique<gtps_ctx>();
       ctx->device = device;
       if (stream)
@@ -1501,6 +1549,31 @@ extern "C"
                               ctx->stream));
       GTPS_HIP(hipStreamSynchronize(ctx->stream));
       *result = ctx->ws.pinned[0];
+    });
+  }
+
+  int
+  gtps_vec_dot_noalloc(gtps_ctx *ctx, long long n, const double *x, const double *y,
+                       double *result)
+  {
+    return guard([&] {
+      GTPS_REQUIRE(ctx && result && n >= 0 && (n == 0 || (x && y)), "bad argument");
+      GTPS_REQUIRE(n <= GTPS_NOALLOC_MAX_N,
+                   "n exceeds GTPS_NOALLOC_MAX_N (2^62) for gtps_vec_dot_noalloc");
+      double *pt = scratch_of(ctx->device);
+      set_device(ctx);
+      if (n == 0)
+        {
+          launch_fill(ctx->stream, 1, GTPS_FP64, result, 0.0);
+          return;
+        }
+      // the same kernels and grid as gtps_vec_dot: the partial count is
+      // capped at GTPS_NOALLOC_PARTIALS work-groups (grid-stride loops), so
+      // every partial slot a launch uses is written before it is read
+      const int nb = launch_dot_partials(ctx->stream, n, x, y, nullptr, pt,
+                                         GTPS_NOALLOC_PARTIALS);
+      GTPS_REQUIRE(nb >= 1 && nb <= GTPS_NOALLOC_PARTIALS, "partial count out of range");
+      launch_finalize_sum(ctx->stream, pt, nb, result, 0);
     });
   }
 

@@ -231,6 +231,23 @@ int gtps_vec_convert_f64_f32(gtps_ctx *ctx, long long n, const double *x,
                              float *y);
 int gtps_vec_convert_f32_f64(gtps_ctx *ctx, long long n, const float *x,
                              double *y);
+/* Allocation-free variant of gtps_vec_dot: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it calls
+ * no hipMalloc/hipFree or allocating routine, does not synchronise, and
+ * writes x.y to `result`, ONE double in DEVICE memory, on the context stream
+ * (same kernels, grid and summation order as gtps_vec_dot, so the same
+ * bits).  Its partial sums live in a static device array of
+ * GTPS_NOALLOC_PARTIALS doubles (16 KiB per GPU, kept until the process
+ * ends); the grid is capped at that many work-groups, so any
+ * 0 <= n <= GTPS_NOALLOC_MAX_N is accepted and larger n is rejected with
+ * GTPS_EINVAL.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race; calls ordered on
+ * one stream do not. */
+#define GTPS_NOALLOC_PARTIALS 2048
+#define GTPS_NOALLOC_MAX_N (1LL << 62)
+int gtps_vec_dot_noalloc(gtps_ctx *ctx, long long n, const double *x,
+                         const double *y, double *result);
 /* v_B = value on the level's Dirichlet DoFs.  Replaces
  * set_inhomogeneous_dofs / set_constrained_dofs (include/solver.cuh:45-62,
  * dealii/cuda_mf.template.cuh:514-528). */
