@@ -174,7 +174,8 @@ def config_report(args):
             line = {"config": "C3", "workload": "3D Q7 Poisson f=1, 32^3 cells, "
                     "multiplicative smoother (FP32: 1D contractions on the matrix cores; "
                     "FP64: LDS/FMA kernel), CG to 1e-8", "n_dofs": b.numel(),
-                    "mfma_utilisation": mfma_report(core, ctx, k, level, reps), **r}
+                    "mfma_utilisation": mfma_report(core, ctx, k, level, reps),
+                    "hbm_per_kernel": hbm_report(core, ctx, k, level, reps), **r}
         elif name == "c4":
             line = c4_report(ctx, core, problems, reps, solve_pair, rel_l2)
         elif name == "c5":
@@ -223,6 +224,55 @@ def mfma_report(core, ctx, k, level, reps):
             "us_per_colour_lds_fma": round(res["fma"], 2), "issued_TFLOPs": round(ach, 1),
             "peak_TFLOPs": 157.3, "frac": round(ach / 157.3, 3),
             "note": "issued FLOPs include the zero padding of 13/15-wide lines to 16"}
+
+
+def hbm_report(core, ctx, k, level, reps):
+    """Finest-level kernels of the 3D solve at (k, level): mean time per
+    launch (HIP events, best of reps) and algorithmic HBM bytes per launch
+    over that time, against the 8 TB/s peak.  Bytes: smoother colour
+    s (interior b + union of the patch boxes), vmult 2 N s, residual 3 N s,
+    prolongate-add s (N_c + 2 N), restrict-add s (N + 2 N_c), FP64 CG
+    matvec with the fused dot 2 N 8."""
+    import torch
+    g = torch.Generator(device="cuda:0").manual_seed(SEED)
+    vals = smoother_colour_values(3, k, level)
+    out = {}
+
+    def rate(name, fn, nbytes, launches):
+        ms = _best_ms(fn, max(3, reps))
+        us = ms * 1e3 / launches
+        gbs = nbytes / launches / (us * 1e-6) / 1e9
+        out[name] = {"us": round(us, 2), "GB_s": round(gbs, 1),
+                     "frac_of_hbm": round(gbs / HBM_PEAK_GBS, 3)}
+
+    for dt, s in ((torch.float32, 4), (torch.float64, 8)):
+        tag = "fp32" if s == 4 else "fp64"
+        op = core.LaplaceOperator(ctx, 3, k, level, dt)
+        sm = core.PatchSmoother(op)
+        u = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+        b = torch.rand(op.n_dofs, generator=g, device="cuda:0", dtype=dt)
+        y = torch.empty_like(u)
+        if s == 4:
+            for mf in (1, 0):
+                prev = core.set_tuning("mfma_smoother", mf)
+                try:
+                    rate(f"smoother_colour_fp32_{'mfma' if mf else 'lds_fma'}",
+                         lambda: sm.step(u, b), s * sum(vals), len(vals))
+                finally:
+                    core.set_tuning("mfma_smoother", prev)
+        else:
+            rate("smoother_colour_fp64", lambda: sm.step(u, b), s * sum(vals), len(vals))
+        rate(f"vmult_{tag}", lambda: op.vmult(y, u), 2 * s * op.n_dofs, 1)
+        rate(f"residual_{tag}", lambda: op.residual(y, b, u), 3 * s * op.n_dofs, 1)
+        tr = core.MGTransferCUDA(ctx, 3, k, 1, level, dt)
+        nc = (k * (1 << (level - 1)) + 1) ** 3
+        c = torch.rand(nc, generator=g, device="cuda:0", dtype=dt)
+        rate(f"prolongate_add_{tag}", lambda: tr.prolongate_and_add(level, u, c),
+             s * (nc + 2 * op.n_dofs), 1)
+        rate(f"restrict_add_{tag}", lambda: tr.restrict_and_add(level, c, u),
+             s * (op.n_dofs + 2 * nc), 1)
+        del op, sm, tr
+    return out
 
 
 def c5_report(ctx, core, problems, reps, rel_l2):
@@ -401,32 +451,44 @@ def traffic_summary(fetch_dir: str, write_dir: str, kernel: str, workload, out_p
     print(json.dumps(entry), flush=True)
 
 
+CPU_NOTE = ("the oracle: an FP64 C++/OpenMP restatement of the reference's algorithm in its "
+            "own formulation (cell-quadrature operator with scatter, full-box local "
+            "residual, valence-weighted transfer), not a tuned CPU code, so the GPU/CPU "
+            "ratio says little about the kernels")
+
+
 def cpu_baseline(args):
-    """FP64 C++ oracle MG-PCG on the host cores (bounded sample)."""
+    """FP64 C++ oracle MG-PCG on the host cores (bounded sample): the full
+    workload solved repeatedly for about 12 s (at most 5 solves, after a
+    probe one level below), best solve reported; one level below when a
+    full solve would not fit in about 40 s."""
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     threads = O.set_threads(threads)
     dim, k = args.dim, args.degree
-    # probe one level below the workload; run the workload itself only when
-    # it fits in about 40 s
     lvl = args.level - 1
     b = O.rhs_random(dim, k, lvl, SEED)
     t0 = time.perf_counter()
     _, it, _, _ = O.pcg(dim, k, lvl, b, rtol=args.rtol, maxit=100)
     t_probe = time.perf_counter() - t0
     n = len(b)
-    sample = f"level {lvl} ({n} DoFs) full FP64 MG-PCG solve, {it} iterations"
     if t_probe * 8.5 < 40.0:
-        b = O.rhs_random(dim, k, args.level, SEED)
-        t0 = time.perf_counter()
-        _, it, _, _ = O.pcg(dim, k, args.level, b, rtol=args.rtol, maxit=100)
-        t = time.perf_counter() - t0
+        lvl = args.level
+        b = O.rhs_random(dim, k, lvl, SEED)
         n = len(b)
-        sample = (f"the full workload (level {args.level}, {n} DoFs) solved once in FP64, "
-                  f"{it} iterations, {t:.1f} s")
-    else:
-        t = t_probe
-        sample += f", {t:.1f} s (the full workload would exceed the time bound)"
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and (not times or time.perf_counter() - t_start < 12.0):
+        t0 = time.perf_counter()
+        _, it, _, _ = O.pcg(dim, k, lvl, b, rtol=args.rtol, maxit=100)
+        times.append(time.perf_counter() - t0)
+        if lvl < args.level:
+            break
+    t = min(times)
+    what = ("the full workload" if lvl == args.level
+            else "one level below the workload (a full solve would exceed the time bound)")
+    sample = (f"{what}: level {lvl} ({n} DoFs), FP64 MG-PCG to {args.rtol:g}, {it} iterations, "
+              f"best of {len(times)} solve(s), {t:.2f} s; {CPU_NOTE}")
     return {"value": round(n / t / 1e6, 3), "unit": "MDoF/s", "cores": threads,
             "kind": "port", "sample": sample}
 
@@ -449,7 +511,7 @@ def cpu_baseline_one_core(args):
     return {"value": round(n / t / 1e6, 3), "unit": "MDoF/s", "cores": 1,
             "kind": "port",
             "sample": f"level {lvl} ({n} DoFs) full FP64 MG-PCG solve, {it} iterations, "
-                      f"{t:.1f} s"}
+                      f"one solve, {t:.1f} s (the oracle, as for cpu_baseline)"}
 
 
 def kernel_bench(args):

@@ -185,6 +185,9 @@ namespace
     if (!lvl->coef)
       lvl->coef = dmalloc(lvl->elem() * ncells);
     ensure_celem(lvl);
+    // the smoother's ratio-grid eigenpairs, built here (setup) and not on
+    // the first smoothing step, which may run under HIP graph capture
+    (void)get_sep_tables(lvl->ctx, lvl->k, lvl->prec);
     if (lvl->prec == GTPS_FP32)
       launch_convert_f64_f32(lvl->ctx->stream, ncells, c, (float *)lvl->coef);
     else
@@ -1231,6 +1234,8 @@ extern "C"
         {
           dfree(kv.second->d64);
           dfree(kv.second->d32);
+          dfree(kv.second->sep64);
+          dfree(kv.second->sep32);
         }
       dfree(ctx->ws.partials);
       dfree(ctx->ws.scalars);
@@ -1749,8 +1754,7 @@ extern "C"
         }
       else if (k == "patches_per_wave")
         {
-          GTPS_REQUIRE(value == 1 || value == 2 || value == 4,
-                       "patches_per_wave must be 1, 2 or 4");
+          GTPS_REQUIRE(value >= 1 && value <= 5, "patches_per_wave must be in 1..5");
           prev                         = tuning().patches_per_wave;
           tuning().patches_per_wave    = (int)value;
         }

@@ -18,10 +18,23 @@
 // an element-vector buffer that a DoF-centred gather adds up in a fixed
 // order (deterministic, no atomics).
 //
-// Smoother: per colour and patch, u_I <- u_I + relax A~_P^{-1} (b - A u)_I
-// with A~_P = cmax_P A^1_P (cmax_P = largest coefficient of the patch's 8
-// cells), so A~_P >= A_P,II and the local step is a contraction; the local
-// residual is formed from the 8 cells of the patch in LDS.
+// Smoother: per colour and patch, u_I <- u_I + relax (sigma_P A~_P)^{-1}
+// (b - A u)_I, the local residual formed from the 8 cells of the patch in
+// LDS.  A~_P is the patch operator of a separable coefficient f_abc =
+// rho_x^a rho_y^b rho_z^c (a, b, c the cell offsets), which is a Kronecker
+// sum of the 2-cell 1D pencils (K_cell0 + rho K_cell1, M_cell0 + rho
+// M_cell1) and is inverted by fast diagonalisation with one eigenbasis per
+// direction (exact when c is separable, c = 1 in particular).  rho_d is the
+// log-space least-squares ratio of the upper to the lower cells along d,
+// rounded to a grid of step SEP_STEP whose eigenpairs are tabulated once per
+// degree (FP64 on the GPU, setup.hip); sigma_P = max(max_K c_K/f_K /
+// SEP_THETA, exp(mean log(c_K/f_K))).  Every A_K is positive semi-definite,
+// so A_P <= max(c/f) sum f_K A_K <= SEP_THETA sigma_P A~_P: the local
+// correction's largest eigenvalue is at most SEP_THETA = 1.5 < 2, each
+// patch update is an A-norm contraction and the symmetric V-cycle stays
+// positive definite.  (The previous local solve, the constant-coefficient
+// inverse scaled by the patch's largest coefficient, took 19 CG iterations
+// at 32^3 Q4 where exact local solves take 7.)
 #include "gtps_internal.h"
 
 namespace gtps
@@ -276,8 +289,8 @@ namespace gtps
     __global__ void __launch_bounds__(CNT)
     patch_coeff_kernel(T *__restrict__ u, const T *__restrict__ bvec,
                        const T *__restrict__ coef, const T *__restrict__ tab,
-                       const int n, const int colour, const T scale,
-                       const T relax)
+                       const T *__restrict__ sep, const int n, const int colour,
+                       const T scale, const T relax)
     {
       using TB         = Tab<K>;
       using C          = PC<K, T>;
@@ -286,10 +299,10 @@ namespace gtps
       __shared__ T sa[8 * N3], sb[8 * N3];
       __shared__ T sr[NI * NI * NI];
       __shared__ T sc[8];
+      __shared__ int sbin[3];
+      __shared__ T   ssig;
       const T     *cM   = tab + TB::cellM;
       const T     *cK   = tab + TB::cellK;
-      const T     *S    = tab + TB::S;
-      const T     *linv = tab + TB::lamInv3;
       const int    tid  = threadIdx.x;
       const int    N1   = K * n + 1;
       const long long P = (long long)N1 * N1;
@@ -334,6 +347,40 @@ namespace gtps
             sbox[tid + it * CNT] = vu[it];
       }
       __syncthreads();
+      if (tid == 0)
+        {
+          // separable fit of the 8 cell coefficients (FP64): per-direction
+          // log ratios on the SEP_STEP grid, then the scale sigma
+          double lc[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            lc[c] = log((double)sc[c]);
+          double lf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            {
+              double d = 0.0;
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                d += ((c >> a) & 1) ? lc[c] : -lc[c];
+              const double q  = 0.25 * d / SEP_STEP;
+              int          bq = (int)(q >= 0.0 ? floor(q + 0.5) : -floor(-q + 0.5));
+              bq              = bq < -SEP_HALF ? -SEP_HALF : (bq > SEP_HALF ? SEP_HALF : bq);
+              sbin[a]         = bq;
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                if ((c >> a) & 1)
+                  lf[c] += bq * SEP_STEP;
+            }
+          double rmax = lc[0] - lf[0], rsum = 0.0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            {
+              rmax = fmax(rmax, lc[c] - lf[c]);
+              rsum += lc[c] - lf[c];
+            }
+          ssig = (T)fmax(exp(rmax) / SEP_THETA, exp(0.125 * rsum));
+        }
       // cell z-rows: task (cell, ix, iy)
       for (int t = tid; t < 8 * N2; t += CNT)
         {
@@ -456,13 +503,14 @@ namespace gtps
                 }
           sr[t] = bres[it] - s;
         }
-      T cmax = sc[0];
-#pragma unroll
-      for (int c = 1; c < 8; ++c)
-        cmax = sc[c] > cmax ? sc[c] : cmax;
       __syncthreads();
-      // fast diagonalisation: S^T along x, y, z; scale; S along z, y, x
-      constexpr int NL = NI * NI;
+      // fast diagonalisation with the per-direction eigenpairs of the fit:
+      // S_x^T, S_y^T, S_z^T; 1/(sigma scale (lx_i + ly_j + lz_l)); S_z, S_y,
+      // S_x (bins are uniform over the work-group: scalar table loads)
+      constexpr int NL = NI * NI, SW = SEP_W(K);
+      const T      *Sx = sep + (__builtin_amdgcn_readfirstlane(sbin[0]) + SEP_HALF) * SW;
+      const T      *Sy = sep + (__builtin_amdgcn_readfirstlane(sbin[1]) + SEP_HALF) * SW;
+      const T      *Sz = sep + (__builtin_amdgcn_readfirstlane(sbin[2]) + SEP_HALF) * SW;
       for (int t = tid; t < NL; t += CNT)  // x-lines (l, j)
         {
           T v[NI];
@@ -475,7 +523,7 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int i = 0; i < NI; ++i)
-                s += S[i * NI + jj] * v[i];
+                s += Sx[i * NI + jj] * v[i];
               sr[t * NI + jj] = s;
             }
         }
@@ -494,16 +542,17 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int j = 0; j < NI; ++j)
-                s += S[j * NI + jj] * v[j];
+                s += Sy[j * NI + jj] * v[j];
               base[jj * NI] = s;
             }
         }
       __syncthreads();
-      const T inv = T(1) / (scale * cmax);
+      const T inv = T(1) / (scale * ssig);
       for (int t = tid; t < NL; t += CNT)  // z-lines (j, i)
         {
-          T *base = sr + t;
-          T  v[NI], w[NI];
+          T      *base = sr + t;
+          T       v[NI], w[NI];
+          const T lxy  = Sx[NL + t % NI] + Sy[NL + t / NI];
 #pragma unroll
           for (int l = 0; l < NI; ++l)
             v[l] = base[l * NL];
@@ -513,8 +562,8 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int l = 0; l < NI; ++l)
-                s += S[l * NI + ll] * v[l];
-              w[ll] = s * linv[ll * NL + t] * inv;
+                s += Sz[l * NI + ll] * v[l];
+              w[ll] = s * inv / (lxy + Sz[NL + ll]);
             }
 #pragma unroll
           for (int l = 0; l < NI; ++l)
@@ -522,7 +571,7 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int ll = 0; ll < NI; ++ll)
-                s += S[l * NI + ll] * w[ll];
+                s += Sz[l * NI + ll] * w[ll];
               base[l * NL] = s;
             }
         }
@@ -541,7 +590,7 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int jj = 0; jj < NI; ++jj)
-                s += S[j * NI + jj] * v[jj];
+                s += Sy[j * NI + jj] * v[jj];
               base[j * NI] = s;
             }
         }
@@ -560,7 +609,7 @@ namespace gtps
               T s = 0;
 #pragma unroll
               for (int jj = 0; jj < NI; ++jj)
-                s += S[i * NI + jj] * v[jj];
+                s += Sx[i * NI + jj] * v[jj];
               const T uold = sbox[((l + 1) * NP + j + 1) * NP + i + 1];
               u[g0 + i]    = uold + relax * s;
             }
@@ -662,9 +711,10 @@ namespace gtps
           if (np == 0)
             return;
           GTPS_REQUIRE(np < (1LL << 31), "mesh too large for the patch kernel");
+          const T *sep = (const T *)get_sep_tables(lvl->ctx, K, lvl->prec);
           hipLaunchKernelGGL((patch_coeff_kernel<K, T>), dim3((unsigned)np),
                              dim3(CNT), 0, lvl->ctx->stream, u, b,
-                             (const T *)lvl->coef, (const T *)lvl->dtab(), n,
+                             (const T *)lvl->coef, (const T *)lvl->dtab(), sep, n,
                              colour, (T)lvl->scale, (T)relax);
           GTPS_LAUNCH_CHECK();
         }

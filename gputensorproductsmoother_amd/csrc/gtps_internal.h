@@ -60,7 +60,31 @@ namespace gtps
     Basis1D basis;
     double *d64 = nullptr;  // packed Tab<K> layout, FP64
     float  *d32 = nullptr;  // same, FP32
+    // eigenpairs of the 2-cell interior pencils (K_cell0 + rho K_cell1,
+    // M_cell0 + rho M_cell1) on the ratio grid of the variable-coefficient
+    // smoother (bins -SEP_HALF..SEP_HALF, built on first use): per bin
+    // S (ni x ni, S^T M S = I) then lambda (ni), SEP_W(k) values
+    double *sep64 = nullptr;
+    float  *sep32 = nullptr;
   };
+
+  // Separable fit of a vertex patch's 2^3 cell coefficients (variable-
+  // coefficient smoother): per-direction log ratios rounded to multiples of
+  // SEP_STEP (|bin| <= SEP_HALF), and the scale
+  //   sigma = max(max_K c_K / f_K / SEP_THETA, exp(mean_K log(c_K / f_K))),
+  // which bounds the largest eigenvalue of the local correction by
+  // SEP_THETA < 2 (see coeff.hip).
+  constexpr int    SEP_HALF  = 64;
+  constexpr int    SEP_BINS  = 2 * SEP_HALF + 1;
+  constexpr double SEP_STEP  = 1.0 / 16.0;
+  constexpr double SEP_THETA = 1.5;
+  constexpr int
+  SEP_W(int k)
+  {
+    return (2 * k - 1) * (2 * k - 1) + (2 * k - 1);
+  }
+  // FP64 or FP32 ratio-grid tables of degree k (built on first use)
+  const void *get_sep_tables(gtps_ctx *ctx, int k, int precision);
 
   struct Workspace
   {

@@ -121,12 +121,17 @@ namespace gtps
 
     // One wavefront: generalized eigenproblem of an n x n SPD pencil (n <= 16)
     // K s = lam M s by M = L L^T, C = L^-1 K L^-T, cyclic Jacobi on C,
-    // S = L^-T Q.  Eigenvalues ascending; S^T M S = I.
+    // S = L^-T Q.  Eigenvalues ascending; S^T M S = I.  Work-group b solves
+    // the b-th of a batch of pencils stored back to back.
     __global__ void __launch_bounds__(64)
     gen_eig_kernel(int n, const double *__restrict__ Kin,
                    const double *__restrict__ Min, double *__restrict__ lam,
                    double *__restrict__ Sout)
     {
+      Kin += (size_t)blockIdx.x * n * n;
+      Min += (size_t)blockIdx.x * n * n;
+      Sout += (size_t)blockIdx.x * n * n;
+      lam += (size_t)blockIdx.x * n;
       __shared__ double L[16 * 16], Li[16 * 16], C[16 * 16], V[16 * 16],
         T[16 * 16];
       __shared__ double cs[2];
@@ -481,6 +486,93 @@ namespace gtps
     for (double v : lam)
       GTPS_REQUIRE(std::isfinite(v) && v > 0.0,
                    "patch eigenproblem produced a non-positive eigenvalue");
+  }
+
+  const void *
+  get_sep_tables(gtps_ctx *ctx, int k, int precision)
+  {
+    DegreeTables *dt = get_tables(ctx, k);
+    if (!dt->sep64)
+      {
+        const Basis1D &b  = dt->basis;
+        const int      n  = k + 1, np = 2 * k + 1, ni = 2 * k - 1, nn = ni * ni;
+        std::vector<double> KI((size_t)SEP_BINS * nn), MI((size_t)SEP_BINS * nn);
+        for (int bin = 0; bin < SEP_BINS; ++bin)
+          {
+            const double        rho = std::exp((bin - SEP_HALF) * SEP_STEP);
+            std::vector<double> Mp(np * np, 0.0), Kp(np * np, 0.0);
+            for (int c = 0; c < 2; ++c)
+              for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j)
+                  {
+                    const double w = c ? rho : 1.0;
+                    Mp[(c * k + i) * np + c * k + j] += w * b.cellM[i * n + j];
+                    Kp[(c * k + i) * np + c * k + j] += w * b.cellK[i * n + j];
+                  }
+            for (int i = 0; i < ni; ++i)
+              for (int j = 0; j < ni; ++j)
+                {
+                  MI[(size_t)bin * nn + i * ni + j] = Mp[(i + 1) * np + j + 1];
+                  KI[(size_t)bin * nn + i * ni + j] = Kp[(i + 1) * np + j + 1];
+                }
+          }
+        const size_t        nb = SEP_BINS;
+        std::vector<double> lam(nb * ni), S(nb * nn);
+        double             *dbuf = nullptr;
+        GTPS_HIP(hipMalloc(&dbuf, sizeof(double) * nb * (3 * nn + ni)));
+        double *dK = dbuf, *dM = dbuf + nb * nn, *dS = dbuf + 2 * nb * nn,
+               *dl = dbuf + 3 * nb * nn;
+        try
+          {
+            GTPS_HIP(hipMemcpyAsync(dK, KI.data(), sizeof(double) * nb * nn,
+                                    hipMemcpyHostToDevice, ctx->stream));
+            GTPS_HIP(hipMemcpyAsync(dM, MI.data(), sizeof(double) * nb * nn,
+                                    hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(gen_eig_kernel, dim3((unsigned)nb), dim3(64), 0,
+                               ctx->stream, ni, dK, dM, dl, dS);
+            GTPS_LAUNCH_CHECK();
+            GTPS_HIP(hipMemcpyAsync(lam.data(), dl, sizeof(double) * nb * ni,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+            GTPS_HIP(hipMemcpyAsync(S.data(), dS, sizeof(double) * nb * nn,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+            GTPS_HIP(hipStreamSynchronize(ctx->stream));
+          }
+        catch (...)
+          {
+            (void)hipFree(dbuf);
+            throw;
+          }
+        GTPS_HIP(hipFree(dbuf));
+        for (double v : lam)
+          GTPS_REQUIRE(std::isfinite(v) && v > 0.0,
+                       "ratio-grid eigenproblem produced a non-positive eigenvalue");
+        const int           w = SEP_W(k);
+        std::vector<double> t64(nb * w);
+        for (size_t bin = 0; bin < nb; ++bin)
+          {
+            std::copy(S.begin() + bin * nn, S.begin() + (bin + 1) * nn, t64.begin() + bin * w);
+            std::copy(lam.begin() + bin * ni, lam.begin() + (bin + 1) * ni,
+                      t64.begin() + bin * w + nn);
+          }
+        std::vector<float> t32(t64.begin(), t64.end());
+        double *d64 = nullptr;
+        float  *d32 = nullptr;
+        GTPS_HIP(hipMalloc(&d64, sizeof(double) * t64.size()));
+        if (hipMalloc(&d32, sizeof(float) * t32.size()) != hipSuccess)
+          {
+            (void)hipGetLastError();
+            (void)hipFree(d64);
+            throw Error(GTPS_ENOMEM, "hipMalloc of the ratio-grid tables failed");
+          }
+        GTPS_HIP(hipMemcpyAsync(d64, t64.data(), sizeof(double) * t64.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+        GTPS_HIP(hipMemcpyAsync(d32, t32.data(), sizeof(float) * t32.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+        GTPS_HIP(hipStreamSynchronize(ctx->stream));
+        dt->sep64 = d64;
+        dt->sep32 = d32;
+      }
+    return precision == GTPS_FP32 ? (const void *)dt->sep32 : (const void *)dt->sep64;
   }
 
   DegreeTables *

@@ -1900,8 +1900,12 @@ namespace gtps
     {                                                                         \
       if (npw == 2)                                                           \
         GTPS_SG(M, 2);                                                        \
+      else if (npw == 3)                                                      \
+        GTPS_SG(M, 3);                                                        \
       else if (npw == 4)                                                      \
         GTPS_SG(M, 4);                                                        \
+      else if (npw == 5)                                                      \
+        GTPS_SG(M, 5);                                                        \
       else                                                                    \
         GTPS_SG(M, 1);                                                        \
     }                                                                         \

@@ -81,7 +81,7 @@ const char *gtps_version(void);
  *   "group_min_patches"     colours with more patches than this pack
  *                           several patches into each wavefront (default
  *                           2048: the finest two levels at 64^3 Q3);
- *   "patches_per_wave"      how many (1, 2 or 4; default 2);
+ *   "patches_per_wave"      how many (1 to 5; default 2);
  *   "box_vmult_max_n"       3D operator with k <= 4 on levels with at most
  *                           this many cells per direction: one-round box
  *                           kernel instead of the z-marching one (default
@@ -364,7 +364,10 @@ int gtps_dg_residual(gtps_dg_level *lvl, void *dst, const void *b,
 /* One multiplicative sweep of the vertex-patch smoother (patch = the 2^3
  * cells around an interior vertex, all their DoFs; 27 colours = patch index
  * mod 3 per direction; exact local solves by fast diagonalisation).
- * variant GTPS_MULT_FWD or GTPS_MULT_REV; u <- (1-w) u + w u_P per patch. */
+ * variant GTPS_MULT_FWD or GTPS_MULT_REV; u <- (1-w) u + w u_P per patch.
+ * Degrees: k <= 5 in FP64 and k <= 7 in FP32 (the patch of (2k+2)^3 DoFs
+ * and its tables must fit one work-group's LDS); other levels are rejected
+ * with GTPS_EINVAL.  The operator and transfer take every k <= 7. */
 int gtps_dg_smoother_step(gtps_dg_level *lvl, void *u, const void *b,
                           int variant, double relaxation);
 /* Cell-wise embedding from n/2 to n cells (fine_lvl is the fine level):
@@ -377,7 +380,8 @@ int gtps_dg_restrict_add(gtps_dg_level *fine_lvl, void *coarse,
 /* DG multigrid on n_coarse 2^l cells per direction, l = 0..max_level:
  * V(1,1) (forward pre-, reverse post-smoothing) in vcycle_precision, the
  * coarsest level solved approximately by coarse_sweeps symmetric sweep pairs
- * from zero; FP64 CG on the finest level. */
+ * from zero; FP64 CG on the finest level.  Same degree limits as
+ * gtps_dg_smoother_step (FP64 k <= 5, FP32 k <= 7). */
 int gtps_dg_mg_create(gtps_ctx *ctx, int dim, int degree, int n_coarse,
                       int max_level, int vcycle_precision, double penalty,
                       int coarse_sweeps, gtps_dg_mg **out);

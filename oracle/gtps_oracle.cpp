@@ -34,9 +34,10 @@
 //   * variable-coefficient diffusion (no reference counterpart: the
 //     reference's MultigridSolvers ignores its coefficient, solver.cuh:1548):
 //     per-cell coefficient in the same cell loop, the transpose of the
-//     constrained operator, a vertex-patch smoother with the scaled
-//     constant-coefficient FD inverse as inexact local solve, coefficient
-//     coarsening by averaging, and dJ/dc_K = -lambda_K^T A^1_K u_K.
+//     constrained operator, a vertex-patch smoother whose inexact local
+//     solve is the FD inverse of a separable (rank-1) fit of the patch's
+//     cell coefficients, coefficient coarsening by averaging, and
+//     dJ/dc_K = -lambda_K^T A^1_K u_K.
 //   * post-processing: load-vector assembly from f at the Gauss points
 //     (laplace_operator.cuh:140-173) and the L2 / H1-seminorm error against
 //     prod sin(w pi x_d) (poisson.cu:44-74, 353-382), CG and DG layouts.
@@ -49,8 +50,10 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <map>
 
 #include <omp.h>
 
@@ -713,21 +716,155 @@ namespace
       }
   }
 
+  // ------------------------ separable approximation of a patch operator
+  // The patch operator sum_K c_K A_K over the 2^d cells of a vertex patch is
+  // a Kronecker sum exactly when the coefficient is separable,
+  // c_abc = s p_a q_b r_c (a, b, c the cell offsets along x, y, z): then
+  //   sum c_abc A_abc = s (K_p (x) M_q (x) M_r + M_p (x) K_q (x) M_r + ...)
+  // with K_p = K_cell0 + (p_1/p_0) K_cell1 the 2-cell 1D patch matrix whose
+  // second cell is weighted by the ratio rho = p_1/p_0 (p_0 = 1), and the
+  // fast-diagonalisation inverse applies with per-direction generalized
+  // eigenpairs of (K_rho, M_rho).  For a general coefficient the local solve
+  // uses the log-space least-squares fit: log rho_d = mean log c over the
+  // upper cells along d minus mean over the lower ones, rounded to a grid of
+  // step SEP_STEP (|log rho| <= SEP_HALF SEP_STEP; the eigenpairs of every
+  // grid point are tabulated once per degree), fit f_abc = prod_d rho_d^bit,
+  // and the scale
+  //   sigma = max( max_K c_K / f_K / SEP_THETA, exp(mean_K log(c_K / f_K)) ).
+  // A_P = sum c_K A_K <= max_K(c_K/f_K) sum f_K A_K (every A_K is positive
+  // semi-definite), so sigma A~ >= A_P / SEP_THETA: the local correction has
+  // lambda_max(A~^{-1} A_P,II) <= SEP_THETA < 2 and every patch update is an
+  // A-norm contraction (the symmetric V-cycle stays SPD); SEP_THETA = 1.5
+  // was the best of 1.0-1.95 in the oracle's MG-PCG iteration counts.  A
+  // separable coefficient (c = 1 in particular) gives the exact local solve.
+  constexpr int    SEP_HALF  = 64;
+  constexpr double SEP_STEP  = 1.0 / 16.0;
+  constexpr double SEP_THETA = 1.5;
+
+  struct SepPencil
+  {
+    std::vector<double> lam, S;
+  };
+  const SepPencil &
+  sep_pencil(int k, int bin)
+  {
+    static std::map<std::pair<int, int>, SepPencil> cache;
+    const SepPencil *res = nullptr;
+#pragma omp critical(gtps_oracle_sep)
+    {
+      auto it = cache.find({k, bin});
+      if (it == cache.end())
+        {
+          const Basis &b   = basis(k);
+          const int    n   = k + 1, np = 2 * k + 1, ni = 2 * k - 1;
+          const double rho = std::exp(bin * SEP_STEP);
+          std::vector<double> MPr(np * np, 0.0), KPr(np * np, 0.0);
+          for (int c = 0; c < 2; ++c)
+            for (int i = 0; i < n; ++i)
+              for (int j = 0; j < n; ++j)
+                {
+                  const double w = c ? rho : 1.0;
+                  MPr[(c * k + i) * np + c * k + j] += w * b.M[i * n + j];
+                  KPr[(c * k + i) * np + c * k + j] += w * b.K[i * n + j];
+                }
+          std::vector<double> MI(ni * ni), KI(ni * ni);
+          for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < ni; ++j)
+              {
+                MI[i * ni + j] = MPr[(i + 1) * np + j + 1];
+                KI[i * ni + j] = KPr[(i + 1) * np + j + 1];
+              }
+          SepPencil p;
+          p.lam.resize(ni);
+          p.S.resize(ni * ni);
+          gen_eig(ni, KI.data(), MI.data(), p.lam.data(), p.S.data());
+          it = cache.emplace(std::make_pair(k, bin), std::move(p)).first;
+        }
+      res = &it->second;
+    }
+    return *res;
+  }
+
+  // grid bins of the per-direction log ratios and the scale sigma of the
+  // patch whose lower-left cell is (ix, iy, iz)
+  void
+  separable_fit(const Grid &g, const double *coef, int ix, int iy, int iz, int *bins,
+                double *sigma)
+  {
+    const int dim = g.dim, nc = 1 << dim;
+    double    lc[8];
+    for (int c = 0; c < nc; ++c)
+      lc[c] = std::log(coef[cell_index(g, ix + (c & 1), iy + ((c >> 1) & 1),
+                                       iz + ((c >> 2) & 1))]);
+    bins[0] = bins[1] = bins[2] = 0;
+    for (int a = 0; a < dim; ++a)
+      {
+        double d = 0.0;
+        for (int c = 0; c < nc; ++c)
+          d += ((c >> a) & 1) ? lc[c] : -lc[c];
+        d /= nc / 2;
+        bins[a] = (int)std::max<long>(-SEP_HALF, std::min<long>(SEP_HALF, std::lround(d / SEP_STEP)));
+      }
+    double rmax = -1e300, rsum = 0.0;
+    for (int c = 0; c < nc; ++c)
+      {
+        double lf = 0.0;
+        for (int a = 0; a < dim; ++a)
+          if ((c >> a) & 1)
+            lf += bins[a] * SEP_STEP;
+        rmax = std::max(rmax, lc[c] - lf);
+        rsum += lc[c] - lf;
+      }
+    *sigma = std::max(std::exp(rmax) / SEP_THETA, std::exp(rsum / nc));
+  }
+
+  // r <- (sigma A~)^{-1} r with A~ = scale sum_d K_rho_d (x) M_rho_others
+  void
+  fd_inverse_sep(const Grid &g, const int *bins, double sigma, double *r, double *tmp)
+  {
+    const int    ni       = 2 * g.k - 1;
+    const int    shape[3] = {ni, ni, g.dim == 3 ? ni : 1};
+    const double scale    = std::pow(g.h, g.dim - 2);
+    const SepPencil *p[3];
+    for (int a = 0; a < g.dim; ++a)
+      p[a] = &sep_pencil(g.k, bins[a]);
+    double *bufs[2] = {r, tmp};
+    int     cur     = 0;
+    for (int a = 0; a < g.dim; ++a)
+      {
+        apply_axis(p[a]->S.data(), ni, ni, bufs[cur], bufs[cur ^ 1], shape, a, false, true);
+        cur ^= 1;
+      }
+    for (int l = 0; l < shape[2]; ++l)
+      for (int j = 0; j < ni; ++j)
+        for (int i = 0; i < ni; ++i)
+          {
+            const double lam =
+              p[0]->lam[i] + p[1]->lam[j] + (g.dim == 3 ? p[2]->lam[l] : 0.0);
+            bufs[cur][(l * ni + j) * ni + i] /= sigma * scale * lam;
+          }
+    for (int a = 0; a < g.dim; ++a)
+      {
+        apply_axis(p[a]->S.data(), ni, ni, bufs[cur], bufs[cur ^ 1], shape, a, false, false);
+        cur ^= 1;
+      }
+    if (cur != 0)
+      std::memcpy(r, bufs[cur], sizeof(double) * shape[0] * shape[1] * shape[2]);
+  }
+
   // ------------------------------------------- variable-coefficient smoother
   // Multiplicative vertex-patch Schwarz for A = sum_K c_K A_K (no reference
   // counterpart: the reference ignores its coefficient, solver.cuh:1548).
   // The patch operator is no longer a Kronecker sum, so the local solve is
-  // the fast-diagonalisation inverse of the constant-coefficient patch
-  // operator scaled by the largest coefficient of the patch's 2^d cells,
-  // A~_P = cmax_P A^1_P >= A_P,II, applied in correction form
-  //   u_I <- u_I + relax A~_P^{-1} (b - A u)_I,
+  // the fast-diagonalisation inverse of the separable approximation
+  // sigma_P A~_P (separable_fit above), applied in correction form
+  //   u_I <- u_I + relax (sigma_P A~_P)^{-1} (b - A u)_I,
   // with the local residual from the 2^d cells of the patch (the only cells
   // that touch its interior DoFs).
   void
   smoother_step_coeff(const Grid &g, const double *coef, int variant,
                       double relax, const double *b_, double *u)
   {
-    const Basis &b      = basis(g.k);
     const int    k      = g.k;
     const int    n1d    = k + 1;
     const int    np     = 2 * k + 1;
@@ -765,13 +902,11 @@ namespace
                       for (int i = 0; i < np; ++i)
                         box[(l * np + j) * np + i] = u[gidx(i, j, l)];
                   std::fill(acc.begin(), acc.end(), 0.0);
-                  double cmax = 0.0;
                   for (int c = 0; c < (1 << dim); ++c)
                     {
                       const int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
                       const double cK =
                         coef[cell_index(g, ix + ox, iy + oy, iz + oz)];
-                      cmax = std::max(cmax, cK);
                       for (int l = 0; l < op.shape[2]; ++l)
                         for (int j = 0; j < n1d; ++j)
                           for (int i = 0; i < n1d; ++i)
@@ -793,12 +928,15 @@ namespace
                             b_[gidx(i + 1, j + 1, lz)] -
                             acc[(lz * np + j + 1) * np + i + 1];
                         }
-                  fd_inverse(g, b, delta.data(), tmp.data());
+                  int    bins[3];
+                  double sigma;
+                  separable_fit(g, coef, ix, iy, iz, bins, &sigma);
+                  fd_inverse_sep(g, bins, sigma, delta.data(), tmp.data());
                   for (int l = 0; l < shi[2]; ++l)
                     for (int j = 0; j < ni; ++j)
                       for (int i = 0; i < ni; ++i)
                         u[gidx(i + 1, j + 1, l + (dim == 3))] +=
-                          relax * delta[(l * ni + j) * ni + i] / cmax;
+                          relax * delta[(l * ni + j) * ni + i];
                 }
         }
       }

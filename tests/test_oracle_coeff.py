@@ -6,7 +6,8 @@ generators) without trusting its own formulation:
   50-digit golden 1D cell matrices (tests/golden/basis_1d.json), with the
   identity Dirichlet rows;
 * block Gauss-Seidel over the patch colours with the inexact local solve
-  (cmax_P * constant-coefficient patch matrix)^{-1}, dense;
+  (sigma_P sum_K f_K A_K)^{-1} of the separable fit of the patch's cell
+  coefficients, assembled densely from the golden cell matrices;
 * exact identities: c = 1 gives the Poisson operator and the exact smoother,
   sum_K c_K dJ/dc_K = -lambda~ . A(c) u, <A x, y> = <x, A^T y>;
 * central finite differences of J(c) = 1/2 ||u(c) - u_d||^2 with dense
@@ -85,10 +86,32 @@ def _patches(dim, level, colour):
                 yield ix, iy, iz
 
 
+SEP_STEP, SEP_HALF, SEP_THETA = 1.0 / 16.0, 64, 1.5
+
+
+def separable_fit(dim, cells, coef):
+    """Per-cell weights f_K of the rank-1 fit of the patch's cell
+    coefficients (cells listed with the x offset fastest) and the scale
+    sigma: log-ratios per direction rounded to the grid, then
+    sigma = max(max c/f / theta, exp(mean log(c/f)))."""
+    lc = np.log(np.array([coef[c] for c in cells]))
+    nc = len(cells)
+    bits = np.array([[(c >> a) & 1 for a in range(dim)] for c in range(nc)])
+    lr = np.array([(lc[bits[:, a] == 1].mean() - lc[bits[:, a] == 0].mean())
+                   for a in range(dim)])
+    q = np.clip(np.rint(lr / SEP_STEP), -SEP_HALF, SEP_HALF) * SEP_STEP
+    lf = bits @ q
+    sigma = max(np.exp((lc - lf).max()) / SEP_THETA, np.exp((lc - lf).mean()))
+    return np.exp(lf), sigma
+
+
 def dense_coeff_sweep(dim, k, level, coef, b, u, reverse=False, relax=1.0):
-    """Block Gauss-Seidel over the colours, local solve (cmax A^1_P,II)^{-1}."""
+    """Block Gauss-Seidel over the colours; local solve with the dense
+    sigma sum_K f_K A_K over the patch's cells (interior rows and columns),
+    the separable fit of the patch's coefficients, assembled from the golden
+    cell matrices (no eigenvectors)."""
     A = dense_constrained(dim, k, level, coef)
-    A1 = dense_full(dim, k, level, None)
+    Ac = _cell_matrix(dim, k, level)
     n1 = k * (1 << level) + 1
     n = 1 << level
     u = u.copy()
@@ -108,9 +131,15 @@ def dense_coeff_sweep(dim, k, level, coef, b, u, reverse=False, relax=1.0):
                      + (k * ix + i)[None, None, :]).ravel()
                 cells = [(ix + a) + n * (iy + b_) + n * n * (iz + c)
                          for c in (0, 1) for b_ in (0, 1) for a in (0, 1)]
-            cmax = max(coef[c] for c in cells)
+            f, sigma = separable_fit(dim, cells, coef)
+            At = np.zeros((A.shape[0], A.shape[0]))
+            for w, K in zip(f, cells):
+                cx, cy = K % n, (K // n) % n
+                cz = K // (n * n) if dim == 3 else 0
+                d = _cell_dofs(dim, k, level, cx, cy, cz)
+                At[np.ix_(d, d)] += sigma * w * Ac
             r = b - A @ u
-            u[I] += relax * np.linalg.solve(cmax * A1[np.ix_(I, I)], r[I])
+            u[I] += relax * np.linalg.solve(At[np.ix_(I, I)], r[I])
     return u
 
 
@@ -163,6 +192,75 @@ def test_coeff_smoother_dense(dim, k, level, reverse):
     got = O.coeff_smoother_step(dim, k, level, c, b, u,
                                 variant=O.MULT_REV if reverse else O.MULT_FWD, relax=0.8)
     assert rel_err(got, ref) < 1e-12
+
+
+def _exact_coeff_sweep(dim, k, level, coef, b, u):
+    """Block Gauss-Seidel with exact local solves A_P,II^{-1}."""
+    A = dense_constrained(dim, k, level, coef)
+    n1, n = k * (1 << level) + 1, 1 << level
+    u = u.copy()
+    for col in range(1 << dim):
+        if dim == 2 and col >= 4:
+            continue
+        for ix, iy, iz in _patches(dim, level, col):
+            i = np.arange(1, 2 * k)
+            if dim == 2:
+                I = ((k * iy + i)[:, None] * n1 + (k * ix + i)[None, :]).ravel()
+            else:
+                I = ((k * iz + i)[:, None, None] * n1 * n1 + (k * iy + i)[None, :, None] * n1
+                     + (k * ix + i)[None, None, :]).ravel()
+            r = b - A @ u
+            u[I] += np.linalg.solve(A[np.ix_(I, I)], r[I])
+    return u
+
+
+@pytest.mark.parametrize("dim,k,level", [(3, 2, 2), (3, 1, 3), (2, 3, 3)])
+def test_coeff_smoother_separable_coefficient_is_exact(dim, k, level):
+    """A separable coefficient c = p(cx) q(cy) r(cz) whose neighbour ratios lie
+    on the fit's grid makes every patch's fit exact: the sweep is block
+    Gauss-Seidel with exact local solves."""
+    n = 1 << level
+    g = rng(13)
+    logs = [SEP_STEP * g.integers(-12, 13, n) for _ in range(dim)]
+    cz_ = np.exp(logs[2]) if dim == 3 else np.ones(1)
+    c = (cz_[:, None, None] * np.exp(logs[1])[None, :, None]
+         * np.exp(logs[0])[None, None, :]).ravel()
+    b, u = g.uniform(-1, 1, (2, n_dofs(dim, k, level)))
+    got = O.coeff_smoother_step(dim, k, level, c, b, u)
+    assert rel_err(got, _exact_coeff_sweep(dim, k, level, c, b, u)) < 1e-11
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_separable_local_solve_bounds(k):
+    """For random patch coefficients (lognormal, sigma up to 1.5) the local
+    operator sigma A~ of the fit satisfies A_P,II <= 1.5 sigma A~ (largest
+    generalized eigenvalue at most the fit's theta), the bound that keeps the
+    symmetric V-cycle positive definite."""
+    dim, level = 3, 1
+    n1 = 2 * k + 1
+    Ac = _cell_matrix(dim, k, level)
+    i = np.arange(1, 2 * k)
+    I = ((i)[:, None, None] * n1 * n1 + i[None, :, None] * n1 + i[None, None, :]).ravel()
+    g = rng(17)
+    worst = 0.0
+    for trial in range(40):
+        sd = [0.25, 0.5, 1.0, 1.5][trial % 4]
+        c = np.exp(sd * g.standard_normal(8))
+        f, sigma = separable_fit(dim, list(range(8)), c)
+        A = np.zeros((n1 ** 3, n1 ** 3))
+        At = np.zeros_like(A)
+        for K in range(8):
+            d = _cell_dofs(dim, k, level, K & 1, (K >> 1) & 1, K >> 2)
+            A[np.ix_(d, d)] += c[K] * Ac
+            At[np.ix_(d, d)] += sigma * f[K] * Ac
+        Ai, Ati = A[np.ix_(I, I)], At[np.ix_(I, I)]
+        L = np.linalg.cholesky(Ati)
+        Li = np.linalg.inv(L)
+        lam = np.linalg.eigvalsh(Li @ Ai @ Li.T)
+        worst = max(worst, lam.max())
+        assert lam.max() <= SEP_THETA * (1 + 1e-12)
+        assert lam.min() > 0
+    assert worst > 1.0
 
 
 @pytest.mark.parametrize("dim,k,level", [(3, 2, 2), (2, 3, 3)])
