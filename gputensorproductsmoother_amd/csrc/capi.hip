@@ -1218,6 +1218,10 @@ extern "C"
       GTPS_HIP(hipHostMalloc((void **)&ctx->ws.pinned,
                              sizeof(double) * cg::N_SLOTS,
                              hipHostMallocDefault));
+      GTPS_HIP(hipHostMalloc((void **)&ctx->ws.readback, sizeof(double) * cg::N_SLOTS,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+      GTPS_HIP(hipHostGetDevicePointer((void **)&ctx->ws.readback_dev,
+                                       ctx->ws.readback, 0));
       *out = ctx.release();
     });
   }
@@ -1241,6 +1245,8 @@ extern "C"
       dfree(ctx->ws.scalars);
       if (ctx->ws.pinned)
         (void)hipHostFree(ctx->ws.pinned);
+      if (ctx->ws.readback)
+        (void)hipHostFree(ctx->ws.readback);
       if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->stream);
       delete ctx;
@@ -2010,43 +2016,59 @@ extern "C"
       const bool     f32 = mg->prec == GTPS_FP32;
       float         *r32 = f32 ? (float *)mg->d[mg->maxl] : nullptr;
 
-      auto read_rr = [&]() {
-        GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, sc + cg::RR, sizeof(double),
-                                hipMemcpyDeviceToHost, s));
+      // the finalize kernel of every residual norm also writes it to mapped
+      // host memory; the host waits on an event recorded right after it
+      double *rb      = ctx->ws.readback_dev;
+      auto    read_rr = [&](int nb) {
+        launch_finalize_sum(s, pt, nb, sc, cg::RR, rb);
         GTPS_HIP(hipEventRecord(mg->ev, s));
       };
       auto wait_rr = [&]() {
         GTPS_HIP(hipEventSynchronize(mg->ev));
-        return std::sqrt(ctx->ws.pinned[0]);
+        return std::sqrt(ctx->ws.readback[0]);
       };
 
-      // x0 = lift(b), r0 = b - A x0
-      launch_lift_boundary(mg->fine64, b, x);
-      launch_laplace(mg->fine64, mg->r, x, b, false);
+      // x0 = lift(b), r0 = b - A x0.  When b vanishes on the Dirichlet DoFs
+      // (the usual right-hand side) this is x0 = 0, r0 = b, one fused pass;
+      // a sum of b^2 over the boundary faces decides.
       {
-        const int nb = launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap);
-        launch_finalize_sum(s, pt, nb, sc, cg::RR);
+        const int nbb = launch_boundary_sumsq(mg->fine64, b, pt, cap);
+        launch_finalize_sum(s, pt, nbb, sc, cg::BB, rb + 1);
+        GTPS_HIP(hipEventRecord(mg->ev, s));
+        GTPS_HIP(hipEventSynchronize(mg->ev));
       }
-      if (f32)
-        launch_convert_f64_f32(s, N, mg->r, r32);
-      // p = 0 and rz = 1 make the first direction update p = z + beta 0
-      launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
-      launch_cg_scalar(s, sc, cg::OP_INIT);
-      read_rr();
+      if (ctx->ws.readback[1] == 0.0)
+        read_rr(launch_cg_init(s, N, b, x, mg->r, r32, mg->p, sc, pt, cap));
+      else
+        {
+          launch_lift_boundary(mg->fine64, b, x);
+          launch_laplace(mg->fine64, mg->r, x, b, false);
+          if (f32)
+            launch_convert_f64_f32(s, N, mg->r, r32);
+          // p = 0 and rz = 1 make the first direction update p = z + beta 0
+          launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
+          launch_cg_scalar(s, sc, cg::OP_INIT);
+          read_rr(launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap));
+        }
+      // the first preconditioning step is enqueued before the host learns
+      // r0 (a zero or non-finite r0 discards it)
+      if (maxit > 0)
+        mg->direction();
       const double r0 = wait_rr();
       if (history)
         history[0] = r0;
       if (!std::isfinite(r0))
         {
           nonfinite = true;
+          GTPS_HIP(hipStreamSynchronize(s));
           return;
         }
       if (r0 == 0.0)
         {
           converged = true;
+          GTPS_HIP(hipStreamSynchronize(s));
           return;
         }
-      mg->direction();
       double rn = r0, rn_prev = -1.0;
       while (it_done < maxit)
         {
@@ -2055,10 +2077,7 @@ extern "C"
           const int nbq = launch_laplace(mg->fine64, mg->q, mg->p, nullptr, true);
           mg->toc(GTPS_PROF_CG_MATVEC, e);
           launch_finalize_op(s, pt, nbq, sc, cg::PQ, cg::OP_ALPHA);
-          const int nbr =
-            launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap);
-          launch_finalize_sum(s, pt, nbr, sc, cg::RR);
-          read_rr();
+          read_rr(launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap));
           ++it_done;
           // Enqueue the next preconditioning step while the host checks,
           // unless the residual history predicts that this iteration
@@ -2412,50 +2431,44 @@ extern "C"
       const int       cap = ctx->ws.capacity;
       const bool      f32 = mg->prec == GTPS_FP32;
       float          *r32 = f32 ? (float *)mg->d[mg->maxl] : nullptr;
-      auto read_rr = [&]() {
-        GTPS_HIP(hipMemcpyAsync(ctx->ws.pinned, sc + cg::RR, sizeof(double),
-                                hipMemcpyDeviceToHost, s));
+      // residual norms reach the host through mapped memory written by
+      // their finalize kernel (see gtps_mg_pcg_solve)
+      double *rb      = ctx->ws.readback_dev;
+      auto    read_rr = [&](int nb) {
+        launch_finalize_sum(s, pt, nb, sc, cg::RR, rb);
         GTPS_HIP(hipEventRecord(mg->ev, s));
       };
       auto wait_rr = [&]() {
         GTPS_HIP(hipEventSynchronize(mg->ev));
-        return std::sqrt(ctx->ws.pinned[0]);
+        return std::sqrt(ctx->ws.readback[0]);
       };
-      // x0 = 0, r0 = b (no Dirichlet rows: boundary conditions are weak)
-      launch_fill(s, N, GTPS_FP64, x, 0.0);
-      GTPS_HIP(hipMemcpyAsync(mg->r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-      {
-        const int nb = launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap);
-        launch_finalize_sum(s, pt, nb, sc, cg::RR);
-      }
-      if (f32)
-        launch_convert_f64_f32(s, N, mg->r, r32);
-      launch_fill(s, N, GTPS_FP64, mg->p, 0.0);
-      launch_cg_scalar(s, sc, cg::OP_INIT);
-      read_rr();
+      // x0 = 0, r0 = b (no Dirichlet rows: boundary conditions are weak),
+      // p = 0, rz = 1 in one pass; the first preconditioning step is
+      // enqueued before the host learns r0
+      read_rr(launch_cg_init(s, N, b, x, mg->r, r32, mg->p, sc, pt, cap));
+      if (maxit > 0)
+        mg->direction();
       const double r0 = wait_rr();
       if (history)
         history[0] = r0;
       if (!std::isfinite(r0))
         {
           nonfinite = true;
+          GTPS_HIP(hipStreamSynchronize(s));
           return;
         }
       if (r0 == 0.0)
         {
           converged = true;
+          GTPS_HIP(hipStreamSynchronize(s));
           return;
         }
-      mg->direction();
       double rn = r0, rn_prev = -1.0;
       while (it_done < maxit)
         {
           const int nbq = launch_dg_apply(mg->fine64, mg->q, mg->p, nullptr, true);
           launch_finalize_op(s, pt, nbq, sc, cg::PQ, cg::OP_ALPHA);
-          const int nbr =
-            launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap);
-          launch_finalize_sum(s, pt, nbr, sc, cg::RR);
-          read_rr();
+          read_rr(launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap));
           ++it_done;
           bool speculate = it_done < maxit;
           if (speculate && rn_prev > 0.0 && rn < rn_prev)

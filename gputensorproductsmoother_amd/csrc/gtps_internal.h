@@ -92,6 +92,11 @@ namespace gtps
     int     capacity = 0;
     double *scalars  = nullptr;  // device scalars (CG state)
     double *pinned   = nullptr;  // host-pinned mirror
+    // coherent, device-mapped host memory that reduction kernels write
+    // directly (residual norms read by the host without a copy);
+    // readback_dev is its device address
+    double *readback     = nullptr;
+    double *readback_dev = nullptr;
   };
 } // namespace gtps
 
@@ -347,9 +352,21 @@ namespace gtps
   int  launch_dot_partials(hipStream_t s, long long n, const double *x,
                            const double *y, const float *y32, double *partials,
                            int capacity);
-  // sum partials[0..nb) into scalars[slot] (deterministic, one block)
+  // sum partials[0..nb) into scalars[slot] (deterministic, one block); a
+  // non-NULL host_out (device address of mapped host memory) receives the
+  // same value
   void launch_finalize_sum(hipStream_t s, const double *partials, int nb,
-                           double *scalars, int slot);
+                           double *scalars, int slot, double *host_out = nullptr);
+  // partial sums of b^2 over the Dirichlet DoFs of the level (the faces
+  // enumerated one by one, edges counted more than once); returns #blocks
+  int launch_boundary_sumsq(const gtps_level *lvl, const double *b, double *partials,
+                            int capacity);
+  // CG start from x = 0: x = 0, r = b, r32 = (float) b (r32 may be NULL),
+  // p = 0, partial sums of b^2, and the CG scalars rz = 1, alpha = 0;
+  // returns #blocks
+  int launch_cg_init(hipStream_t s, long long n, const double *b, double *x, double *r,
+                     float *r32, double *p, double *scalars, double *partials,
+                     int capacity);
   // the same, followed by the scalar op (cg::OP_*) in the same kernel
   void launch_finalize_op(hipStream_t s, const double *partials, int nb,
                           double *scalars, int slot, int op);
@@ -374,13 +391,13 @@ namespace gtps
   namespace cg
   {
     // device scalar slots
-    constexpr int RZ = 0, RZ_OLD = 1, PQ = 2, ALPHA = 3, BETA = 4, RR = 5,
+    constexpr int RZ = 0, RZ_OLD = 1, PQ = 2, ALPHA = 3, BETA = 4, RR = 5, BB = 6,
                   N_SLOTS = 8;
     // scalar ops
     constexpr int OP_ALPHA = 0;  // alpha = rz / pq
     constexpr int OP_BETA  = 1;  // beta = rz / rz_old
     constexpr int OP_SHIFT = 3;  // rz_old = rz
-    constexpr int OP_INIT  = 4;  // rz = 1 (first direction: p = 0 + z)
+    constexpr int OP_INIT  = 4;  // rz = 1, alpha = 0 (first direction: p = 0 + z)
     // finalize_op only: rz_old = rz; rz = sum; beta = rz / rz_old
     constexpr int OP_SHIFT_BETA = 5;
   } // namespace cg

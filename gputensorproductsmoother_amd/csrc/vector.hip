@@ -112,12 +112,81 @@ namespace gtps
 
     __global__ void __launch_bounds__(VNT)
     finalize_kernel(const double *__restrict__ partials, int nb,
-                    double *__restrict__ scalars, int slot)
+                    double *__restrict__ scalars, int slot,
+                    double *__restrict__ host_out)
     {
       __shared__ double red[VNT / 64];
       const double      s = sum_partials(partials, nb, red);
       if (threadIdx.x == 0)
-        scalars[slot] = s;
+        {
+          scalars[slot] = s;
+          if (host_out)
+            *host_out = s;
+        }
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    boundary_sumsq_kernel(int dim, int N1, const double *__restrict__ b,
+                          double *__restrict__ partials)
+    {
+      __shared__ double red[VNT / 64];
+      const long long   face  = dim == 3 ? (long long)N1 * N1 : N1;
+      const long long   total = 2 * dim * face;
+      double            s     = 0.0;
+      for (long long e = (long long)blockIdx.x * VNT + threadIdx.x; e < total;
+           e += (long long)gridDim.x * VNT)
+        {
+          const int       f  = (int)(e / face);
+          const long long w  = e - f * face;
+          const int       a  = (int)(w % N1), c = (int)(w / N1);
+          const int       hi = (f & 1) ? N1 - 1 : 0;
+          long long       i;
+          if (dim == 3)
+            {
+              const long long P = (long long)N1 * N1;
+              i = f < 2 ? a + (long long)c * N1 + hi * P        // z face
+                : f < 4 ? a + (long long)hi * N1 + c * P        // y face
+                        : hi + (long long)a * N1 + c * P;       // x face
+            }
+          else
+            i = f < 2 ? a + (long long)hi * N1 : hi + (long long)a * N1;
+          const double v = b[i];
+          s += v * v;
+        }
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x == 0)
+        partials[blockIdx.x] = s;
+    }
+
+    __global__ void __launch_bounds__(VNT)
+    cg_init_kernel(long long n, const double *__restrict__ b, double *__restrict__ x,
+                   double *__restrict__ r, float *__restrict__ r32,
+                   double *__restrict__ p, double *__restrict__ sc,
+                   double *__restrict__ partials)
+    {
+      __shared__ double red[VNT / 64];
+      double            s = 0.0;
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        {
+          const double bi = b[i];
+          x[i]            = 0.0;
+          p[i]            = 0.0;
+          r[i]            = bi;
+          if (r32)
+            r32[i] = (float)bi;
+          s += bi * bi;
+        }
+      s = block_sum<VNT>(s, red);
+      if (threadIdx.x == 0)
+        {
+          partials[blockIdx.x] = s;
+          if (blockIdx.x == 0)
+            {
+              sc[cg::RZ]    = 1.0;
+              sc[cg::ALPHA] = 0.0;
+            }
+        }
     }
 
     // sum of the partials into scalars[slot], then one scalar op; op < 0:
@@ -160,7 +229,8 @@ namespace gtps
             sc[cg::RZ_OLD] = sc[cg::RZ];
             break;
           case cg::OP_INIT:
-            sc[cg::RZ] = 1.0;
+            sc[cg::RZ]    = 1.0;
+            sc[cg::ALPHA] = 0.0;
             break;
           default:
             break;
@@ -327,11 +397,34 @@ namespace gtps
 
   void
   launch_finalize_sum(hipStream_t s, const double *partials, int nb,
-                      double *scalars, int slot)
+                      double *scalars, int slot, double *host_out)
   {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(VNT), 0, s, partials, nb,
-                       scalars, slot);
+                       scalars, slot, host_out);
     GTPS_LAUNCH_CHECK();
+  }
+
+  int
+  launch_boundary_sumsq(const gtps_level *lvl, const double *b, double *partials,
+                        int capacity)
+  {
+    const long long face = lvl->dim == 3 ? (long long)lvl->N1 * lvl->N1 : lvl->N1;
+    const int       nb   = grid_for(2 * lvl->dim * face, std::min(capacity, 2048));
+    hipLaunchKernelGGL(boundary_sumsq_kernel, dim3(nb), dim3(VNT), 0, lvl->ctx->stream,
+                       lvl->dim, lvl->N1, b, partials);
+    GTPS_LAUNCH_CHECK();
+    return nb;
+  }
+
+  int
+  launch_cg_init(hipStream_t s, long long n, const double *b, double *x, double *r,
+                 float *r32, double *p, double *scalars, double *partials, int capacity)
+  {
+    const int nb = grid_for(n, std::min(capacity, 2048));
+    hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(VNT), 0, s, n, b, x, r, r32, p,
+                       scalars, partials);
+    GTPS_LAUNCH_CHECK();
+    return nb;
   }
 
   void
