@@ -193,6 +193,18 @@ namespace
     else
       GTPS_HIP(hipMemcpyAsync(lvl->coef, c, sizeof(double) * ncells,
                               hipMemcpyDeviceToDevice, lvl->ctx->stream));
+    // the smoother's per-patch separable fit, from the level-precision
+    // coefficient the operator uses
+    const long long np = (long long)(lvl->n - 1) * (lvl->n - 1) * (lvl->n - 1);
+    if (np > 0)
+      {
+        if (!lvl->sep_bins)
+          {
+            lvl->sep_bins  = (int *)dmalloc(sizeof(int) * np);
+            lvl->sep_sigma = dmalloc(lvl->elem() * np);
+          }
+        launch_separable_fit(lvl);
+      }
   }
 
   void
@@ -203,6 +215,10 @@ namespace
         GTPS_HIP(hipStreamSynchronize(lvl->ctx->stream));
         dfree(lvl->coef);
         lvl->coef = nullptr;
+        dfree(lvl->sep_bins);
+        dfree(lvl->sep_sigma);
+        lvl->sep_bins  = nullptr;
+        lvl->sep_sigma = nullptr;
       }
   }
 

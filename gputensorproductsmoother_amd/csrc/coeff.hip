@@ -284,12 +284,66 @@ namespace gtps
       return (n - parity) / 2;
     }
 
+    // One thread per vertex patch (lower-left cell (px, py, pz), patches
+    // lexicographic over (n-1)^3): the log-space least-squares ratios of the
+    // 8 cell coefficients per direction, rounded to the SEP_STEP grid, and
+    // sigma = max(max_K c_K/f_K / SEP_THETA, exp(mean log(c_K/f_K))), in
+    // FP64 from the level-precision coefficient.
+    template <typename T>
+    __global__ void __launch_bounds__(CNT)
+    separable_fit_kernel(const T *__restrict__ coef, const int n,
+                         int *__restrict__ bins, T *__restrict__ sigma)
+    {
+      const int       m  = n - 1;
+      const long long np = (long long)m * m * m;
+      for (long long q = (long long)blockIdx.x * CNT + threadIdx.x; q < np;
+           q += (long long)gridDim.x * CNT)
+        {
+          const int px = (int)(q % m), py = (int)((q / m) % m), pz = (int)(q / ((long long)m * m));
+          double    lc[8], lf[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            {
+              lc[c] = log((double)coef[(px + (c & 1)) + (long long)n * (py + ((c >> 1) & 1)) +
+                                       (long long)n * n * (pz + (c >> 2))]);
+              lf[c] = 0.0;
+            }
+          int packed = 0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            {
+              double d = 0.0;
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                d += ((c >> a) & 1) ? lc[c] : -lc[c];
+              const double qq = 0.25 * d / SEP_STEP;
+              int          bq = (int)(qq >= 0.0 ? floor(qq + 0.5) : -floor(-qq + 0.5));
+              bq              = bq < -SEP_HALF ? -SEP_HALF : (bq > SEP_HALF ? SEP_HALF : bq);
+              packed |= (bq + SEP_HALF) << (8 * a);
+#pragma unroll
+              for (int c = 0; c < 8; ++c)
+                if ((c >> a) & 1)
+                  lf[c] += bq * SEP_STEP;
+            }
+          double rmax = lc[0] - lf[0], rsum = 0.0;
+#pragma unroll
+          for (int c = 0; c < 8; ++c)
+            {
+              rmax = fmax(rmax, lc[c] - lf[c]);
+              rsum += lc[c] - lf[c];
+            }
+          bins[q]  = packed;
+          sigma[q] = (T)fmax(exp(rmax) / SEP_THETA, exp(0.125 * rsum));
+        }
+    }
+
     // one vertex patch per work-group
     template <int K, typename T>
     __global__ void __launch_bounds__(CNT)
     patch_coeff_kernel(T *__restrict__ u, const T *__restrict__ bvec,
                        const T *__restrict__ coef, const T *__restrict__ tab,
-                       const T *__restrict__ sep, const int n, const int colour,
+                       const T *__restrict__ sep, const int *__restrict__ fit_bins,
+                       const T *__restrict__ fit_sigma, const int n, const int colour,
                        const T scale, const T relax)
     {
       using TB         = Tab<K>;
@@ -349,37 +403,13 @@ namespace gtps
       __syncthreads();
       if (tid == 0)
         {
-          // separable fit of the 8 cell coefficients (FP64): per-direction
-          // log ratios on the SEP_STEP grid, then the scale sigma
-          double lc[8];
-#pragma unroll
-          for (int c = 0; c < 8; ++c)
-            lc[c] = log((double)sc[c]);
-          double lf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-          for (int a = 0; a < 3; ++a)
-            {
-              double d = 0.0;
-#pragma unroll
-              for (int c = 0; c < 8; ++c)
-                d += ((c >> a) & 1) ? lc[c] : -lc[c];
-              const double q  = 0.25 * d / SEP_STEP;
-              int          bq = (int)(q >= 0.0 ? floor(q + 0.5) : -floor(-q + 0.5));
-              bq              = bq < -SEP_HALF ? -SEP_HALF : (bq > SEP_HALF ? SEP_HALF : bq);
-              sbin[a]         = bq;
-#pragma unroll
-              for (int c = 0; c < 8; ++c)
-                if ((c >> a) & 1)
-                  lf[c] += bq * SEP_STEP;
-            }
-          double rmax = lc[0] - lf[0], rsum = 0.0;
-#pragma unroll
-          for (int c = 0; c < 8; ++c)
-            {
-              rmax = fmax(rmax, lc[c] - lf[c]);
-              rsum += lc[c] - lf[c];
-            }
-          ssig = (T)fmax(exp(rmax) / SEP_THETA, exp(0.125 * rsum));
+          // this patch's separable fit (computed with the coefficient)
+          const long long pq = c0x + (long long)(n - 1) * (c0y + (long long)(n - 1) * c0z);
+          const int       pk = fit_bins[pq];
+          sbin[0]            = (pk & 255) - SEP_HALF;
+          sbin[1]            = ((pk >> 8) & 255) - SEP_HALF;
+          sbin[2]            = ((pk >> 16) & 255) - SEP_HALF;
+          ssig               = fit_sigma[pq];
         }
       // cell z-rows: task (cell, ix, iy)
       for (int t = tid; t < 8 * N2; t += CNT)
@@ -563,7 +593,11 @@ namespace gtps
 #pragma unroll
               for (int l = 0; l < NI; ++l)
                 s += Sz[l * NI + ll] * v[l];
-              w[ll] = s * inv / (lxy + Sz[NL + ll]);
+              // FP32: hardware reciprocal (1 ulp); FP64: exact division
+              if constexpr (sizeof(T) == 4)
+                w[ll] = s * inv * __builtin_amdgcn_rcpf(lxy + Sz[NL + ll]);
+              else
+                w[ll] = s * inv / (lxy + Sz[NL + ll]);
             }
 #pragma unroll
           for (int l = 0; l < NI; ++l)
@@ -711,10 +745,12 @@ namespace gtps
           if (np == 0)
             return;
           GTPS_REQUIRE(np < (1LL << 31), "mesh too large for the patch kernel");
+          GTPS_REQUIRE(lvl->sep_bins && lvl->sep_sigma, "level has no separable fit");
           const T *sep = (const T *)get_sep_tables(lvl->ctx, K, lvl->prec);
           hipLaunchKernelGGL((patch_coeff_kernel<K, T>), dim3((unsigned)np),
                              dim3(CNT), 0, lvl->ctx->stream, u, b,
-                             (const T *)lvl->coef, (const T *)lvl->dtab(), sep, n,
+                             (const T *)lvl->coef, (const T *)lvl->dtab(), sep,
+                             (const int *)lvl->sep_bins, (const T *)lvl->sep_sigma, n,
                              colour, (T)lvl->scale, (T)relax);
           GTPS_LAUNCH_CHECK();
         }
@@ -799,6 +835,25 @@ namespace gtps
                                              colour, relax);
         return 0;
       });
+  }
+
+  void
+  launch_separable_fit(const gtps_level *lvl)
+  {
+    const long long m  = lvl->n - 1;
+    const long long np = m * m * m;
+    if (np <= 0)
+      return;
+    const unsigned grid = (unsigned)std::min<long long>((np + CNT - 1) / CNT, 4096);
+    if (lvl->prec == GTPS_FP32)
+      hipLaunchKernelGGL(separable_fit_kernel<float>, dim3(grid), dim3(CNT), 0,
+                         lvl->ctx->stream, (const float *)lvl->coef, lvl->n, lvl->sep_bins,
+                         (float *)lvl->sep_sigma);
+    else
+      hipLaunchKernelGGL(separable_fit_kernel<double>, dim3(grid), dim3(CNT), 0,
+                         lvl->ctx->stream, (const double *)lvl->coef, lvl->n, lvl->sep_bins,
+                         (double *)lvl->sep_sigma);
+    GTPS_LAUNCH_CHECK();
   }
 
   void

@@ -122,12 +122,22 @@ struct gtps_level
   // operator; both null for the constant-coefficient Laplacian
   void                    *coef  = nullptr;
   void                    *celem = nullptr;
+  // per vertex patch (lexicographic over (n-1)^3): the separable fit of the
+  // variable-coefficient smoother, ratio bins packed as (bx + SEP_HALF) |
+  // (by + SEP_HALF) << 8 | (bz + SEP_HALF) << 16, and the scale sigma (level
+  // precision); set with the coefficient
+  int                     *sep_bins  = nullptr;
+  void                    *sep_sigma = nullptr;
   ~gtps_level()
   {
     if (coef)
       (void)hipFree(coef);
     if (celem)
       (void)hipFree(celem);
+    if (sep_bins)
+      (void)hipFree(sep_bins);
+    if (sep_sigma)
+      (void)hipFree(sep_sigma);
   }
   const void *
   dtab() const
@@ -256,6 +266,9 @@ namespace gtps
   // one colour of the inexact-FD multiplicative smoother (lvl->coef set)
   void launch_patch_coeff_colour(const gtps_level *lvl, void *u,
                                  const void *b, int colour, double relax);
+  // the per-patch separable fit of the level's coefficient into
+  // lvl->sep_bins / lvl->sep_sigma (allocated by the caller)
+  void launch_separable_fit(const gtps_level *lvl);
   // coarse[c] = mean of the 8 children (FP64), nc coarse cells per direction
   void launch_coarsen_coefficient(hipStream_t s, int nc, const double *fine,
                                   double *coarse);
