@@ -403,6 +403,97 @@ def c4_report(ctx, core, problems, reps, solve_pair, rel_l2):
                          "max_rel_dev_adjoint_vs_fd": dev_max}}
 
 
+def solve_trace(args):
+    """Workload for a kernel trace: the configuration's FP32-V-cycle solve
+    (C5: also the batch of 4 right-hand sides) three times, a 1 s pause,
+    then --steps timed solves (one pause before each further group), so
+    that --trace-summary can cut the timed dispatches out of the trace."""
+    import torch
+    from gputensorproductsmoother_amd import core, problems
+    ctx = core.Context(0)
+    dev = "cuda:0"
+    name = args.solve_trace.lower()
+
+    def cell_mass(k):
+        op = core.LaplaceOperator(ctx, 3, k, 1, torch.float64)
+        return op.tensor_product_data()["cell_mass"]
+
+    groups = []
+    if name in ("c2", "c3", "c4"):
+        k, level = {"c2": (3, 6), "c3": (7, 5), "c4": (4, 5)}[name]
+        if name == "c2":
+            b = torch.tensor(problems.rhs_random(3, k, level), device=dev)
+        else:
+            b = torch.tensor(problems.rhs_constant(3, k, level, cell_mass(k)), device=dev)
+        mg = core.MultigridSolvers(ctx, 3, k, level, torch.float32)
+        if name == "c4":
+            mg.set_coefficient(torch.tensor(problems.coeff_problem(3, k, level)[0], device=dev))
+        x = torch.empty_like(b)
+        groups.append((f"{name} fp32 solve", lambda: mg.solve_cg(b, x, rtol=1e-8)))
+    elif name == "c5":
+        k, n0, maxl = 4, 3, 3
+        B = torch.tensor(problems.rhs_dg_random(3, k, n0 << maxl, batch=4), device=dev)
+        mg = core.DGMultigrid(ctx, 3, k, n0, maxl, torch.float32)
+        b = B[0].contiguous()
+        x = torch.empty_like(b)
+        X = torch.empty_like(B)
+        groups.append(("c5 fp32 solve", lambda: mg.solve_cg(b, x, rtol=1e-8)))
+        groups.append(("c5 fp32 batch-4 solve", lambda: mg.solve_cg_batch(B, X, rtol=1e-8)))
+    else:
+        raise SystemExit(f"unknown config {name}")
+    for label, fn in groups:
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        time.sleep(1.0)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        print(json.dumps({"trace_group": label, "calls": args.steps,
+                          "ms_per_call": round((time.perf_counter() - t0) * 1e3 / args.steps, 3)}),
+              flush=True)
+        time.sleep(1.0)
+
+
+def trace_summary(csv_path: str, calls: int, top: int = 30):
+    """Per-call kernel table of the timed groups of a --solve-trace run:
+    the rocprofv3 kernel-trace CSV is cut at gaps over 0.5 s, the segments
+    after the first (setup and warm-up) are reported, kernels grouped by
+    name and grid size, times divided by the number of calls."""
+    import collections
+    import csv
+    rows = []
+    for r in csv.DictReader(open(csv_path)):
+        name = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "")
+        name = name.replace("gtps::", "").split("(")[0]
+        grid = 1
+        for ax in ("X", "Y", "Z"):
+            grid *= int(r.get(f"Grid_Size_{ax}") or 1)
+        rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name, grid))
+    rows.sort()
+    segs, cur = [], [rows[0]]
+    for a, b in zip(rows, rows[1:]):
+        if b[0] - a[1] > 5e8:
+            segs.append(cur)
+            cur = []
+        cur.append(b)
+    segs.append(cur)
+    for i, seg in enumerate(segs[1:], start=1):
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for s0, e0, n, g in seg:
+            agg[(n, g)][0] += 1
+            agg[(n, g)][1] += (e0 - s0) / 1e3
+        busy = sum(v[1] for v in agg.values())
+        span = (seg[-1][1] - seg[0][0]) / 1e3
+        print(f"segment {i}: {len(seg) / calls:.1f} dispatches per call, kernel time "
+              f"{busy / calls:.1f} us per call, span {span / calls:.1f} us per call")
+        print("| us per call | launches per call | us per launch | grid | kernel |")
+        print("|---:|---:|---:|---:|---|")
+        for (n, g), (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+            print(f"| {t / calls:.1f} | {c / calls:.1f} | {t / c:.2f} | {g} | `{n}` |")
+
+
 def traffic_summary(fetch_dir: str, write_dir: str, kernel: str, workload, out_path: str):
     """Turn two rocprofv3 --pmc runs of `bench.py --kernels` (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch) into HBM bytes per launch of `kernel`:
@@ -630,7 +721,17 @@ def main():
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs of "
                          "--kernels (no GPU work)")
     ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2>")
+    ap.add_argument("--solve-trace", metavar="c2|c3|c4|c5",
+                    help="run a configuration's FP32 solve for a kernel trace (warm-up, "
+                         "1 s pause, --steps timed solves)")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="per-call kernel table of a --solve-trace rocprofv3 kernel trace "
+                         "(--steps = calls per group; no GPU work)")
     args = ap.parse_args()
+
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.steps)
+        return
 
     if args.traffic_summary:
         traffic_summary(args.traffic_summary[0], args.traffic_summary[1], args.traffic_kernel,
@@ -652,6 +753,9 @@ def main():
 
     if args.kernels:
         kernel_bench(args)
+        return
+    if args.solve_trace:
+        solve_trace(args)
         return
     if args.configs:
         config_report(args)
