@@ -904,6 +904,10 @@ struct gtps_dg_mg
   hipStream_t                   cap_stream = nullptr;
   hipGraphExec_t                dir_exec   = nullptr;
   unsigned long long            dir_gen    = 0;  // tuning generation at capture
+  // the batched V-cycle as a graph, for batches of vb_nb rows
+  hipGraphExec_t                vb_exec    = nullptr;
+  int                           vb_nb      = 0;
+  unsigned long long            vb_gen     = 0;
   // the coarsest solve (symmetric sweep pairs from zero, a fixed linear
   // map) tabulated as a dense matrix on first use (coarse.hip)
   void                         *dense0       = nullptr;
@@ -926,6 +930,11 @@ struct gtps_dg_mg
       {
         (void)hipGraphExecDestroy(dir_exec);
         dir_exec = nullptr;
+      }
+    if (vb_exec)
+      {
+        (void)hipGraphExecDestroy(vb_exec);
+        vb_exec = nullptr;
       }
     const int    nc = std::max(nb, bcap);
     const size_t es = prec == GTPS_FP32 ? sizeof(float) : sizeof(double);
@@ -986,20 +995,80 @@ struct gtps_dg_mg
     launch_dg_sweep(L, ul, dl, GTPS_MULT_REV, relax, nb);
   }
 
-  // z_j = V(r_j); per vector: rz_old = rz; rz = r.z; beta; p = z + beta p
+  // the batched V-cycle z_j = V(r_j) of every row
   void
+  enqueue_vcycle_batch(int nb)
+  {
+    if (prec == GTPS_FP32)
+      vcycle_batch(maxl, d[maxl], u[maxl], nb);
+    else
+      vcycle_batch(maxl, r, u[maxl], nb);
+  }
+
+  // the batched V-cycle replayed as one HIP graph per batch size (it uses
+  // only the hierarchy's own buffers); captured again when the batch size or
+  // the tuning knobs change, dropped when the buffers are reallocated
+  void
+  run_vcycle_batch(int nb)
+  {
+    if (!use_graph)
+      {
+        enqueue_vcycle_batch(nb);
+        return;
+      }
+    if (vb_exec && (vb_nb != nb || vb_gen != tuning().generation))
+      {
+        GTPS_HIP(hipStreamSynchronize(ctx->stream));
+        (void)hipGraphExecDestroy(vb_exec);
+        vb_exec = nullptr;
+      }
+    if (!vb_exec)
+      {
+        if (!cap_stream)
+          GTPS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+        hipStream_t saved = ctx->stream;
+        ctx->stream       = cap_stream;
+        hipGraph_t graph  = nullptr;
+        try
+          {
+            GTPS_HIP(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+            enqueue_vcycle_batch(nb);
+            GTPS_HIP(hipStreamEndCapture(cap_stream, &graph));
+          }
+        catch (...)
+          {
+            ctx->stream = saved;
+            if (!graph)
+              {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cap_stream, &g);
+                if (g)
+                  (void)hipGraphDestroy(g);
+              }
+            else
+              (void)hipGraphDestroy(graph);
+            throw;
+          }
+        ctx->stream  = saved;
+        hipError_t e = hipGraphInstantiate(&vb_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        GTPS_HIP(e);
+        vb_nb  = nb;
+        vb_gen = tuning().generation;
+      }
+    GTPS_HIP(hipGraphLaunch(vb_exec, ctx->stream));
+  }
+
   // z = V(r) for the whole batch, then rz, beta and p = z + beta p for the
   // rows still iterating (itc[j] >= 0: converged, left frozen)
+  void
   direction_batch(int nb, const std::vector<int> &itc)
   {
     hipStream_t     s   = ctx->stream;
     const long long N   = fine64->N;
     const bool      f32 = prec == GTPS_FP32;
     double         *pt  = ctx->ws.partials;
-    if (f32)
-      vcycle_batch(maxl, d[maxl], u[maxl], nb);
-    else
-      vcycle_batch(maxl, r, u[maxl], nb);
+    run_vcycle_batch(nb);
     for (int j = 0; j < nb; ++j)
       {
         if (itc[j] >= 0)
@@ -1175,6 +1244,8 @@ struct gtps_dg_mg
       (void)hipHostFree(bpinned);
     if (dir_exec)
       (void)hipGraphExecDestroy(dir_exec);
+    if (vb_exec)
+      (void)hipGraphExecDestroy(vb_exec);
     if (cap_stream)
       (void)hipStreamDestroy(cap_stream);
     for (size_t l = 0; l < lv.size(); ++l)
