@@ -86,7 +86,10 @@ def dominant_kernel_name(core, dim: int, k: int, level: int) -> str:
         npw = 1
         if patches > core.get_tuning("group_min_patches"):
             npw = core.get_tuning("patches_per_wave")
-        return f"patch3d_group_kernel<{k}, float, 0, {npw}>"
+        waves = core.get_tuning("group_waves") if npw <= 2 else 1
+        if waves == 0:
+            waves = 4  # automatic: 4 wavefronts per work-group in FP32
+        return f"patch3d_group_kernel<{k}, float, 0, {npw}, {waves}>"
     return f"patch3d_kernel<{k}, float, false>"
 
 
@@ -720,7 +723,7 @@ def main():
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs of "
                          "--kernels (no GPU work)")
-    ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2>")
+    ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2, 1>")
     ap.add_argument("--solve-trace", metavar="c2|c3|c4|c5",
                     help="run a configuration's FP32 solve for a kernel trace (warm-up, "
                          "1 s pause, --steps timed solves)")

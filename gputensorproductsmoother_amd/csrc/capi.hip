@@ -313,11 +313,13 @@ struct gtps_mg : LaunchTimer
   bool                       own_tr64 = false;
 
   // the per-iteration preconditioning sequence (V-cycle, r.z, beta,
-  // p update) replayed as one HIP graph; captured lazily on a private stream
+  // deferred x update, p update) replayed as one HIP graph; captured lazily
+  // on a private stream, again when the solution vector changes
   bool                       use_graph  = true;
   hipStream_t                cap_stream = nullptr;
   hipGraphExec_t             dir_exec   = nullptr;
   unsigned long long         dir_gen    = 0;  // tuning generation at capture
+  double                    *dir_x      = nullptr;  // solution at capture
 
   // the V-cycle from level 2 down as a dense matrix (coarse.hip), built
   // on first use and dropped when the operators change
@@ -457,10 +459,12 @@ struct gtps_mg : LaunchTimer
     toc(GTPS_PROF_SMOOTH_COARSE, e);
   }
 
-  // z = V(r), rz_old = rz, rz = r.z, beta = rz / rz_old, p = z + beta p;
-  // every buffer is owned by the hierarchy, so the sequence can be captured
+  // z = V(r), rz_old = rz, rz = r.z, beta = rz / rz_old, x += alpha p (the
+  // update of the last step, deferred so that p is read once), p = z +
+  // beta p; every other buffer is owned by the hierarchy, so the sequence
+  // can be captured
   void
-  enqueue_direction()
+  enqueue_direction(double *x)
   {
     hipStream_t     s   = ctx->stream;
     double         *sc  = ctx->ws.scalars;
@@ -477,15 +481,16 @@ struct gtps_mg : LaunchTimer
     toc(GTPS_PROF_VCYCLE, e);
     const int nb = launch_dot_partials(s, N, r, z64, z32, pt, ctx->ws.capacity);
     launch_finalize_op(s, pt, nb, sc, cg::RZ, cg::OP_SHIFT_BETA);
-    launch_cg_update_p(s, N, p, z64, z32, sc);
+    launch_cg_update_p(s, N, p, z64, z32, sc, x);
   }
 
   // a graph captured before the last gtps_set_tuning may replay other
-  // kernels than an eager launch would: capture it again
+  // kernels than an eager launch would, and one captured for another
+  // solution vector writes there: capture it again
   void
-  drop_stale_graph()
+  drop_stale_graph(const double *x)
   {
-    if (dir_exec && dir_gen != tuning().generation)
+    if (dir_exec && (dir_gen != tuning().generation || dir_x != x))
       {
         GTPS_HIP(hipStreamSynchronize(ctx->stream));
         (void)hipGraphExecDestroy(dir_exec);
@@ -494,7 +499,7 @@ struct gtps_mg : LaunchTimer
   }
 
   void
-  capture_direction()
+  capture_direction(double *x)
   {
     if (!cap_stream)
       GTPS_HIP(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
@@ -505,7 +510,7 @@ struct gtps_mg : LaunchTimer
       {
         GTPS_HIP(hipStreamBeginCapture(cap_stream,
                                        hipStreamCaptureModeThreadLocal));
-        enqueue_direction();
+        enqueue_direction(x);
         GTPS_HIP(hipStreamEndCapture(cap_stream, &graph));
       }
     catch (...)
@@ -527,21 +532,23 @@ struct gtps_mg : LaunchTimer
     (void)hipGraphDestroy(graph);
     GTPS_HIP(e);
     dir_gen = tuning().generation;
+    dir_x   = x;
   }
 
+  // x: the solution, which receives the deferred x += alpha p
   void
-  direction()
+  direction(double *x)
   {
     ensure_dense();
     // stage timers need eager launches
     if (!use_graph || prof_mask != 0)
       {
-        enqueue_direction();
+        enqueue_direction(x);
         return;
       }
-    drop_stale_graph();
+    drop_stale_graph(x);
     if (!dir_exec)
-      capture_direction();
+      capture_direction(x);
     GTPS_HIP(hipGraphLaunch(dir_exec, ctx->stream));
   }
 
@@ -1851,6 +1858,12 @@ extern "C"
           prev                         = tuning().patches_per_wave;
           tuning().patches_per_wave    = (int)value;
         }
+      else if (k == "group_waves")
+        {
+          GTPS_REQUIRE(value == 0 || value == 1 || value == 4, "group_waves must be 0, 1 or 4");
+          prev                  = tuning().group_waves;
+          tuning().group_waves  = (int)value;
+        }
       else if (k == "box_vmult_max_n")
         {
           GTPS_REQUIRE(value >= 0, "value must be >= 0");
@@ -1907,6 +1920,8 @@ extern "C"
         v = tuning().group_min_patches;
       else if (k == "patches_per_wave")
         v = tuning().patches_per_wave;
+      else if (k == "group_waves")
+        v = tuning().group_waves;
       else if (k == "box_vmult_max_n")
         v = tuning().box_vmult_max_n;
       else if (k == "row_staged")
@@ -2138,9 +2153,10 @@ extern "C"
           read_rr(launch_dot_partials(s, N, mg->r, mg->r, nullptr, pt, cap));
         }
       // the first preconditioning step is enqueued before the host learns
-      // r0 (a zero or non-finite r0 discards it)
+      // r0 (a zero or non-finite r0 discards it; alpha = 0 here, so its x
+      // update leaves x as it is)
       if (maxit > 0)
-        mg->direction();
+        mg->direction(x);
       const double r0 = wait_rr();
       if (history)
         history[0] = r0;
@@ -2157,6 +2173,9 @@ extern "C"
           return;
         }
       double rn = r0, rn_prev = -1.0;
+      // x += alpha p of the last step not yet enqueued (the next direction
+      // step applies it)
+      bool x_pending = false;
       while (it_done < maxit)
         {
           // q = A p with fused p.q, alpha = rz / pq
@@ -2164,7 +2183,8 @@ extern "C"
           const int nbq = launch_laplace(mg->fine64, mg->q, mg->p, nullptr, true);
           mg->toc(GTPS_PROF_CG_MATVEC, e);
           launch_finalize_op(s, pt, nbq, sc, cg::PQ, cg::OP_ALPHA);
-          read_rr(launch_cg_update_xr(s, N, x, mg->r, mg->p, mg->q, r32, sc, pt, cap));
+          read_rr(launch_cg_update_xr(s, N, nullptr, mg->r, mg->p, mg->q, r32, sc, pt, cap));
+          x_pending = true;
           ++it_done;
           // Enqueue the next preconditioning step while the host checks,
           // unless the residual history predicts that this iteration
@@ -2175,7 +2195,10 @@ extern "C"
           if (speculate && rn_prev > 0.0 && rn < rn_prev)
             speculate = rn * (rn / rn_prev) > 0.25 * rtol * r0;
           if (speculate)
-            mg->direction();
+            {
+              mg->direction(x);
+              x_pending = false;
+            }
           rn_prev = rn;
           rn      = wait_rr();
           if (history)
@@ -2191,8 +2214,13 @@ extern "C"
               break;
             }
           if (!speculate && it_done < maxit)
-            mg->direction();
+            {
+              mg->direction(x);
+              x_pending = false;
+            }
         }
+      if (x_pending)
+        launch_cg_add_alpha_p(s, N, x, mg->p, sc);
       rel = rn / r0;
       GTPS_HIP(hipStreamSynchronize(s));
       mg->collect();

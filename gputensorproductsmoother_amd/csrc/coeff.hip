@@ -337,7 +337,16 @@ namespace gtps
         }
     }
 
-    // one vertex patch per work-group
+    // One vertex patch per work-group.  Only the residual at the patch's
+    // interior DoFs is needed, and along each direction those are K of a
+    // cell's K+1 local nodes: 1..K in the lower cell, 0..K-1 in the upper
+    // one.  Each cell pass therefore forms K outputs per line, the y pass
+    // runs only over the interior z outputs and the x pass over the
+    // interior (y, z) ones.  The lines of a pass are split into two halves
+    // of 128 threads by the cell offset along the pass direction, so that
+    // offset, and with it the coefficient row, is uniform per wavefront
+    // (scalar loads).  The patch's fit (ratio bins, scale) is read with the
+    // first loads.
     template <int K, typename T>
     __global__ void __launch_bounds__(CNT)
     patch_coeff_kernel(T *__restrict__ u, const T *__restrict__ bvec,
@@ -349,12 +358,11 @@ namespace gtps
       using TB         = Tab<K>;
       using C          = PC<K, T>;
       constexpr int NN = C::NN, N2 = C::N2, N3 = C::N3, NP = C::NP, NI = C::NI;
+      constexpr int NL = NI * NI, SW = SEP_W(K), HALF = CNT / 2;
       __shared__ T sbox[NP * NP * NP];
       __shared__ T sa[8 * N3], sb[8 * N3];
       __shared__ T sr[NI * NI * NI];
       __shared__ T sc[8];
-      __shared__ int sbin[3];
-      __shared__ T   ssig;
       const T     *cM   = tab + TB::cellM;
       const T     *cK   = tab + TB::cellK;
       const int    tid  = threadIdx.x;
@@ -367,6 +375,11 @@ namespace gtps
       const int c0x = 2 * qx + px, c0y = 2 * qy + py, c0z = 2 * qz + pz;
       const long long first = (long long)K * c0x + (long long)K * c0y * N1 +
                               (long long)K * c0z * P;
+      // this patch's separable fit (computed with the coefficient): ratio
+      // bins and scale, uniform over the work-group
+      const long long pq  = c0x + (long long)(n - 1) * (c0y + (long long)(n - 1) * c0z);
+      const int       pk  = fit_bins[pq];
+      const T         sig = fit_sigma[pq];
       if (tid < 8)
         {
           const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
@@ -401,30 +414,23 @@ namespace gtps
             sbox[tid + it * CNT] = vu[it];
       }
       __syncthreads();
-      if (tid == 0)
+      const int h  = __builtin_amdgcn_readfirstlane(tid / HALF);  // wave-uniform
+      const int hr = tid - h * HALF;
+      // cell z-rows: task (cell with z offset h, ix, iy), z outputs lz = m + 1 - h
+      for (int t = hr; t < 4 * N2; t += HALF)
         {
-          // this patch's separable fit (computed with the coefficient)
-          const long long pq = c0x + (long long)(n - 1) * (c0y + (long long)(n - 1) * c0z);
-          const int       pk = fit_bins[pq];
-          sbin[0]            = (pk & 255) - SEP_HALF;
-          sbin[1]            = ((pk >> 8) & 255) - SEP_HALF;
-          sbin[2]            = ((pk >> 16) & 255) - SEP_HALF;
-          ssig               = fit_sigma[pq];
-        }
-      // cell z-rows: task (cell, ix, iy)
-      for (int t = tid; t < 8 * N2; t += CNT)
-        {
-          const int c = t / N2, r = t - c * N2;
-          const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+          const int cl = t / N2, r = t - cl * N2;
+          const int ox = cl & 1, oy = cl >> 1, c = cl + 4 * h;
           const int ix = r % NN, iy = r / NN;
           T         v[NN];
 #pragma unroll
           for (int iz = 0; iz < NN; ++iz)
-            v[iz] = sbox[((oz * K + iz) * NP + oy * K + iy) * NP + ox * K + ix];
+            v[iz] = sbox[((h * K + iz) * NP + oy * K + iy) * NP + ox * K + ix];
 #pragma unroll
-          for (int lz = 0; lz < NN; ++lz)
+          for (int m = 0; m < K; ++m)
             {
-              T a = 0, b = 0;
+              const int lz = m + 1 - h;
+              T         a = 0, b = 0;
 #pragma unroll
               for (int iz = 0; iz < NN; ++iz)
                 {
@@ -436,11 +442,13 @@ namespace gtps
             }
         }
       __syncthreads();
-      // cell y-rows in place: task (cell, ix, lz)
-      for (int t = tid; t < 8 * N2; t += CNT)
+      // cell y-rows in place: task (cell with y offset h, ix, interior lz),
+      // y outputs ly = m + 1 - h
+      for (int t = hr; t < 4 * NN * K; t += HALF)
         {
-          const int c = t / N2, r = t - c * N2;
-          const int ix = r % NN, lz = r / NN;
+          const int cl = t / (NN * K), r = t - cl * (NN * K);
+          const int ox = cl & 1, oz = cl >> 1, c = ox + 2 * h + 4 * oz;
+          const int ix = r % NN, lz = r / NN + 1 - oz;
           T        *ca = sa + c * N3, *cb = sb + c * N3;
           T         a[NN], b[NN];
 #pragma unroll
@@ -450,9 +458,10 @@ namespace gtps
               b[iy] = cb[(lz * NN + iy) * NN + ix];
             }
 #pragma unroll
-          for (int ly = 0; ly < NN; ++ly)
+          for (int m = 0; m < K; ++m)
             {
-              T c1 = 0, c2 = 0;
+              const int ly = m + 1 - h;
+              T         c1 = 0, c2 = 0;
 #pragma unroll
               for (int iy = 0; iy < NN; ++iy)
                 {
@@ -464,11 +473,13 @@ namespace gtps
             }
         }
       __syncthreads();
-      // cell x-rows in place into sa: task (cell, ly, lz)
-      for (int t = tid; t < 8 * N2; t += CNT)
+      // cell x-rows in place into sa: task (cell with x offset h, interior
+      // ly, lz), x outputs lx = m + 1 - h, times the cell coefficient
+      for (int t = hr; t < 4 * K * K; t += HALF)
         {
-          const int c = t / N2, r = t - c * N2;
-          const int ly = r % NN, lz = r / NN;
+          const int cl = t / (K * K), r = t - cl * (K * K);
+          const int oy = cl & 1, oz = cl >> 1, c = h + 2 * oy + 4 * oz;
+          const int ly = r % K + 1 - oy, lz = r / K + 1 - oz;
           T        *ca = sa + c * N3, *cb = sb + c * N3;
           T         c1[NN], c2[NN];
 #pragma unroll
@@ -479,9 +490,10 @@ namespace gtps
             }
           const T f = sc[c] * scale;
 #pragma unroll
-          for (int lx = 0; lx < NN; ++lx)
+          for (int m = 0; m < K; ++m)
             {
-              T y = 0;
+              const int lx = m + 1 - h;
+              T         y  = 0;
 #pragma unroll
               for (int ix = 0; ix < NN; ++ix)
                 y += cK[lx * NN + ix] * c1[ix] + cM[lx * NN + ix] * c2[ix];
@@ -536,11 +548,10 @@ namespace gtps
       __syncthreads();
       // fast diagonalisation with the per-direction eigenpairs of the fit:
       // S_x^T, S_y^T, S_z^T; 1/(sigma scale (lx_i + ly_j + lz_l)); S_z, S_y,
-      // S_x (bins are uniform over the work-group: scalar table loads)
-      constexpr int NL = NI * NI, SW = SEP_W(K);
-      const T      *Sx = sep + (__builtin_amdgcn_readfirstlane(sbin[0]) + SEP_HALF) * SW;
-      const T      *Sy = sep + (__builtin_amdgcn_readfirstlane(sbin[1]) + SEP_HALF) * SW;
-      const T      *Sz = sep + (__builtin_amdgcn_readfirstlane(sbin[2]) + SEP_HALF) * SW;
+      // S_x (bins uniform over the work-group: scalar table loads)
+      const T *Sx = sep + (pk & 255) * SW;
+      const T *Sy = sep + ((pk >> 8) & 255) * SW;
+      const T *Sz = sep + ((pk >> 16) & 255) * SW;
       for (int t = tid; t < NL; t += CNT)  // x-lines (l, j)
         {
           T v[NI];
@@ -577,7 +588,7 @@ namespace gtps
             }
         }
       __syncthreads();
-      const T inv = T(1) / (scale * ssig);
+      const T inv = T(1) / (scale * sig);
       for (int t = tid; t < NL; t += CNT)  // z-lines (j, i)
         {
           T      *base = sr + t;

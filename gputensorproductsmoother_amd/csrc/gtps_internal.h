@@ -209,6 +209,12 @@ namespace gtps
     // pack patches_per_wave (1, 2 or 4) patches into each wavefront
     long long group_min_patches = 2048;
     int       patches_per_wave  = 2;
+    // patch-group kernel: wavefronts per work-group (1 or 4; 4 applies to 1
+    // or 2 patches per wavefront, each wavefront with its own patches; 0:
+    // 4 in FP32, 1 in FP64).  Measured on MI355X at 64^3 Q3, level-6 colour
+    // (HIP events): FP32 21.0 us with 4 against 21.4-23.1 us with 1 (fewer,
+    // larger work-groups to dispatch), FP64 34.7 against 32.8-34.2 us.
+    int       group_waves = 0;
     // 3D operator (k <= 4): levels with at most this many cells per
     // direction use the one-round box kernel instead of z-marching
     long long box_vmult_max_n = 16;
@@ -397,9 +403,13 @@ namespace gtps
                            const double *p, const double *q, float *r32,
                            const double *scalars, double *partials,
                            int capacity);
+  // x == nullptr in launch_cg_update_xr defers x += alpha p to the next
+  // launch_cg_update_p with x (or to launch_cg_add_alpha_p)
   void launch_cg_update_p(hipStream_t s, long long n, double *p,
                           const double *z, const float *z32,
-                          const double *scalars);
+                          const double *scalars, double *x = nullptr);
+  void launch_cg_add_alpha_p(hipStream_t s, long long n, double *x,
+                             const double *p, const double *scalars);
 
   namespace cg
   {

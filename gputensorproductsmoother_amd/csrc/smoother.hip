@@ -981,8 +981,28 @@ namespace gtps
       static_assert(NI * NI * NI <= YOFF, "FD buffer overlaps the y-rows");
     };
 
-    template <int K, typename T, int MODE, int NPW>
-    __global__ void __launch_bounds__(64)
+    // Stage boundary of the patch-group kernel.  Every wavefront owns its
+    // own LDS region, so only its own lanes' LDS stores have to be seen: a
+    // wavefront's LDS operations are performed in issue order, and the
+    // fences keep the compiler from moving them across the boundary.
+    template <int W>
+    __device__ __forceinline__ void
+    group_sync()
+    {
+      if constexpr (W == 1)
+        __syncthreads();
+      else
+        {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+
+    // W wavefronts per work-group, each with its own NPW patches and LDS
+    // region.
+    template <int K, typename T, int MODE, int NPW, int W>
+    __global__ void __launch_bounds__(64 * W)
     patch3d_group_kernel(T *__restrict__ u, const T *__restrict__ src,
                          const T *__restrict__ tab, const int n,
                          const int colour, const T scale, const T inv_scale,
@@ -994,14 +1014,16 @@ namespace gtps
       static_assert(NP * NP <= 64, "one lane per z-pencil");
       constexpr int P2 = (NPW * NI * NP + 63) / 64;  // passes per stage
       constexpr int P3 = (NPW * NI * NI + 63) / 64;
-      __shared__ T      sm[NPW * PER];
+      __shared__ T      sm_all[W * NPW * PER];
       const int       N1   = K * n + 1;
       const int       P    = N1 * N1;  // level sizes < 2^31 (host check)
-      const int       lane = threadIdx.x;
+      const int       lane = W > 1 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+      const int       wv   = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+      T              *sm   = sm_all + wv * (NPW * PER);
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
                 npz = n_patches_1d(n, pz);
-      const int blk = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int blk = xcd_logical_id(blockIdx.x, gridDim.x) * W + wv;
       // patch origins: wave-uniform arithmetic, divisions by the patch
       // counts as multiply-high + shift (divisors set up on the host)
       int fst[NPW];
@@ -1170,7 +1192,7 @@ namespace gtps
                     }
                 }
             }
-          __syncthreads();
+          group_sync<W>();
           // stage 2: y-rows, C1 = My Mz u, C2 = Ky Mz u + My Kz u.  Every
           // pass reads its z-rows before the first y-row is stored (the
           // y-rows overwrite the end of the z-row region).
@@ -1191,7 +1213,7 @@ namespace gtps
                   b2[pp][iy] = sz[(NI + zi) * ZST + iy * NP + ix];
                 }
             }
-          __syncthreads();
+          group_sync<W>();
 #pragma unroll
           for (int pp = 0; pp < P2; ++pp)
             {
@@ -1221,7 +1243,7 @@ namespace gtps
                   sy[((NI + zi) * NI + yi) * NP + ix] = c2;
                 }
             }
-          __syncthreads();
+          group_sync<W>();
         }
       if (MODE == 2)
         {
@@ -1284,7 +1306,7 @@ namespace gtps
           for (int j = 0; j < NI; ++j)
             sr[r * NI + j] = tt[j];
         }
-      __syncthreads();
+      group_sync<W>();
       // stage 4: S^T along y
 #pragma unroll
       for (int pp = 0; pp < P3; ++pp)
@@ -1306,7 +1328,7 @@ namespace gtps
           for (int j = 0; j < NI; ++j)
             base[j * NI] = tt[j];
         }
-      __syncthreads();
+      group_sync<W>();
       // stage 5: S^T along z, eigenvalue scaling, S along z
 #pragma unroll
       for (int pp = 0; pp < P3; ++pp)
@@ -1332,7 +1354,7 @@ namespace gtps
           for (int i = 0; i < NI; ++i)
             base[i * NI * NI] = v[i];
         }
-      __syncthreads();
+      group_sync<W>();
       // stage 6: S along y
 #pragma unroll
       for (int pp = 0; pp < P3; ++pp)
@@ -1354,7 +1376,7 @@ namespace gtps
           for (int i = 0; i < NI; ++i)
             base[i * NI] = o[i];
         }
-      __syncthreads();
+      group_sync<W>();
       // stage 7: S along x, store
 #pragma unroll
       for (int pp = 0; pp < P3; ++pp)
@@ -1882,6 +1904,13 @@ namespace gtps
               return rows ? (unsigned)((long long)((npx_ + w - 1) / w) * npy_ * npz_)
                           : (unsigned)((np + w - 1) / w);
             };
+            // patch-group kernel: wavefronts per work-group (automatic: 4 in
+            // FP32, 1 in FP64)
+            const int gw     = tuning().group_waves != 0 ? tuning().group_waves
+                                                          : (sizeof(T) == 4 ? 4 : 1);
+            auto      wgs_of = [&](int w, int waves) {
+              return (unsigned)((np + (long long)w * waves - 1) / ((long long)w * waves));
+            };
 #define GTPS_SG(M, W)                                                         \
   do                                                                          \
     {                                                                         \
@@ -1889,10 +1918,14 @@ namespace gtps
         hipLaunchKernelGGL((patch3d_row_kernel<K, T, M, W>), dim3(grid_of(W)), \
                            dim3(64), 0, s, up, sp, tab, n, colour, scale, inv,  \
                            (T)relax);                                         \
+      else if (gw == 4 && W <= 2)                                             \
+        hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W, W <= 2 ? 4 : 1>), \
+                           dim3(wgs_of(W, 4)), dim3(256), 0, s, up, sp, tab,   \
+                           n, colour, scale, inv, (T)relax, dx, dxy);          \
       else                                                                    \
-        hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W>),                \
-                           dim3(grid_of(W)), dim3(64), 0, s, up, sp, tab, n,   \
-                           colour, scale, inv, (T)relax, dx, dxy);            \
+        hipLaunchKernelGGL((patch3d_group_kernel<K, T, M, W, 1>),             \
+                           dim3(wgs_of(W, 1)), dim3(64), 0, s, up, sp, tab, n, \
+                           colour, scale, inv, (T)relax, dx, dxy);             \
     }                                                                         \
   while (0)
 #define GTPS_SG_W(M)                                                          \

@@ -237,7 +237,8 @@ namespace gtps
         }
     }
 
-    // x += alpha p; r -= alpha q; partial ||r||^2; r32 = (float) r
+    // x += alpha p (x == nullptr: deferred to the next p update); r -=
+    // alpha q; partial ||r||^2; r32 = (float) r
     __global__ void __launch_bounds__(VNT)
     cg_update_xr_kernel(long long n, double *__restrict__ x,
                         double *__restrict__ r, const double *__restrict__ p,
@@ -251,7 +252,8 @@ namespace gtps
       for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
            i += (long long)gridDim.x * VNT)
         {
-          x[i] += alpha * p[i];
+          if (x)
+            x[i] += alpha * p[i];
           const double ri = r[i] - alpha * q[i];
           r[i]            = ri;
           if (r32)
@@ -263,20 +265,36 @@ namespace gtps
         partials[blockIdx.x] = s;
     }
 
-    // p = z + beta p, z given in FP64 or FP32
+    // p = z + beta p, z given in FP64 or FP32; x != nullptr: first the
+    // deferred x += alpha p of the last step with the old p (read once for
+    // both)
     __global__ void __launch_bounds__(VNT)
     cg_update_p_kernel(long long n, double *__restrict__ p,
                        const double *__restrict__ z,
                        const float *__restrict__ z32,
-                       const double *__restrict__ sc)
+                       const double *__restrict__ sc, double *__restrict__ x)
     {
-      const double beta = sc[cg::BETA];
+      const double beta = sc[cg::BETA], alpha = sc[cg::ALPHA];
       for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
            i += (long long)gridDim.x * VNT)
         {
           const double zi = z32 ? (double)z32[i] : z[i];
-          p[i]            = zi + beta * p[i];
+          const double pi = p[i];
+          if (x)
+            x[i] += alpha * pi;
+          p[i] = zi + beta * pi;
         }
+    }
+
+    // x += alpha p: the deferred update of the last step
+    __global__ void __launch_bounds__(VNT)
+    cg_add_alpha_p_kernel(long long n, double *__restrict__ x,
+                          const double *__restrict__ p, const double *__restrict__ sc)
+    {
+      const double alpha = sc[cg::ALPHA];
+      for (long long i = (long long)blockIdx.x * VNT + threadIdx.x; i < n;
+           i += (long long)gridDim.x * VNT)
+        x[i] += alpha * p[i];
     }
 
     template <typename T>
@@ -457,10 +475,19 @@ namespace gtps
 
   void
   launch_cg_update_p(hipStream_t s, long long n, double *p, const double *z,
-                     const float *z32, const double *scalars)
+                     const float *z32, const double *scalars, double *x)
   {
     hipLaunchKernelGGL(cg_update_p_kernel, dim3(grid_for(n)), dim3(VNT), 0, s,
-                       n, p, z, z32, scalars);
+                       n, p, z, z32, scalars, x);
+    GTPS_LAUNCH_CHECK();
+  }
+
+  void
+  launch_cg_add_alpha_p(hipStream_t s, long long n, double *x, const double *p,
+                        const double *scalars)
+  {
+    hipLaunchKernelGGL(cg_add_alpha_p_kernel, dim3(grid_for(n)), dim3(VNT), 0, s,
+                       n, x, p, scalars);
     GTPS_LAUNCH_CHECK();
   }
 

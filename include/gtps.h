@@ -82,6 +82,10 @@ const char *gtps_version(void);
  *                           several patches into each wavefront (default
  *                           2048: the finest two levels at 64^3 Q3);
  *   "patches_per_wave"      how many (1 to 5; default 2);
+ *   "group_waves"           wavefronts per work-group of the patch-group
+ *                           kernel, each with its own patches: 1 or 4 (4
+ *                           applies to 1 or 2 patches per wavefront); 0
+ *                           (default): 4 in FP32, 1 in FP64;
  *   "box_vmult_max_n"       3D operator with k <= 4 on levels with at most
  *                           this many cells per direction: one-round box
  *                           kernel instead of the z-marching one (default

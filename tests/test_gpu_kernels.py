@@ -181,14 +181,17 @@ def test_tensor_product_data(core, ctx):
 
 
 SMOOTHER_PATHS = {
-    # small_colour_patches, group_min_patches, patches_per_wave, row_staged
-    "single": (1 << 40, 1 << 40, 2, 1),
-    "group2": (1 << 40, 0, 2, 1),
-    "group4": (1 << 40, 0, 4, 1),
-    "single_lane": (1 << 40, 1 << 40, 2, 0),
-    "group2_lane": (1 << 40, 0, 2, 0),
-    "group4_lane": (1 << 40, 0, 4, 0),
-    "multi": (0, 4096, 2, 1),
+    # small_colour_patches, group_min_patches, patches_per_wave, row_staged,
+    # group_waves
+    "single": (1 << 40, 1 << 40, 2, 1, 1),
+    "group2": (1 << 40, 0, 2, 1, 1),
+    "group4": (1 << 40, 0, 4, 1, 1),
+    "single_lane": (1 << 40, 1 << 40, 2, 0, 1),
+    "group2_lane": (1 << 40, 0, 2, 0, 1),
+    "group4_lane": (1 << 40, 0, 4, 0, 1),
+    "single_lane_4waves": (1 << 40, 1 << 40, 2, 0, 4),
+    "group2_lane_4waves": (1 << 40, 0, 2, 0, 4),
+    "multi": (0, 4096, 2, 1, 1),
 }
 
 
@@ -198,9 +201,10 @@ SMOOTHER_PATHS = {
 @pytest.mark.parametrize("dim,k,level", [(3, 1, 4), (3, 2, 3), (3, 3, 3), (3, 3, 4)])
 def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, path):
     """The patch-group kernel at 1, 2 and 4 patches per wavefront (coarse and
-    fine levels), with row-staged or per-lane global accesses, and the
-    multi-wave kernel agree with the oracle on the same level; odd patch
-    counts per row leave a partly empty last group."""
+    fine levels), with row-staged or per-lane global accesses, with one or
+    four wavefronts per work-group, and the multi-wave kernel agree with the
+    oracle on the same level; odd patch counts per row leave a partly empty
+    last group."""
     _announce("smoother-path", dim, k, level, variant, prec, path)
     g = rng(level * 7 + k)
     N = n_dofs(dim, k, level)
@@ -211,7 +215,8 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
     ov, cv, relax = {"fwd": (O.MULT_FWD, core.PatchSmoother.MULTIPLICATIVE, 1.0),
                      "add": (O.ADDITIVE, core.PatchSmoother.ADDITIVE, 0.5)}[variant]
     ref = O.smoother_step(dim, k, level, b, u, variant=ov, relax=relax)
-    keys = ("small_colour_patches", "group_min_patches", "patches_per_wave", "row_staged")
+    keys = ("small_colour_patches", "group_min_patches", "patches_per_wave", "row_staged",
+            "group_waves")
     prev = [core.set_tuning(kk, v) for kk, v in zip(keys, SMOOTHER_PATHS[path])]
     try:
         op = core.LaplaceOperator(ctx, dim, k, level, _dt(prec))
