@@ -723,7 +723,9 @@ def main():
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/traffic.json from two rocprofv3 --pmc runs of "
                          "--kernels (no GPU work)")
-    ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2, 1>")
+    ap.add_argument("--traffic-kernel", default="patch3d_group_kernel<3, float, 0, 2, 4>")
+    ap.add_argument("--traffic-out", default=os.path.join(ROOT, "profiles", "traffic.json"),
+                    help="output of --traffic-summary")
     ap.add_argument("--solve-trace", metavar="c2|c3|c4|c5",
                     help="run a configuration's FP32 solve for a kernel trace (warm-up, "
                          "1 s pause, --steps timed solves)")
@@ -738,8 +740,7 @@ def main():
 
     if args.traffic_summary:
         traffic_summary(args.traffic_summary[0], args.traffic_summary[1], args.traffic_kernel,
-                        (args.dim, args.degree, args.level),
-                        os.path.join(ROOT, "profiles", "traffic.json"))
+                        (args.dim, args.degree, args.level), args.traffic_out)
         return
 
     if args.gpus != 1:

@@ -355,19 +355,22 @@ namespace gtps
         }
     }
 
-    // LDS plan of the patch kernel: r, t (patch box, ping-pong), NS face
-    // slabs and their intermediate (NS = 6: all faces in one phase; 2: one
-    // direction per phase; 1: one face per phase, whichever fits 64 KB), and
-    // the 1D tables of this patch (S and lambda per direction, M, L-, L+).
+    // LDS plan of the patch kernel: r and t (patch box, ping-pong in the
+    // local solve), NS face slabs and their intermediate (NS = 6: all faces
+    // in one round; 2: one direction per round; 1: one face per round,
+    // whichever fits 64 KB; t shares the slab buffer, which the local solve
+    // no longer needs), and the 1D tables of this patch (S and lambda per
+    // direction, M, L-, L+).
     template <int K, typename T>
     struct DP
     {
       static constexpr int    NN = K + 1, PD = 2 * NN, P3 = PD * PD * PD;
       static constexpr int    SL   = PD * PD * NN;  // one slab
       static constexpr int    TABN = 3 * PD * PD + 3 * PD + 3 * NN * NN;
+      static constexpr int    sa_len(int ns) { return ns * SL > P3 ? ns * SL : P3; }
       static constexpr size_t bytes(int ns)
       {
-        return sizeof(T) * (2 * P3 + 2 * ns * SL + TABN);
+        return sizeof(T) * (P3 + sa_len(ns) + ns * SL + TABN);
       }
       static constexpr int    NS  = bytes(6) <= 65536 ? 6 : (bytes(2) <= 65536 ? 2 : 1);
       // threads per patch: about one patch DoF per thread (the kernel is
@@ -381,19 +384,22 @@ namespace gtps
     // (2k+2)^3 DoFs), u_P <- (1-w) u_P + w A_PP^{-1} (b_P - A_P,ext u_ext):
     //  1. b_P into r; the face-neighbour cell slabs of u into LDS;
     //  2. per face, L-/L+ along the face normal, then the cell mass along the
-    //     two in-plane directions, gathered into r (one thread per patch DoF,
-    //     so the faces of different directions never race);
-    //  3. fast diagonalisation, ping-ponging r and t, the eigenvalue scaling
-    //     fused into the last forward pass and the last inverse pass written
-    //     straight to u.
+    //     two in-plane directions, each pass one line of k+1 values per
+    //     thread; then every patch DoF subtracts the face of each direction
+    //     on its side (so the faces of different directions never race);
+    //  3. fast diagonalisation, ping-ponging r and t, one task per line and
+    //     half of its outputs, the eigenvalue scaling fused into the last
+    //     forward pass; the result is stored coalesced along x.
     // Patch arrays are [z][y][x] with PD per direction; slab f (direction
     // D = f / 2, side f % 2: below / above the patch) is [a][b][j] with j
-    // along D and (a, b) the other two directions, a the slower one.
+    // along D and (a, b) the other two directions, a the slower one.  Line
+    // tasks are counted in whole wavefronts per face (or output half), so
+    // the face and its tables are uniform over a wavefront.
     template <int K, typename T>
     struct DPShared
     {
       using C = DP<K, T>;
-      T r[C::P3], t[C::P3], sa[C::NS * C::SL], sw[C::NS * C::SL];
+      T r[C::P3], sa[C::sa_len(C::NS)], sw[C::NS * C::SL];  // t = sa
       T Ssh[3][C::PD * C::PD], lsh[3][C::PD], Msh[C::NN * C::NN], Lsh[2][C::NN * C::NN];
     };
 
@@ -405,181 +411,247 @@ namespace gtps
                    const T *__restrict__ tab, const int n, const int pc[3],
                    const T scale, const T relax, DPShared<K, T> &sh)
     {
-      using TB           = DTab<K>;
       using C            = DP<K, T>;
+      using TB           = DTab<K>;
       constexpr int NN   = C::NN, PD = C::PD, P3 = C::P3, SL = C::SL, NS = C::NS;
-      T *r = sh.r, *t = sh.t, *sa = sh.sa, *sw = sh.sw;
+      constexpr int NT   = C::NT;
+      // lines per slab (or per direction of the box), padded to wavefronts
+      constexpr int NL = PD * PD, NLP = (NL + 63) / 64 * 64;
+      static_assert(NT % 64 == 0, "whole wavefronts");
+      T *r = sh.r, *t = sh.sa, *sa = sh.sa, *sw = sh.sw;
       auto &Ssh = sh.Ssh;
       auto &lsh = sh.lsh;
       auto &Msh = sh.Msh;
       auto &Lsh = sh.Lsh;
-      const int       N1 = n * NN;
-      const long long P  = (long long)N1 * N1;
-      const long long st[3] = {1, N1, P};
-      const long long g0    = (long long)pc[0] * NN + pc[1] * NN * st[1] + pc[2] * NN * st[2];
-      const int       tid   = threadIdx.x;
+      // level sizes < 2^31 (host check): offsets from the patch's first DoF
+      // are 32-bit
+      const int       N1    = n * NN, P = N1 * N1;
+      const int       st[3] = {1, N1, P};
+      const long long g0    = (long long)pc[0] * NN + (long long)pc[1] * NN * N1 +
+                           (long long)pc[2] * NN * P;
+      T *__restrict__ up       = u + g0;
+      const T *__restrict__ bp = bvec + g0;
+      const int tid            = threadIdx.x;
       auto variant = [&](int p) { return (p == 0 ? 1 : 0) + (p + 2 == n ? 2 : 0); };
       // face f exists if the neighbour cell slab lies inside the mesh
       auto has_face = [&](int f) {
         const int p = pc[f >> 1];
         return (f & 1) ? p + 2 < n : p > 0;
       };
+      auto box_offset = [&](int e) {
+        const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
+        return x + y * N1 + z * P;
+      };
       // tables
-      for (int e = tid; e < 3 * PD * PD; e += C::NT)
+      for (int e = tid; e < 3 * PD * PD; e += NT)
         {
           const int d = e / (PD * PD);
           Ssh[d][e - d * PD * PD] = tab[TB::S + variant(pc[d]) * PD * PD + e - d * PD * PD];
         }
-      for (int e = tid; e < 3 * PD; e += C::NT)
+      for (int e = tid; e < 3 * PD; e += NT)
         {
           const int d = e / PD;
           lsh[d][e - d * PD] = tab[TB::lam + variant(pc[d]) * PD + e - d * PD];
         }
-      for (int e = tid; e < NN * NN; e += C::NT)
+      for (int e = tid; e < NN * NN; e += NT)
         {
           Msh[e]    = tab[TB::M + e];
           Lsh[0][e] = tab[TB::Lm + e];
           Lsh[1][e] = tab[TB::Lp + e];
         }
-      // b_P and the slabs of one face group: every load of a thread issued
-      // before the first wait
-      constexpr int ITB = (P3 + C::NT - 1) / C::NT, ITS = (NS * SL + C::NT - 1) / C::NT;
-      auto slab_value = [&](int f0, int e) -> T {
-        const int f = f0 + e / SL;
-        if (e >= NS * SL || !has_face(f))
-          return T(0);
-        const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-        const int l = e % SL, j = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
-        const long long o = g0 + ((f & 1) ? (long long)PD : -(long long)NN) * st[D];
-        return u[o + aa * st[da] + bb * st[db] + j * st[D]];
-      };
-      {
-        T vb[ITB], vs[ITS];
+      // b_P and the old patch values (relax != 1), issued before the slab
+      // loads and their first wait
+      constexpr int ITB   = (P3 + NT - 1) / NT;
+      const bool    blend = relax != T(1);
+      T             vb[ITB], uold[ITB];
 #pragma unroll
-        for (int it = 0; it < ITB; ++it)
+      for (int it = 0; it < ITB; ++it)
+        {
+          const int e = tid + it * NT;
+          const int o = box_offset(e < P3 ? e : 0);
+          vb[it]      = e < P3 ? bp[o] : T(0);
+          uold[it]    = e < P3 && blend ? up[o] : T(0);
+        }
+      // slab lines of faces f0 .. f0+NS-1 into sa: task (face, a, b)
+      auto load_slabs = [&](int f0) {
+        for (int tk = tid; tk < NS * NLP; tk += NT)
           {
-            const int e = tid + it * C::NT;
-            const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
-            vb[it] = e < P3 ? bvec[g0 + x + (long long)y * N1 + (long long)z * P] : T(0);
+            const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
+            const int f  = f0 + ff;
+            if (ln >= NL || !has_face(f))
+              continue;
+            const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
+            const int aa = ln / PD, bb = ln - aa * PD;
+            const int o  = ((f & 1) ? PD : -NN) * st[D] + aa * st[da] + bb * st[db];
+            T         v[NN];
+#pragma unroll
+            for (int j = 0; j < NN; ++j)
+              v[j] = up[o + j * st[D]];
+            T *sl = sa + ff * SL + ln * NN;
+#pragma unroll
+            for (int j = 0; j < NN; ++j)
+              sl[j] = v[j];
           }
+      };
+      load_slabs(0);
 #pragma unroll
-        for (int it = 0; it < ITS; ++it)
-          vs[it] = slab_value(0, tid + it * C::NT);
-#pragma unroll
-        for (int it = 0; it < ITB; ++it)
-          if (tid + it * C::NT < P3)
-            r[tid + it * C::NT] = vb[it];
-#pragma unroll
-        for (int it = 0; it < ITS; ++it)
-          if (tid + it * C::NT < NS * SL)
-            sa[tid + it * C::NT] = vs[it];
-      }
-      // 2. external coupling, NS faces per phase
+      for (int it = 0; it < ITB; ++it)
+        if (tid + it * NT < P3)
+          r[tid + it * NT] = vb[it];
+      // 2. external coupling, NS faces per round
       for (int f0 = 0; f0 < 6; f0 += NS)
         {
           if (f0 > 0)
-            {
-              T vs[ITS];
-#pragma unroll
-              for (int it = 0; it < ITS; ++it)
-                vs[it] = slab_value(f0, tid + it * C::NT);
-#pragma unroll
-              for (int it = 0; it < ITS; ++it)
-                if (tid + it * C::NT < NS * SL)
-                  sa[tid + it * C::NT] = vs[it];
-            }
+            load_slabs(f0);
           __syncthreads();
-          // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]
-          for (int e = tid; e < NS * SL; e += C::NT)
+          // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]: line (a, b)
+          for (int tk = tid; tk < NS * NLP; tk += NT)
             {
-              const int f = f0 + e / SL;
-              if (!has_face(f))
+              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
+              const int f  = f0 + ff;
+              if (ln >= NL || !has_face(f))
                 continue;
-              const int l = e % SL, i = l % NN, base = e - i;
-              const T  *Lo = Lsh[f & 1] + i * NN;
-              T         v  = 0;
+              const T *sl = sa + ff * SL + ln * NN;
+              const T *Lo = Lsh[f & 1];
+              T        v[NN];
 #pragma unroll
               for (int j = 0; j < NN; ++j)
-                v += Lo[j] * sa[base + j];
-              sw[e] = v;
+                v[j] = sl[j];
+              T *w = sw + ff * SL + ln * NN;
+#pragma unroll
+              for (int i = 0; i < NN; ++i)
+                {
+                  T acc = 0;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    acc += Lo[i * NN + j] * v[j];
+                  w[i] = acc;
+                }
             }
           __syncthreads();
-          // cell mass along b
-          for (int e = tid; e < NS * SL; e += C::NT)
+          // cell mass along b: line (a, cell cb of b, i)
+          for (int tk = tid; tk < NS * NLP; tk += NT)
             {
-              const int f = f0 + e / SL;
-              if (!has_face(f))
+              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
+              const int f  = f0 + ff;
+              if (ln >= NL || !has_face(f))
                 continue;
-              const int l = e % SL, i = l % NN, ab = l / NN, bb = ab % PD, aa = ab / PD;
-              const int cb = bb / NN, ib = bb - cb * NN;
-              const T  *w0 = sw + (e - l) + (aa * PD + cb * NN) * NN + i;
-              T         v  = 0;
+              const int i = ln % NN, acb = ln / NN, cb = acb & 1, aa = acb >> 1;
+              const int o = ff * SL + (aa * PD + cb * NN) * NN + i;
+              T         v[NN];
 #pragma unroll
               for (int j = 0; j < NN; ++j)
-                v += Msh[ib * NN + j] * w0[j * NN];
-              sa[e] = v;
+                v[j] = sw[o + j * NN];
+#pragma unroll
+              for (int ib = 0; ib < NN; ++ib)
+                {
+                  T acc = 0;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    acc += Msh[ib * NN + j] * v[j];
+                  sa[o + ib * NN] = acc;
+                }
             }
           __syncthreads();
-          // cell mass along a, gathered per patch DoF over this phase's faces
-          for (int e = tid; e < P3; e += C::NT)
+          // cell mass along a: line (cell ca of a, b, i), into sw[a][b][i]
+          for (int tk = tid; tk < NS * NLP; tk += NT)
+            {
+              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
+              const int f  = f0 + ff;
+              if (ln >= NL || !has_face(f))
+                continue;
+              const int i = ln % NN, cbb = ln / NN, bb = cbb % PD, ca = cbb / PD;
+              const int o = ff * SL + (ca * NN * PD + bb) * NN + i;
+              T         v[NN];
+#pragma unroll
+              for (int j = 0; j < NN; ++j)
+                v[j] = sa[o + j * PD * NN];
+#pragma unroll
+              for (int ia = 0; ia < NN; ++ia)
+                {
+                  T acc = 0;
+#pragma unroll
+                  for (int j = 0; j < NN; ++j)
+                    acc += Msh[ia * NN + j] * v[j];
+                  sw[o + ia * PD * NN] = acc;
+                }
+            }
+          __syncthreads();
+          // r -= scale (face terms): per direction, the face on the DoF's side
+          for (int e = tid; e < P3; e += NT)
             {
               const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
               T         acc   = 0;
 #pragma unroll
-              for (int ff = 0; ff < NS; ++ff)
+              for (int D = 0; D < 3; ++D)
                 {
-                  const int f = f0 + ff, D = f >> 1;
-                  const int pD = c3[D];
-                  if (!has_face(f) || ((f & 1) ? pD < NN : pD >= NN))
+                  const int pD = c3[D], side = pD >= NN ? 1 : 0, f = 2 * D + side;
+                  if (f < f0 || f >= f0 + NS || !has_face(f))
                     continue;
                   const int da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-                  const int i = (f & 1) ? pD - NN : pD, aa = c3[da], bb = c3[db];
-                  const int ca = aa / NN, ia = aa - ca * NN;
-                  const T  *s0 = sa + ff * SL + ((ca * NN) * PD + bb) * NN + i;
-                  T         v  = 0;
-#pragma unroll
-                  for (int j = 0; j < NN; ++j)
-                    v += Msh[ia * NN + j] * s0[j * PD * NN];
-                  acc += v;
+                  acc += sw[(f - f0) * SL + (c3[da] * PD + c3[db]) * NN + pD - side * NN];
                 }
               r[e] -= scale * acc;
             }
           __syncthreads();
         }
-      // 3. fast diagonalisation: forward S^T along x, y, z, scale, inverse
-      auto pass = [&](const T *in, T *out, int D, bool tr, bool divide) {
-        const int str = D == 0 ? 1 : (D == 1 ? PD : PD * PD);
-        const T  *S   = Ssh[D];
-        for (int e = tid; e < P3; e += C::NT)
+      // 3. fast diagonalisation, ping-ponging r and t: S^T along x, y, z,
+      // the eigenvalue scaling, S along z, y, x.  Task (half h of the
+      // outputs, line): the line's PD values read once, NN outputs.
+      auto pass = [&](const T *in, T *out, auto dirc, auto trc, auto divc) {
+        constexpr int  D      = decltype(dirc)::value;
+        constexpr bool TR     = decltype(trc)::value;
+        constexpr bool DIVIDE = decltype(divc)::value;
+        constexpr int  str    = D == 0 ? 1 : (D == 1 ? PD : PD * PD);
+        for (int tk = tid; tk < 2 * NLP; tk += NT)
           {
-            const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
-            const int qq    = c3[D];
-            const T  *line  = in + e - qq * str;
-            T         v     = 0;
+            const int h = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - h * NLP;
+            if (ln >= NL)
+              continue;
+            // (lp, lq): the other two coordinates, lp the slower
+            const int lp = ln / PD, lq = ln - lp * PD;
+            const int base =
+              D == 0 ? (lp * PD + lq) * PD : (D == 1 ? lp * PD * PD + lq : lp * PD + lq);
+            // rows (S) or columns (S^T) h NN .. h NN + NN-1 of this direction
+            const T *S = TR ? Ssh[D] + h * NN : Ssh[D] + h * NN * PD;
+            T        v[PD];
 #pragma unroll
             for (int j = 0; j < PD; ++j)
-              v += (tr ? S[j * PD + qq] : S[qq * PD + j]) * line[j * str];
-            if (divide)
-              v /= scale * (lsh[0][c3[0]] + lsh[1][c3[1]] + lsh[2][c3[2]]);
-            out[e] = v;
+              v[j] = in[base + j * str];
+#pragma unroll
+            for (int m = 0; m < NN; ++m)
+              {
+                T acc = 0;
+#pragma unroll
+                for (int j = 0; j < PD; ++j)
+                  acc += (TR ? S[j * PD + m] : S[m * PD + j]) * v[j];
+                if constexpr (DIVIDE)
+                  acc /= scale * (lsh[0][lq] + lsh[1][lp] + lsh[2][h * NN + m]);
+                out[base + (h * NN + m) * str] = acc;
+              }
           }
         __syncthreads();
       };
-      pass(r, t, 0, true, false);
-      pass(t, r, 1, true, false);
-      pass(r, t, 2, true, true);
-      pass(t, r, 2, false, false);
-      pass(r, t, 1, false, false);
-      for (int e = tid; e < P3; e += C::NT)
-        {
-          const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
-          const T  *line = t + e - x;
-          T         v    = 0;
+      using I0 = std::integral_constant<int, 0>;
+      using I1 = std::integral_constant<int, 1>;
+      using I2 = std::integral_constant<int, 2>;
+      using BT = std::true_type;
+      using BF = std::false_type;
+      pass(r, t, I0{}, BT{}, BF{});
+      pass(t, r, I1{}, BT{}, BF{});
+      pass(r, t, I2{}, BT{}, BT{});
+      pass(t, r, I2{}, BF{}, BF{});
+      pass(r, t, I1{}, BF{}, BF{});
+      pass(t, r, I0{}, BF{}, BF{});
+      // the patch values, coalesced along x
 #pragma unroll
-          for (int j = 0; j < PD; ++j)
-            v += Ssh[0][x * PD + j] * line[j];
-          const long long g = g0 + x + (long long)y * N1 + (long long)z * P;
-          u[g]              = relax == T(1) ? v : (T(1) - relax) * u[g] + relax * v;
+      for (int it = 0; it < ITB; ++it)
+        {
+          const int e = tid + it * NT;
+          if (e >= P3)
+            continue;
+          const T v          = r[e];
+          up[box_offset(e)] = blend ? (T(1) - relax) * uold[it] + relax * v : v;
         }
     }
 
@@ -928,6 +1000,7 @@ namespace gtps
           const int   n = lvl->n;
           hipStream_t s = lvl->ctx->stream;
           GTPS_REQUIRE(columns == 1 || n <= 4, "batched sweeps need at most 4 cells per direction");
+          GTPS_REQUIRE(lvl->N < (1LL << 31), "DG level too large for the patch kernel");
           if (n <= 4 && nvec > 1)
             columns = nvec;  // independent vectors: one work-group each
           if (n <= 4)
