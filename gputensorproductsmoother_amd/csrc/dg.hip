@@ -498,26 +498,47 @@ namespace gtps
       for (int it = 0; it < ITB; ++it)
         if (tid + it * NT < P3)
           r[tid + it * NT] = vb[it];
-      // 2. external coupling, NS faces per round
+      // 2. external coupling, NS faces per round.  A thread keeps its task
+      // (face ff, line ln) in every pass of a round, so the 1D matrices it
+      // needs (L- or L+ of its face, M) are read into registers once, with
+      // the first pass's slab values.
+      // lines per face: whole wavefronts (uniform face) where that fits the
+      // work-group, else packed
+      constexpr int CL = NS * NLP <= NT ? NLP : NL;
+      static_assert(NS * CL <= NT, "one coupling task per thread");
+      const int  cff  = CL % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / CL) : tid / CL;
+      const int  cln  = tid - cff * CL;
+      const bool cact = tid < NS * CL && cln < NL;
       for (int f0 = 0; f0 < 6; f0 += NS)
         {
           if (f0 > 0)
             load_slabs(f0);
           __syncthreads();
-          // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]: line (a, b)
-          for (int tk = tid; tk < NS * NLP; tk += NT)
+          const int  f  = f0 + cff;
+          const bool on = cact && has_face(f);
+          // small degrees (FP32): L and M in registers; else read from LDS
+          constexpr bool CPRE = 2 * NN * NN * sizeof(T) <= 256;
+          T              Lr[CPRE ? NN * NN : 1], Mr[CPRE ? NN * NN : 1];
+          const T       *Lo = CPRE ? Lr : Lsh[f & 1];
+          const T       *Mc = CPRE ? Mr : Msh;
+          if (on)
             {
-              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
-              const int f  = f0 + ff;
-              if (ln >= NL || !has_face(f))
-                continue;
-              const T *sl = sa + ff * SL + ln * NN;
-              const T *Lo = Lsh[f & 1];
+              if constexpr (CPRE)
+                {
+#pragma unroll
+                  for (int e = 0; e < NN * NN; ++e)
+                    {
+                      Lr[e] = Lsh[f & 1][e];
+                      Mr[e] = Msh[e];
+                    }
+                }
+              // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]: line (a, b)
+              const T *sl = sa + cff * SL + cln * NN;
               T        v[NN];
 #pragma unroll
               for (int j = 0; j < NN; ++j)
                 v[j] = sl[j];
-              T *w = sw + ff * SL + ln * NN;
+              T *w = sw + cff * SL + cln * NN;
 #pragma unroll
               for (int i = 0; i < NN; ++i)
                 {
@@ -530,14 +551,10 @@ namespace gtps
             }
           __syncthreads();
           // cell mass along b: line (a, cell cb of b, i)
-          for (int tk = tid; tk < NS * NLP; tk += NT)
+          if (on)
             {
-              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
-              const int f  = f0 + ff;
-              if (ln >= NL || !has_face(f))
-                continue;
-              const int i = ln % NN, acb = ln / NN, cb = acb & 1, aa = acb >> 1;
-              const int o = ff * SL + (aa * PD + cb * NN) * NN + i;
+              const int i = cln % NN, acb = cln / NN, cb = acb & 1, aa = acb >> 1;
+              const int o = cff * SL + (aa * PD + cb * NN) * NN + i;
               T         v[NN];
 #pragma unroll
               for (int j = 0; j < NN; ++j)
@@ -548,20 +565,16 @@ namespace gtps
                   T acc = 0;
 #pragma unroll
                   for (int j = 0; j < NN; ++j)
-                    acc += Msh[ib * NN + j] * v[j];
+                    acc += Mc[ib * NN + j] * v[j];
                   sa[o + ib * NN] = acc;
                 }
             }
           __syncthreads();
           // cell mass along a: line (cell ca of a, b, i), into sw[a][b][i]
-          for (int tk = tid; tk < NS * NLP; tk += NT)
+          if (on)
             {
-              const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
-              const int f  = f0 + ff;
-              if (ln >= NL || !has_face(f))
-                continue;
-              const int i = ln % NN, cbb = ln / NN, bb = cbb % PD, ca = cbb / PD;
-              const int o = ff * SL + (ca * NN * PD + bb) * NN + i;
+              const int i = cln % NN, cbb = cln / NN, bb = cbb % PD, ca = cbb / PD;
+              const int o = cff * SL + (ca * NN * PD + bb) * NN + i;
               T         v[NN];
 #pragma unroll
               for (int j = 0; j < NN; ++j)
@@ -572,7 +585,7 @@ namespace gtps
                   T acc = 0;
 #pragma unroll
                   for (int j = 0; j < NN; ++j)
-                    acc += Msh[ia * NN + j] * v[j];
+                    acc += Mc[ia * NN + j] * v[j];
                   sw[o + ia * PD * NN] = acc;
                 }
             }
@@ -596,38 +609,69 @@ namespace gtps
           __syncthreads();
         }
       // 3. fast diagonalisation, ping-ponging r and t: S^T along x, y, z,
-      // the eigenvalue scaling, S along z, y, x.  Task (half h of the
-      // outputs, line): the line's PD values read once, NN outputs.
-      auto pass = [&](const T *in, T *out, auto dirc, auto trc, auto divc) {
+      // the eigenvalue scaling, S along z, y, x.  Task (output block h,
+      // line): the line's PD values, OB outputs; in FP32 blocks of 2 while
+      // the tasks fit the work-group once (more wavefronts share a pass,
+      // which is what the latency of a lone patch needs), else halves
+      // (measured at Q4 24^3: FP32 7.77 against 7.87 ms per solve, FP64
+      // 10.10 against 9.76 ms).  Every thread has one task, the same in all
+      // passes, so the next pass's matrix entries are read while the
+      // current pass computes: a pass waits for its line values only.
+      constexpr int OB = (PD / 2) * NLP <= NT ? 2 : NN, NB = PD / OB;
+      constexpr int FL = NB * NLP <= NT ? NLP : NL;
+      static_assert(NB * FL <= NT, "one FD task per thread");
+      const int  fh   = FL % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / FL) : tid / FL;
+      const int  fln  = tid - fh * FL;
+      const bool fact = tid < NB * FL && fln < NL;
+      // (lp, lq): the other two coordinates of the line, lp the slower
+      const int flp = fln / PD, flq = fln - flp * PD;
+      // entries (j, m) of the direction's S^T (TR) or S for output block fh
+      auto load_coef = [&](auto &cf, int D, bool TR) {
+        const T *S = Ssh[D];
+#pragma unroll
+        for (int j = 0; j < PD; ++j)
+#pragma unroll
+          for (int m = 0; m < OB; ++m)
+            cf[j * OB + m] = TR ? S[j * PD + fh * OB + m] : S[(fh * OB + m) * PD + j];
+      };
+      // small blocks: the next pass's entries are read into registers during
+      // this pass (two register sets); large ones are read from LDS as used
+      constexpr bool AHEAD = PD * OB * sizeof(T) <= 80;
+      auto pass = [&](const T *in, T *out, auto dirc, auto trc, auto divc, auto &cf,
+                      auto &next, int nD, bool nTR, bool has_next) {
         constexpr int  D      = decltype(dirc)::value;
         constexpr bool TR     = decltype(trc)::value;
         constexpr bool DIVIDE = decltype(divc)::value;
         constexpr int  str    = D == 0 ? 1 : (D == 1 ? PD : PD * PD);
-        for (int tk = tid; tk < 2 * NLP; tk += NT)
+        if (fact)
           {
-            const int h = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - h * NLP;
-            if (ln >= NL)
-              continue;
-            // (lp, lq): the other two coordinates, lp the slower
-            const int lp = ln / PD, lq = ln - lp * PD;
-            const int base =
-              D == 0 ? (lp * PD + lq) * PD : (D == 1 ? lp * PD * PD + lq : lp * PD + lq);
-            // rows (S) or columns (S^T) h NN .. h NN + NN-1 of this direction
-            const T *S = TR ? Ssh[D] + h * NN : Ssh[D] + h * NN * PD;
+            const int base = D == 0 ? (flp * PD + flq) * PD
+                                    : (D == 1 ? flp * PD * PD + flq : flp * PD + flq);
+            const T *S = TR ? Ssh[D] + fh * OB : Ssh[D] + fh * OB * PD;
             T        v[PD];
 #pragma unroll
             for (int j = 0; j < PD; ++j)
               v[j] = in[base + j * str];
+            if constexpr (AHEAD)
+              if (has_next)
+                load_coef(next, nD, nTR);
 #pragma unroll
-            for (int m = 0; m < NN; ++m)
+            for (int m = 0; m < OB; ++m)
               {
                 T acc = 0;
 #pragma unroll
                 for (int j = 0; j < PD; ++j)
-                  acc += (TR ? S[j * PD + m] : S[m * PD + j]) * v[j];
+                  {
+                    T c;
+                    if constexpr (AHEAD)
+                      c = cf[j * OB + m];
+                    else
+                      c = TR ? S[j * PD + m] : S[m * PD + j];
+                    acc += c * v[j];
+                  }
                 if constexpr (DIVIDE)
-                  acc /= scale * (lsh[0][lq] + lsh[1][lp] + lsh[2][h * NN + m]);
-                out[base + (h * NN + m) * str] = acc;
+                  acc /= scale * (lsh[0][flq] + lsh[1][flp] + lsh[2][fh * OB + m]);
+                out[base + (fh * OB + m) * str] = acc;
               }
           }
         __syncthreads();
@@ -637,12 +681,15 @@ namespace gtps
       using I2 = std::integral_constant<int, 2>;
       using BT = std::true_type;
       using BF = std::false_type;
-      pass(r, t, I0{}, BT{}, BF{});
-      pass(t, r, I1{}, BT{}, BF{});
-      pass(r, t, I2{}, BT{}, BT{});
-      pass(t, r, I2{}, BF{}, BF{});
-      pass(r, t, I1{}, BF{}, BF{});
-      pass(t, r, I0{}, BF{}, BF{});
+      T ca[AHEAD ? PD * OB : 1], cb[AHEAD ? PD * OB : 1];
+      if (AHEAD && fact)
+        load_coef(ca, 0, true);
+      pass(r, t, I0{}, BT{}, BF{}, ca, cb, 1, true, true);
+      pass(t, r, I1{}, BT{}, BF{}, cb, ca, 2, true, true);
+      pass(r, t, I2{}, BT{}, BT{}, ca, cb, 2, false, true);
+      pass(t, r, I2{}, BF{}, BF{}, cb, ca, 1, false, true);
+      pass(r, t, I1{}, BF{}, BF{}, ca, cb, 0, false, true);
+      pass(t, r, I0{}, BF{}, BF{}, cb, ca, 0, false, false);
       // the patch values, coalesced along x
 #pragma unroll
       for (int it = 0; it < ITB; ++it)
