@@ -1018,7 +1018,7 @@ namespace gtps
       const int       N1   = K * n + 1;
       const int       P    = N1 * N1;  // level sizes < 2^31 (host check)
       const int       lane = W > 1 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
-      const int       wv   = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+      const int       wv   = W > 1 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
       T              *sm   = sm_all + wv * (NPW * PER);
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
       const int npx = n_patches_1d(n, px), npy = n_patches_1d(n, py),
