@@ -137,7 +137,8 @@ namespace gtps
               for (int iy = 0; iy < NN; ++iy)
                 {
                   c1 += cM[ly * NN + iy] * a[iy];
-                  c2 += cK[ly * NN + iy] * a[iy] + cM[ly * NN + iy] * b[iy];
+                  c2 += cK[ly * NN + iy] * a[iy];
+                  c2 += cM[ly * NN + iy] * b[iy];
                 }
               ca[(lz * NN + ly) * NN + ix] = c1;
               cb[(lz * NN + ly) * NN + ix] = c2;
@@ -163,7 +164,10 @@ namespace gtps
               T y = 0;
 #pragma unroll
               for (int ix = 0; ix < NN; ++ix)
-                y += cK[lx * NN + ix] * c1[ix] + cM[lx * NN + ix] * c2[ix];
+                {
+                  y += cK[lx * NN + ix] * c1[ix];
+                  y += cM[lx * NN + ix] * c2[ix];
+                }
               if (MODE == 0)
                 out[cell * N3 + (lz * NN + ly) * NN + lx] = f * y;
               else
@@ -366,15 +370,18 @@ namespace gtps
       const T     *cM   = tab + TB::cellM;
       const T     *cK   = tab + TB::cellK;
       const int    tid  = threadIdx.x;
+      // level sizes < 2^31 (host check): offsets within the patch box are
+      // 32-bit, from the box's first DoF
       const int    N1   = K * n + 1;
-      const long long P = (long long)N1 * N1;
+      const int    P    = N1 * N1;
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
       const int npx = n_patches(n, px), npy = n_patches(n, py);
       const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
       const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
       const int c0x = 2 * qx + px, c0y = 2 * qy + py, c0z = 2 * qz + pz;
-      const long long first = (long long)K * c0x + (long long)K * c0y * N1 +
-                              (long long)K * c0z * P;
+      const int first = K * c0x + K * c0y * N1 + K * c0z * P;
+      T *__restrict__ ub       = u + first;
+      const T *__restrict__ bp = bvec + first;
       // this patch's separable fit (computed with the coefficient): ratio
       // bins and scale, uniform over the work-group
       const long long pq  = c0x + (long long)(n - 1) * (c0y + (long long)(n - 1) * c0z);
@@ -397,7 +404,7 @@ namespace gtps
           {
             const int t  = tid + it * CNT;
             const int ix = t % NP, iy = (t / NP) % NP, iz = t / (NP * NP);
-            vu[it] = t < NP * NP * NP ? u[first + ix + (long long)iy * N1 + iz * P] : T(0);
+            vu[it] = t < NP * NP * NP ? ub[ix + iy * N1 + iz * P] : T(0);
           }
 #pragma unroll
         for (int it = 0; it < ITR; ++it)
@@ -405,7 +412,7 @@ namespace gtps
             const int t = tid + it * CNT;
             const int i = t % NI, j = (t / NI) % NI, l = t / (NI * NI);
             bres[it]    = t < NI * NI * NI
-                            ? bvec[first + (i + 1) + (long long)(j + 1) * N1 + (l + 1) * P]
+                            ? bp[(i + 1) + (j + 1) * N1 + (l + 1) * P]
                             : T(0);
           }
 #pragma unroll
@@ -466,7 +473,8 @@ namespace gtps
               for (int iy = 0; iy < NN; ++iy)
                 {
                   c1 += cM[ly * NN + iy] * a[iy];
-                  c2 += cK[ly * NN + iy] * a[iy] + cM[ly * NN + iy] * b[iy];
+                  c2 += cK[ly * NN + iy] * a[iy];
+                  c2 += cM[ly * NN + iy] * b[iy];
                 }
               ca[(lz * NN + ly) * NN + ix] = c1;
               cb[(lz * NN + ly) * NN + ix] = c2;
@@ -496,53 +504,38 @@ namespace gtps
               T         y  = 0;
 #pragma unroll
               for (int ix = 0; ix < NN; ++ix)
-                y += cK[lx * NN + ix] * c1[ix] + cM[lx * NN + ix] * c2[ix];
+                {
+                  y += cK[lx * NN + ix] * c1[ix];
+                  y += cM[lx * NN + ix] * c2[ix];
+                }
               ca[(lz * NN + ly) * NN + lx] = f * y;
             }
         }
       __syncthreads();
-      // interior residual r = b - sum of the cells containing the DoF
+      // interior residual r = b - sum of the cells containing the DoF.  Per
+      // direction the DoF at patch index p is node p of the lower cell
+      // (p <= K) and node p - K of the upper one (p >= K); summed over the 8
+      // cells in cell order with the absent ones masked, each read at a
+      // fixed offset from the DoF's index in the lower cell (x + y NN + z N2,
+      // within the 8-cell buffer for every cell).
 #pragma unroll
       for (int it = 0; it < ITR; ++it)
         {
           const int t = tid + it * CNT;
           if (t >= NI * NI * NI)
             break;
-          const int i = t % NI, j = (t / NI) % NI, l = t / (NI * NI);
-          const int p[3] = {i + 1, j + 1, l + 1};
-          int       co[3][2], lo[3][2], cnt[3];
+          const int  i = t % NI + 1, j = (t / NI) % NI + 1, l = t / (NI * NI) + 1;
+          const int  base = (l * NN + j) * NN + i;
+          const bool lo[3] = {i <= K, j <= K, l <= K}, hi[3] = {i >= K, j >= K, l >= K};
+          T          s = 0;
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
+          for (int c = 0; c < 8; ++c)
             {
-              if (p[d] < K)
-                {
-                  co[d][0] = 0;
-                  lo[d][0] = p[d];
-                  cnt[d]   = 1;
-                }
-              else if (p[d] == K)
-                {
-                  co[d][0] = 0;
-                  lo[d][0] = K;
-                  co[d][1] = 1;
-                  lo[d][1] = 0;
-                  cnt[d]   = 2;
-                }
-              else
-                {
-                  co[d][0] = 1;
-                  lo[d][0] = p[d] - K;
-                  cnt[d]   = 1;
-                }
+              const int  ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;
+              const bool on = (ox ? hi[0] : lo[0]) && (oy ? hi[1] : lo[1]) && (oz ? hi[2] : lo[2]);
+              const T    v  = sa[base + c * N3 - K * ((oz * NN + oy) * NN + ox)];
+              s += on ? v : T(0);
             }
-          T s = 0;
-          for (int a = 0; a < cnt[2]; ++a)
-            for (int bb = 0; bb < cnt[1]; ++bb)
-              for (int c = 0; c < cnt[0]; ++c)
-                {
-                  const int cell = co[0][c] + 2 * co[1][bb] + 4 * co[2][a];
-                  s += sa[cell * N3 + (lo[2][a] * NN + lo[1][bb]) * NN + lo[0][c]];
-                }
           sr[t] = bres[it] - s;
         }
       __syncthreads();
@@ -647,7 +640,7 @@ namespace gtps
 #pragma unroll
           for (int jj = 0; jj < NI; ++jj)
             v[jj] = sr[t * NI + jj];
-          const long long g0 = first + 1 + (long long)(j + 1) * N1 + (l + 1) * P;
+          const int g0 = 1 + (j + 1) * N1 + (l + 1) * P;
 #pragma unroll
           for (int i = 0; i < NI; ++i)
             {
@@ -656,7 +649,7 @@ namespace gtps
               for (int jj = 0; jj < NI; ++jj)
                 s += Sx[i * NI + jj] * v[jj];
               const T uold = sbox[((l + 1) * NP + j + 1) * NP + i + 1];
-              u[g0 + i]    = uold + relax * s;
+              ub[g0 + i]   = uold + relax * s;
             }
         }
     }
@@ -755,7 +748,8 @@ namespace gtps
             (long long)((n - px) / 2) * ((n - py) / 2) * ((n - pz) / 2);
           if (np == 0)
             return;
-          GTPS_REQUIRE(np < (1LL << 31), "mesh too large for the patch kernel");
+          GTPS_REQUIRE(np < (1LL << 31) && lvl->N < (1LL << 31),
+                       "mesh too large for the patch kernel");
           GTPS_REQUIRE(lvl->sep_bins && lvl->sep_sigma, "level has no separable fit");
           const T *sep = (const T *)get_sep_tables(lvl->ctx, K, lvl->prec);
           hipLaunchKernelGGL((patch_coeff_kernel<K, T>), dim3((unsigned)np),
