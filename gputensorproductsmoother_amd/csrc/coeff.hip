@@ -282,12 +282,6 @@ namespace gtps
       static constexpr bool ok = lds <= 65536;
     };
 
-    __device__ __forceinline__ int
-    n_patches(int n, int parity)
-    {
-      return (n - parity) / 2;
-    }
-
     // One thread per vertex patch (lower-left cell (px, py, pz), patches
     // lexicographic over (n-1)^3): the log-space least-squares ratios of the
     // 8 cell coefficients per direction, rounded to the SEP_STEP grid, and
@@ -357,7 +351,8 @@ namespace gtps
                        const T *__restrict__ coef, const T *__restrict__ tab,
                        const T *__restrict__ sep, const int *__restrict__ fit_bins,
                        const T *__restrict__ fit_sigma, const int n, const int colour,
-                       const T scale, const T relax)
+                       const T scale, const T relax, const FastDiv div_x,
+                       const FastDiv div_xy)
     {
       using TB         = Tab<K>;
       using C          = PC<K, T>;
@@ -375,9 +370,11 @@ namespace gtps
       const int    N1   = K * n + 1;
       const int    P    = N1 * N1;
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
-      const int npx = n_patches(n, px), npy = n_patches(n, py);
-      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
-      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      // patch coordinates: divisions by the patch counts as multiply-high
+      // and shift (divisors set up on the host)
+      const int q  = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qz = (int)fastdiv((unsigned)q, div_xy), rq = q - qz * (int)div_xy.d;
+      const int qy = (int)fastdiv((unsigned)rq, div_x), qx = rq - qy * (int)div_x.d;
       const int c0x = 2 * qx + px, c0y = 2 * qy + py, c0z = 2 * qz + pz;
       const int first = K * c0x + K * c0y * N1 + K * c0z * P;
       T *__restrict__ ub       = u + first;
@@ -752,11 +749,13 @@ namespace gtps
                        "mesh too large for the patch kernel");
           GTPS_REQUIRE(lvl->sep_bins && lvl->sep_sigma, "level has no separable fit");
           const T *sep = (const T *)get_sep_tables(lvl->ctx, K, lvl->prec);
+          const int npx = (n - px) / 2, npy = (n - py) / 2;
           hipLaunchKernelGGL((patch_coeff_kernel<K, T>), dim3((unsigned)np),
                              dim3(CNT), 0, lvl->ctx->stream, u, b,
                              (const T *)lvl->coef, (const T *)lvl->dtab(), sep,
                              (const int *)lvl->sep_bins, (const T *)lvl->sep_sigma, n,
-                             colour, (T)lvl->scale, (T)relax);
+                             colour, (T)lvl->scale, (T)relax, make_fastdiv((unsigned)npx),
+                             make_fastdiv((unsigned)(npx * npy)));
           GTPS_LAUNCH_CHECK();
         }
       else

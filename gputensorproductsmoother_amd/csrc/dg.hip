@@ -707,16 +707,19 @@ namespace gtps
     __global__ void __launch_bounds__((DP<K, T>::NT))
     dg_patch_kernel(T *__restrict__ u, const T *__restrict__ bvec,
                     const T *__restrict__ tab, const int n, const int colour,
-                    const T scale, const T relax)
+                    const T scale, const T relax, const FastDiv div_x,
+                    const FastDiv div_xy)
     {
       __shared__ DPShared<K, T> sh;
       const long long NNN = (long long)n * (K + 1);
       u += (long long)blockIdx.y * NNN * NNN * NNN;
       bvec += (long long)blockIdx.y * NNN * NNN * NNN;
       const int cx = colour % 3, cy = (colour / 3) % 3, cz = colour / 9;
-      const int npx = (n - 1 - cx + 2) / 3, npy = (n - 1 - cy + 2) / 3;
-      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
-      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      // patch coordinates: divisions by the patch counts as multiply-high
+      // and shift (divisors set up on the host)
+      const int q  = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qz = (int)fastdiv((unsigned)q, div_xy), rq = q - qz * (int)div_xy.d;
+      const int qy = (int)fastdiv((unsigned)rq, div_x), qx = rq - qy * (int)div_x.d;
       const int pc[3] = {cx + 3 * qx, cy + 3 * qy, cz + 3 * qz};
       dg_patch_solve<K, T>(u, bvec, tab, n, pc, scale, relax, sh);
     }
@@ -1068,12 +1071,13 @@ namespace gtps
                   const int cx = c % 3, cy = (c / 3) % 3, cz = c / 9;
                   if (cx >= n - 1 || cy >= n - 1 || cz >= n - 1)
                     continue;
-                  const long long np = (long long)((n - 1 - cx + 2) / 3) *
-                                       ((n - 1 - cy + 2) / 3) * ((n - 1 - cz + 2) / 3);
+                  const int       npx = (n - 1 - cx + 2) / 3, npy = (n - 1 - cy + 2) / 3;
+                  const long long np  = (long long)npx * npy * ((n - 1 - cz + 2) / 3);
                   hipLaunchKernelGGL((dg_patch_kernel<K, T>), dim3((unsigned)np, nvec),
                                      dim3(DP<K, T>::NT),
                                      0, s, u, b, (const T *)lvl->tab, n, c, (T)lvl->scale,
-                                     (T)relax);
+                                     (T)relax, make_fastdiv((unsigned)npx),
+                                     make_fastdiv((unsigned)(npx * npy)));
                   GTPS_LAUNCH_CHECK();
                 }
             }
