@@ -335,25 +335,36 @@ namespace gtps
       const int w0x = 2 * K * c0x - 2 * K + 1, w0y = 2 * K * c0y - 2 * K + 1,
                 w0z = 2 * K * c0z - 2 * K + 1;
       {
-        // the whole window in one round of loads (unrolled), then to LDS
-        constexpr int TOT = WZ * WY * WX, IT = (TOT + NT - 1) / NT;
-        T             v[IT];
+        // the whole window in one round of loads, one z-column (y, x) of WZ
+        // values per thread and pass: the column's address and x/y bounds
+        // are set up once, the z bounds are uniform
+        constexpr int NC = WY * WX, IT = (NC + NT - 1) / NT;
+        T             v[IT][WZ];
 #pragma unroll
         for (int j = 0; j < IT; ++j)
           {
-            const int i  = tid + j * NT;
-            const int x  = i % WX, y = (i / WX) % WY, z = i / (WX * WY);
-            const int gx = w0x + x, gy = w0y + y, gz = w0z + z;
-            const bool in = i < TOT && gx >= 0 && gx < Nf1 && gy >= 0 &&
-                            gy < Nf1 && gz >= 0 && gz < Nf1;
-            v[j] = in ? fine[gx + (long long)gy * Nf1 + gz * Pf] : T(0);
+            const int  col = tid + j * NT;
+            const int  y = col / WX, x = col - y * WX;
+            const int  gx = w0x + x, gy = w0y + y;
+            const bool in = col < NC && gx >= 0 && gx < Nf1 && gy >= 0 && gy < Nf1;
+            const T   *fc = fine + (in ? gx + (long long)gy * Nf1 : 0);
+#pragma unroll
+            for (int z = 0; z < WZ; ++z)
+              {
+                const int gz = w0z + z;
+                v[j][z]      = in && gz >= 0 && gz < Nf1 ? fc[gz * Pf] : T(0);
+              }
           }
 #pragma unroll
         for (int j = 0; j < IT; ++j)
           {
-            const int i = tid + j * NT;
-            if (i < TOT)
-              sF[i] = v[j];
+            const int col = tid + j * NT;
+            if (col < NC)
+              {
+#pragma unroll
+                for (int z = 0; z < WZ; ++z)
+                  sF[z * NC + col] = v[j][z];
+              }
           }
       }
       __syncthreads();
