@@ -194,62 +194,55 @@ namespace gtps
     }
 
     // DoF-centred sum of the element vectors, fixed order (lower cell
-    // first per direction).  Rows: RESID y = b - A x; TRANSPOSE boundary
-    // rows y_B = sum + x_B; otherwise identity rows.  DOT: per-block
-    // partial sums of x.y.
+    // first per direction, z outermost).  Per direction the DoF at node g
+    // is node g - K c of the lower cell c = (g-1)/K (g >= 1), and, on a cell
+    // boundary (g = K c', c' < n), node 0 of the upper cell c'; the up to 8
+    // contributions are read at offsets summed per direction, the absent
+    // ones masked.  Rows: RESID y = b - A x; TRANSPOSE boundary rows y_B =
+    // sum + x_B; otherwise identity rows.  DOT: per-block partial sums of
+    // x.y.  Level and element-buffer sizes < 2^31 (host check).
     template <int K, typename T, bool RESID, bool TRANSPOSE, bool DOT>
     __global__ void __launch_bounds__(CNT)
-    gather_kernel(const long long N, const int N1, const int n,
+    gather_kernel(const int N, const int N1, const int n,
                   const T *__restrict__ E, const T *__restrict__ x,
                   const T *__restrict__ b, T *__restrict__ y,
-                  double *__restrict__ partials)
+                  double *__restrict__ partials, const FastDiv div_n1,
+                  const FastDiv div_p)
     {
       constexpr int NN = K + 1, N3 = NN * NN * NN;
       __shared__ double red[CNT / 64];
       double            dsum = 0.0;
-      const long long   P    = (long long)N1 * N1;
-      for (long long i = (long long)blockIdx.x * CNT + threadIdx.x; i < N;
-           i += (long long)gridDim.x * CNT)
+      const int         P    = N1 * N1;
+      // element-buffer stride of one cell step and one node step per direction
+      const int cs[3] = {N3, n * N3, n * n * N3}, ns[3] = {1, NN, NN * NN};
+      for (int i = blockIdx.x * CNT + threadIdx.x; i < N; i += gridDim.x * CNT)
         {
-          const int gx = (int)(i % N1), gy = (int)((i / N1) % N1),
-                    gz = (int)(i / P);
-          int cxs[2], lxs[2], cys[2], lys[2], czs[2], lzs[2];
-          int nx = 0, ny = 0, nz = 0;
-          auto split = [&](int g, int *cs, int *ls, int &cnt) {
-            const int c = g / K, l = g - c * K;
-            if (l == 0)
-              {
-                if (c > 0)
-                  {
-                    cs[cnt] = c - 1;
-                    ls[cnt] = K;
-                    ++cnt;
-                  }
-                if (c < n)
-                  {
-                    cs[cnt] = c;
-                    ls[cnt] = 0;
-                    ++cnt;
-                  }
-              }
-            else
-              {
-                cs[cnt] = c;
-                ls[cnt] = l;
-                ++cnt;
-              }
-          };
-          split(gx, cxs, lxs, nx);
-          split(gy, cys, lys, ny);
-          split(gz, czs, lzs, nz);
+          const int gz = (int)fastdiv((unsigned)i, div_p), r = i - gz * P;
+          const int gy = (int)fastdiv((unsigned)r, div_n1), gx = r - gy * N1;
+          const int g[3] = {gx, gy, gz};
+          int       off[3][2];
+          bool      on[3][2];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            {
+              const int cl = (g[d] - 1) / K;  // lower cell (g >= 1)
+              const int cu = g[d] / K;        // upper cell on a cell boundary
+              on[d][0]     = g[d] >= 1;
+              on[d][1]     = g[d] == K * cu && cu < n;
+              off[d][0]    = cl * cs[d] + (g[d] - K * cl) * ns[d];
+              off[d][1]    = cu * cs[d];
+            }
           T s = 0;
-          for (int a = 0; a < nz; ++a)
-            for (int bb = 0; bb < ny; ++bb)
-              for (int c = 0; c < nx; ++c)
+#pragma unroll
+          for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+            for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+              for (int ox = 0; ox < 2; ++ox)
                 {
-                  const long long cell = cxs[c] + (long long)n * cys[bb] +
-                                         (long long)n * n * czs[a];
-                  s += E[cell * N3 + (lzs[a] * NN + lys[bb]) * NN + lxs[c]];
+                  const bool ok = on[0][ox] && on[1][oy] && on[2][oz];
+                  const T    v  = ok ? E[off[0][ox] + off[1][oy] + off[2][oz]] : T(0);
+                  s += v;
                 }
           const bool bd = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1 ||
                           gz == 0 || gz == N1 - 1;
@@ -712,9 +705,14 @@ namespace gtps
       hipStream_t s   = lvl->ctx->stream;
       double     *pt  = lvl->ctx->ws.partials;
       const int   nb  = grid_cap(lvl->N, dot ? std::min(lvl->ctx->ws.capacity, 2048) : 8192);
+      GTPS_REQUIRE(lvl->N < (1LL << 31) &&
+                     (long long)lvl->n * lvl->n * lvl->n * (K + 1) * (K + 1) * (K + 1) < (1LL << 31),
+                   "mesh too large for the gather kernel");
+      const FastDiv dn1 = make_fastdiv((unsigned)lvl->N1);
+      const FastDiv dp  = make_fastdiv((unsigned)(lvl->N1 * lvl->N1));
 #define GTPS_G(R, TR, D)                                                      \
   hipLaunchKernelGGL((gather_kernel<K, T, R, TR, D>), dim3(nb), dim3(CNT), 0, \
-                     s, lvl->N, lvl->N1, lvl->n, (const T *)E, x, b, y, pt)
+                     s, (int)lvl->N, lvl->N1, lvl->n, (const T *)E, x, b, y, pt, dn1, dp)
       if (transpose)
         {
           GTPS_REQUIRE(b == nullptr && !dot, "transpose form has no residual/dot");
