@@ -107,10 +107,10 @@ namespace gtps
       for (int j = tid; j < ld; j += CNT)
         sd[j] = j < ni ? d[interior_index<DIM>(j, N1)] : T(0);
       // Dirichlet rows: zero (spread over the grid)
-      for (long long g = (long long)blockIdx.x * CNT + tid; DIM != 0 && g < N;
-           g += (long long)gridDim.x * CNT)
+      // (the dense form only serves levels of a few thousand DoFs: 32-bit)
+      for (int g = blockIdx.x * CNT + tid; DIM != 0 && g < (int)N; g += gridDim.x * CNT)
         {
-          const long long x = g % N1, y = (g / N1) % N1, z = DIM == 3 ? g / ((long long)N1 * N1) : 1;
+          const int x = g % N1, y = (g / N1) % N1, z = DIM == 3 ? g / (N1 * N1) : 1;
           bool bd = x == 0 || x == N1 - 1 || y == 0 || y == N1 - 1;
           if (DIM == 3)
             bd = bd || z == 0 || z == N1 - 1;

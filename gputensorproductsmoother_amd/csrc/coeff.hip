@@ -65,7 +65,8 @@ namespace gtps
     cell_kernel(const T *__restrict__ x, const T *__restrict__ lam,
                 const T *__restrict__ coef, T *__restrict__ out,
                 const T *__restrict__ tab, const int n, const int N1,
-                const T scale, const int mask)
+                const T scale, const int mask, const FastDiv div_n,
+                const FastDiv div_nn)
     {
       using TB          = Tab<K>;
       constexpr int NN  = K + 1, N2 = NN * NN, N3 = N2 * NN;
@@ -76,19 +77,20 @@ namespace gtps
       const T      *cK   = tab + TB::cellK;
       const int     tid  = threadIdx.x;
       const int     lc   = tid / N2, t = tid - lc * N2;
-      const long long ncells = (long long)n * n * n;
-      const long long cell =
-        (long long)xcd_logical_id(blockIdx.x, gridDim.x) * CPB + lc;
+      // level and element-buffer sizes < 2^31 (host check): 32-bit indices,
+      // cell coordinates by host-computed multiply-high divisors
+      const int  ncells = n * n * n;
+      const int  cell   = xcd_logical_id(blockIdx.x, gridDim.x) * CPB + lc;
       const bool active = lc < CPB && cell < ncells;
-      const long long P = (long long)N1 * N1;
-      int cx = 0, cy = 0, cz = 0;
-      long long base = 0;
+      const int  P      = N1 * N1;
+      int        cx = 0, cy = 0, cz = 0, base = 0;
       if (active)
         {
-          cx   = (int)(cell % n);
-          cy   = (int)((cell / n) % n);
-          cz   = (int)(cell / ((long long)n * n));
-          base = (long long)K * cx + (long long)K * cy * N1 + (long long)K * cz * P;
+          cz   = (int)fastdiv((unsigned)cell, div_nn);
+          const int rc = cell - cz * n * n;
+          cy   = (int)fastdiv((unsigned)rc, div_n);
+          cx   = rc - cy * n;
+          base = K * cx + K * cy * N1 + K * cz * P;
         }
       T *ca = sa + (active ? lc : 0) * N3;
       T *cb = sb + (active ? lc : 0) * N3;
@@ -101,7 +103,7 @@ namespace gtps
           for (int iz = 0; iz < NN; ++iz)
             {
               const bool m = mask && on_bdry(K * cx + ix, K * cy + iy, K * cz + iz, N1);
-              v[iz] = m ? T(0) : x[base + ix + (long long)iy * N1 + iz * P];
+              v[iz] = m ? T(0) : x[base + ix + iy * N1 + iz * P];
             }
 #pragma unroll
           for (int lz = 0; lz < NN; ++lz)
@@ -174,7 +176,7 @@ namespace gtps
                 {
                   const int gx = K * cx + lx, gy = K * cy + ly, gz = K * cz + lz;
                   if (!on_bdry(gx, gy, gz, N1))
-                    part += (double)lam[base + lx + (long long)ly * N1 + lz * P] *
+                    part += (double)lam[base + lx + ly * N1 + lz * P] *
                             (double)(f * y);
                 }
             }
@@ -681,17 +683,20 @@ namespace gtps
       const long long ncells = (long long)lvl->n * lvl->n * lvl->n;
       constexpr int   CPB    = cells_per_block(K);
       const long long nb     = (ncells + CPB - 1) / CPB;
-      GTPS_REQUIRE(nb < (1LL << 31), "mesh too large for the cell kernel");
-      const T *tab = (const T *)lvl->dtab();
+      GTPS_REQUIRE(lvl->N < (1LL << 31) && ncells * (K + 1) * (K + 1) * (K + 1) < (1LL << 31),
+                   "mesh too large for the cell kernel");
+      const T      *tab = (const T *)lvl->dtab();
+      const FastDiv dn  = make_fastdiv((unsigned)lvl->n);
+      const FastDiv dnn = make_fastdiv((unsigned)(lvl->n * lvl->n));
       if (mode == 0)
         hipLaunchKernelGGL((cell_kernel<K, T, 0>), dim3((unsigned)nb), dim3(CNT),
                            0, lvl->ctx->stream, x, nullptr,
                            (const T *)lvl->coef, out, tab, lvl->n, lvl->N1,
-                           (T)lvl->scale, mask ? 1 : 0);
+                           (T)lvl->scale, mask ? 1 : 0, dn, dnn);
       else
         hipLaunchKernelGGL((cell_kernel<K, T, 1>), dim3((unsigned)nb), dim3(CNT),
                            0, lvl->ctx->stream, x, lam, nullptr, out, tab,
-                           lvl->n, lvl->N1, (T)lvl->scale, 0);
+                           lvl->n, lvl->N1, (T)lvl->scale, 0, dn, dnn);
       GTPS_LAUNCH_CHECK();
     }
 

@@ -123,13 +123,17 @@ namespace gtps
                  const T *__restrict__ tab, const T *__restrict__ b,
                  const T *__restrict__ x0, const int n, const int N1,
                  const long long N, const T scale,
-                 double *__restrict__ partials)
+                 double *__restrict__ partials, const FastDiv div_n1,
+                 const FastDiv div_n)
     {
       using TB         = DTab<K>;
       constexpr int NN = TB::NN;
       __shared__ double red[DNT / 64];
       double            dot = 0.0;
-      const long long   P   = (long long)N1 * N1;
+      // level sizes < 2^31 (host check): 32-bit indices within a vector,
+      // divisions by N1 and n as multiply-high and shift
+      const int         P   = N1 * N1;
+      const int         Ni  = (int)N;
       // batch: blockIdx.y selects one of several contiguous vectors
       const long long   bo  = (long long)blockIdx.y * N;
       i1 += bo;
@@ -147,11 +151,10 @@ namespace gtps
         }
       if constexpr (PASS == 2)
         {
-          for (long long g = (long long)blockIdx.x * DNT + threadIdx.x; g < N;
-               g += (long long)gridDim.x * DNT)
+          for (int g = blockIdx.x * DNT + threadIdx.x; g < Ni; g += gridDim.x * DNT)
             {
-              const int       X    = (int)(g % N1);
-              const long long base = g - X;
+              const int X    = g - (int)fastdiv((unsigned)g, div_n1) * N1;
+              const int base = g - X;
               T v = scale * (dg_L_line<K>(tab, i1, base, 1, X, n) +
                              dg_M_line<K>(tab, i2, base, 1, X));
               if (RESID)
@@ -161,33 +164,33 @@ namespace gtps
                 dot += (double)x0[g] * (double)v;
             }
         }
-      const long long st = PASS == 0 ? P : (long long)N1;
-      const long long NL = PASS == 2 ? 0 : N / NN;
-      for (long long q = (long long)blockIdx.x * DNT + threadIdx.x; q < NL;
-           q += (long long)gridDim.x * DNT)
+      const int st = PASS == 0 ? P : N1;
+      const int NL = PASS == 2 ? 0 : Ni / NN;
+      for (int q = blockIdx.x * DNT + threadIdx.x; q < NL; q += gridDim.x * DNT)
         {
-          int       c;
-          long long base;
+          int c, base;
           if (PASS == 0)
             {
-              // lanes along x, lines along z
-              const long long r = q / N1;
-              base = q - r * N1 + (r % N1) * N1;
-              c    = (int)(r / N1);
+              // lanes along x, lines along z: q = x + N1 (y + N1 c)
+              const int r  = (int)fastdiv((unsigned)q, div_n1);
+              const int r2 = (int)fastdiv((unsigned)r, div_n1);
+              base = q - r * N1 + (r - r2 * N1) * N1;
+              c    = r2;
             }
           else if (PASS == 1)
             {
-              // lanes along x, lines along y
-              const long long r = q / N1;
-              base = q - r * N1 + (r / n) * P;
-              c    = (int)(r % n);
+              // lanes along x, lines along y: q = x + N1 (c + n z)
+              const int r  = (int)fastdiv((unsigned)q, div_n1);
+              const int z  = (int)fastdiv((unsigned)r, div_n);
+              base = q - r * N1 + z * P;
+              c    = r - z * n;
             }
           else
             {
               base = 0;
               c    = 0;
             }
-          const long long c0 = base + (long long)c * NN * st;
+          const int c0 = base + c * NN * st;
           T a[NN], pm[NN], pp[NN], bb[NN];
 #pragma unroll
           for (int j = 0; j < NN; ++j)
@@ -220,7 +223,7 @@ namespace gtps
                     for (int j = 0; j < NN; ++j)
                       l += Lp[i * NN + j] * pp[j];
                   }
-                const long long g = c0 + i * st;
+                const int g = c0 + i * st;
                 if (PASS == 0)
                   {
                     T m = 0;
@@ -1001,12 +1004,14 @@ namespace gtps
       const dim3 nbd(dgrid(lvl->N, std::min(lvl->ctx->ws.capacity / nvec, 2048)), nvec);
       double    *pt = lvl->ctx->ws.partials;
       const T    sc = (T)lvl->scale;
+      GTPS_REQUIRE(lvl->N < (1LL << 31), "DG level too large for the operator kernels");
+      const FastDiv dn1 = make_fastdiv((unsigned)lvl->N1), dn = make_fastdiv((unsigned)lvl->n);
       hipLaunchKernelGGL((dg_op_kernel<K, T, 0, false, false>), nbl, dim3(DNT), 0,
                          s, x, nullptr, t0, t1, tab, nullptr, nullptr, lvl->n,
-                         lvl->N1, lvl->N, sc, pt);
+                         lvl->N1, lvl->N, sc, pt, dn1, dn);
       hipLaunchKernelGGL((dg_op_kernel<K, T, 1, false, false>), nbl, dim3(DNT), 0,
                          s, t0, t1, t2, t3, tab, nullptr, nullptr, lvl->n,
-                         lvl->N1, lvl->N, sc, pt);
+                         lvl->N1, lvl->N, sc, pt, dn1, dn);
       // x pass: whole rows staged in LDS when a row fits, else per DoF
       const bool rows_fit = lvl->N1 <= DG_ROW_CAP;
       const int  RB       = std::max(1, 1024 / lvl->N1);
@@ -1022,7 +1027,8 @@ namespace gtps
                            s, t2, t3, y, tab, b, x, lvl->n, lvl->N1, lvl->N, RB, sc, pt); \
       else                                                                           \
         hipLaunchKernelGGL((dg_op_kernel<K, T, 2, R, D>), G, dim3(DNT), 0, s, t2, t3, \
-                           y, nullptr, tab, b, x, lvl->n, lvl->N1, lvl->N, sc, pt);  \
+                           y, nullptr, tab, b, x, lvl->n, lvl->N1, lvl->N, sc, pt,   \
+                           dn1, dn);                                                 \
     }                                                                                \
   while (0)
       if (b && dot)
