@@ -400,24 +400,35 @@ namespace gtps
       const int ox0 = (L % nbx) * B, oy0 = ((L / nbx) % nbx) * B,
                 oz0 = (L / (nbx * nbx)) * B;
       {
-        constexpr int IT = (NBOX + NT - 1) / NT;
-        T             v[IT];
+        // one z-column (x, y) of the box per thread: the column's address and
+        // x/y bounds set up once, the z bounds uniform
+        constexpr int NC = WB * WB, IT = (NC + NT - 1) / NT;
+        T             v[IT][WB];
 #pragma unroll
         for (int j = 0; j < IT; ++j)
           {
-            const int i  = tid + j * NT;
-            const int bx = i % WB, by = (i / WB) % WB, bz = i / (WB * WB);
-            const int gx = ox0 - K + bx, gy = oy0 - K + by, gz = oz0 - K + bz;
-            const bool in = i < NBOX && gx >= 0 && gx < N1 && gy >= 0 &&
-                            gy < N1 && gz >= 0 && gz < N1;
-            v[j] = in ? x[gx + (long long)gy * N1 + gz * P] : T(0);
+            const int  col = tid + j * NT;
+            const int  by = col / WB, bx = col - by * WB;
+            const int  gx = ox0 - K + bx, gy = oy0 - K + by;
+            const bool in = col < NC && gx >= 0 && gx < N1 && gy >= 0 && gy < N1;
+            const T   *xc = x + (in ? gx + (long long)gy * N1 : 0);
+#pragma unroll
+            for (int bz = 0; bz < WB; ++bz)
+              {
+                const int gz = oz0 - K + bz;
+                v[j][bz]     = in && gz >= 0 && gz < N1 ? xc[gz * P] : T(0);
+              }
           }
 #pragma unroll
         for (int j = 0; j < IT; ++j)
           {
-            const int i = tid + j * NT;
-            if (i < NBOX)
-              sBox[i] = v[j];
+            const int col = tid + j * NT;
+            if (col < NC)
+              {
+#pragma unroll
+                for (int bz = 0; bz < WB; ++bz)
+                  sBox[bz * NC + col] = v[j][bz];
+              }
           }
       }
       // the x-pass outputs of this thread (row (ly, lz) = tid < B*B): their
@@ -450,8 +461,9 @@ namespace gtps
             {
               const int p = lz % K;
               T         a = 0, b = 0;
+              // band entries outside band_lo .. band_hi are zero
 #pragma unroll
-              for (int o = -K; o <= K; ++o)
+              for (int o = band_lo<K>(p); o <= band_hi<K>(p); ++o)
                 {
                   const T m = bM[p * W + o + K], kk = bK[p * W + o + K];
                   a += m * c[lz + K + o];
@@ -480,11 +492,12 @@ namespace gtps
               const int p = ly % K;
               T         c1 = 0, c2 = 0;
 #pragma unroll
-              for (int o = -K; o <= K; ++o)
+              for (int o = band_lo<K>(p); o <= band_hi<K>(p); ++o)
                 {
                   const T m = bM[p * W + o + K], kk = bK[p * W + o + K];
                   c1 += m * a[ly + K + o];
-                  c2 += kk * a[ly + K + o] + m * b[ly + K + o];
+                  c2 += kk * a[ly + K + o];
+                  c2 += m * b[ly + K + o];
                 }
               sC1[(lz * B + ly) * WB + bx] = c1;
               sC2[(lz * B + ly) * WB + bx] = c2;
@@ -513,9 +526,11 @@ namespace gtps
               const int p = lx % K;
               T         v = 0;
 #pragma unroll
-              for (int o = -K; o <= K; ++o)
-                v += bK[p * W + o + K] * c1[lx + K + o] +
-                     bM[p * W + o + K] * c2[lx + K + o];
+              for (int o = band_lo<K>(p); o <= band_hi<K>(p); ++o)
+                {
+                  v += bK[p * W + o + K] * c1[lx + K + o];
+                  v += bM[p * W + o + K] * c2[lx + K + o];
+                }
               const bool bd = gx == 0 || gx == N1 - 1 || gy == 0 || gy == N1 - 1 ||
                               gz == 0 || gz == N1 - 1;
               const long long g  = gx + (long long)gy * N1 + gz * P;
