@@ -367,12 +367,24 @@ namespace gtps
     // ((B+2k)^3 / B^3) come from L2.
     // B is a multiple of k, so block origins sit on cell boundaries and the
     // node position inside a cell (which selects the band coefficients) is
-    // a compile-time function of the unrolled output index.
+    // a compile-time function of the unrolled output index.  From k = 5 on
+    // a block is a single cell (halo volume 27x the block), so the box form
+    // pays only up to 8 cells per direction; k = 6, 7 in FP64 exceed the
+    // 64 KiB of static LDS and always march.
     template <int K>
     struct VB
     {
-      static constexpr int B = K <= 3 ? K * ((8 + K - 1) / K) : 4, NT = 256;
+      static constexpr int B = K <= 3 ? K * ((8 + K - 1) / K) : (K == 4 ? 4 : K), NT = 256;
       static constexpr int WB = B + 2 * K;
+      // static LDS of the box kernel (box, the two z-contractions)
+      template <typename T>
+      static constexpr bool fits()
+      {
+        constexpr int NBOX = WB * WB * WB, NCC = WB * B * B;
+        return sizeof(T) * ((NBOX > 2 * NCC ? NBOX : 2 * NCC) + 2 * WB * WB * B) +
+                 sizeof(double) * (NT / 64) <=
+               65536;
+      }
     };
 
     template <int K, typename T, bool RESID, bool DOT>
@@ -678,8 +690,8 @@ namespace gtps
       const T   *xp    = reinterpret_cast<const T *>(x);
       T         *yp    = reinterpret_cast<T *>(y);
       const T   *bp    = reinterpret_cast<const T *>(b);
-      if constexpr (K <= 4)
-        if (lvl->dim == 3 && n <= tuning().box_vmult_max_n)
+      if constexpr (VB<K>::template fits<T>())
+        if (lvl->dim == 3 && n <= (K >= 5 ? tuning().box_vmult_max_n / 2 : tuning().box_vmult_max_n))
           {
             using C        = VB<K>;
             const int nbx  = (lvl->N1 + C::B - 1) / C::B;

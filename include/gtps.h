@@ -86,10 +86,11 @@ const char *gtps_version(void);
  *                           kernel, each with its own patches: 1 or 4 (4
  *                           applies to 1 or 2 patches per wavefront); 0
  *                           (default): 4 in FP32, 1 in FP64;
- *   "box_vmult_max_n"       3D operator with k <= 4 on levels with at most
- *                           this many cells per direction: one-round box
- *                           kernel instead of the z-marching one (default
- *                           16);
+ *   "box_vmult_max_n"       3D operator on levels with at most this many
+ *                           cells per direction (half as many for k >= 5):
+ *                           one-round box kernel instead of the z-marching
+ *                           one (default 16; k = 6, 7 in FP64 always
+ *                           march);
  *   "row_staged"            patch-group smoother with its global accesses
  *                           staged through contiguous x-runs (1) or
  *                           per-lane pencils and rows (0, default).

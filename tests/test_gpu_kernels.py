@@ -233,9 +233,9 @@ def test_smoother_both_kernel_paths(core, ctx, dim, k, level, variant, prec, pat
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("path", ["box", "march"])
-@pytest.mark.parametrize("k,level", [(1, 3), (2, 3), (3, 2), (3, 4), (4, 3), (3, 5)])
+@pytest.mark.parametrize("k,level", [(1, 3), (2, 3), (3, 2), (3, 4), (4, 3), (3, 5), (5, 2), (6, 2), (7, 2)])
 def test_vmult_box_and_march_paths(core, ctx, k, level, path, prec):
-    """The one-round box kernel (small levels, k <= 4) and the z-marching
+    """The one-round box kernel (small levels; k >= 6 in FP32 only) and the z-marching
     kernel agree with the oracle on the same level (vmult and residual)."""
     _announce("vmult-path", k, level, path, prec)
     g = rng(level * 5 + k)
