@@ -445,23 +445,41 @@ namespace gtps
         const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
         return x + y * N1 + z * P;
       };
-      // tables
-      for (int e = tid; e < 3 * PD * PD; e += NT)
-        {
-          const int d = e / (PD * PD);
-          Ssh[d][e - d * PD * PD] = tab[TB::S + variant(pc[d]) * PD * PD + e - d * PD * PD];
-        }
-      for (int e = tid; e < 3 * PD; e += NT)
-        {
-          const int d = e / PD;
-          lsh[d][e - d * PD] = tab[TB::lam + variant(pc[d]) * PD + e - d * PD];
-        }
-      for (int e = tid; e < NN * NN; e += NT)
-        {
-          Msh[e]    = tab[TB::M + e];
-          Lsh[0][e] = tab[TB::Lm + e];
-          Lsh[1][e] = tab[TB::Lp + e];
-        }
+      // Coupling tasks (face ff, line ln) and FD tasks (output block fh,
+      // line fln): lines per face or block padded to whole wavefronts where
+      // that fits the work-group, else packed.
+      constexpr int CL = NS * NLP <= NT ? NLP : NL;
+      static_assert(NS * CL <= NT, "one coupling task per thread");
+      constexpr int OB = (PD / 2) * NLP <= NT ? 2 : NN, NB = PD / OB;
+      constexpr int FL = NB * NLP <= NT ? NLP : NL;
+      static_assert(NB * FL <= NT, "one FD task per thread");
+      // With padded tasks a wavefront's face (block) is uniform, and so are
+      // the 1D matrix entries it multiplies by: they are read from the
+      // global table at wave-uniform addresses (scalar loads, used as SGPR
+      // operands of the multiply-adds).  Packed tasks read LDS copies.
+      constexpr bool CU = CL % 64 == 0, FU = FL % 64 == 0;
+      const T *Sg[3] = {tab + TB::S + variant(pc[0]) * PD * PD,
+                        tab + TB::S + variant(pc[1]) * PD * PD,
+                        tab + TB::S + variant(pc[2]) * PD * PD};
+      if constexpr (!FU)
+        for (int e = tid; e < 3 * PD * PD; e += NT)
+          {
+            const int d = e / (PD * PD);
+            Ssh[d][e - d * PD * PD] = Sg[d][e - d * PD * PD];
+          }
+      // the eigenvalues: loaded here, stored to LDS once the patch loads are
+      // in flight (a store now would wait for this load first)
+      static_assert(3 * PD <= NT, "one eigenvalue per thread");
+      const bool lam_t = tid < 3 * PD;
+      const int  lam_d = lam_t ? tid / PD : 0, lam_i = tid - lam_d * PD;
+      const T    lam_v = lam_t ? tab[TB::lam + variant(pc[lam_d]) * PD + lam_i] : T(0);
+      if constexpr (!CU)
+        for (int e = tid; e < NN * NN; e += NT)
+          {
+            Msh[e]    = tab[TB::M + e];
+            Lsh[0][e] = tab[TB::Lm + e];
+            Lsh[1][e] = tab[TB::Lp + e];
+          }
       // b_P and the old patch values (relax != 1), issued before the slab
       // loads and their first wait
       constexpr int ITB   = (P3 + NT - 1) / NT;
@@ -497,19 +515,15 @@ namespace gtps
           }
       };
       load_slabs(0);
+      if (lam_t)
+        lsh[lam_d][lam_i] = lam_v;
 #pragma unroll
       for (int it = 0; it < ITB; ++it)
         if (tid + it * NT < P3)
           r[tid + it * NT] = vb[it];
-      // 2. external coupling, NS faces per round.  A thread keeps its task
-      // (face ff, line ln) in every pass of a round, so the 1D matrices it
-      // needs (L- or L+ of its face, M) are read into registers once, with
-      // the first pass's slab values.
-      // lines per face: whole wavefronts (uniform face) where that fits the
-      // work-group, else packed
-      constexpr int CL = NS * NLP <= NT ? NLP : NL;
-      static_assert(NS * CL <= NT, "one coupling task per thread");
-      const int  cff  = CL % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / CL) : tid / CL;
+      // 2. external coupling, NS faces per round; a thread keeps its task
+      // (face ff, line ln) in every pass of a round
+      const int  cff  = CU ? __builtin_amdgcn_readfirstlane(tid / CL) : tid / CL;
       const int  cln  = tid - cff * CL;
       const bool cact = tid < NS * CL && cln < NL;
       for (int f0 = 0; f0 < 6; f0 += NS)
@@ -519,22 +533,10 @@ namespace gtps
           __syncthreads();
           const int  f  = f0 + cff;
           const bool on = cact && has_face(f);
-          // small degrees (FP32): L and M in registers; else read from LDS
-          constexpr bool CPRE = 2 * NN * NN * sizeof(T) <= 256;
-          T              Lr[CPRE ? NN * NN : 1], Mr[CPRE ? NN * NN : 1];
-          const T       *Lo = CPRE ? Lr : Lsh[f & 1];
-          const T       *Mc = CPRE ? Mr : Msh;
+          const T   *Lo = CU ? tab + ((f & 1) ? TB::Lp : TB::Lm) : Lsh[f & 1];
+          const T   *Mc = CU ? tab + TB::M : Msh;
           if (on)
             {
-              if constexpr (CPRE)
-                {
-#pragma unroll
-                  for (int e = 0; e < NN * NN; ++e)
-                    {
-                      Lr[e] = Lsh[f & 1][e];
-                      Mr[e] = Msh[e];
-                    }
-                }
               // w[a][b][i] = sum_j L-/+[i][j] s[a][b][j]: line (a, b)
               const T *sl = sa + cff * SL + cln * NN;
               T        v[NN];
@@ -593,21 +595,26 @@ namespace gtps
                 }
             }
           __syncthreads();
-          // r -= scale (face terms): per direction, the face on the DoF's side
+          // r -= scale (face terms): per direction, the face on the DoF's
+          // side; all reads issued together (absent faces read entry 0 and
+          // are masked out)
           for (int e = tid; e < P3; e += NT)
             {
               const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
-              T         acc   = 0;
+              T         v[3];
+              bool      on[3];
 #pragma unroll
               for (int D = 0; D < 3; ++D)
                 {
                   const int pD = c3[D], side = pD >= NN ? 1 : 0, f = 2 * D + side;
-                  if (f < f0 || f >= f0 + NS || !has_face(f))
-                    continue;
                   const int da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-                  acc += sw[(f - f0) * SL + (c3[da] * PD + c3[db]) * NN + pD - side * NN];
+                  on[D] = f >= f0 && f < f0 + NS && has_face(f);
+                  v[D]  = sw[on[D] ? (f - f0) * SL + (c3[da] * PD + c3[db]) * NN + pD - side * NN
+                                   : 0];
                 }
-              r[e] -= scale * acc;
+              const T re  = r[e];
+              const T acc = (on[0] ? v[0] : T(0)) + (on[1] ? v[1] : T(0)) + (on[2] ? v[2] : T(0));
+              r[e]        = re - scale * acc;
             }
           __syncthreads();
         }
@@ -618,19 +625,16 @@ namespace gtps
       // which is what the latency of a lone patch needs), else halves
       // (measured at Q4 24^3: FP32 7.77 against 7.87 ms per solve, FP64
       // 10.10 against 9.76 ms).  Every thread has one task, the same in all
-      // passes, so the next pass's matrix entries are read while the
-      // current pass computes: a pass waits for its line values only.
-      constexpr int OB = (PD / 2) * NLP <= NT ? 2 : NN, NB = PD / OB;
-      constexpr int FL = NB * NLP <= NT ? NLP : NL;
-      static_assert(NB * FL <= NT, "one FD task per thread");
-      const int  fh   = FL % 64 == 0 ? __builtin_amdgcn_readfirstlane(tid / FL) : tid / FL;
+      // passes; with packed tasks the next pass's matrix entries are read
+      // from LDS while the current pass computes.
+      const int  fh   = FU ? __builtin_amdgcn_readfirstlane(tid / FL) : tid / FL;
       const int  fln  = tid - fh * FL;
       const bool fact = tid < NB * FL && fln < NL;
       // (lp, lq): the other two coordinates of the line, lp the slower
       const int flp = fln / PD, flq = fln - flp * PD;
       // entries (j, m) of the direction's S^T (TR) or S for output block fh
       auto load_coef = [&](auto &cf, int D, bool TR) {
-        const T *S = Ssh[D];
+        const T *S = FU ? Sg[D] : Ssh[D];
 #pragma unroll
         for (int j = 0; j < PD; ++j)
 #pragma unroll
@@ -639,7 +643,7 @@ namespace gtps
       };
       // small blocks: the next pass's entries are read into registers during
       // this pass (two register sets); large ones are read from LDS as used
-      constexpr bool AHEAD = PD * OB * sizeof(T) <= 80;
+      constexpr bool AHEAD = !FU && PD * OB * sizeof(T) <= 80;
       auto pass = [&](const T *in, T *out, auto dirc, auto trc, auto divc, auto &cf,
                       auto &next, int nD, bool nTR, bool has_next) {
         constexpr int  D      = decltype(dirc)::value;
@@ -650,7 +654,8 @@ namespace gtps
           {
             const int base = D == 0 ? (flp * PD + flq) * PD
                                     : (D == 1 ? flp * PD * PD + flq : flp * PD + flq);
-            const T *S = TR ? Ssh[D] + fh * OB : Ssh[D] + fh * OB * PD;
+            const T *S0 = FU ? Sg[D] : Ssh[D];
+            const T *S  = TR ? S0 + fh * OB : S0 + fh * OB * PD;
             T        v[PD];
 #pragma unroll
             for (int j = 0; j < PD; ++j)
@@ -673,7 +678,14 @@ namespace gtps
                     acc += c * v[j];
                   }
                 if constexpr (DIVIDE)
-                  acc /= scale * (lsh[0][flq] + lsh[1][flp] + lsh[2][fh * OB + m]);
+                  {
+                    const T den = scale * (lsh[0][flq] + lsh[1][flp] + lsh[2][fh * OB + m]);
+                    // FP32: hardware reciprocal (1 ulp), FP64: IEEE division
+                    if constexpr (sizeof(T) == 4)
+                      acc *= __builtin_amdgcn_rcpf(den);
+                    else
+                      acc /= den;
+                  }
                 out[base + (fh * OB + m) * str] = acc;
               }
           }
