@@ -379,12 +379,11 @@ namespace gtps
       const long long pq  = c0x + (long long)(n - 1) * (c0y + (long long)(n - 1) * c0z);
       const int       pk  = fit_bins[pq];
       const T         sig = fit_sigma[pq];
-      if (tid < 8)
-        {
-          const int ox = tid & 1, oy = (tid >> 1) & 1, oz = tid >> 2;
-          sc[tid] = coef[(c0x + ox) + (long long)n * (c0y + oy) +
-                         (long long)n * n * (c0z + oz)];
-        }
+      // the 8 cell coefficients: loaded here, stored to LDS with the box
+      // (a store now would wait for this load before the box loads issue)
+      const int tc = tid < 8 ? tid : 0;
+      const T   cv = coef[(c0x + (tc & 1)) + (long long)n * (c0y + ((tc >> 1) & 1)) +
+                        (long long)n * n * (c0z + (tc >> 2))];
       // the box of u and this thread's interior b values: every load issued
       // before the first wait
       constexpr int ITU = (NP * NP * NP + CNT - 1) / CNT, ITR = (NI * NI * NI + CNT - 1) / CNT;
@@ -411,6 +410,8 @@ namespace gtps
         for (int it = 0; it < ITU; ++it)
           if (tid + it * CNT < NP * NP * NP)
             sbox[tid + it * CNT] = vu[it];
+        if (tid < 8)
+          sc[tid] = cv;
       }
       __syncthreads();
       const int h  = __builtin_amdgcn_readfirstlane(tid / HALF);  // wave-uniform

@@ -796,6 +796,8 @@ namespace gtps
       const int       ncd = nf / 2;  // coarse cells per direction
       const int       N1f = nf * NN, N1c = ncd * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
+      // level sizes < 2^31 (host check): offsets inside a brick are 32-bit
+      const int       Pf32 = (int)Pf, Pc32 = (int)Pc;
       fine += (long long)blockIdx.y * Nf;
       coarse += (long long)blockIdx.y * Pc * N1c;
       const int cc  = blockIdx.x;
@@ -804,13 +806,39 @@ namespace gtps
                     (long long)czc * NN * Pc;
       T        *f0  = fine + (long long)cxc * FB + (long long)cyc * FB * N1f +
                     (long long)czc * FB * Pf;
-      for (int e = tid; e < FB * NN; e += DNT)
-        sE[e] = tab[TB::E + e];
-      for (int e = tid; e < C::C3; e += DNT)
+      // every global load (embedding table, coarse cell, the fine values
+      // the last pass adds to) issued before the first wait
+      constexpr int ITE = (FB * NN + DNT - 1) / DNT, ITC = (C::C3 + DNT - 1) / DNT,
+                    ITF = (C::F3 + DNT - 1) / DNT;
+      T             ve[ITE], vc[ITC], vf[ITF];
+#pragma unroll
+      for (int it = 0; it < ITE; ++it)
         {
-          const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
-          sc[e] = c0[a + (long long)b * N1c + (long long)c * Pc];
+          const int e = tid + it * DNT;
+          ve[it]      = e < FB * NN ? tab[TB::E + e] : T(0);
         }
+#pragma unroll
+      for (int it = 0; it < ITC; ++it)
+        {
+          const int e = tid + it * DNT;
+          const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
+          vc[it] = e < C::C3 ? c0[a + b * N1c + c * Pc32] : T(0);
+        }
+#pragma unroll
+      for (int it = 0; it < ITF; ++it)
+        {
+          const int e  = tid + it * DNT;
+          const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
+          vf[it] = e < C::F3 ? f0[xf + yf * N1f + zf * Pf32] : T(0);
+        }
+#pragma unroll
+      for (int it = 0; it < ITE; ++it)
+        if (tid + it * DNT < FB * NN)
+          sE[tid + it * DNT] = ve[it];
+#pragma unroll
+      for (int it = 0; it < ITC; ++it)
+        if (tid + it * DNT < C::C3)
+          sc[tid + it * DNT] = vc[it];
       __syncthreads();
       // x: s1[c][b][xf] = sum_a E[xf][a] sc[c][b][a]
       for (int e = tid; e < C::S2; e += DNT)
@@ -835,14 +863,18 @@ namespace gtps
         }
       __syncthreads();
       // z: fine[zf][yf][xf] += sum_c E[zf][c] s2[c][yf][xf]
-      for (int e = tid; e < C::F3; e += DNT)
+#pragma unroll
+      for (int it = 0; it < ITF; ++it)
         {
+          const int e = tid + it * DNT;
+          if (e >= C::F3)
+            continue;
           const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
-          T         v  = 0;
+          T         v  = vf[it];
 #pragma unroll
           for (int c = 0; c < NN; ++c)
             v += sE[zf * NN + c] * s2[(c * FB + yf) * FB + xf];
-          f0[xf + (long long)yf * N1f + (long long)zf * Pf] += v;
+          f0[xf + yf * N1f + zf * Pf32] = v;
         }
     }
 
@@ -861,6 +893,8 @@ namespace gtps
       const int       ncd = nf / 2;
       const int       N1f = nf * NN, N1c = ncd * NN;
       const long long Pf = (long long)N1f * N1f, Pc = (long long)N1c * N1c;
+      // level sizes < 2^31 (host check): offsets inside a brick are 32-bit
+      const int       Pf32 = (int)Pf, Pc32 = (int)Pc;
       fine += (long long)blockIdx.y * Pf * N1f;
       coarse += (long long)blockIdx.y * Nc;
       const int cc  = blockIdx.x;
@@ -869,13 +903,40 @@ namespace gtps
                     (long long)czc * NN * Pc;
       const T  *f0  = fine + (long long)cxc * FB + (long long)cyc * FB * N1f +
                      (long long)czc * FB * Pf;
-      for (int e = tid; e < FB * NN; e += DNT)
-        sE[e] = tab[TB::E + e];
-      for (int e = tid; e < C::F3; e += DNT)
+      // every global load (embedding table, fine brick, the coarse values
+      // the last pass adds to) issued before the first wait
+      constexpr int ITE = (FB * NN + DNT - 1) / DNT, ITF = (C::F3 + DNT - 1) / DNT,
+                    ITC = (C::C3 + DNT - 1) / DNT;
+      T             ve[ITE], vf[ITF], vc[STORE ? 1 : ITC];
+#pragma unroll
+      for (int it = 0; it < ITE; ++it)
         {
-          const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
-          sf[e] = f0[xf + (long long)yf * N1f + (long long)zf * Pf];
+          const int e = tid + it * DNT;
+          ve[it]      = e < FB * NN ? tab[TB::E + e] : T(0);
         }
+#pragma unroll
+      for (int it = 0; it < ITF; ++it)
+        {
+          const int e  = tid + it * DNT;
+          const int xf = e % FB, yf = (e / FB) % FB, zf = e / (FB * FB);
+          vf[it] = e < C::F3 ? f0[xf + yf * N1f + zf * Pf32] : T(0);
+        }
+      if constexpr (!STORE)
+#pragma unroll
+        for (int it = 0; it < ITC; ++it)
+          {
+            const int e = tid + it * DNT;
+            const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
+            vc[it] = e < C::C3 ? c0[a + b * N1c + c * Pc32] : T(0);
+          }
+#pragma unroll
+      for (int it = 0; it < ITE; ++it)
+        if (tid + it * DNT < FB * NN)
+          sE[tid + it * DNT] = ve[it];
+#pragma unroll
+      for (int it = 0; it < ITF; ++it)
+        if (tid + it * DNT < C::F3)
+          sf[tid + it * DNT] = vf[it];
       __syncthreads();
       // x: s1[zf][yf][a] = sum_xf E[xf][a] sf[zf][yf][xf]
       for (int e = tid; e < C::S1; e += DNT)
@@ -900,15 +961,22 @@ namespace gtps
         }
       __syncthreads();
       // z: coarse[c][b][a] (+)= sum_zf E[zf][c] s2[zf][b][a]
-      for (int e = tid; e < C::C3; e += DNT)
+#pragma unroll
+      for (int it = 0; it < ITC; ++it)
         {
+          const int e = tid + it * DNT;
+          if (e >= C::C3)
+            continue;
           const int a = e % NN, b = (e / NN) % NN, c = e / (NN * NN);
           T         v = 0;
 #pragma unroll
           for (int zf = 0; zf < FB; ++zf)
             v += sE[zf * NN + c] * s2[(zf * NN + b) * NN + a];
-          T *o = c0 + a + (long long)b * N1c + (long long)c * Pc;
-          *o   = STORE ? v : *o + v;
+          T *o = c0 + (a + b * N1c + c * Pc32);
+          if constexpr (STORE)
+            *o = v;
+          else
+            *o = vc[it] + v;
         }
     }
 
@@ -1218,6 +1286,7 @@ namespace gtps
   launch_dg_prolongate_add(const gtps_dg_level *fl, void *fine, const void *coarse, int nvec)
   {
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
+    GTPS_REQUIRE(fl->N < (1LL << 31), "DG level too large for the transfer kernels");
     // one work-group per coarse cell
     const unsigned ncc = (unsigned)((fl->n / 2) * (fl->n / 2) * (fl->n / 2));
     dispatch_k(fl->k, [&](auto kk) {
@@ -1240,6 +1309,7 @@ namespace gtps
                      bool store, int nvec)
   {
     GTPS_REQUIRE(fl->n % 2 == 0, "fine level must have an even cell count");
+    GTPS_REQUIRE(fl->N < (1LL << 31), "DG level too large for the transfer kernels");
     const long long N1c = (long long)(fl->n / 2) * (fl->k + 1);
     const long long Nc  = N1c * N1c * N1c;
     // one work-group per coarse cell

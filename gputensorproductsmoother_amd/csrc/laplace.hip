@@ -411,6 +411,11 @@ namespace gtps
       const int L   = xcd_logical_id(blockIdx.x, gridDim.x);
       const int ox0 = (L % nbx) * B, oy0 = ((L / nbx) % nbx) * B,
                 oz0 = (L / (nbx * nbx)) * B;
+      constexpr int NRB  = RESID ? B : 1;
+      T             xo[B], bo[NRB];
+      const int     rly = tid % B, rlz = tid / B;
+      const int     rgy = oy0 + rly, rgz = oz0 + rlz;
+      const bool    rrow = tid < B * B && rgy < N1 && rgz < N1;
       {
         // one z-column (x, y) of the box per thread: the column's address and
         // x/y bounds set up once, the z bounds uniform
@@ -431,6 +436,17 @@ namespace gtps
                 v[j][bz]     = in && gz >= 0 && gz < N1 ? xc[gz * P] : T(0);
               }
           }
+        // the x-pass outputs of this thread (row (ly, lz) = tid < B*B):
+        // their input values and right-hand side, issued with the box loads
+#pragma unroll
+        for (int lx = 0; lx < B; ++lx)
+          {
+            const int       gx = ox0 + lx;
+            const long long g  = gx + (long long)rgy * N1 + rgz * P;
+            xo[lx]             = (rrow && gx < N1) ? x[g] : T(0);
+            if constexpr (RESID)
+              bo[lx] = (rrow && gx < N1) ? bv[g] : T(0);
+          }
 #pragma unroll
         for (int j = 0; j < IT; ++j)
           {
@@ -443,22 +459,6 @@ namespace gtps
               }
           }
       }
-      // the x-pass outputs of this thread (row (ly, lz) = tid < B*B): their
-      // input values and right-hand side, loaded with the box
-      constexpr int NRB = RESID ? B : 1;
-      T             xo[B], bo[NRB];
-      const int     rly = tid % B, rlz = tid / B;
-      const int     rgy = oy0 + rly, rgz = oz0 + rlz;
-      const bool    rrow = tid < B * B && rgy < N1 && rgz < N1;
-#pragma unroll
-      for (int lx = 0; lx < B; ++lx)
-        {
-          const int       gx = ox0 + lx;
-          const long long g  = gx + (long long)rgy * N1 + rgz * P;
-          xo[lx]             = (rrow && gx < N1) ? x[g] : T(0);
-          if constexpr (RESID)
-            bo[lx] = (rrow && gx < N1) ? bv[g] : T(0);
-        }
       static_assert(B * B <= NT, "one x-pass row per thread");
       __syncthreads();
       // z-rows on every (x, y) column of the box, B output planes
