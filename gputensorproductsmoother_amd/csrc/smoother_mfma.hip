@@ -112,23 +112,30 @@ namespace gtps
     __global__ void __launch_bounds__(MF_NT)
     patch3d_mfma_kernel(float *__restrict__ u, const float *__restrict__ bvec,
                         const float *__restrict__ tab, const int n, const int colour,
-                        const float scale, const float inv_scale, const float relax)
+                        const float scale, const float inv_scale, const float relax,
+                        const FastDiv div_x, const FastDiv div_xy)
     {
       using TB         = Tab<K>;
       constexpr int NP = TB::NP, NI = TB::NI;
       static_assert(NP <= 16 && NI <= NZ, "patch box must fit a 16-wide tile");
       __shared__ float sA[ARR], sB[ARR], sC[ARR];
 
-      const int       N1   = K * n + 1;
-      const long long P    = (long long)N1 * N1;
-      const int       tid  = threadIdx.x;
-      const int       wave = tid >> 6, l = tid & 63, ln = l & 15, kq = l >> 4;
+      // level sizes < 2^31 (host check): offsets from the patch's first
+      // DoF are 32-bit
+      const int N1   = K * n + 1;
+      const int P    = N1 * N1;
+      const int tid  = threadIdx.x;
+      const int wave = tid >> 6, l = tid & 63, ln = l & 15, kq = l >> 4;
       const int px = colour & 1, py = (colour >> 1) & 1, pz = (colour >> 2) & 1;
-      const int npx = (n - px) / 2, npy = (n - py) / 2;
-      const int q   = xcd_logical_id(blockIdx.x, gridDim.x);
-      const int qx = q % npx, qy = (q / npx) % npy, qz = q / (npx * npy);
+      // patch coordinates: divisions by the patch counts as multiply-high
+      // and shift (divisors set up on the host)
+      const int q  = xcd_logical_id(blockIdx.x, gridDim.x);
+      const int qz = (int)fastdiv((unsigned)q, div_xy), rq = q - qz * (int)div_xy.d;
+      const int qy = (int)fastdiv((unsigned)rq, div_x), qx = rq - qy * (int)div_x.d;
       const long long first = (long long)K * (2 * qx + px) + (long long)K * (2 * qy + py) * N1 +
                               (long long)K * (2 * qz + pz) * P;
+      float *__restrict__ up       = u + first;
+      const float *__restrict__ bp = bvec + first;
 
       const float *pM = tab + TB::patchM, *pK = tab + TB::patchK, *S = tab + TB::S;
       const float *linv = tab + TB::lamInv3;
@@ -152,7 +159,7 @@ namespace gtps
             const bool inner = ln > 0 && ln < NP - 1 && t > 0 && t < NP - 1 && z > 0 &&
                                z < NP - 1;
             uz[tt][s] = (ln < NP && t < NP && z < NP && !inner)
-                          ? u[first + ln + (long long)t * N1 + (long long)z * P]
+                          ? up[ln + t * N1 + z * P]
                           : 0.f;
           }
 #pragma unroll
@@ -162,8 +169,7 @@ namespace gtps
           {
             const int t = wave + 4 * tt, xi = 4 * kq + r;
             bv[tt][r]   = (xi < NI && ln < NI && t < NI)
-                            ? bvec[first + (xi + 1) + (long long)(ln + 1) * N1 +
-                                   (long long)(t + 1) * P]
+                            ? bp[(xi + 1) + (ln + 1) * N1 + (t + 1) * P]
                             : 0.f;
           }
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -276,8 +282,7 @@ namespace gtps
                 {
                   const int xi = 4 * kq + r;
                   if (xi < NI && ln < NI)
-                    u[first + (xi + 1) + (long long)(ln + 1) * N1 + (long long)(t + 1) * P] =
-                      relax * acc[r];
+                    up[(xi + 1) + (ln + 1) * N1 + (t + 1) * P] = relax * acc[r];
                 }
             }
         }
@@ -302,13 +307,17 @@ namespace gtps
     const long long np = (long long)((n - px) / 2) * ((n - py) / 2) * ((n - pz) / 2);
     if (np <= 0)
       return;
+    GTPS_REQUIRE(lvl->N < (1LL << 31), "level too large for the matrix-core smoother");
+    const int     npx = (n - px) / 2, npy = (n - py) / 2;
+    const FastDiv dx  = make_fastdiv((unsigned)npx);
+    const FastDiv dxy = make_fastdiv((unsigned)(npx * npy));
     const float *tab   = reinterpret_cast<const float *>(lvl->dtab());
     const float  scale = (float)lvl->scale, inv = (float)(1.0 / lvl->scale);
     hipStream_t  s     = lvl->ctx->stream;
 #define GTPS_MF(KK)                                                                   \
   hipLaunchKernelGGL((patch3d_mfma_kernel<KK>), dim3((unsigned)np), dim3(MF_NT), 0, s, \
                      (float *)u, (const float *)src, tab, n, colour, scale, inv,       \
-                     (float)relax)
+                     (float)relax, dx, dxy)
     switch (lvl->k)
       {
         case 4: GTPS_MF(4); break;
