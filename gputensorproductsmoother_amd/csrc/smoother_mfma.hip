@@ -147,9 +147,10 @@ namespace gtps
       load_weights(wS, [&](int i, int j) { return i < NI && j < NI ? S[i * NI + j] : 0.f; });
 
       // the z pass's B operands: u on the box skeleton (interior zeroed) at
-      // x = ln, y = t, z = 4(l/16) + s; and this lane's interior b values
-      // (x pass): every load issued before the first wait
-      float uz[4][4], bv[4][4];
+      // x = ln, y = t, z = 4(l/16) + s; this lane's interior b values (x
+      // pass) and inverse eigenvalue sums (z scaling): every load issued
+      // before the first wait
+      float uz[4][4], bv[4][4], lv[4][4];
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
@@ -171,6 +172,14 @@ namespace gtps
             bv[tt][r]   = (xi < NI && ln < NI && t < NI)
                             ? bp[(xi + 1) + (ln + 1) * N1 + (t + 1) * P]
                             : 0.f;
+          }
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          {
+            const int t = wave + 4 * tt, zi = 4 * kq + r;
+            lv[tt][r]   = (t < NI && zi < NI && ln < NI) ? linv[(zi * NI + t) * NI + ln] : 0.f;
           }
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
       // z pass (lines x = ln, y = t): A = Mz u, B = Kz u
@@ -252,8 +261,7 @@ namespace gtps
               for (int r = 0; r < 4; ++r)
                 {
                   const int zi = 4 * kq + r;
-                  acc[r] = (zi < NI && ln < NI) ? acc[r] * linv[(zi * NI + t) * NI + ln] * inv_scale
-                                                : 0.f;
+                  acc[r] = (zi < NI && ln < NI) ? acc[r] * lv[tt][r] * inv_scale : 0.f;
                 }
               out = contract_regs(acc, wS, zero);
             }
