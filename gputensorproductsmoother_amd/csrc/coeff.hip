@@ -388,6 +388,13 @@ namespace gtps
       // before the first wait
       constexpr int ITU = (NP * NP * NP + CNT - 1) / CNT, ITR = (NI * NI * NI + CNT - 1) / CNT;
       T             bres[ITR];
+      // the fit's per-direction eigenbases and eigenvalues (bins uniform
+      // over the work-group: scalar table loads in the FD passes)
+      const T *Sx = sep + (pk & 255) * SW;
+      const T *Sy = sep + ((pk >> 8) & 255) * SW;
+      const T *Sz = sep + ((pk >> 16) & 255) * SW;
+      static_assert(NL <= CNT, "one z-line per thread");
+      T lxy;
       {
         T vu[ITU];
 #pragma unroll
@@ -406,6 +413,8 @@ namespace gtps
                             ? bp[(i + 1) + (j + 1) * N1 + (l + 1) * P]
                             : T(0);
           }
+        // this thread's x + y eigenvalue sum for the z-line stage
+        lxy = tid < NL ? Sx[NL + tid % NI] + Sy[NL + tid / NI] : T(0);
 #pragma unroll
         for (int it = 0; it < ITU; ++it)
           if (tid + it * CNT < NP * NP * NP)
@@ -534,10 +543,7 @@ namespace gtps
       __syncthreads();
       // fast diagonalisation with the per-direction eigenpairs of the fit:
       // S_x^T, S_y^T, S_z^T; 1/(sigma scale (lx_i + ly_j + lz_l)); S_z, S_y,
-      // S_x (bins uniform over the work-group: scalar table loads)
-      const T *Sx = sep + (pk & 255) * SW;
-      const T *Sy = sep + ((pk >> 8) & 255) * SW;
-      const T *Sz = sep + ((pk >> 16) & 255) * SW;
+      // S_x
       for (int t = tid; t < NL; t += CNT)  // x-lines (l, j)
         {
           T v[NI];
@@ -575,11 +581,10 @@ namespace gtps
         }
       __syncthreads();
       const T inv = T(1) / (scale * sig);
-      for (int t = tid; t < NL; t += CNT)  // z-lines (j, i)
+      if (tid < NL)  // z-lines (j, i)
         {
-          T      *base = sr + t;
-          T       v[NI], w[NI];
-          const T lxy  = Sx[NL + t % NI] + Sy[NL + t / NI];
+          T *base = sr + tid;
+          T  v[NI], w[NI];
 #pragma unroll
           for (int l = 0; l < NI; ++l)
             v[l] = base[l * NL];

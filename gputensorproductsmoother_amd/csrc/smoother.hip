@@ -1054,8 +1054,9 @@ namespace gtps
       const T *linv = tab + TB::lamInv3p;
 
       // interior right-hand side (or residual) rows of this lane's stage-3
-      // tasks, loaded up front
-      T rb[P3][NI];
+      // tasks and the inverse eigenvalue sums of its stage-5 tasks (the same
+      // task index r), loaded up front
+      T rb[P3][NI], lv[P3][NI];
 #pragma unroll
       for (int pp = 0; pp < P3; ++pp)
         {
@@ -1072,6 +1073,9 @@ namespace gtps
                   for (int xi = 0; xi < NI; ++xi)
                     rb[pp][xi] = src[g0 + xi];
                 }
+#pragma unroll
+              for (int j = 0; j < NI; ++j)
+                lv[pp][j] = linv[j * NI * NI + r];
             }
         }
       if (MODE == 0)
@@ -1348,7 +1352,7 @@ namespace gtps
           fd_fwd<K>(SE, SO, v, tt);
 #pragma unroll
           for (int j = 0; j < NI; ++j)
-            tt[j] *= linv[(j * NI + yi) * NI + xi] * inv_scale;
+            tt[j] *= lv[pp][j] * inv_scale;
           fd_inv<K>(SE, SO, tt, v);
 #pragma unroll
           for (int i = 0; i < NI; ++i)

@@ -158,6 +158,26 @@ namespace gtps
                                      gridDim.x * gridDim.y * gridDim.z);
       const int c0x = (L % gridDim.x) * CX, c0y = ((L / gridDim.x) % gridDim.y) * CY,
                 c0z = (L / (gridDim.x * gridDim.y)) * CZ;
+      // the fine values the z pass adds to (its columns (yf, xf), 2k CZ
+      // planes each), loaded with the coarse block so that the
+      // read-modify-write costs no round trip of its own
+      constexpr int ITZ = (FY * FX + NT - 1) / NT, NZO = CZ * 2 * K;
+      T             old[ITZ][NZO];
+#pragma unroll
+      for (int jz = 0; jz < ITZ; ++jz)
+        {
+          const int  t  = tid + jz * NT;
+          const int  yf = t / FX, xf = t - yf * FX;
+          const int  gx = 2 * K * c0x + xf, gy = 2 * K * c0y + yf;
+          const bool in = t < FY * FX && gx < Nf1 - 1 && gy < Nf1 - 1;
+          const T   *col = fine + (in ? gx + (long long)gy * Nf1 : 0);
+#pragma unroll
+          for (int o = 0; o < NZO; ++o)
+            {
+              const int gz = 2 * K * c0z + o;
+              old[jz][o]   = in && gz < Nf1 - 1 ? col[gz * Pf] : T(0);
+            }
+        }
       {
         // coarse block in one round of loads (unrolled), then to LDS
         constexpr int TOT = EZ * EY * EX, IT = (TOT + NT - 1) / NT;
@@ -223,8 +243,12 @@ namespace gtps
         }
       __syncthreads();
       // z: one column (yf, xf) per thread, accumulated into the fine vector
-      for (int t = tid; t < FY * FX; t += NT)
+#pragma unroll
+      for (int jz = 0; jz < ITZ; ++jz)
         {
+          const int t = tid + jz * NT;
+          if (t >= FY * FX)
+            continue;
           const int yf = t / FX, xf = t - yf * FX;
           const int gx = 2 * K * c0x + xf, gy = 2 * K * c0y + yf;
           if (gx >= Nf1 - 1 || gy >= Nf1 - 1)
@@ -234,17 +258,6 @@ namespace gtps
           for (int i = 0; i < EZ; ++i)
             r[i] = sY[(i * FY + yf) * FX + xf];
           T *out = fine + gx + (long long)gy * Nf1;
-          // all read-modify-write loads first, so that they are in flight
-          // together instead of one latency per output
-          T old[CZ * 2 * K];
-#pragma unroll
-          for (int cl = 0; cl < CZ; ++cl)
-#pragma unroll
-            for (int q = 0; q < 2 * K; ++q)
-              {
-                const int gz = 2 * K * (c0z + cl) + q;
-                old[cl * 2 * K + q] = gz < Nf1 - 1 ? out[gz * Pf] : T(0);
-              }
 #pragma unroll
           for (int cl = 0; cl < CZ; ++cl)
 #pragma unroll
@@ -257,7 +270,7 @@ namespace gtps
 #pragma unroll
                 for (int a = 0; a < N; ++a)
                   v += E[q * N + a] * r[K * cl + a];
-                out[gz * Pf] = old[cl * 2 * K + q] + v;
+                out[gz * Pf] = old[jz][cl * 2 * K + q] + v;
               }
         }
     }
