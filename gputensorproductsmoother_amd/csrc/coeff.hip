@@ -393,8 +393,11 @@ namespace gtps
       const T *Sx = sep + (pk & 255) * SW;
       const T *Sy = sep + ((pk >> 8) & 255) * SW;
       const T *Sz = sep + ((pk >> 16) & 255) * SW;
-      static_assert(NL <= CNT, "one z-line per thread");
-      T lxy;
+      // lines of an FD pass per lane, wavefronts, outputs per wavefront
+      constexpr int NW = CNT / 64, NCH = (NL + 63) / 64, OPW = (NI + NW - 1) / NW;
+      const int     lane = tid & 63;
+      const int     wv   = __builtin_amdgcn_readfirstlane(tid >> 6);
+      T             lxy[NCH];
       {
         T vu[ITU];
 #pragma unroll
@@ -413,8 +416,14 @@ namespace gtps
                             ? bp[(i + 1) + (j + 1) * N1 + (l + 1) * P]
                             : T(0);
           }
-        // this thread's x + y eigenvalue sum for the z-line stage
-        lxy = tid < NL ? Sx[NL + tid % NI] + Sy[NL + tid / NI] : T(0);
+        // this lane's x + y eigenvalue sums for the z-line pass (z-line
+        // t = (j, i))
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+          {
+            const int t = ch * 64 + lane;
+            lxy[ch]     = t < NL ? Sx[NL + t % NI] + Sy[NL + t / NI] : T(0);
+          }
 #pragma unroll
         for (int it = 0; it < ITU; ++it)
           if (tid + it * CNT < NP * NP * NP)
@@ -542,106 +551,86 @@ namespace gtps
         }
       __syncthreads();
       // fast diagonalisation with the per-direction eigenpairs of the fit:
-      // S_x^T, S_y^T, S_z^T; 1/(sigma scale (lx_i + ly_j + lz_l)); S_z, S_y,
-      // S_x
-      for (int t = tid; t < NL; t += CNT)  // x-lines (l, j)
-        {
-          T v[NI];
-#pragma unroll
-          for (int i = 0; i < NI; ++i)
-            v[i] = sr[t * NI + i];
-#pragma unroll
-          for (int jj = 0; jj < NI; ++jj)
-            {
-              T s = 0;
-#pragma unroll
-              for (int i = 0; i < NI; ++i)
-                s += Sx[i * NI + jj] * v[i];
-              sr[t * NI + jj] = s;
-            }
-        }
-      __syncthreads();
-      for (int t = tid; t < NL; t += CNT)  // y-lines (l, i)
-        {
-          const int i = t % NI, l = t / NI;
-          T        *base = sr + l * NL + i;
-          T         v[NI];
-#pragma unroll
-          for (int j = 0; j < NI; ++j)
-            v[j] = base[j * NI];
-#pragma unroll
-          for (int jj = 0; jj < NI; ++jj)
-            {
-              T s = 0;
-#pragma unroll
-              for (int j = 0; j < NI; ++j)
-                s += Sy[j * NI + jj] * v[j];
-              base[jj * NI] = s;
-            }
-        }
-      __syncthreads();
+      // S_x^T, S_y^T, S_z^T with 1/(sigma scale (lx_i + ly_j + lz_l)), then
+      // S_z, S_y, S_x.  In every pass the lanes run over the NI^2 lines and
+      // wavefront w forms outputs w, w + NW, ... of each line, so all four
+      // wavefronts share the pass and the matrix entries a wavefront
+      // multiplies by are uniform (scalar loads); the passes ping-pong
+      // between sr and sa (free after the gather).
       const T inv = T(1) / (scale * sig);
-      if (tid < NL)  // z-lines (j, i)
+      // D = 0 (x): line (l, j), D = 1 (y): line (l, i), D = 2 (z): line (j, i)
+      auto line_base = [&](int D, int t) {
+        return D == 0 ? t * NI : (D == 1 ? (t / NI) * NL + t % NI : t);
+      };
+      auto fd_pass = [&](const T *in, T *out, auto dirc, auto trc, auto scc, const T *Sm) {
+        constexpr int  D     = decltype(dirc)::value;
+        constexpr bool TR    = decltype(trc)::value;
+        constexpr bool SCALE = decltype(scc)::value;
+        constexpr int  str   = D == 0 ? 1 : (D == 1 ? NI : NL);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+          {
+            const int t = ch * 64 + lane;
+            if (t >= NL)
+              continue;
+            const int base = line_base(D, t);
+            T         v[NI];
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+              v[i] = in[base + i * str];
+#pragma unroll
+            for (int m = 0; m < OPW; ++m)
+              {
+                const int q = wv + NW * m;
+                if (q >= NI)
+                  break;
+                T acc = 0;
+#pragma unroll
+                for (int i = 0; i < NI; ++i)
+                  acc += (TR ? Sm[i * NI + q] : Sm[q * NI + i]) * v[i];
+                if constexpr (SCALE)
+                  {
+                    // FP32: hardware reciprocal (1 ulp); FP64: exact division
+                    if constexpr (sizeof(T) == 4)
+                      acc = acc * inv * __builtin_amdgcn_rcpf(lxy[ch] + Sm[NL + q]);
+                    else
+                      acc = acc * inv / (lxy[ch] + Sm[NL + q]);
+                  }
+                out[base + q * str] = acc;
+              }
+          }
+        __syncthreads();
+      };
+      using I0 = std::integral_constant<int, 0>;
+      using I1 = std::integral_constant<int, 1>;
+      using I2 = std::integral_constant<int, 2>;
+      using BT = std::true_type;
+      using BF = std::false_type;
+      fd_pass(sr, sa, I0{}, BT{}, BF{}, Sx);
+      fd_pass(sa, sr, I1{}, BT{}, BF{}, Sy);
+      fd_pass(sr, sa, I2{}, BT{}, BT{}, Sz);
+      fd_pass(sa, sr, I2{}, BF{}, BF{}, Sz);
+      fd_pass(sr, sa, I1{}, BF{}, BF{}, Sy);
+      // S_x and the store: output i of x-line (l, j) is DoF (i + 1, j + 1,
+      // l + 1) of the patch box
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch)
         {
-          T *base = sr + tid;
-          T  v[NI], w[NI];
-#pragma unroll
-          for (int l = 0; l < NI; ++l)
-            v[l] = base[l * NL];
-#pragma unroll
-          for (int ll = 0; ll < NI; ++ll)
-            {
-              T s = 0;
-#pragma unroll
-              for (int l = 0; l < NI; ++l)
-                s += Sz[l * NI + ll] * v[l];
-              // FP32: hardware reciprocal (1 ulp); FP64: exact division
-              if constexpr (sizeof(T) == 4)
-                w[ll] = s * inv * __builtin_amdgcn_rcpf(lxy + Sz[NL + ll]);
-              else
-                w[ll] = s * inv / (lxy + Sz[NL + ll]);
-            }
-#pragma unroll
-          for (int l = 0; l < NI; ++l)
-            {
-              T s = 0;
-#pragma unroll
-              for (int ll = 0; ll < NI; ++ll)
-                s += Sz[l * NI + ll] * w[ll];
-              base[l * NL] = s;
-            }
-        }
-      __syncthreads();
-      for (int t = tid; t < NL; t += CNT)  // y-lines
-        {
-          const int i = t % NI, l = t / NI;
-          T        *base = sr + l * NL + i;
-          T         v[NI];
-#pragma unroll
-          for (int jj = 0; jj < NI; ++jj)
-            v[jj] = base[jj * NI];
-#pragma unroll
-          for (int j = 0; j < NI; ++j)
-            {
-              T s = 0;
-#pragma unroll
-              for (int jj = 0; jj < NI; ++jj)
-                s += Sy[j * NI + jj] * v[jj];
-              base[j * NI] = s;
-            }
-        }
-      __syncthreads();
-      for (int t = tid; t < NL; t += CNT)  // x-lines and store
-        {
+          const int t = ch * 64 + lane;
+          if (t >= NL)
+            continue;
           const int j = t % NI, l = t / NI;
           T         v[NI];
 #pragma unroll
           for (int jj = 0; jj < NI; ++jj)
-            v[jj] = sr[t * NI + jj];
+            v[jj] = sa[t * NI + jj];
           const int g0 = 1 + (j + 1) * N1 + (l + 1) * P;
 #pragma unroll
-          for (int i = 0; i < NI; ++i)
+          for (int m = 0; m < OPW; ++m)
             {
+              const int i = wv + NW * m;
+              if (i >= NI)
+                break;
               T s = 0;
 #pragma unroll
               for (int jj = 0; jj < NI; ++jj)
