@@ -100,7 +100,7 @@ def test_coeff_gradient_kernel(core, ctx, k, level, prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("variant", ["fwd", "rev"])
-@pytest.mark.parametrize("k,level", [(1, 3), (2, 2), (3, 2), (4, 2), (5, 1)])
+@pytest.mark.parametrize("k,level", [(1, 3), (2, 2), (3, 2), (4, 2), (5, 1), (6, 1), (7, 1)])
 def test_coeff_smoother_step(core, ctx, k, level, variant, prec):
     if prec == "fp64" and k > 5:
         pytest.skip("FP64 coefficient smoother supports k <= 5")
