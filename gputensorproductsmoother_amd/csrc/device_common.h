@@ -153,8 +153,10 @@ namespace gtps
   __host__ __device__ __forceinline__ int
   xcd_logical_id(int hw, int n)
   {
-    const int q = n / N_XCD, r = n % N_XCD;
-    const int xcd = hw % N_XCD, slot = hw / N_XCD;
-    return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+    // unsigned: the divisions by 8 are shifts and masks
+    const unsigned un = (unsigned)n, uh = (unsigned)hw;
+    const unsigned q = un / N_XCD, r = un % N_XCD;
+    const unsigned xcd = uh % N_XCD, slot = uh / N_XCD;
+    return (int)((xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot);
   }
 } // namespace gtps

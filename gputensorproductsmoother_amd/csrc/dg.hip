@@ -430,8 +430,7 @@ namespace gtps
       // are 32-bit
       const int       N1    = n * NN, P = N1 * N1;
       const int       st[3] = {1, N1, P};
-      const long long g0    = (long long)pc[0] * NN + (long long)pc[1] * NN * N1 +
-                           (long long)pc[2] * NN * P;
+      const int       g0    = pc[0] * NN + pc[1] * NN * N1 + pc[2] * NN * P;
       T *__restrict__ up       = u + g0;
       const T *__restrict__ bp = bvec + g0;
       const int tid            = threadIdx.x;
@@ -729,7 +728,8 @@ namespace gtps
       const long long NNN = (long long)n * (K + 1);
       u += (long long)blockIdx.y * NNN * NNN * NNN;
       bvec += (long long)blockIdx.y * NNN * NNN * NNN;
-      const int cx = colour % 3, cy = (colour / 3) % 3, cz = colour / 9;
+      const unsigned uc = (unsigned)colour;
+      const int      cx = (int)(uc % 3), cy = (int)((uc / 3) % 3), cz = (int)(uc / 9);
       // patch coordinates: divisions by the patch counts as multiply-high
       // and shift (divisors set up on the host)
       const int q  = xcd_logical_id(blockIdx.x, gridDim.x);
