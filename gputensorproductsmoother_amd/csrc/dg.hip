@@ -440,10 +440,19 @@ namespace gtps
         const int p = pc[f >> 1];
         return (f & 1) ? p + 2 < n : p > 0;
       };
-      auto box_offset = [&](int e) {
-        const int x = e % PD, y = (e / PD) % PD, z = e / (PD * PD);
-        return x + y * N1 + z * P;
-      };
+      // patch coordinates (x fastest) of this thread's DoFs tid + it NT,
+      // used by the patch loads, the face terms and the store
+      constexpr int ITB = (P3 + NT - 1) / NT;
+      int           ec[ITB][3];
+#pragma unroll
+      for (int it = 0; it < ITB; ++it)
+        {
+          const int e = tid + it * NT < P3 ? tid + it * NT : 0;
+          ec[it][0]   = e % PD;
+          ec[it][1]   = (e / PD) % PD;
+          ec[it][2]   = e / (PD * PD);
+        }
+      auto box_offset = [&](int it) { return ec[it][0] + ec[it][1] * N1 + ec[it][2] * P; };
       // Coupling tasks (face ff, line ln) and FD tasks (output block fh,
       // line fln): lines per face or block padded to whole wavefronts where
       // that fits the work-group, else packed.
@@ -481,14 +490,13 @@ namespace gtps
           }
       // b_P and the old patch values (relax != 1), issued before the slab
       // loads and their first wait
-      constexpr int ITB   = (P3 + NT - 1) / NT;
       const bool    blend = relax != T(1);
       T             vb[ITB], uold[ITB];
 #pragma unroll
       for (int it = 0; it < ITB; ++it)
         {
           const int e = tid + it * NT;
-          const int o = box_offset(e < P3 ? e : 0);
+          const int o = box_offset(it);
           vb[it]      = e < P3 ? bp[o] : T(0);
           uold[it]    = e < P3 && blend ? up[o] : T(0);
         }
@@ -574,32 +582,48 @@ namespace gtps
                 }
             }
           __syncthreads();
-          // cell mass along a: line (cell ca of a, b, i), into sw[a][b][i]
-          if (on)
+          // cell mass along a: line (cell ca of a, b, i), into sw[a][b][i];
+          // an absent face's slab is set to zero, so that the face terms
+          // below are read without a mask
+          if (cact)
             {
               const int i = cln % NN, cbb = cln / NN, bb = cbb % PD, ca = cbb / PD;
               const int o = cff * SL + (ca * NN * PD + bb) * NN + i;
-              T         v[NN];
-#pragma unroll
-              for (int j = 0; j < NN; ++j)
-                v[j] = sa[o + j * PD * NN];
-#pragma unroll
-              for (int ia = 0; ia < NN; ++ia)
+              if (on)
                 {
-                  T acc = 0;
+                  T v[NN];
 #pragma unroll
                   for (int j = 0; j < NN; ++j)
-                    acc += Mc[ia * NN + j] * v[j];
-                  sw[o + ia * PD * NN] = acc;
+                    v[j] = sa[o + j * PD * NN];
+#pragma unroll
+                  for (int ia = 0; ia < NN; ++ia)
+                    {
+                      T acc = 0;
+#pragma unroll
+                      for (int j = 0; j < NN; ++j)
+                        acc += Mc[ia * NN + j] * v[j];
+                      sw[o + ia * PD * NN] = acc;
+                    }
+                }
+              else
+                {
+#pragma unroll
+                  for (int ia = 0; ia < NN; ++ia)
+                    sw[o + ia * PD * NN] = T(0);
                 }
             }
           __syncthreads();
           // r -= scale (face terms): per direction, the face on the DoF's
-          // side; all reads issued together (absent faces read entry 0 and
-          // are masked out)
-          for (int e = tid; e < P3; e += NT)
+          // side (absent faces hold zeros); all reads issued together, with
+          // faces outside this round masked out when the faces take several
+          // rounds
+#pragma unroll
+          for (int it = 0; it < ITB; ++it)
             {
-              const int c3[3] = {e % PD, (e / PD) % PD, e / (PD * PD)};
+              const int e = tid + it * NT;
+              if (e >= P3)
+                continue;
+              const int(&c3)[3] = ec[it];
               T         v[3];
               bool      on[3];
 #pragma unroll
@@ -607,7 +631,7 @@ namespace gtps
                 {
                   const int pD = c3[D], side = pD >= NN ? 1 : 0, f = 2 * D + side;
                   const int da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-                  on[D] = f >= f0 && f < f0 + NS && has_face(f);
+                  on[D] = NS == 6 || (f >= f0 && f < f0 + NS);
                   v[D]  = sw[on[D] ? (f - f0) * SL + (c3[da] * PD + c3[db]) * NN + pD - side * NN
                                    : 0];
                 }
@@ -712,7 +736,7 @@ namespace gtps
           if (e >= P3)
             continue;
           const T v          = r[e];
-          up[box_offset(e)] = blend ? (T(1) - relax) * uold[it] + relax * v : v;
+          up[box_offset(it)] = blend ? (T(1) - relax) * uold[it] + relax * v : v;
         }
     }
 
