@@ -385,7 +385,7 @@ namespace gtps
 
     // One work-group per patch (2x2x2 cells around an interior vertex, all
     // (2k+2)^3 DoFs), u_P <- (1-w) u_P + w A_PP^{-1} (b_P - A_P,ext u_ext):
-    //  1. b_P into r; the face-neighbour cell slabs of u into LDS;
+    //  1. b_P into registers; the face-neighbour cell slabs of u into LDS;
     //  2. per face, L-/L+ along the face normal, then the cell mass along the
     //     two in-plane directions, each pass one line of k+1 values per
     //     thread; then every patch DoF subtracts the face of each direction
@@ -475,8 +475,47 @@ namespace gtps
             const int d = e / (PD * PD);
             Ssh[d][e - d * PD * PD] = Sg[d][e - d * PD * PD];
           }
-      // the eigenvalues: loaded here, stored to LDS once the patch loads are
-      // in flight (a store now would wait for this load first)
+      // slab lines of faces f0 .. f0+NS-1: task (face, a, b), at most NTK
+      // per thread.  issue_slabs loads a thread's lines into registers,
+      // commit_slabs stores them to sa, so that the first round's slab
+      // loads go out before any other load (their index arithmetic waits
+      // for nothing) and every load is in flight before the first wait.
+      constexpr int NTK = (NS * NLP + NT - 1) / NT;
+      T             slv[NTK][NN];
+      int           slo[NTK];
+      auto issue_slabs = [&](int f0) {
+#pragma unroll
+        for (int j = 0; j < NTK; ++j)
+          {
+            const int tk = tid + j * NT;
+            slo[j]       = -1;
+            if (tk >= NS * NLP)
+              continue;
+            const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
+            const int f  = f0 + ff;
+            if (ln >= NL || !has_face(f))
+              continue;
+            const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
+            const int aa = ln / PD, bb = ln - aa * PD;
+            const int o  = ((f & 1) ? PD : -NN) * st[D] + aa * st[da] + bb * st[db];
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+              slv[j][i] = up[o + i * st[D]];
+            slo[j] = ff * SL + ln * NN;
+          }
+      };
+      auto commit_slabs = [&]() {
+#pragma unroll
+        for (int j = 0; j < NTK; ++j)
+          if (slo[j] >= 0)
+            {
+#pragma unroll
+              for (int i = 0; i < NN; ++i)
+                sa[slo[j] + i] = slv[j][i];
+            }
+      };
+      issue_slabs(0);
+      // the eigenvalues, b_P and the old patch values (relax != 1)
       static_assert(3 * PD <= NT, "one eigenvalue per thread");
       const bool lam_t = tid < 3 * PD;
       const int  lam_d = lam_t ? tid / PD : 0, lam_i = tid - lam_d * PD;
@@ -488,10 +527,8 @@ namespace gtps
             Lsh[0][e] = tab[TB::Lm + e];
             Lsh[1][e] = tab[TB::Lp + e];
           }
-      // b_P and the old patch values (relax != 1), issued before the slab
-      // loads and their first wait
-      const bool    blend = relax != T(1);
-      T             vb[ITB], uold[ITB];
+      const bool blend = relax != T(1);
+      T          vb[ITB], uold[ITB];
 #pragma unroll
       for (int it = 0; it < ITB; ++it)
         {
@@ -500,34 +537,9 @@ namespace gtps
           vb[it]      = e < P3 ? bp[o] : T(0);
           uold[it]    = e < P3 && blend ? up[o] : T(0);
         }
-      // slab lines of faces f0 .. f0+NS-1 into sa: task (face, a, b)
-      auto load_slabs = [&](int f0) {
-        for (int tk = tid; tk < NS * NLP; tk += NT)
-          {
-            const int ff = __builtin_amdgcn_readfirstlane(tk / NLP), ln = tk - ff * NLP;
-            const int f  = f0 + ff;
-            if (ln >= NL || !has_face(f))
-              continue;
-            const int D = f >> 1, da = D == 2 ? 1 : 2, db = D == 0 ? 1 : 0;
-            const int aa = ln / PD, bb = ln - aa * PD;
-            const int o  = ((f & 1) ? PD : -NN) * st[D] + aa * st[da] + bb * st[db];
-            T         v[NN];
-#pragma unroll
-            for (int j = 0; j < NN; ++j)
-              v[j] = up[o + j * st[D]];
-            T *sl = sa + ff * SL + ln * NN;
-#pragma unroll
-            for (int j = 0; j < NN; ++j)
-              sl[j] = v[j];
-          }
-      };
-      load_slabs(0);
+      commit_slabs();
       if (lam_t)
         lsh[lam_d][lam_i] = lam_v;
-#pragma unroll
-      for (int it = 0; it < ITB; ++it)
-        if (tid + it * NT < P3)
-          r[tid + it * NT] = vb[it];
       // 2. external coupling, NS faces per round; a thread keeps its task
       // (face ff, line ln) in every pass of a round
       const int  cff  = CU ? __builtin_amdgcn_readfirstlane(tid / CL) : tid / CL;
@@ -536,7 +548,10 @@ namespace gtps
       for (int f0 = 0; f0 < 6; f0 += NS)
         {
           if (f0 > 0)
-            load_slabs(f0);
+            {
+              issue_slabs(f0);
+              commit_slabs();
+            }
           __syncthreads();
           const int  f  = f0 + cff;
           const bool on = cact && has_face(f);
@@ -635,7 +650,9 @@ namespace gtps
                   v[D]  = sw[on[D] ? (f - f0) * SL + (c3[da] * PD + c3[db]) * NN + pD - side * NN
                                    : 0];
                 }
-              const T re  = r[e];
+              // b_P enters with the first round (r holds the partial
+              // residual of the earlier rounds otherwise)
+              const T re  = f0 == 0 ? vb[it] : r[e];
               const T acc = (on[0] ? v[0] : T(0)) + (on[1] ? v[1] : T(0)) + (on[2] ? v[2] : T(0));
               r[e]        = re - scale * acc;
             }
