@@ -36,6 +36,21 @@ namespace gtps
       return x + (long long)N1 * (y + (long long)N1 * z);
     }
 
+    // interior_index with host-computed divisors (ni1 = N1 - 2 and ni1^2;
+    // the dense form only serves levels of a few thousand DoFs: 32-bit)
+    template <int DIM>
+    __device__ __forceinline__ int
+    interior_index_fast(int j, int N1, const FastDiv &da, const FastDiv &db)
+    {
+      if (DIM == 0)
+        return j;
+      const int q = (int)fastdiv((unsigned)j, da), x = j - q * (int)da.d + 1;
+      if (DIM == 2)
+        return x + N1 * (q + 1);
+      const int z = (int)fastdiv((unsigned)j, db), y = q - z * (int)da.d + 1;
+      return x + N1 * (y + N1 * (z + 1));
+    }
+
     // D[j][*] = e_{interior j}: column j's unit defect
     template <typename T, int DIM>
     __global__ void __launch_bounds__(CNT)
@@ -95,7 +110,8 @@ namespace gtps
     template <typename T, int DIM>
     __global__ void __launch_bounds__(CNT)
     dense_apply_kernel(const T *__restrict__ B, const T *__restrict__ d, T *__restrict__ u,
-                       const int N1, const long long N, const int ni, const int ld)
+                       const int N1, const long long N, const int ni, const int ld,
+                       const FastDiv da, const FastDiv db)
     {
       using V  = typename Vec16<T>::type;
       constexpr int PER = 16 / sizeof(T);
@@ -104,8 +120,22 @@ namespace gtps
       // batch: blockIdx.y selects one of several contiguous vectors
       d += (long long)blockIdx.y * N;
       u += (long long)blockIdx.y * N;
-      for (int j = tid; j < ld; j += CNT)
-        sd[j] = j < ni ? d[interior_index<DIM>(j, N1)] : T(0);
+      {
+        // the defect's interior values: every load issued before the first
+        // wait (a load-store loop waited for each element in turn)
+        constexpr int ITD = (MAX_DENSE_COARSE + CNT - 1) / CNT;
+        T             v[ITD];
+#pragma unroll
+        for (int q = 0; q < ITD; ++q)
+          {
+            const int j = tid + q * CNT;
+            v[q]        = j < ni ? d[interior_index_fast<DIM>(j, N1, da, db)] : T(0);
+          }
+#pragma unroll
+        for (int q = 0; q < ITD; ++q)
+          if (tid + q * CNT < ld)
+            sd[tid + q * CNT] = v[q];
+      }
       // Dirichlet rows: zero (spread over the grid)
       // (the dense form only serves levels of a few thousand DoFs: 32-bit)
       for (int g = blockIdx.x * CNT + tid; DIM != 0 && g < (int)N; g += gridDim.x * CNT)
@@ -145,7 +175,7 @@ namespace gtps
         }
       const double s = wave_sum((double)acc);
       if (lane == 0)
-        u[interior_index<DIM>(i, N1)] = (T)s;
+        u[interior_index_fast<DIM>(i, N1, da, db)] = (T)s;
     }
 
     int
@@ -219,11 +249,11 @@ namespace gtps
     if (prec == GTPS_FP32)
       hipLaunchKernelGGL((dense_apply_kernel<float, 0>), grid, dim3(CNT), 0, s,
                          (const float *)B, (const float *)d, (float *)u, 0, (long long)n, n,
-                         ld);
+                         ld, make_fastdiv(1), make_fastdiv(1));
     else
       hipLaunchKernelGGL((dense_apply_kernel<double, 0>), grid, dim3(CNT), 0, s,
                          (const double *)B, (const double *)d, (double *)u, 0, (long long)n,
-                         n, ld);
+                         n, ld, make_fastdiv(1), make_fastdiv(1));
     GTPS_LAUNCH_CHECK();
   }
 
@@ -269,10 +299,12 @@ namespace gtps
     GTPS_REQUIRE(ni >= 1 && ld <= MAX_DENSE_COARSE, "dense coarse operator too large");
     const long long N  = dim == 3 ? (long long)N1 * N1 * N1 : (long long)N1 * N1;
     const int       nb = (ni + CNT / 64 - 1) / (CNT / 64);
+    const FastDiv   da = make_fastdiv((unsigned)(N1 - 2)),
+                  db = make_fastdiv((unsigned)((N1 - 2) * (N1 - 2)));
     by_type(dim, prec, [&](auto tp, auto dd) {
       using T = std::remove_pointer_t<decltype(tp)>;
       hipLaunchKernelGGL((dense_apply_kernel<T, decltype(dd)::value>), dim3(nb), dim3(CNT), 0,
-                         s, (const T *)B, (const T *)d, (T *)u, N1, N, ni, ld);
+                         s, (const T *)B, (const T *)d, (T *)u, N1, N, ni, ld, da, db);
     });
     GTPS_LAUNCH_CHECK();
   }
