@@ -18,7 +18,7 @@
 // is exactly the B operand of a following contraction along the same
 // direction: x-residual -> S^T along x and the eigenvalue scaling -> S
 // along z are chained in registers without an LDS round trip.
-// LDS holds three arrays of the 13 interior z-planes (16 x 16, x fastest).
+// LDS holds two arrays of the 13 interior z-planes (16 x 16, x fastest).
 #include "gtps_internal.h"
 
 namespace gtps
@@ -105,7 +105,7 @@ namespace gtps
     // Tiles of a stage: t = wave + 4 tt.  Every LDS array holds the NZ = 13
     // interior z-planes (16 x 16 each, x fastest): the z pass reads the box
     // straight from global memory into its B operands, so no array holds
-    // the 15 box planes, and 3 arrays (39 KiB) leave room for 4 work-groups
+    // the 15 box planes, and 2 arrays (26 KiB) leave room for 6 work-groups
     // per CU.  Tiles whose lines lie outside the interior are skipped (or
     // stored as zeros where a later stage reads them).
     template <int K>
@@ -118,7 +118,7 @@ namespace gtps
       using TB         = Tab<K>;
       constexpr int NP = TB::NP, NI = TB::NI;
       static_assert(NP <= 16 && NI <= NZ, "patch box must fit a 16-wide tile");
-      __shared__ float sA[ARR], sB[ARR], sC[ARR];
+      __shared__ float sA[ARR], sB[ARR];
 
       // level sizes < 2^31 (host check): offsets from the patch's first
       // DoF are 32-bit
@@ -198,29 +198,32 @@ namespace gtps
           store_tile<2>(sB, t, k);
         }
       __syncthreads();
-      // y pass (lines x = ln, z = t < NZ): C = My A, then A = Ky A + My B
-      // once every wave has read A
-      floatx4 c2[4];
+      // The y pass, the x pass and S^T along y read and write the z-planes
+      // t of their own tiles only (tile t: lines along y or x in plane
+      // z = t), so they follow each other within the wavefront, in place,
+      // with no work-group barrier: a wavefront's LDS operations complete
+      // in order, and the fence keeps the compiler from moving them.
+      auto wave_fence = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      };
+      // y pass (lines x = ln, z = t < NZ): C = My A into B, A' = Ky A + My B
+      // into A
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
           const int t = wave + 4 * tt;
           if (t < NZ)
             {
-              store_tile<1>(sC, t, contract<1>(sA, t, wM, zero));
-              c2[tt] = contract<1>(sB, t, wM, contract<1>(sA, t, wK, zero));
+              const floatx4 c1 = contract<1>(sA, t, wM, zero);
+              const floatx4 c2 = contract<1>(sB, t, wM, contract<1>(sA, t, wK, zero));
+              store_tile<1>(sB, t, c1);
+              store_tile<1>(sA, t, c2);
             }
         }
-      __syncthreads();
-#pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
-        {
-          const int t = wave + 4 * tt;
-          if (t < NZ)
-            store_tile<1>(sA, t, c2[tt]);
-        }
-      __syncthreads();
-      // x pass (lines y = ln, z = t): r = b_I - h (Kx C + Mx A), then S^T
+      wave_fence();
+      // x pass (lines y = ln, z = t): r = b_I - h (Kx C + Mx A'), then S^T
       // along x in registers, into B
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
@@ -228,7 +231,7 @@ namespace gtps
           const int t = wave + 4 * tt;
           if (t < NZ)
             {
-              const floatx4 acc = contract<0>(sA, t, wM, contract<0>(sC, t, wK, zero));
+              const floatx4 acc = contract<0>(sA, t, wM, contract<0>(sB, t, wK, zero));
               floatx4       res;
 #pragma unroll
               for (int r = 0; r < 4; ++r)
@@ -237,18 +240,18 @@ namespace gtps
               store_tile<0>(sB, t, contract_regs(res, wST, zero));
             }
         }
-      __syncthreads();
-      // S^T along y (lines x = ln, z = t): B -> C
+      wave_fence();
+      // S^T along y (lines x = ln, z = t): B -> A
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
           const int t = wave + 4 * tt;
           if (t < NZ)
-            store_tile<1>(sC, t, contract<1>(sB, t, wST, zero));
+            store_tile<1>(sA, t, contract<1>(sB, t, wST, zero));
         }
       __syncthreads();
       // S^T along z (lines x = ln, y = t), eigenvalue scaling, S along z in
-      // registers: C -> A (rows y >= NI stored as zeros)
+      // registers: A -> B (rows y >= NI stored as zeros)
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
@@ -256,7 +259,7 @@ namespace gtps
           floatx4   out = zero;
           if (t < NI)
             {
-              floatx4 acc = contract<2>(sC, t, wST, zero);
+              floatx4 acc = contract<2>(sA, t, wST, zero);
 #pragma unroll
               for (int r = 0; r < 4; ++r)
                 {
@@ -265,18 +268,19 @@ namespace gtps
                 }
               out = contract_regs(acc, wS, zero);
             }
-          store_tile<2>(sA, t, out);
+          store_tile<2>(sB, t, out);
         }
       __syncthreads();
-      // S along y (lines x = ln, z = t): A -> B
+      // S along y (lines x = ln, z = t): B -> A, and S along x on the same
+      // planes of this wavefront
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
         {
           const int t = wave + 4 * tt;
           if (t < NZ)
-            store_tile<1>(sB, t, contract<1>(sA, t, wS, zero));
+            store_tile<1>(sA, t, contract<1>(sB, t, wS, zero));
         }
-      __syncthreads();
+      wave_fence();
       // S along x (lines y = ln, z = t), write the interior
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
@@ -284,7 +288,7 @@ namespace gtps
           const int t = wave + 4 * tt;
           if (t < NI)
             {
-              const floatx4 acc = contract<0>(sB, t, wS, zero);
+              const floatx4 acc = contract<0>(sA, t, wS, zero);
 #pragma unroll
               for (int r = 0; r < 4; ++r)
                 {
