@@ -150,7 +150,7 @@ namespace gtps
       // x = ln, y = t, z = 4(l/16) + s; this lane's interior b values (x
       // pass) and inverse eigenvalue sums (z scaling): every load issued
       // before the first wait
-      float uz[4][4], bv[4][4], lv[4][4];
+      float uz[4][4], bv[4][4];
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
@@ -172,14 +172,6 @@ namespace gtps
             bv[tt][r]   = (xi < NI && ln < NI && t < NI)
                             ? bp[(xi + 1) + (ln + 1) * N1 + (t + 1) * P]
                             : 0.f;
-          }
-#pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          {
-            const int t = wave + 4 * tt, zi = 4 * kq + r;
-            lv[tt][r]   = (t < NI && zi < NI && ln < NI) ? linv[(zi * NI + t) * NI + ln] : 0.f;
           }
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
       // z pass (lines x = ln, y = t): A = Mz u, B = Kz u
@@ -241,6 +233,17 @@ namespace gtps
             }
         }
       wave_fence();
+      // the inverse eigenvalue sums of the scaling stage, loaded here (the b
+      // values are no longer live): their latency passes behind S^T along y
+      float lv[4][4];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          {
+            const int t = wave + 4 * tt, zi = 4 * kq + r;
+            lv[tt][r]   = (t < NI && zi < NI && ln < NI) ? linv[(zi * NI + t) * NI + ln] : 0.f;
+          }
       // S^T along y (lines x = ln, z = t): B -> A
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
