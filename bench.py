@@ -223,7 +223,12 @@ def mfma_report(core, ctx, k, level, reps):
         res["mfma" if mf else "fma"] = ms * 1e3 / 8
     fl = mfma_flops_per_patch(k) * patches
     ach = fl / (res["mfma"] * 8 * 1e-6) / 1e12
-    return {"kernel": f"patch3d_mfma_kernel<{k}>", "us_per_colour_mfma": round(res["mfma"], 2),
+    # wavefronts per patch: 16 on colours of at most mfma_wide_max_patches
+    wide = core.get_tuning("mfma_wide_max_patches")
+    most, least = (n // 2) ** 3, ((n - 1) // 2) ** 3
+    waves = "16" if most <= wide else ("4" if least > wide else "4|16")
+    return {"kernel": f"patch3d_mfma_kernel<{k}, {waves}>",
+            "us_per_colour_mfma": round(res["mfma"], 2),
             "us_per_colour_lds_fma": round(res["fma"], 2), "issued_TFLOPs": round(ach, 1),
             "peak_TFLOPs": 157.3, "frac": round(ach / 157.3, 3),
             "note": "issued FLOPs include the zero padding of 13/15-wide lines to 16"}

@@ -1900,6 +1900,12 @@ extern "C"
           prev                   = tuning().mfma_smoother;
           tuning().mfma_smoother = (int)value;
         }
+      else if (k == "mfma_wide_max_patches")
+        {
+          GTPS_REQUIRE(value >= 0, "value must be >= 0");
+          prev                           = tuning().mfma_wide_max_patches;
+          tuning().mfma_wide_max_patches = value;
+        }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
       ++tuning().generation;
@@ -1930,6 +1936,8 @@ extern "C"
         v = tuning().dense_coarse;
       else if (k == "mfma_smoother")
         v = tuning().mfma_smoother;
+      else if (k == "mfma_wide_max_patches")
+        v = tuning().mfma_wide_max_patches;
       else if (k == "vmult_zdiv")
         v = tuning().vmult_zdiv;
       else if (k == "vmult_tile")

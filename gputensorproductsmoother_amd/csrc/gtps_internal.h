@@ -232,6 +232,10 @@ namespace gtps
     int       dense_coarse = 1;
     // 3D FP32 degree 4..7 multiplicative smoother on the matrix cores
     int       mfma_smoother = 1;
+    // matrix-core smoother: colours with at most this many patches run 16
+    // wavefronts per patch instead of 4 (the coarse levels, where a colour
+    // launch is one patch chain)
+    long long mfma_wide_max_patches = 1024;
     // z-marching vmult: target work-group count divisor (cells per chunk =
     // ceil(n tx ty / vmult_zdiv)); 0: by precision
     long long vmult_zdiv = 0;

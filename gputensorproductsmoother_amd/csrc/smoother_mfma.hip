@@ -6,7 +6,7 @@
 // interior ones.  Padded to 16, each step of the local solve is one 16x16
 // matrix applied along one direction to the 256 lines of a 16^3 array:
 // 16 GEMM tiles of v_mfma_f32_16x16x4f32 (4 k-steps each), 4 tiles per
-// wavefront.  The steps are those of the LDS/FMA kernel
+// wavefront (1 per wavefront in the 16-wavefront form of small colours).  The steps are those of the LDS/FMA kernel
 // (loop_kernel.cuh:969-1060 in the reference; same BD form here):
 //   residual  r_I = b_I - h (Kx My Mz + Mx Ky Mz + Mx My Kz)_IB u_B
 //             (z pass: Mz, Kz; y pass: My Mz, Ky Mz + My Kz; x pass),
@@ -27,7 +27,6 @@ namespace gtps
   {
     typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-    constexpr int MF_NT = 256;  // 4 wavefronts, one patch
     constexpr int NZ    = 13;   // z-planes of an LDS array (interior rows)
     constexpr int ARR   = NZ * 256;
 
@@ -102,14 +101,18 @@ namespace gtps
         w[s] = f(i, 4 * kq + s);
     }
 
-    // Tiles of a stage: t = wave + 4 tt.  Every LDS array holds the NZ = 13
+    // NW wavefronts per patch (4, or 16 for colours too small to fill the
+    // GPU: the matrix-core work per SIMD is the same, but four wavefronts
+    // per SIMD overlap each other's LDS and global latency on the patch
+    // chain).  Tiles of a stage: t = wave + NW tt, tt < 16 / NW; a wavefront
+    // owns the same z-planes t in every stage.  Every LDS array holds the NZ = 13
     // interior z-planes (16 x 16 each, x fastest): the z pass reads the box
     // straight from global memory into its B operands, so no array holds
     // the 15 box planes, and 2 arrays (26 KiB) leave room for 6 work-groups
     // per CU.  Tiles whose lines lie outside the interior are skipped (or
     // stored as zeros where a later stage reads them).
-    template <int K>
-    __global__ void __launch_bounds__(MF_NT)
+    template <int K, int NW>
+    __global__ void __launch_bounds__(64 * NW)
     patch3d_mfma_kernel(float *__restrict__ u, const float *__restrict__ bvec,
                         const float *__restrict__ tab, const int n, const int colour,
                         const float scale, const float inv_scale, const float relax,
@@ -118,6 +121,8 @@ namespace gtps
       using TB         = Tab<K>;
       constexpr int NP = TB::NP, NI = TB::NI;
       static_assert(NP <= 16 && NI <= NZ, "patch box must fit a 16-wide tile");
+      static_assert(NW == 4 || NW == 16, "4 or 16 wavefronts per patch");
+      constexpr int TT = 16 / NW;  // tiles per wavefront and stage
       __shared__ float sA[ARR], sB[ARR];
 
       // level sizes < 2^31 (host check): offsets from the patch's first
@@ -150,13 +155,13 @@ namespace gtps
       // x = ln, y = t, z = 4(l/16) + s; this lane's interior b values (x
       // pass) and inverse eigenvalue sums (z scaling): every load issued
       // before the first wait
-      float uz[4][4], bv[4][4];
+      float uz[TT][4], bv[TT][4];
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int s = 0; s < 4; ++s)
           {
-            const int  t = wave + 4 * tt, z = 4 * kq + s;
+            const int  t = wave + NW * tt, z = 4 * kq + s;
             const bool inner = ln > 0 && ln < NP - 1 && t > 0 && t < NP - 1 && z > 0 &&
                                z < NP - 1;
             uz[tt][s] = (ln < NP && t < NP && z < NP && !inner)
@@ -164,11 +169,11 @@ namespace gtps
                           : 0.f;
           }
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           {
-            const int t = wave + 4 * tt, xi = 4 * kq + r;
+            const int t = wave + NW * tt, xi = 4 * kq + r;
             bv[tt][r]   = (xi < NI && ln < NI && t < NI)
                             ? bp[(xi + 1) + (ln + 1) * N1 + (t + 1) * P]
                             : 0.f;
@@ -176,9 +181,9 @@ namespace gtps
       const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
       // z pass (lines x = ln, y = t): A = Mz u, B = Kz u
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           floatx4   m = zero, k = zero;
           if (t < NP)
             {
@@ -203,9 +208,9 @@ namespace gtps
       // y pass (lines x = ln, z = t < NZ): C = My A into B, A' = Ky A + My B
       // into A
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           if (t < NZ)
             {
               const floatx4 c1 = contract<1>(sA, t, wM, zero);
@@ -218,9 +223,9 @@ namespace gtps
       // x pass (lines y = ln, z = t): r = b_I - h (Kx C + Mx A'), then S^T
       // along x in registers, into B
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           if (t < NZ)
             {
               const floatx4 acc = contract<0>(sA, t, wM, contract<0>(sB, t, wK, zero));
@@ -235,20 +240,20 @@ namespace gtps
       wave_fence();
       // the inverse eigenvalue sums of the scaling stage, loaded here (the b
       // values are no longer live): their latency passes behind S^T along y
-      float lv[4][4];
+      float lv[TT][4];
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           {
-            const int t = wave + 4 * tt, zi = 4 * kq + r;
+            const int t = wave + NW * tt, zi = 4 * kq + r;
             lv[tt][r]   = (t < NI && zi < NI && ln < NI) ? linv[(zi * NI + t) * NI + ln] : 0.f;
           }
       // S^T along y (lines x = ln, z = t): B -> A
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           if (t < NZ)
             store_tile<1>(sA, t, contract<1>(sB, t, wST, zero));
         }
@@ -256,9 +261,9 @@ namespace gtps
       // S^T along z (lines x = ln, y = t), eigenvalue scaling, S along z in
       // registers: A -> B (rows y >= NI stored as zeros)
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           floatx4   out = zero;
           if (t < NI)
             {
@@ -277,18 +282,18 @@ namespace gtps
       // S along y (lines x = ln, z = t): B -> A, and S along x on the same
       // planes of this wavefront
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           if (t < NZ)
             store_tile<1>(sA, t, contract<1>(sB, t, wS, zero));
         }
       wave_fence();
       // S along x (lines y = ln, z = t), write the interior
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt)
+      for (int tt = 0; tt < TT; ++tt)
         {
-          const int t = wave + 4 * tt;
+          const int t = wave + NW * tt;
           if (t < NI)
             {
               const floatx4 acc = contract<0>(sA, t, wS, zero);
@@ -329,10 +334,20 @@ namespace gtps
     const float *tab   = reinterpret_cast<const float *>(lvl->dtab());
     const float  scale = (float)lvl->scale, inv = (float)(1.0 / lvl->scale);
     hipStream_t  s     = lvl->ctx->stream;
-#define GTPS_MF(KK)                                                                   \
-  hipLaunchKernelGGL((patch3d_mfma_kernel<KK>), dim3((unsigned)np), dim3(MF_NT), 0, s, \
-                     (float *)u, (const float *)src, tab, n, colour, scale, inv,       \
-                     (float)relax, dx, dxy)
+    const bool wide = np <= tuning().mfma_wide_max_patches;
+#define GTPS_MF(KK)                                                                      \
+  do                                                                                     \
+    {                                                                                    \
+      if (wide)                                                                          \
+        hipLaunchKernelGGL((patch3d_mfma_kernel<KK, 16>), dim3((unsigned)np), dim3(1024), \
+                           0, s, (float *)u, (const float *)src, tab, n, colour, scale,  \
+                           inv, (float)relax, dx, dxy);                                  \
+      else                                                                               \
+        hipLaunchKernelGGL((patch3d_mfma_kernel<KK, 4>), dim3((unsigned)np), dim3(256),   \
+                           0, s, (float *)u, (const float *)src, tab, n, colour, scale,  \
+                           inv, (float)relax, dx, dxy);                                  \
+    }                                                                                    \
+  while (0)
     switch (lvl->k)
       {
         case 4: GTPS_MF(4); break;
