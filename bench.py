@@ -223,10 +223,7 @@ def mfma_report(core, ctx, k, level, reps):
         res["mfma" if mf else "fma"] = ms * 1e3 / 8
     fl = mfma_flops_per_patch(k) * patches
     ach = fl / (res["mfma"] * 8 * 1e-6) / 1e12
-    # wavefronts per patch: 16 on colours of at most mfma_wide_max_patches
-    wide = core.get_tuning("mfma_wide_max_patches")
-    most, least = (n // 2) ** 3, ((n - 1) // 2) ** 3
-    waves = "16" if most <= wide else ("4" if least > wide else "4|16")
+    waves = core.get_tuning("mfma_waves")  # wavefronts per patch
     return {"kernel": f"patch3d_mfma_kernel<{k}, {waves}>",
             "us_per_colour_mfma": round(res["mfma"], 2),
             "us_per_colour_lds_fma": round(res["fma"], 2), "issued_TFLOPs": round(ach, 1),

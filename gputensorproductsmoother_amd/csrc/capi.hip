@@ -1900,11 +1900,12 @@ extern "C"
           prev                   = tuning().mfma_smoother;
           tuning().mfma_smoother = (int)value;
         }
-      else if (k == "mfma_wide_max_patches")
+      else if (k == "mfma_waves")
         {
-          GTPS_REQUIRE(value >= 0, "value must be >= 0");
-          prev                           = tuning().mfma_wide_max_patches;
-          tuning().mfma_wide_max_patches = value;
+          GTPS_REQUIRE(value == 4 || value == 8 || value == 16,
+                       "mfma_waves must be 4, 8 or 16");
+          prev                = tuning().mfma_waves;
+          tuning().mfma_waves = (int)value;
         }
       else
         throw Error(GTPS_EINVAL, "unknown tuning key " + k);
@@ -1936,8 +1937,8 @@ extern "C"
         v = tuning().dense_coarse;
       else if (k == "mfma_smoother")
         v = tuning().mfma_smoother;
-      else if (k == "mfma_wide_max_patches")
-        v = tuning().mfma_wide_max_patches;
+      else if (k == "mfma_waves")
+        v = tuning().mfma_waves;
       else if (k == "vmult_zdiv")
         v = tuning().vmult_zdiv;
       else if (k == "vmult_tile")

@@ -232,10 +232,11 @@ namespace gtps
     int       dense_coarse = 1;
     // 3D FP32 degree 4..7 multiplicative smoother on the matrix cores
     int       mfma_smoother = 1;
-    // matrix-core smoother: colours with at most this many patches run 16
-    // wavefronts per patch instead of 4 (the coarse levels, where a colour
-    // launch is one patch chain)
-    long long mfma_wide_max_patches = 1024;
+    // matrix-core smoother: wavefronts per patch (4, 8 or 16).  Measured on
+    // MI355X at Q7 32^3 (C3 FP32 solve): 12.44 ms with 8, 13.0 with 4,
+    // 13.4 with 16; 8 is at least as fast on every level (level-5 colour
+    // 64-76 us, a level-4 colour 13.4-13.8, a level 1-3 chain 8.2-8.7)
+    int       mfma_waves = 8;
     // z-marching vmult: target work-group count divisor (cells per chunk =
     // ceil(n tx ty / vmult_zdiv)); 0: by precision
     long long vmult_zdiv = 0;

@@ -5,8 +5,8 @@
 // A patch box has NP = 2k+1 <= 15 values per direction and NI = 2k-1
 // interior ones.  Padded to 16, each step of the local solve is one 16x16
 // matrix applied along one direction to the 256 lines of a 16^3 array:
-// 16 GEMM tiles of v_mfma_f32_16x16x4f32 (4 k-steps each), 4 tiles per
-// wavefront (1 per wavefront in the 16-wavefront form of small colours).  The steps are those of the LDS/FMA kernel
+// 16 GEMM tiles of v_mfma_f32_16x16x4f32 (4 k-steps each), 2 tiles per
+// wavefront (8 wavefronts per patch; 4 and 16 are tuning options).  The steps are those of the LDS/FMA kernel
 // (loop_kernel.cuh:969-1060 in the reference; same BD form here):
 //   residual  r_I = b_I - h (Kx My Mz + Mx Ky Mz + Mx My Kz)_IB u_B
 //             (z pass: Mz, Kz; y pass: My Mz, Ky Mz + My Kz; x pass),
@@ -101,15 +101,17 @@ namespace gtps
         w[s] = f(i, 4 * kq + s);
     }
 
-    // NW wavefronts per patch (4, or 16 for colours too small to fill the
-    // GPU: the matrix-core work per SIMD is the same, but four wavefronts
-    // per SIMD overlap each other's LDS and global latency on the patch
-    // chain).  Tiles of a stage: t = wave + NW tt, tt < 16 / NW; a wavefront
-    // owns the same z-planes t in every stage.  Every LDS array holds the NZ = 13
-    // interior z-planes (16 x 16 each, x fastest): the z pass reads the box
-    // straight from global memory into its B operands, so no array holds
-    // the 15 box planes, and 2 arrays (26 KiB) leave room for 6 work-groups
-    // per CU.  Tiles whose lines lie outside the interior are skipped (or
+    // NW wavefronts per patch (4, 8 or 16; 8 by default).  The matrix-core
+    // work of a patch is the same, but with 8 wavefronts 4 patches (and 8
+    // wavefronts per SIMD) are resident per CU, where 4 give 5 patches at 5
+    // wavefronts per SIMD (92 registers at k = 7) and 16 give 2: more
+    // wavefronts overlap each other's LDS and global latency on the patch
+    // chain.  Tiles of a stage: t = wave + NW tt, tt < 16 / NW; a wavefront
+    // owns the same z-planes t in every stage.  Every LDS array holds the
+    // NZ = 13 interior z-planes (16 x 16 each, x fastest): the z pass reads
+    // the box straight from global memory into its B operands, so no array
+    // holds the 15 box planes, and 2 arrays (26 KiB) fit 6 work-groups per
+    // CU.  Tiles whose lines lie outside the interior are skipped (or
     // stored as zeros where a later stage reads them).
     template <int K, int NW>
     __global__ void __launch_bounds__(64 * NW)
@@ -121,7 +123,7 @@ namespace gtps
       using TB         = Tab<K>;
       constexpr int NP = TB::NP, NI = TB::NI;
       static_assert(NP <= 16 && NI <= NZ, "patch box must fit a 16-wide tile");
-      static_assert(NW == 4 || NW == 16, "4 or 16 wavefronts per patch");
+      static_assert(NW == 4 || NW == 8 || NW == 16, "4, 8 or 16 wavefronts per patch");
       constexpr int TT = 16 / NW;  // tiles per wavefront and stage
       __shared__ float sA[ARR], sB[ARR];
 
@@ -334,12 +336,16 @@ namespace gtps
     const float *tab   = reinterpret_cast<const float *>(lvl->dtab());
     const float  scale = (float)lvl->scale, inv = (float)(1.0 / lvl->scale);
     hipStream_t  s     = lvl->ctx->stream;
-    const bool wide = np <= tuning().mfma_wide_max_patches;
+    const int nw = tuning().mfma_waves;
 #define GTPS_MF(KK)                                                                      \
   do                                                                                     \
     {                                                                                    \
-      if (wide)                                                                          \
+      if (nw == 16)                                                                      \
         hipLaunchKernelGGL((patch3d_mfma_kernel<KK, 16>), dim3((unsigned)np), dim3(1024), \
+                           0, s, (float *)u, (const float *)src, tab, n, colour, scale,  \
+                           inv, (float)relax, dx, dxy);                                  \
+      else if (nw == 8)                                                                  \
+        hipLaunchKernelGGL((patch3d_mfma_kernel<KK, 8>), dim3((unsigned)np), dim3(512),   \
                            0, s, (float *)u, (const float *)src, tab, n, colour, scale,  \
                            inv, (float)relax, dx, dxy);                                  \
       else                                                                               \

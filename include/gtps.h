@@ -114,10 +114,8 @@ const char *gtps_version(void);
  *                           degree 4..7 runs its 1D contractions on the
  *                           matrix cores (v_mfma_f32_16x16x4f32, patches
  *                           padded to 16^3); 0: the LDS/FMA kernel.
- *   "mfma_wide_max_patches" matrix-core smoother colours with at most this
- *                           many patches run 16 wavefronts per patch
- *                           instead of 4 (default 1024: the coarse levels,
- *                           where a colour launch is one patch chain).
+ *   "mfma_waves"            matrix-core smoother wavefronts per patch: 4,
+ *                           8 (default) or 16.
  * Results do not depend on the knobs beyond round-off.  No reference
  * counterpart (the reference picks kernels at compile time,
  * scripts/ct_parameter.py.in:146-148). */

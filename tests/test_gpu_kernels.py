@@ -260,14 +260,14 @@ def test_vmult_box_and_march_paths(core, ctx, k, level, path, prec):
     assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec]
 
 
-@pytest.mark.parametrize("mfma", [1, 0, 16])
+@pytest.mark.parametrize("mfma", [4, 0, 8, 16])
 @pytest.mark.parametrize("reverse", [False, True])
 @pytest.mark.parametrize("k,level", [(4, 2), (4, 3), (5, 2), (6, 2), (7, 1), (7, 2), (7, 3)])
 def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
     """3D FP32 multiplicative smoothing of degree 4..7 on the matrix cores
-    (v_mfma_f32_16x16x4f32, patches padded to 16^3) with 4 (mfma = 1) or 16
-    (mfma = 16) wavefronts per patch, and on the LDS/FMA kernel (mfma = 0),
-    agree with the oracle (FP32 tolerance x20, relaxation 0.8)."""
+    (v_mfma_f32_16x16x4f32, patches padded to 16^3) with mfma = 4, 8 or 16
+    wavefronts per patch, and on the LDS/FMA kernel (mfma = 0), agree with
+    the oracle (FP32 tolerance x20, relaxation 0.8)."""
     _announce("smoother-mfma", k, level, reverse, mfma)
     g = rng(level * 11 + k)
     N = n_dofs(3, k, level)
@@ -278,7 +278,7 @@ def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
     ref = O.smoother_step(3, k, level, b, u, variant=O.MULT_REV if reverse else O.MULT_FWD,
                           relax=relax)
     prev = core.set_tuning("mfma_smoother", 1 if mfma else 0)
-    prev_wide = core.set_tuning("mfma_wide_max_patches", 1 << 40 if mfma == 16 else 0)
+    prev_waves = core.set_tuning("mfma_waves", mfma or 8)
     try:
         op = core.LaplaceOperator(ctx, 3, k, level, torch.float32)
         sm = core.PatchSmoother(op, core.PatchSmoother.MULTIPLICATIVE_REVERSE if reverse
@@ -288,7 +288,7 @@ def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
         torch.cuda.synchronize()
     finally:
         core.set_tuning("mfma_smoother", prev)
-        core.set_tuning("mfma_wide_max_patches", prev_wide)
+        core.set_tuning("mfma_waves", prev_waves)
     assert rel_err(ud.cpu().numpy(), ref) < TOL["fp32"] * 20
 
 
