@@ -241,7 +241,8 @@ namespace gtps
     // ceil(n tx ty / vmult_zdiv)); 0: by precision
     long long vmult_zdiv = 0;
     // z-marching vmult tile for k = 3: 0 = 16x8 cells / 512 threads,
-    // 1 = 8x8 / 256, 2 = 16x8 / 256; 3: by precision
+    // 1 = 8x8 / 256, 2 = 16x8 / 256; 3: by precision.  k = 7: 0 = 8x2 /
+    // 512, 1 = 4x4 / 1024, 2 and 3 = 4x4 / 512
     int       vmult_tile = 3;
     // bumped by every successful gtps_set_tuning: HIP graphs captured under
     // an older generation are re-captured before their next replay

@@ -86,6 +86,13 @@ namespace gtps
     {
       static constexpr int TCX = 4, TCY = 4, NT = 512;
     };
+    // k = 7: the same 4x4 cells on 1024 threads (2 columns of the window
+    // per thread instead of 4: about half the registers)
+    template <>
+    struct VT3C<7, 1>
+    {
+      static constexpr int TCX = 4, TCY = 4, NT = 1024;
+    };
 
     template <int K, typename T, bool RESID, bool DOT, int CFG = 0>
     __global__ void __launch_bounds__((VT3C<K, CFG>::NT))
@@ -163,6 +170,21 @@ namespace gtps
         }
 
       double dot = 0.0;
+      // residual, k >= 5: b at this thread's Dirichlet (boundary-column or
+      // first-plane) DoFs, loaded in one plane and stored at the top of the
+      // next, when the load has long completed (stored in pass 1 of its own
+      // plane, the wait for it would also wait for the pass-3 prefetch).
+      // Smaller k keep the pass-1 store: their kernels stay within 128
+      // registers that way.
+      constexpr bool DEFER = RESID && K >= 5;
+      T              bnd[DEFER ? NCOL : 1];
+      bool           bpend[DEFER ? NCOL : 1];
+#pragma unroll
+      for (int c = 0; c < (DEFER ? NCOL : 1); ++c)
+        {
+          bnd[c]   = T(0);
+          bpend[c] = false;
+        }
       for (int cz = cz0; cz < cz1; ++cz)
         {
 #pragma unroll
@@ -170,6 +192,28 @@ namespace gtps
             {
               const int       gz   = K * cz + p;
               const long long zoff = gz * P;
+              if constexpr (DEFER)
+                {
+#pragma unroll
+                  for (int c = 0; c < NCOL; ++c)
+                    {
+                      // previous plane (x in the window just below plane p)
+                      if (bpend[c])
+                        {
+                          const T xv = w[c][K + p - 1];
+                          const T yv = bnd[c] - xv;
+                          y[cidx[c] + zoff - P] = yv;
+                          if (DOT)
+                            dot += (double)xv * (double)yv;
+                        }
+                      // this plane: loaded unconditionally (index 0 where not
+                      // needed), so no load sits behind a branch
+                      const bool      need = own[c] && (bcol[c] || gz == 0);
+                      const long long gi   = need ? cidx[c] + zoff : 0;
+                      bnd[c]               = bv[gi];
+                      bpend[c]             = need;
+                    }
+                }
               // right-hand side of this thread's pass-3 outputs, loaded now
               // so that the loads overlap passes 1 and 2
               // (not for k = 7 in FP64, where the registers would spill)
@@ -212,7 +256,7 @@ namespace gtps
                       sB[col] = b;
                       if (DOT)
                         sX0[col] = w[c][K + p];
-                      if (own[c] && (bcol[c] || gz == 0))
+                      if (!DEFER && own[c] && (bcol[c] || gz == 0))
                         {
                           const long long g  = cidx[c] + zoff;
                           const T         xv = w[c][K + p];
@@ -333,6 +377,20 @@ namespace gtps
                     }
                 }
             }
+        }
+      // the chunk's last plane (x now just below the window centre)
+      if constexpr (DEFER)
+        {
+#pragma unroll
+          for (int c = 0; c < NCOL; ++c)
+            if (bpend[c])
+              {
+                const T xv = w[c][K - 1];
+                const T yv = bnd[c] - xv;
+                y[cidx[c] + (long long)(K * cz1 - 1) * P] = yv;
+                if (DOT)
+                  dot += (double)xv * (double)yv;
+              }
         }
       // last plane of the domain: Dirichlet rows
       if (cz1 == n)
@@ -774,10 +832,13 @@ namespace gtps
             }
           if constexpr (K == 7)
             {
-              // 3 (auto) and 2: 4x4 cells / 512 threads; 0, 1: 8x2 / 512
+              // 3 (auto) and 2: 4x4 cells / 512 threads; 1: 4x4 / 1024;
+              // 0: 8x2 / 512
               const int tile = tuning().vmult_tile;
               if (tile == 2 || tile == 3)
                 return go(std::integral_constant<int, 2>{});
+              if (tile == 1)
+                return go(std::integral_constant<int, 1>{});
             }
           return go(std::integral_constant<int, 0>{});
         }

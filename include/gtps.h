@@ -108,8 +108,8 @@ const char *gtps_version(void);
  *   "vmult_tile"            z-marching vmult tile for k = 3: 0: 16x8 cells
  *                           and 512 threads, 1: 8x8 and 256, 2: 16x8 and
  *                           256, 3 (default): 0 in FP32, 1 in FP64; for
- *                           k = 7: 0 and 1: 8x2 cells and 512 threads, 2 and
- *                           3 (default): 4x4 and 512;
+ *                           k = 7: 0: 8x2 cells and 512 threads, 1: 4x4
+ *                           and 1024, 2 and 3 (default): 4x4 and 512;
  *   "mfma_smoother"         1 (default): 3D FP32 multiplicative smoothing of
  *                           degree 4..7 runs its 1D contractions on the
  *                           matrix cores (v_mfma_f32_16x16x4f32, patches

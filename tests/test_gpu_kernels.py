@@ -294,6 +294,39 @@ def test_smoother_matrix_core_path(core, ctx, k, level, reverse, mfma):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("level", [3, 4])
+def test_vmult_tiles_q7(core, ctx, level, tile, prec):
+    """The three z-marching tiles of the k = 7 vmult (8x2 cells / 512
+    threads, 4x4 / 1024, 4x4 / 512) agree with the oracle, vmult and
+    residual (level 3: a partial 8x2 tile row; the box kernel is off)."""
+    _announce("vmult-tile-q7", level, tile, prec)
+    k = 7
+    g = rng(level * 13 + tile)
+    N = n_dofs(3, k, level)
+    x, b = g.uniform(-1, 1, (2, N))
+    if prec == "fp32":
+        x = x.astype(np.float32).astype(np.float64)
+        b = b.astype(np.float32).astype(np.float64)
+    prev = core.set_tuning("vmult_tile", tile)
+    prev_box = core.set_tuning("box_vmult_max_n", 0)
+    try:
+        op = core.LaplaceOperator(ctx, 3, k, level, _dt(prec))
+        xd, bd = _to_dev(x, prec), _to_dev(b, prec)
+        y = torch.full_like(xd, float("nan"))
+        r = torch.full_like(xd, float("nan"))
+        op.vmult(y, xd)
+        op.residual(r, bd, xd)
+        torch.cuda.synchronize()
+    finally:
+        core.set_tuning("vmult_tile", prev)
+        core.set_tuning("box_vmult_max_n", prev_box)
+    ref = O.laplace_vmult(3, k, level, x)
+    assert rel_err(y.cpu().numpy(), ref) < TOL[prec]
+    assert rel_err(r.cpu().numpy(), b - ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tile", [0, 1, 2])
 @pytest.mark.parametrize("level", [3, 5])
 def test_vmult_tiles(core, ctx, level, tile, prec):
     """The three z-marching tiles of the k = 3 vmult (16x8 cells / 512
