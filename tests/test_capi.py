@@ -87,3 +87,32 @@ def test_solution_file_header_without_gpu(tmp_path):
     wrong_n.write_bytes(hdr[:32] + struct.pack("<q", n + 1) + hdr[40:] + data)
     with pytest.raises(RuntimeError):
         core.vector_info(wrong_n)
+
+
+def documented_tuning_keys():
+    text = open(HEADER).read()
+    block = text[text.index("Process-wide tuning knobs"):]
+    block = block[:block.index("*/")]
+    return re.findall(r'^\s*\*\s+"([a-z0-9_]+)"', block, flags=re.M)
+
+
+def test_tuning_knobs_round_trip_without_gpu():
+    """Every knob documented in gtps.h reads and writes back on a CPU-only
+    host (host state only); unknown keys and out-of-range values are
+    rejected with GTPS_EINVAL and leave the knob unchanged."""
+    from gputensorproductsmoother_amd import _lib
+    lib = _lib.load()
+    keys = documented_tuning_keys()
+    assert {"mfma_smoother", "mfma_waves", "vmult_tile", "patches_per_wave"} <= set(keys)
+    for k in keys:
+        v = lib.gtps_get_tuning(k.encode())
+        assert v >= 0, (k, lib.gtps_last_error())
+        assert lib.gtps_set_tuning(k.encode(), v) == v
+        assert lib.gtps_get_tuning(k.encode()) == v
+    assert lib.gtps_get_tuning(b"no_such_knob") == -1
+    assert lib.gtps_set_tuning(b"no_such_knob", 1) == -1
+    for k, bad in (("mfma_waves", 5), ("patches_per_wave", 0), ("mfma_smoother", 2),
+                   ("group_waves", 3)):
+        before = lib.gtps_get_tuning(k.encode())
+        assert lib.gtps_set_tuning(k.encode(), bad) == -1
+        assert lib.gtps_get_tuning(k.encode()) == before
